@@ -1,0 +1,188 @@
+/*
+ * lapyhip.h -- C ABI of the MI355X (gfx950) FEM hot-path library.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  No
+ * C++ exception crosses this boundary.  Loaded from Python with ctypes by
+ * lapy_amd/_lib.py.
+ *
+ * Each entry point names the reference interface it replaces (LaPy, a pure
+ * Python package; the arithmetic it delegates to numpy/scipy is what these
+ * functions compute on the GPU).
+ *
+ * Threading: one lh_ctx per host thread; a ctx owns one HIP stream and is
+ * not re-entrant.  Handles created from a ctx belong to it.
+ */
+#ifndef LAPYHIP_H
+#define LAPYHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define LH_OK              0
+#define LH_ERR_ARG         1   /* invalid argument (Python: ValueError)      */
+#define LH_ERR_OOM         2   /* device allocation failed                   */
+#define LH_ERR_HIP         3   /* HIP runtime error                          */
+#define LH_ERR_NOCONV      4   /* iterative solver did not converge          */
+#define LH_ERR_NUMERIC     5   /* breakdown (non-SPD Gram, NaN, ...)         */
+
+/* ---- dtypes -------------------------------------------------------------- */
+#define LH_F64 0
+#define LH_F32 1
+
+/* ---- element kinds ------------------------------------------------------- */
+#define LH_TRIA        0   /* lapy/solver.py:106-194  Solver._fem_tria          */
+#define LH_TRIA_ANISO  1   /* lapy/solver.py:197-308  Solver._fem_tria_aniso    */
+#define LH_TRIA_MASS   2   /* lapy/solver.py:311-377  Solver.fem_tria_mass      */
+#define LH_TET         3   /* lapy/solver.py:380-533  Solver._fem_tetra         */
+
+typedef struct lh_ctx  lh_ctx;   /* device, stream, scratch, last error        */
+typedef struct lh_mesh lh_mesh;  /* device-resident mesh + cached topology     */
+typedef struct lh_csr  lh_csr;   /* device-resident CSR matrix (n x n)         */
+typedef struct lh_eig  lh_eig;   /* device-resident eigen result               */
+
+/* Per-solve statistics returned by the solvers. */
+typedef struct lh_stats {
+    int32_t iterations;      /* outer iterations                              */
+    int32_t converged;       /* number of converged wanted pairs / rhs        */
+    int32_t fallback_used;   /* 1 if the learned preconditioner was replaced  */
+    int32_t precision_mode;  /* LH_PREC_*                                     */
+    double  max_resid;       /* largest final relative residual               */
+    double  t_setup_us;      /* host-measured phase times (microseconds)      */
+    double  t_iter_us;
+    double  t_rr_us;
+    double  t_total_us;
+    int32_t spmm_calls;
+    int32_t rr_calls;
+    int32_t restarts;
+    int32_t reserved;
+} lh_stats;
+
+#define LH_PREC_FP64   0   /* FP64 everywhere                                */
+#define LH_PREC_MIXED  1   /* FP32 SpMM + BF16-MFMA Gram, FP64 RR / QR / check */
+
+#define LH_PRECOND_JACOBI  0
+#define LH_PRECOND_LEARNED 1
+#define LH_PRECOND_NONE    2
+
+/* ---- context ------------------------------------------------------------- */
+/* New; the reference runs on the host and has no device context. */
+int  lh_ctx_create(int device, lh_ctx** out);
+void lh_ctx_destroy(lh_ctx* ctx);
+const char* lh_last_error(lh_ctx* ctx);
+int  lh_sync(lh_ctx* ctx);
+/* Number of visible HIP devices (0 when no GPU is present). */
+int  lh_device_count(void);
+
+/* ---- mesh ---------------------------------------------------------------- */
+/* Uploads the arrays LaPy's Solver reads from a mesh: geometry.v (nv x 3,
+ * float32 or float64 -- the element math runs in this dtype, as numpy does at
+ * lapy/solver.py:145-165) and geometry.t (ne x nodes, int64), see
+ * lapy/tria_mesh.py:238-276 and lapy/tet_mesh.py:51-59.  Indices are checked
+ * on the device against [0, nv) before any kernel gathers with them. */
+int lh_mesh_create(lh_ctx* ctx, const void* v, int v_dtype, int64_t nv,
+                   const int64_t* t, int64_t ne, int nodes, lh_mesh** out);
+/* Same, but v / t are already device pointers (v: x,y,z SoA planes of nv
+ * values each, t: int32 ne x nodes row-major); they are copied. */
+int lh_mesh_create_device(lh_ctx* ctx, const void* v_soa, int v_dtype, int64_t nv,
+                          const int32_t* t_dev, int64_t ne, int nodes, lh_mesh** out);
+void lh_mesh_free(lh_mesh* m);
+/* Matrix dimension n = max(t)+1 (scipy's inferred csc shape,
+ * lapy/solver.py:178) and pattern size; builds the topology on first use. */
+int lh_mesh_pattern_info(lh_mesh* m, int64_t* n, int64_t* nnz);
+/* Average edge length: lapy/tria_mesh.py:735-748, lapy/tet_mesh.py:182-195. */
+int lh_mesh_avg_edge_length(lh_mesh* m, double* out);
+
+/* ---- assembly -------------------------------------------------------------
+ * Replaces the numpy element math and scipy's COO->CSC duplicate summation of
+ * lapy/solver.py:106-194 (LH_TRIA), :197-308 (LH_TRIA_ANISO), :311-377
+ * (LH_TRIA_MASS, mass only, A_out ignored) and :380-533 (LH_TET).
+ * out_dtype is the Solver's dtype (lapy/solver.py:61).  lump selects the
+ * diagonal mass (lapy/solver.py:188-193).  For LH_TRIA_ANISO, u1/u2
+ * (ne x 3) and aniso_mat (ne x 2) are host arrays in v's dtype
+ * (lapy/solver.py:76-91); else pass NULL.
+ * The pattern (indptr/indices) equals scipy's exactly; values are summed in
+ * ascending element order, so A is bit-symmetric. */
+int lh_assemble(lh_ctx* ctx, lh_mesh* m, int kind, int lump, int out_dtype,
+                const void* u1, const void* u2, const void* aniso_mat,
+                lh_csr** A_out, lh_csr** B_out);
+/* Numeric phase only (topology reused), results written into existing CSR
+ * handles from an earlier lh_assemble of the same mesh and kind. */
+int lh_assemble_into(lh_ctx* ctx, lh_mesh* m, int kind, int lump,
+                     const void* u1, const void* u2, const void* aniso_mat,
+                     lh_csr* A, lh_csr* B);
+
+/* ---- CSR matrices ----------------------------------------------------------- */
+/* A CSR of a symmetric matrix equals its CSC, which is how the façade hands
+ * back scipy.sparse.csc_matrix (lapy/solver.py:100-101). */
+int  lh_csr_info(lh_csr* a, int64_t* n, int64_t* nnz, int* dtype, int* is_diag);
+int  lh_csr_download(lh_csr* a, int32_t* indptr, int32_t* indices, void* data);
+/* Upload a host CSR (e.g. a mass matrix a caller replaced with sparse.eye,
+ * lapy/diffgeo.py:149).  data dtype LH_F64 or LH_F32. */
+int  lh_csr_create(lh_ctx* ctx, int64_t n, int64_t nnz, const int32_t* indptr,
+                   const int32_t* indices, const void* data, int dtype, lh_csr** out);
+void lh_csr_free(lh_csr* a);
+/* y = a * x for nrhs row-major columns (x, y host arrays, n x nrhs, f64). */
+int  lh_csr_matmul(lh_ctx* ctx, lh_csr* a, const double* x, int nrhs, double* y);
+
+/* ---- generalized eigensolve  A x = lambda B x -------------------------------- */
+typedef struct lh_eig_opts {
+    int32_t precision;     /* LH_PREC_*                                        */
+    int32_t precond;       /* LH_PRECOND_*                                     */
+    int32_t block;         /* block size m >= k (0: automatic)                 */
+    int32_t max_iter;      /* 0: automatic                                     */
+    double  tol;           /* relative residual target (0: automatic)          */
+    uint64_t seed;         /* start block seed                                 */
+    int32_t verbose;
+    int32_t reserved;
+} lh_eig_opts;
+
+/* Replaces Solver.eigs / eigensystem (lapy/solver.py:667-716: splu of
+ * A - sigma*B, then eigsh shift-invert).  Returns the k eigenpairs nearest
+ * sigma for sigma below the spectrum (the smallest k), ascending, vectors
+ * B-orthonormal.  Results stay on the device in *out. */
+int lh_eigs(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma,
+            const lh_eig_opts* opts, lh_eig** out, lh_stats* stats);
+/* Copy results to host: evals (k), evecs (n x k, column-major = numpy
+ * Fortran order, i.e. evecs[:, j] is contiguous). Either may be NULL. */
+int  lh_eig_download(lh_eig* e, double* evals, double* evecs);
+void lh_eig_free(lh_eig* e);
+
+/* ---- linear solves ((alpha*B + beta*A) x = b) ----------------------------------
+ * Replaces the splu solves of heat.diffusion (lapy/heat.py:204-227, alpha=1,
+ * beta=t, B lumped) and Solver.poisson (lapy/solver.py:843-877, alpha=0,
+ * beta=1).  dmask (n bytes, nullable): 1 marks a Dirichlet row/column that is
+ * removed from the system (lapy/solver.py:848-865); those entries of x are
+ * left as given in x on input.  b and x are host, n x nrhs column-major.
+ * With no Dirichlet rows and alpha == 0 the operator is singular (Neumann):
+ * the right-hand side and the iterates are projected onto the complement of
+ * the constant vector and the mean-free solution is returned. */
+typedef struct lh_pcg_opts {
+    int32_t precond;       /* LH_PRECOND_*                                      */
+    int32_t max_iter;      /* 0: automatic                                      */
+    double  rtol;          /* ||r|| / ||b||                                     */
+    int32_t verbose;
+    int32_t reserved;
+} lh_pcg_opts;
+int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta,
+           const double* b, int nrhs, const uint8_t* dmask, double* x,
+           const lh_pcg_opts* opts, lh_stats* stats);
+
+/* ---- measurement helpers (no counterpart in the reference) -------------------
+ * Average duration in microseconds of one launch of a hot kernel, timed with
+ * HIP events on the context's stream over `iters` back-to-back launches.
+ * lh_time_assembly: the numeric assembly kernel of lh_assemble_into (topology
+ * already built, A/B from an lh_assemble of the same mesh/kind/lump).
+ * lh_time_spmm: the fused A/B SpMM on an n x ncol row-major block (FP64 or
+ * FP32 per `precision`). */
+int lh_time_assembly(lh_ctx* ctx, lh_mesh* m, int kind, int lump, lh_csr* A, lh_csr* B, int iters,
+                     double* avg_us);
+int lh_time_spmm(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int precision, int iters, double* avg_us);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LAPYHIP_H */

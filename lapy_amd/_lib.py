@@ -1,0 +1,374 @@
+"""ctypes binding of liblapyhip.so (the C ABI declared in include/lapyhip.h).
+
+The library is required: importing the package works without it (so that
+``build()`` can run), but every call that needs the GPU raises
+``RuntimeError`` when the shared library is missing or no HIP device is
+visible.  There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "liblapyhip.so")
+
+LH_OK, LH_ERR_ARG, LH_ERR_OOM, LH_ERR_HIP, LH_ERR_NOCONV, LH_ERR_NUMERIC = range(6)
+LH_F64, LH_F32 = 0, 1
+LH_TRIA, LH_TRIA_ANISO, LH_TRIA_MASS, LH_TET = 0, 1, 2, 3
+LH_PREC_FP64, LH_PREC_MIXED = 0, 1
+LH_PRECOND_JACOBI, LH_PRECOND_LEARNED, LH_PRECOND_NONE = 0, 1, 2
+
+_vp = C.c_void_p
+_i32 = C.c_int32
+_i64 = C.c_int64
+_dp = C.POINTER(C.c_double)
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("converged", C.c_int32),
+        ("fallback_used", C.c_int32),
+        ("precision_mode", C.c_int32),
+        ("max_resid", C.c_double),
+        ("t_setup_us", C.c_double),
+        ("t_iter_us", C.c_double),
+        ("t_rr_us", C.c_double),
+        ("t_total_us", C.c_double),
+        ("spmm_calls", C.c_int32),
+        ("rr_calls", C.c_int32),
+        ("restarts", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+class EigOpts(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32),
+        ("precond", C.c_int32),
+        ("block", C.c_int32),
+        ("max_iter", C.c_int32),
+        ("tol", C.c_double),
+        ("seed", C.c_uint64),
+        ("verbose", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class PcgOpts(C.Structure):
+    _fields_ = [
+        ("precond", C.c_int32),
+        ("max_iter", C.c_int32),
+        ("rtol", C.c_double),
+        ("verbose", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+# name -> (restype, argtypes); every symbol of include/lapyhip.h
+SIGNATURES = {
+    "lh_device_count": (C.c_int, []),
+    "lh_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "lh_ctx_destroy": (None, [_vp]),
+    "lh_last_error": (C.c_char_p, [_vp]),
+    "lh_sync": (C.c_int, [_vp]),
+    "lh_mesh_create": (C.c_int, [_vp, _vp, C.c_int, _i64, _vp, _i64, C.c_int, C.POINTER(_vp)]),
+    "lh_mesh_create_device": (C.c_int, [_vp, _vp, C.c_int, _i64, _vp, _i64, C.c_int, C.POINTER(_vp)]),
+    "lh_mesh_free": (None, [_vp]),
+    "lh_mesh_pattern_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
+    "lh_mesh_avg_edge_length": (C.c_int, [_vp, _dp]),
+    "lh_assemble": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.POINTER(_vp), C.POINTER(_vp)]),
+    "lh_assemble_into": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "lh_csr_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "lh_csr_download": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "lh_csr_create": (C.c_int, [_vp, _i64, _i64, _vp, _vp, _vp, C.c_int, C.POINTER(_vp)]),
+    "lh_csr_free": (None, [_vp]),
+    "lh_csr_matmul": (C.c_int, [_vp, _vp, _vp, C.c_int, _vp]),
+    "lh_eigs": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_double, C.POINTER(EigOpts), C.POINTER(_vp), C.POINTER(Stats)]),
+    "lh_eig_download": (C.c_int, [_vp, _vp, _vp]),
+    "lh_eig_free": (None, [_vp]),
+    "lh_pcg": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp, C.c_int, _vp, _vp, C.POINTER(PcgOpts),
+                         C.POINTER(Stats)]),
+    "lh_time_assembly": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, _dp]),
+    "lh_time_spmm": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _dp]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """Load liblapyhip.so (raises RuntimeError when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"liblapyhip.so not found at {LIB_PATH}; build it with "
+                    "`python -c 'import __graft_entry__ as g; g.build()'` (hipcc, gfx950)")
+            h = C.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                f = getattr(h, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = h
+    return _lib
+
+
+class LapyHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class NoConvergence(LapyHipError):
+    """Raised when an iterative solve stops before reaching its tolerance."""
+
+
+def check(code, ctx=None, what=""):
+    if code == LH_OK:
+        return
+    msg = lib().lh_last_error(ctx.handle if ctx is not None else None)
+    msg = (msg.decode() if msg else "") or f"error {code}"
+    if what:
+        msg = f"{what}: {msg}"
+    if code == LH_ERR_ARG:
+        raise ValueError(msg)
+    if code == LH_ERR_NOCONV:
+        raise NoConvergence(code, msg)
+    raise LapyHipError(code, msg)
+
+
+class Context:
+    """One HIP device + stream (lh_ctx)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        nd = L.lh_device_count()
+        if nd <= 0:
+            raise RuntimeError("no HIP device visible: the lapy_amd GPU path needs an MI355X (gfx950)")
+        h = _vp()
+        check(L.lh_ctx_create(device, C.byref(h)), None, "lh_ctx_create")
+        self.handle = h
+        self.device = device
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().lh_ctx_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib().lh_sync(self.handle), self, "lh_sync")
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        with _lock:
+            if _default_ctx is None:
+                _default_ctx = Context(int(os.environ.get("LAPY_AMD_DEVICE", "0")))
+    return _default_ctx
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+class DeviceCSR:
+    """Device-resident CSR matrix (lh_csr); symmetric, so its CSR is its CSC."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self.handle = handle
+        n, nnz, dt, isd = _i64(), _i64(), C.c_int(), C.c_int()
+        check(lib().lh_csr_info(handle, C.byref(n), C.byref(nnz), C.byref(dt), C.byref(isd)), ctx)
+        self.n, self.nnz = n.value, nnz.value
+        self.dtype = np.float64 if dt.value == LH_F64 else np.float32
+        self.is_diag = bool(isd.value)
+
+    @classmethod
+    def from_scipy(cls, ctx: Context, mat, dtype=None):
+        from scipy import sparse
+
+        m = sparse.csr_matrix(mat)
+        m.sum_duplicates()
+        m.sort_indices()
+        if dtype is None:
+            dtype = np.float32 if m.dtype == np.float32 else np.float64
+        data = np.ascontiguousarray(m.data, dtype=dtype)
+        indptr = np.ascontiguousarray(m.indptr, dtype=np.int32)
+        indices = np.ascontiguousarray(m.indices, dtype=np.int32)
+        if m.shape[0] != m.shape[1]:
+            raise ValueError("matrix must be square")
+        h = _vp()
+        check(lib().lh_csr_create(ctx.handle, m.shape[0], indices.size, _ptr(indptr), _ptr(indices), _ptr(data),
+                                  LH_F64 if dtype == np.float64 else LH_F32, C.byref(h)), ctx, "lh_csr_create")
+        return cls(ctx, h)
+
+    def download(self):
+        indptr = np.empty(self.n + 1, dtype=np.int32)
+        indices = np.empty(self.nnz, dtype=np.int32)
+        data = np.empty(self.nnz, dtype=self.dtype)
+        check(lib().lh_csr_download(self.handle, _ptr(indptr), _ptr(indices), _ptr(data)), self.ctx)
+        return indptr, indices, data
+
+    def to_scipy_csc(self):
+        """scipy.sparse.csc_matrix of this (symmetric) matrix."""
+        from scipy import sparse
+
+        indptr, indices, data = self.download()
+        m = sparse.csc_matrix((data, indices, indptr), shape=(self.n, self.n))
+        m.has_sorted_indices = True
+        return m
+
+    def matmul(self, x):
+        x = np.asarray(x, dtype=np.float64)
+        squeeze = x.ndim == 1
+        x2 = x[:, None] if squeeze else x
+        xr = np.ascontiguousarray(x2)  # row-major n x nrhs
+        y = np.empty_like(xr)
+        check(lib().lh_csr_matmul(self.ctx.handle, self.handle, _ptr(xr), xr.shape[1], _ptr(y)), self.ctx)
+        return y[:, 0] if squeeze else y
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().lh_csr_free(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class DeviceMesh:
+    """Device-resident mesh (lh_mesh) with its cached topology."""
+
+    def __init__(self, ctx: Context, v, t):
+        v = np.asarray(v)
+        t = np.asarray(t)
+        if v.dtype == np.float32:
+            vd, vdt = np.ascontiguousarray(v, dtype=np.float32), LH_F32
+        else:
+            vd, vdt = np.ascontiguousarray(v, dtype=np.float64), LH_F64
+        if vd.ndim != 2 or vd.shape[1] != 3:
+            raise ValueError("vertices must have shape (n, 3)")
+        td = np.ascontiguousarray(t, dtype=np.int64)
+        if td.ndim != 2 or td.shape[1] not in (3, 4):
+            raise ValueError("elements must have shape (m, 3) or (m, 4)")
+        h = _vp()
+        check(lib().lh_mesh_create(ctx.handle, _ptr(vd), vdt, vd.shape[0], _ptr(td), td.shape[0], td.shape[1],
+                                   C.byref(h)), ctx, "lh_mesh_create")
+        self.ctx = ctx
+        self.handle = h
+        self.nodes = td.shape[1]
+        self.vdtype = vd.dtype
+
+    def pattern_info(self):
+        n, nnz = _i64(), _i64()
+        check(lib().lh_mesh_pattern_info(self.handle, C.byref(n), C.byref(nnz)), self.ctx)
+        return n.value, nnz.value
+
+    def avg_edge_length(self):
+        out = C.c_double()
+        check(lib().lh_mesh_avg_edge_length(self.handle, C.byref(out)), self.ctx)
+        return out.value
+
+    def assemble(self, kind, lump, dtype=np.float64, u1=None, u2=None, aniso_mat=None):
+        a, b = _vp(), _vp()
+        odt = LH_F64 if np.dtype(dtype) == np.float64 else LH_F32
+        keep = []
+
+        def arr(x, cols):
+            if x is None:
+                return None
+            y = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, cols)
+            keep.append(y)
+            return _ptr(y)
+
+        pa = arr(u1, 3)
+        pb = arr(u2, 3)
+        pc = arr(aniso_mat, 2)
+        check(lib().lh_assemble(self.ctx.handle, self.handle, kind, int(bool(lump)), odt, pa, pb, pc,
+                                C.byref(a) if kind != LH_TRIA_MASS else None, C.byref(b)), self.ctx, "lh_assemble")
+        A = DeviceCSR(self.ctx, a) if kind != LH_TRIA_MASS else None
+        return A, DeviceCSR(self.ctx, b)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().lh_mesh_free(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def eigs(ctx: Context, A: DeviceCSR, B: DeviceCSR, k: int, sigma: float = -0.01, precision=LH_PREC_FP64,
+         precond=LH_PRECOND_JACOBI, block=0, max_iter=0, tol=0.0, seed=0, verbose=0, keep_device=False):
+    """Generalized eigenpairs on the GPU; returns (evals, evecs, stats)."""
+    opts = EigOpts(precision, precond, block, max_iter, tol, seed, verbose, 0)
+    st = Stats()
+    h = _vp()
+    code = lib().lh_eigs(ctx.handle, A.handle, B.handle, int(k), float(sigma), C.byref(opts), C.byref(h), C.byref(st))
+    try:
+        check(code, ctx, "lh_eigs")
+        evals = np.empty(k, dtype=np.float64)
+        if keep_device:
+            check(lib().lh_eig_download(h, _ptr(evals), None), ctx)
+            return evals, EigHandle(ctx, h), st
+        evecs_t = np.empty((k, A.n), dtype=np.float64)
+        check(lib().lh_eig_download(h, _ptr(evals), _ptr(evecs_t)), ctx)
+        return evals, evecs_t.T, st
+    finally:
+        if h and not keep_device:
+            lib().lh_eig_free(h)
+
+
+class EigHandle:
+    def __init__(self, ctx, h):
+        self.ctx, self.handle = ctx, h
+
+    def download(self, n, k):
+        evals = np.empty(k, dtype=np.float64)
+        evecs_t = np.empty((k, n), dtype=np.float64)
+        check(lib().lh_eig_download(self.handle, _ptr(evals), _ptr(evecs_t)), self.ctx)
+        return evals, evecs_t.T
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().lh_eig_free(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def pcg(ctx: Context, A: DeviceCSR, B, alpha: float, beta: float, b, x0=None, dmask=None, rtol=1e-12,
+        max_iter=0, precond=LH_PRECOND_JACOBI, verbose=0):
+    """Solve (alpha*B + beta*A) x = b (b: (n,) or (n, nrhs)); returns (x, stats)."""
+    b = np.asarray(b, dtype=np.float64)
+    squeeze = b.ndim == 1
+    b2 = b[:, None] if squeeze else b
+    bf = np.asfortranarray(b2)
+    x = np.asfortranarray(np.zeros_like(b2) if x0 is None else np.asarray(x0, dtype=np.float64).reshape(b2.shape))
+    dm = None if dmask is None else np.ascontiguousarray(dmask, dtype=np.uint8)
+    opts = PcgOpts(precond, max_iter, rtol, verbose, 0)
+    st = Stats()
+    code = lib().lh_pcg(ctx.handle, A.handle, B.handle if B is not None else None, float(alpha), float(beta),
+                        _ptr(bf), bf.shape[1], _ptr(dm), _ptr(x), C.byref(opts), C.byref(st))
+    check(code, ctx, "lh_pcg")
+    return (x[:, 0] if squeeze else np.ascontiguousarray(x)), st

@@ -1,0 +1,786 @@
+// Dense kernels of the block eigensolver.
+//
+//  * gram:  C = U^T V for tall-skinny U (n x p), V (n x q), row-major blocks.
+//    Split over row chunks; each workgroup writes a 64x64 partial, a second
+//    kernel adds the partials in fixed chunk order (deterministic).
+//  * gemm:  Y = alpha * U C + beta * Y, U tall-skinny, C small.
+//  * rr:    one workgroup solves the small dense generalized symmetric
+//    eigenproblem GA c = theta GM c of Rayleigh-Ritz in FP64: Cholesky of GM,
+//    H = L^-1 GA L^-T, Householder tridiagonalisation, bisection for the
+//    wanted (smallest) eigenvalues, inverse iteration for their vectors,
+//    back-transformation, C = L^-T Z.
+//  * cholinv: T = L^-T for G = L L^T (Cholesky-QR of a block).
+//
+// These replace the dense steps ARPACK performs inside eigsh
+// (lapy/solver.py:713-715); the algorithm is block LOBPCG, not Lanczos.
+#include "lh_dense.hpp"
+
+#include <cmath>
+
+namespace lh {
+
+// ---------------------------------------------------------------------------
+// Gram C = U^T V
+// ---------------------------------------------------------------------------
+constexpr int GT = 64;   // output tile
+constexpr int GR = 32;   // rows per LDS stage
+
+template <typename T, typename AT>
+__global__ void __launch_bounds__(256)
+k_gram_partial(int64_t n, int64_t rows_per_chunk, const T* __restrict__ U, int ldu, int p,
+               const T* __restrict__ V, int ldv, int q, AT* __restrict__ part) {
+    __shared__ AT su[GR][GT + 1];
+    __shared__ AT sv[GR][GT + 1];
+    const int tid = threadIdx.x;
+    const int tx = tid % 16, ty = tid / 16;
+    const int u0 = blockIdx.y * GT, v0 = blockIdx.z * GT;
+    const int64_t rb = (int64_t)blockIdx.x * rows_per_chunk;
+    const int64_t re = (rb + rows_per_chunk < n) ? rb + rows_per_chunk : n;
+    AT acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = AT(0);
+    for (int64_t r0 = rb; r0 < re; r0 += GR) {
+        // load GR x 64 of U and V (zero beyond bounds)
+        for (int idx = tid; idx < GR * GT; idx += 256) {
+            int rr = idx / GT, cc = idx % GT;
+            int64_t r = r0 + rr;
+            AT uu = AT(0), vv = AT(0);
+            if (r < re) {
+                if (u0 + cc < p) uu = (AT)U[r * ldu + u0 + cc];
+                if (v0 + cc < q) vv = (AT)V[r * ldv + v0 + cc];
+            }
+            su[rr][cc] = uu;
+            sv[rr][cc] = vv;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int rr = 0; rr < GR; ++rr) {
+            AT a[4], b[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) { a[s] = su[rr][ty * 4 + s]; b[s] = sv[rr][tx * 4 + s]; }
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[s][t] += a[s] * b[t];
+        }
+        __syncthreads();
+    }
+    const int ntile = gridDim.y * gridDim.z;
+    const int tile = blockIdx.y * gridDim.z + blockIdx.z;
+    AT* dst = part + ((int64_t)blockIdx.x * ntile + tile) * (GT * GT);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) dst[(ty * 4 + s) * GT + tx * 4 + t] = acc[s][t];
+}
+
+template <typename AT, typename CT>
+__global__ void k_gram_reduce(int nchunk, int p, int q, int ntz, const AT* __restrict__ part, CT* __restrict__ C,
+                              int ldc, double alpha) {
+    // one thread per output entry
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= p * q) return;
+    const int i = idx / q, j = idx % q;
+    const int ty = i / GT, tz = j / GT;
+    const int tile = ty * ntz + tz;
+    const int ntile = ((p + GT - 1) / GT) * ntz;
+    const int off = (i % GT) * GT + (j % GT);
+    AT s = AT(0);
+    for (int c = 0; c < nchunk; ++c) s += part[((int64_t)c * ntile + tile) * (GT * GT) + off];
+    C[(int64_t)i * ldc + j] = (CT)(alpha * (double)s);
+}
+
+template <typename T>
+void gram(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, int ldv, int q, double* C, int ldc,
+          DevBuf& ws, double alpha) {
+    const int tu = (p + GT - 1) / GT, tv = (q + GT - 1) / GT;
+    int64_t nchunk = std::min<int64_t>(std::max<int64_t>(1, (int64_t)ctx->n_cu * 4 / (tu * tv)), (n + 255) / 256);
+    if (nchunk < 1) nchunk = 1;
+    int64_t rpc = (n + nchunk - 1) / nchunk;
+    rpc = ((rpc + GR - 1) / GR) * GR;
+    nchunk = (n + rpc - 1) / rpc;
+    if (nchunk < 1) nchunk = 1;
+    ws.ensure((size_t)nchunk * tu * tv * GT * GT * sizeof(double));
+    dim3 grid((unsigned)nchunk, tu, tv);
+    k_gram_partial<T, double><<<grid, 256, 0, ctx->stream>>>(n, rpc, U, ldu, p, V, ldv, q, ws.as<double>());
+    LH_CHECK_LAUNCH();
+    k_gram_reduce<double, double><<<nblocks((int64_t)p * q, 256), 256, 0, ctx->stream>>>(
+        (int)nchunk, p, q, tv, ws.as<double>(), C, ldc, alpha);
+    LH_CHECK_LAUNCH();
+}
+template void gram<double>(lh_ctx*, int64_t, const double*, int, int, const double*, int, int, double*, int, DevBuf&,
+                           double);
+template void gram<float>(lh_ctx*, int64_t, const float*, int, int, const float*, int, int, double*, int, DevBuf&,
+                          double);
+
+// ---------------------------------------------------------------------------
+// Y = alpha U C + beta Y     U: n x p (ldu), C: p x q (ldc, FP64), Y: n x q (ldy)
+// ---------------------------------------------------------------------------
+constexpr int MT = 64, NT = 64, KT = 32;
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_nn(int64_t n, int p, int q, const T* __restrict__ U, int ldu, const double* __restrict__ Cm, int ldc,
+          T* __restrict__ Y, int ldy, double alpha, double beta) {
+    __shared__ double sU[MT][KT + 1];
+    __shared__ double sC[KT][NT + 1];
+    const int tid = threadIdx.x;
+    const int tx = tid % 16, ty = tid / 16;
+    const int64_t row0 = (int64_t)blockIdx.x * MT;
+    const int col0 = blockIdx.y * NT;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int k0 = 0; k0 < p; k0 += KT) {
+        for (int idx = tid; idx < MT * KT; idx += 256) {
+            int r = idx / KT, kk = idx % KT;
+            int64_t gr = row0 + r;
+            sU[r][kk] = (gr < n && k0 + kk < p) ? (double)U[gr * ldu + k0 + kk] : 0.0;
+        }
+        for (int idx = tid; idx < KT * NT; idx += 256) {
+            int kk = idx / NT, c = idx % NT;
+            sC[kk][c] = (k0 + kk < p && col0 + c < q) ? Cm[(int64_t)(k0 + kk) * ldc + col0 + c] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < KT; ++kk) {
+            double a[4], b[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) { a[s] = sU[ty * 4 + s][kk]; b[s] = sC[kk][tx * 4 + s]; }
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[s][t] += a[s] * b[t];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        int64_t gr = row0 + ty * 4 + s;
+        if (gr >= n) continue;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            int gc = col0 + tx * 4 + t;
+            if (gc >= q) continue;
+            double v = alpha * acc[s][t];
+            if (beta != 0.0) v += beta * (double)Y[gr * ldy + gc];
+            Y[gr * ldy + gc] = (T)v;
+        }
+    }
+}
+
+template <typename T>
+void gemm_nn(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const double* C, int ldc, T* Y, int ldy,
+             double alpha, double beta) {
+    if (n == 0 || q == 0) return;
+    dim3 grid(nblocks(n, MT), (q + NT - 1) / NT);
+    k_gemm_nn<T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta);
+    LH_CHECK_LAUNCH();
+}
+template void gemm_nn<double>(lh_ctx*, int64_t, int, int, const double*, int, const double*, int, double*, int, double,
+                              double);
+template void gemm_nn<float>(lh_ctx*, int64_t, int, int, const float*, int, const double*, int, float*, int, double,
+                             double);
+
+// ---------------------------------------------------------------------------
+// single-workgroup dense routines (1024 threads)
+// ---------------------------------------------------------------------------
+constexpr int RB = 1024;
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+// block-wide sum; red must hold RB/64 doubles; all threads get the result
+__device__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    const int nw = blockDim.x >> 6;
+    for (int i = 0; i < nw; ++i) s += red[i];
+    return s;
+}
+
+// Cholesky G = L L^T in place (lower, row-major N x N, ld N).  Returns false
+// on a non-positive pivot relative to the largest diagonal.
+__device__ bool chol_inplace(double* G, int N, double* red, int* flag) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        double mx = 0.0;
+        for (int i = 0; i < N; ++i) mx = fmax(mx, fabs(G[i * N + i]));
+        red[0] = mx;
+    }
+    __syncthreads();
+    const double dmax = red[0];
+    __syncthreads();
+    for (int j = 0; j < N; ++j) {
+        if (tid == 0) {
+            double d = G[j * N + j];
+            if (!(d > 1e-14 * dmax)) { flag[0] = 1; d = fmax(fabs(d), 1e-300); }
+            G[j * N + j] = sqrt(d);
+        }
+        __syncthreads();
+        const double ljj = G[j * N + j];
+        for (int i = j + 1 + tid; i < N; i += RB) G[i * N + j] /= ljj;
+        __syncthreads();
+        const int L = N - j - 1;
+        for (int idx = tid; idx < L * L; idx += RB) {
+            int i = j + 1 + idx / L, k = j + 1 + idx % L;
+            if (k <= i) G[i * N + k] -= G[i * N + j] * G[k * N + j];
+        }
+        __syncthreads();
+    }
+    // zero the strict upper triangle
+    for (int idx = tid; idx < N * N; idx += RB) {
+        int i = idx / N, k = idx % N;
+        if (k > i) G[idx] = 0.0;
+    }
+    __syncthreads();
+    return flag[0] == 0;
+}
+
+// Solve L X = B in place on B's columns (N x ncols, ld ldb), L lower (ld N)
+__device__ void trsm_lower_cols(const double* L, int N, double* B, int ldb, int ncols) {
+    for (int c = threadIdx.x; c < ncols; c += RB) {
+        for (int i = 0; i < N; ++i) {
+            double s = B[(int64_t)i * ldb + c];
+            for (int k = 0; k < i; ++k) s -= L[i * N + k] * B[(int64_t)k * ldb + c];
+            B[(int64_t)i * ldb + c] = s / L[i * N + i];
+        }
+    }
+    __syncthreads();
+}
+// Solve L^T X = B in place on B's columns
+__device__ void trsm_lowerT_cols(const double* L, int N, double* B, int ldb, int ncols) {
+    for (int c = threadIdx.x; c < ncols; c += RB) {
+        for (int i = N - 1; i >= 0; --i) {
+            double s = B[(int64_t)i * ldb + c];
+            for (int k = i + 1; k < N; ++k) s -= L[k * N + i] * B[(int64_t)k * ldb + c];
+            B[(int64_t)i * ldb + c] = s / L[i * N + i];
+        }
+    }
+    __syncthreads();
+}
+
+// Householder tridiagonalisation of symmetric H (N x N, ld N) in place.
+// On exit d[0..N), e[0..N-1), tau[0..N-2); reflector k is stored in
+// H[k+1..N-1][k] (its first entry explicitly).
+__device__ void tridiag(double* H, int N, double* d, double* e, double* tau, double* v, double* p, double* red) {
+    const int tid = threadIdx.x;
+    for (int k = 0; k < N - 2; ++k) {
+        const int L = N - k - 1;
+        double part = 0.0;
+        for (int i = tid; i < L; i += RB) {
+            double x = H[(k + 1 + i) * N + k];
+            v[i] = x;
+            part += x * x;
+        }
+        const double xn2 = block_sum(part, red);
+        const double x0 = v[0];
+        double alpha, t;
+        if (xn2 == 0.0 || (L == 1)) {
+            // nothing to annihilate below the subdiagonal
+            if (tid == 0) { d[k] = H[k * N + k]; e[k] = x0; tau[k] = 0.0; }
+            __syncthreads();
+            continue;
+        }
+        const double nx = sqrt(xn2);
+        alpha = (x0 >= 0.0) ? -nx : nx;
+        const double v0 = x0 - alpha;
+        const double vn2 = xn2 - x0 * x0 + v0 * v0;
+        t = 2.0 / vn2;
+        __syncthreads();
+        if (tid == 0) {
+            v[0] = v0;
+            d[k] = H[k * N + k];
+            e[k] = alpha;
+            tau[k] = t;
+            H[(k + 1) * N + k] = v0;
+        }
+        __syncthreads();
+        // p = t * H22 v   (groups of 8 threads per row)
+        const int G = 8;
+        const int gi = tid / G, gl = tid % G;
+        for (int r = gi; r < L; r += RB / G) {
+            const double* hr = H + (int64_t)(k + 1 + r) * N + (k + 1);
+            double s = 0.0;
+            for (int j = gl; j < L; j += G) s += hr[j] * v[j];
+            for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            if (gl == 0) p[r] = t * s;
+        }
+        __syncthreads();
+        double pp = 0.0;
+        for (int i = tid; i < L; i += RB) pp += p[i] * v[i];
+        const double pv = block_sum(pp, red);
+        const double K = 0.5 * t * pv;
+        for (int i = tid; i < L; i += RB) p[i] -= K * v[i];  // p := w
+        __syncthreads();
+        for (int idx = tid; idx < L * L; idx += RB) {
+            int i = idx / L, j = idx % L;
+            H[(int64_t)(k + 1 + i) * N + (k + 1 + j)] -= v[i] * p[j] + p[i] * v[j];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (N >= 2) {
+            d[N - 2] = H[(N - 2) * N + (N - 2)];
+            e[N - 2] = H[(N - 1) * N + (N - 2)];
+            tau[N - 2] = 0.0;
+        }
+        d[N - 1] = H[(N - 1) * N + (N - 1)];
+    }
+    __syncthreads();
+}
+
+// number of eigenvalues of T(d, e2 = e^2) strictly below x
+__device__ int sturm_count(const double* d, const double* e2, int N, double x, double pivmin) {
+    int cnt = 0;
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    if (q < 0) ++cnt;
+    for (int i = 1; i < N; ++i) {
+        q = (d[i] - x) - e2[i - 1] / q;
+        if (fabs(q) < pivmin) q = -pivmin;
+        if (q < 0) ++cnt;
+    }
+    return cnt;
+}
+
+// Solve (T - lam I) y = y in place by LU with partial pivoting; ws holds 5N.
+__device__ void tri_solve_shifted(const double* d, const double* e, int N, double lam, double tiny, double* y,
+                                  double* ws) {
+    double* u0 = ws;
+    double* u1 = ws + N;
+    double* u2 = ws + 2 * N;
+    double* lm = ws + 3 * N;
+    double* pv = ws + 4 * N;
+    double c0 = d[0] - lam, c1 = (N > 1) ? e[0] : 0.0, c2 = 0.0;
+    for (int i = 0; i < N - 1; ++i) {
+        const double sub = e[i];
+        const double a1 = d[i + 1] - lam;
+        const double b1 = (i + 1 < N - 1) ? e[i + 1] : 0.0;
+        if (fabs(c0) >= fabs(sub)) {
+            double l = (c0 != 0.0) ? sub / c0 : 0.0;
+            u0[i] = (c0 != 0.0) ? c0 : tiny;
+            u1[i] = c1;
+            u2[i] = c2;
+            lm[i] = l;
+            pv[i] = 0.0;
+            c0 = a1 - l * c1;
+            c1 = b1 - l * c2;
+            c2 = 0.0;
+        } else {
+            double l = c0 / sub;
+            u0[i] = sub;
+            u1[i] = a1;
+            u2[i] = b1;
+            lm[i] = l;
+            pv[i] = 1.0;
+            c0 = c1 - l * a1;
+            c1 = c2 - l * b1;
+            c2 = 0.0;
+        }
+    }
+    u0[N - 1] = (fabs(c0) > tiny) ? c0 : tiny;
+    // forward
+    for (int i = 0; i < N - 1; ++i) {
+        if (pv[i] != 0.0) { double tmp = y[i]; y[i] = y[i + 1]; y[i + 1] = tmp; }
+        y[i + 1] -= lm[i] * y[i];
+    }
+    // back
+    y[N - 1] /= u0[N - 1];
+    if (N >= 2) y[N - 2] = (y[N - 2] - u1[N - 2] * y[N - 1]) / u0[N - 2];
+    for (int i = N - 3; i >= 0; --i) y[i] = (y[i] - u1[i] * y[i + 1] - u2[i] * y[i + 2]) / u0[i];
+}
+
+// Smallest nev eigenpairs of T (d, e): lam[0..nev), Z (N x nev, ld nev).
+__device__ void tri_eig_lowest(const double* d, const double* e, int N, int nev, double* lam, double* Z, double* e2,
+                               double* ws /* nev * 6N */, double* red) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < N - 1; i += RB) e2[i] = e[i] * e[i];
+    if (tid == 0) {
+        double lo = d[0], hi = d[0], nrm = 0.0;
+        for (int i = 0; i < N; ++i) {
+            double r = (i > 0 ? fabs(e[i - 1]) : 0.0) + (i < N - 1 ? fabs(e[i]) : 0.0);
+            lo = fmin(lo, d[i] - r);
+            hi = fmax(hi, d[i] + r);
+            nrm = fmax(nrm, fabs(d[i]) + r);
+        }
+        red[0] = lo;
+        red[1] = hi;
+        red[2] = fmax(nrm, 1e-300);
+    }
+    __syncthreads();
+    const double glo = red[0], ghi = red[1], tnorm = red[2];
+    const double pivmin = 1e-300 + 4.0 * 2.2e-308 * tnorm;
+    __syncthreads();
+    // bisection, one thread per wanted eigenvalue
+    for (int j = tid; j < nev; j += RB) {
+        double a = glo - 1e-14 * tnorm, b = ghi + 1e-14 * tnorm;
+        for (int it = 0; it < 200; ++it) {
+            double mid = 0.5 * (a + b);
+            if (mid <= a || mid >= b) break;
+            if (b - a <= 2.2e-16 * (fabs(a) + fabs(b)) + 1e-17 * tnorm) break;
+            if (sturm_count(d, e2, N, mid, pivmin) > j) b = mid;
+            else a = mid;
+        }
+        lam[j] = 0.5 * (a + b);
+    }
+    __syncthreads();
+    // inverse iteration; groups of numerically equal eigenvalues are handled
+    // by their first member, which orthogonalises within the group.
+    const double ctol = 1e-10 * tnorm;
+    const double tiny = 2.2e-16 * tnorm;
+    for (int j = tid; j < nev; j += RB) {
+        const bool leader = (j == 0) || (lam[j] - lam[j - 1] > ctol);
+        if (!leader) continue;
+        int gend = j + 1;
+        while (gend < nev && lam[gend] - lam[gend - 1] <= ctol) ++gend;
+        double* w = ws + (int64_t)j * 6 * N;
+        double* y = w + 5 * N;
+        for (int g = j; g < gend; ++g) {
+            // start vector: deterministic, not orthogonal to anything special
+            for (int i = 0; i < N; ++i) y[i] = 1.0 + 0.37 * sin(0.71 * (i + 1) * (g + 1));
+            const double shift = lam[g] + (double)(g - j) * 4.0 * tiny;
+            for (int it = 0; it < 3; ++it) {
+                tri_solve_shifted(d, e, N, shift, tiny, y, w);
+                for (int h = j; h < g; ++h) {  // orthogonalise against group
+                    double s = 0.0;
+                    for (int i = 0; i < N; ++i) s += y[i] * Z[(int64_t)i * nev + h];
+                    for (int i = 0; i < N; ++i) y[i] -= s * Z[(int64_t)i * nev + h];
+                }
+                double nn = 0.0;
+                for (int i = 0; i < N; ++i) nn += y[i] * y[i];
+                nn = 1.0 / sqrt(fmax(nn, 1e-300));
+                for (int i = 0; i < N; ++i) y[i] *= nn;
+            }
+            for (int i = 0; i < N; ++i) Z[(int64_t)i * nev + g] = y[i];
+        }
+    }
+    __syncthreads();
+}
+
+// Z := Q Z with Q = H_0 ... H_{N-3} (reflectors stored below H's diagonal)
+__device__ void apply_q(const double* H, const double* tau, int N, double* Z, int nev, double* dots) {
+    const int tid = threadIdx.x;
+    for (int k = N - 3; k >= 0; --k) {
+        const double t = tau[k];
+        if (t == 0.0) continue;
+        const int L = N - k - 1;
+        // dots[c] = v^T Z[k+1:, c]
+        const int G = 16;
+        const int gi = tid / G, gl = tid % G;
+        for (int c = gi; c < nev; c += RB / G) {
+            double s = 0.0;
+            for (int i = gl; i < L; i += G) s += H[(int64_t)(k + 1 + i) * N + k] * Z[(int64_t)(k + 1 + i) * nev + c];
+            for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            if (gl == 0) dots[c] = t * s;
+        }
+        __syncthreads();
+        for (int idx = tid; idx < L * nev; idx += RB) {
+            int i = idx / nev, c = idx % nev;
+            Z[(int64_t)(k + 1 + i) * nev + c] -= H[(int64_t)(k + 1 + i) * N + k] * dots[c];
+        }
+        __syncthreads();
+    }
+}
+
+// Rayleigh-Ritz: GA, GM (N x N) -> C (N x nev), theta (nev).  status bit 1:
+// GM not positive definite.
+__global__ void __launch_bounds__(RB) k_rr(int N, int nev, double* __restrict__ GA, double* __restrict__ GM,
+                                           double* __restrict__ C, double* __restrict__ theta,
+                                           double* __restrict__ work, int* __restrict__ status) {
+    __shared__ double red[64];
+    __shared__ int flag[1];
+    const int tid = threadIdx.x;
+    if (tid == 0) flag[0] = 0;
+    // symmetrise
+    for (int idx = tid; idx < N * N; idx += RB) {
+        int i = idx / N, j = idx % N;
+        if (j > i) {
+            double a = 0.5 * (GA[i * N + j] + GA[j * N + i]);
+            GA[i * N + j] = a;
+            GA[j * N + i] = a;
+            double b = 0.5 * (GM[i * N + j] + GM[j * N + i]);
+            GM[i * N + j] = b;
+            GM[j * N + i] = b;
+        }
+    }
+    __syncthreads();
+    chol_inplace(GM, N, red, flag);
+    if (flag[0]) {
+        if (tid == 0) status[0] = 1;
+        return;
+    }
+    // H = L^-1 GA L^-T : Y = L^-1 GA (in place in GA), then H = L^-1 Y^T
+    trsm_lower_cols(GM, N, GA, N, N);
+    double* H = work;  // N x N
+    for (int idx = tid; idx < N * N; idx += RB) {
+        int i = idx / N, j = idx % N;
+        H[idx] = GA[j * N + i];
+    }
+    __syncthreads();
+    trsm_lower_cols(GM, N, H, N, N);
+    for (int idx = tid; idx < N * N; idx += RB) {
+        int i = idx / N, j = idx % N;
+        if (j > i) {
+            double a = 0.5 * (H[i * N + j] + H[j * N + i]);
+            H[i * N + j] = a;
+            H[j * N + i] = a;
+        }
+    }
+    __syncthreads();
+    double* d = work + N * N;
+    double* e = d + N;
+    double* tau = e + N;
+    double* v = tau + N;
+    double* p = v + N;
+    double* e2 = p + N;
+    double* dots = e2 + N;     // nev
+    double* Z = dots + N + nev;  // N x nev
+    double* ws = Z + (int64_t)N * nev;  // nev * 6N
+    tridiag(H, N, d, e, tau, v, p, red);
+    tri_eig_lowest(d, e, N, nev, theta, Z, e2, ws, red);
+    apply_q(H, tau, N, Z, nev, dots);
+    // C = L^-T Z
+    for (int idx = tid; idx < N * nev; idx += RB) C[idx] = Z[idx];
+    __syncthreads();
+    trsm_lowerT_cols(GM, N, C, nev, nev);
+    if (tid == 0) status[0] = 0;
+}
+
+size_t rr_work_doubles(int N, int nev) {
+    return (size_t)N * N + 7 * (size_t)N + nev + (size_t)N * nev + (size_t)nev * 6 * N + 64;
+}
+
+void rr_solve(lh_ctx* ctx, int N, int nev, double* GA, double* GM, double* C, double* theta, double* work,
+              int* status) {
+    k_rr<<<1, RB, 0, ctx->stream>>>(N, nev, GA, GM, C, theta, work, status);
+    LH_CHECK_LAUNCH();
+}
+
+// T = L^-T where G = L L^T (m x m).  status 1 on failure.  Optional diagonal
+// shift (relative to the largest diagonal) before factorising.
+__global__ void __launch_bounds__(RB) k_cholinv(int m, double* __restrict__ G, double* __restrict__ T,
+                                                double shift, int* __restrict__ status) {
+    __shared__ double red[64];
+    __shared__ int flag[1];
+    const int tid = threadIdx.x;
+    if (tid == 0) flag[0] = 0;
+    for (int idx = tid; idx < m * m; idx += RB) {
+        int i = idx / m, j = idx % m;
+        if (j > i) {
+            double a = 0.5 * (G[i * m + j] + G[j * m + i]);
+            G[i * m + j] = a;
+            G[j * m + i] = a;
+        }
+    }
+    __syncthreads();
+    if (shift > 0.0) {
+        if (tid == 0) {
+            double mx = 0.0;
+            for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(G[i * m + i]));
+            red[0] = mx;
+        }
+        __syncthreads();
+        const double s = shift * red[0];
+        __syncthreads();
+        for (int i = tid; i < m; i += RB) G[i * m + i] += s;
+        __syncthreads();
+    }
+    chol_inplace(G, m, red, flag);
+    if (flag[0]) {
+        if (tid == 0) status[0] = 1;
+        return;
+    }
+    for (int idx = tid; idx < m * m; idx += RB) T[idx] = ((idx / m) == (idx % m)) ? 1.0 : 0.0;
+    __syncthreads();
+    trsm_lowerT_cols(G, m, T, m, m);
+    if (tid == 0) status[0] = 0;
+}
+
+void cholinv(lh_ctx* ctx, int m, double* G, double* T, double shift, int* status) {
+    k_cholinv<<<1, RB, 0, ctx->stream>>>(m, G, T, shift, status);
+    LH_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+// block utilities
+// ---------------------------------------------------------------------------
+// R = AX - MX diag(theta), R written with ldr; X blocks with ldx
+template <typename T>
+__global__ void k_residual(int64_t n, int m, const T* __restrict__ AX, const T* __restrict__ MX, int ldx,
+                           const double* __restrict__ theta, T* __restrict__ R, int ldr) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = gid / m;
+    int c = (int)(gid % m);
+    if (i >= n) return;
+    R[i * ldr + c] = (T)((double)AX[i * ldx + c] - theta[c] * (double)MX[i * ldx + c]);
+}
+template <typename T>
+void residual(lh_ctx* ctx, int64_t n, int m, const T* AX, const T* MX, int ldx, const double* theta, T* R, int ldr) {
+    k_residual<T><<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, AX, MX, ldx, theta, R, ldr);
+    LH_CHECK_LAUNCH();
+}
+template void residual<double>(lh_ctx*, int64_t, int, const double*, const double*, int, const double*, double*, int);
+template void residual<float>(lh_ctx*, int64_t, int, const float*, const float*, int, const double*, float*, int);
+
+// per-column sums of squares of a block (n x m, ld), deterministic
+template <typename T>
+__global__ void k_colsq_partial(int64_t n, int m, const T* __restrict__ X, int ld, int64_t rpc,
+                                double* __restrict__ part) {
+    const int c = threadIdx.x % 64 + 64 * blockIdx.y;
+    const int rg = threadIdx.x / 64;  // 4 row groups
+    __shared__ double s[4][64];
+    double acc = 0.0;
+    if (c < m) {
+        int64_t rb = (int64_t)blockIdx.x * rpc, re = min(rb + rpc, n);
+        for (int64_t i = rb + rg; i < re; i += 4) {
+            double v = (double)X[i * ld + c];
+            acc += v * v;
+        }
+    }
+    s[rg][threadIdx.x % 64] = acc;
+    __syncthreads();
+    if (rg == 0 && c < m) part[(int64_t)blockIdx.x * m + c] = ((s[0][threadIdx.x] + s[1][threadIdx.x]) + s[2][threadIdx.x]) + s[3][threadIdx.x];
+}
+__global__ void k_colsq_reduce(int nchunk, int m, const double* __restrict__ part, double* __restrict__ out) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    double s = 0.0;
+    for (int k = 0; k < nchunk; ++k) s += part[(int64_t)k * m + c];
+    out[c] = s;
+}
+template <typename T>
+void colsq(lh_ctx* ctx, int64_t n, int m, const T* X, int ld, double* out, DevBuf& ws) {
+    int64_t nchunk = std::min<int64_t>(512, std::max<int64_t>(1, n / 256));
+    int64_t rpc = (n + nchunk - 1) / nchunk;
+    nchunk = (n + rpc - 1) / rpc;
+    ws.ensure(nchunk * m * sizeof(double));
+    dim3 grid((unsigned)nchunk, (m + 63) / 64);
+    k_colsq_partial<T><<<grid, 256, 0, ctx->stream>>>(n, m, X, ld, rpc, ws.as<double>());
+    LH_CHECK_LAUNCH();
+    k_colsq_reduce<<<nblocks(m, 64), 64, 0, ctx->stream>>>((int)nchunk, m, ws.as<double>(), out);
+    LH_CHECK_LAUNCH();
+}
+template void colsq<double>(lh_ctx*, int64_t, int, const double*, int, double*, DevBuf&);
+template void colsq<float>(lh_ctx*, int64_t, int, const float*, int, double*, DevBuf&);
+
+// X[:, c] *= s[c]
+template <typename T>
+__global__ void k_colscale(int64_t n, int m, T* __restrict__ X, int ld, const double* __restrict__ s) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = gid / m;
+    int c = (int)(gid % m);
+    if (i >= n) return;
+    X[i * ld + c] = (T)((double)X[i * ld + c] * s[c]);
+}
+template <typename T>
+void colscale(lh_ctx* ctx, int64_t n, int m, T* X, int ld, const double* s) {
+    k_colscale<T><<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, X, ld, s);
+    LH_CHECK_LAUNCH();
+}
+template void colscale<double>(lh_ctx*, int64_t, int, double*, int, const double*);
+template void colscale<float>(lh_ctx*, int64_t, int, float*, int, const double*);
+
+// X[i, c] *= dinv[i]  (Jacobi)
+template <typename T>
+__global__ void k_rowscale(int64_t n, int m, T* __restrict__ X, int ld, const double* __restrict__ dinv) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = gid / m;
+    int c = (int)(gid % m);
+    if (i >= n) return;
+    X[i * ld + c] = (T)((double)X[i * ld + c] * dinv[i]);
+}
+template <typename T>
+void rowscale(lh_ctx* ctx, int64_t n, int m, T* X, int ld, const double* dinv) {
+    k_rowscale<T><<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, X, ld, dinv);
+    LH_CHECK_LAUNCH();
+}
+template void rowscale<double>(lh_ctx*, int64_t, int, double*, int, const double*);
+template void rowscale<float>(lh_ctx*, int64_t, int, float*, int, const double*);
+
+// copy a column block
+template <typename S, typename D>
+__global__ void k_copy_block(int64_t n, int m, const S* __restrict__ src, int lds, D* __restrict__ dst, int ldd) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = gid / m;
+    int c = (int)(gid % m);
+    if (i >= n) return;
+    dst[i * ldd + c] = (D)src[i * lds + c];
+}
+template <typename S, typename D>
+void copy_block(lh_ctx* ctx, int64_t n, int m, const S* src, int lds, D* dst, int ldd) {
+    if (n * m == 0) return;
+    k_copy_block<S, D><<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, src, lds, dst, ldd);
+    LH_CHECK_LAUNCH();
+}
+template void copy_block<double, double>(lh_ctx*, int64_t, int, const double*, int, double*, int);
+template void copy_block<double, float>(lh_ctx*, int64_t, int, const double*, int, float*, int);
+template void copy_block<float, double>(lh_ctx*, int64_t, int, const float*, int, double*, int);
+template void copy_block<float, float>(lh_ctx*, int64_t, int, const float*, int, float*, int);
+
+// seeded standard-normal block (splitmix64 + Box-Muller)
+__device__ __forceinline__ uint64_t splitmix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+template <typename T>
+__global__ void k_randn(int64_t n, int m, T* __restrict__ X, int ld, uint64_t seed) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = gid / m;
+    int c = (int)(gid % m);
+    if (i >= n) return;
+    uint64_t h1 = splitmix(seed ^ (uint64_t)gid * 0x9E3779B97F4A7C15ull);
+    uint64_t h2 = splitmix(h1);
+    double u1 = ((h1 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    double u2 = ((h2 >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    X[i * ld + c] = (T)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+}
+template <typename T>
+void randn(lh_ctx* ctx, int64_t n, int m, T* X, int ld, uint64_t seed) {
+    k_randn<T><<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, X, ld, seed);
+    LH_CHECK_LAUNCH();
+}
+template void randn<double>(lh_ctx*, int64_t, int, double*, int, uint64_t);
+template void randn<float>(lh_ctx*, int64_t, int, float*, int, uint64_t);
+
+// dinv[i] = 1 / (diag(A)[i] - sigma * diag(B)[i])
+template <typename VT>
+__global__ void k_jacobi_diag(int64_t n, const int32_t* __restrict__ adiag, const VT* __restrict__ avals,
+                              const int32_t* __restrict__ bdiag, const VT* __restrict__ bvals, int b_dense,
+                              const VT* __restrict__ bdense, double alpha, double beta, double* __restrict__ dinv) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = adiag[i] >= 0 ? (double)avals[adiag[i]] : 0.0;
+    double b = 0.0;
+    if (b_dense) b = (double)bdense[i];
+    else if (bdiag && bdiag[i] >= 0) b = (double)bvals[bdiag[i]];
+    double dd = alpha * b + beta * a;
+    dinv[i] = (dd != 0.0) ? 1.0 / dd : 0.0;
+}
+void jacobi_diag(lh_ctx* ctx, const lh_csr* A, const lh_csr* B, double alpha, double beta, double* dinv) {
+    const int64_t n = A->n();
+    if (A->dtype == LH_F64)
+        k_jacobi_diag<double><<<nblocks(n, 256), 256, 0, ctx->stream>>>(
+            n, A->pat->diagpos.as<int32_t>(), A->vals.as<double>(), B ? B->pat->diagpos.as<int32_t>() : nullptr,
+            B ? B->vals.as<double>() : nullptr, B && B->is_diag, B && B->is_diag ? B->dense_diag.as<double>() : nullptr,
+            alpha, beta, dinv);
+    else
+        k_jacobi_diag<float><<<nblocks(n, 256), 256, 0, ctx->stream>>>(
+            n, A->pat->diagpos.as<int32_t>(), A->vals.as<float>(), B ? B->pat->diagpos.as<int32_t>() : nullptr,
+            B ? B->vals.as<float>() : nullptr, B && B->is_diag, B && B->is_diag ? B->dense_diag.as<float>() : nullptr,
+            alpha, beta, dinv);
+    LH_CHECK_LAUNCH();
+}
+
+}  // namespace lh

@@ -1,0 +1,34 @@
+// Declarations of the dense block kernels (lh_dense.hip).
+#pragma once
+#include "lh_internal.hpp"
+
+namespace lh {
+
+// C (p x q, ldc, FP64) = alpha * U^T V over n rows
+template <typename T>
+void gram(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, int ldv, int q, double* C, int ldc,
+          DevBuf& ws, double alpha = 1.0);
+// Y = alpha * U C + beta * Y
+template <typename T>
+void gemm_nn(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const double* C, int ldc, T* Y, int ldy,
+             double alpha, double beta);
+size_t rr_work_doubles(int N, int nev);
+void rr_solve(lh_ctx* ctx, int N, int nev, double* GA, double* GM, double* C, double* theta, double* work,
+              int* status);
+void cholinv(lh_ctx* ctx, int m, double* G, double* T, double shift, int* status);
+template <typename T>
+void residual(lh_ctx* ctx, int64_t n, int m, const T* AX, const T* MX, int ldx, const double* theta, T* R, int ldr);
+template <typename T>
+void colsq(lh_ctx* ctx, int64_t n, int m, const T* X, int ld, double* out, DevBuf& ws);
+template <typename T>
+void colscale(lh_ctx* ctx, int64_t n, int m, T* X, int ld, const double* s);
+template <typename T>
+void rowscale(lh_ctx* ctx, int64_t n, int m, T* X, int ld, const double* dinv);
+template <typename S, typename D>
+void copy_block(lh_ctx* ctx, int64_t n, int m, const S* src, int lds, D* dst, int ldd);
+template <typename T>
+void randn(lh_ctx* ctx, int64_t n, int m, T* X, int ld, uint64_t seed);
+// dinv = 1 / (alpha * diag(B) + beta * diag(A))
+void jacobi_diag(lh_ctx* ctx, const lh_csr* A, const lh_csr* B, double alpha, double beta, double* dinv);
+
+}  // namespace lh

@@ -1,0 +1,352 @@
+// Generalized symmetric eigensolver  A x = lambda B x  (smallest k pairs).
+//
+// Replaces Solver.eigs (lapy/solver.py:667-716), which factorises A - sigma*B
+// with SuperLU and runs ARPACK's shift-invert Lanczos (eigsh).  Here: block
+// LOBPCG with soft guard vectors, on the GPU, FP64:
+//
+//   S = [X | W | P]  (n x 3m, row-major, one array so that a single Gram and a
+//                    single GEMM cover the whole trial subspace)
+//   AS, MS           A*S and B*S, recomputed with SpMM after every change of a
+//                    block (more accurate than the usual GEMM recurrences and
+//                    not slower: the SpMM reads the block once, a GEMM update
+//                    reads the 3m-wide S)
+//   W = D^-1 (AX - BX Theta),  D = diag(A - sigma*B)      (Jacobi)
+//   W <- W - X (BX)^T W (twice),  Cholesky-QR in the B inner product
+//   P <- P - X (BX)^T P - W (BW)^T P,  Cholesky-QR (dropped on breakdown)
+//   Rayleigh-Ritz on span(S) with the one-workgroup FP64 solver (lh_dense)
+//   X <- S C,  P <- S[:, m:] C[m:]
+//
+// Convergence: ||A x_j - theta_j B x_j||_2 / ||B x_j||_2 <= tol * scale_j with
+// scale_j = max(|theta_j|, 1e-2 * |theta_{k-1}|) for the k wanted columns.
+#include "lh_dense.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+using namespace lh;
+
+namespace {
+
+double now_us() {
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Lobpcg {
+    lh_ctx* ctx;
+    const lh_csr* A;
+    const lh_csr* B;
+    int64_t n;
+    int k, m, ld;
+    double sigma, tol;
+    int max_iter, verbose;
+    uint64_t seed;
+    DevBuf S, S2, AS, MS, tmp, dinv;
+    DevBuf GA, GM, Cm, Gs, Ts, theta_d, rrwork, status, cs1, cs2, wsg, wsc;
+    std::vector<double> theta, res;
+    lh_stats st{};
+
+    double* Sp() { return S.as<double>(); }
+    double* ASp() { return AS.as<double>(); }
+    double* MSp() { return MS.as<double>(); }
+
+    void spmm_AB(const double* X, double* AX, double* MX, int ncol) {
+        if (A->dtype == LH_F64) spmm_pair<double, double>(ctx, A, B, X, AX, MX, ncol, ld, ld);
+        else LH_THROW(LH_ERR_ARG, "FP64 eigensolver needs FP64 matrices");
+        st.spmm_calls++;
+    }
+    void spmm_B(const double* X, double* MX, int ncol) {
+        spmm_ld<double, double>(ctx, B, X, MX, ncol, ld, ld);
+        st.spmm_calls++;
+    }
+    int read_status() {
+        int s = 0;
+        d2h(ctx, &s, status.p, sizeof(int));
+        return s;
+    }
+
+    // X block (cols [off, off+c)) <- X T, T = L^-T of G = X^T (B X); returns
+    // false on breakdown.  AX/MX of the block are recomputed afterwards.
+    bool cholqr(int off, int c, bool recompute_A) {
+        double* X = Sp() + off;
+        double* MX = MSp() + off;
+        gram<double>(ctx, n, X, ld, c, MX, ld, c, Gs.as<double>(), c, wsg);
+        cholinv(ctx, c, Gs.as<double>(), Ts.as<double>(), 0.0, status.as<int>());
+        if (read_status()) {
+            gram<double>(ctx, n, X, ld, c, MX, ld, c, Gs.as<double>(), c, wsg);
+            cholinv(ctx, c, Gs.as<double>(), Ts.as<double>(), 1e-13, status.as<int>());
+            if (read_status()) return false;
+        }
+        gemm_nn<double>(ctx, n, c, c, X, ld, Ts.as<double>(), c, tmp.as<double>(), c, 1.0, 0.0);
+        copy_block<double, double>(ctx, n, c, tmp.as<double>(), c, X, ld);
+        if (recompute_A) spmm_AB(X, ASp() + off, MX, c);
+        else spmm_B(X, MX, c);
+        return true;
+    }
+
+    // Y -= X C with C = (B X)^T Y  (X block at xoff, Y block at yoff, widths cx, cy)
+    void project_out(int xoff, int cx, int yoff, int cy) {
+        gram<double>(ctx, n, MSp() + xoff, ld, cx, Sp() + yoff, ld, cy, Cm.as<double>(), cy, wsg);
+        gemm_nn<double>(ctx, n, cx, cy, Sp() + xoff, ld, Cm.as<double>(), cy, Sp() + yoff, ld, -1.0, 1.0);
+    }
+
+    void normalize_cols(int off, int c) {
+        colsq<double>(ctx, n, c, Sp() + off, ld, cs1.as<double>(), wsc);
+        std::vector<double> h(c);
+        d2h(ctx, h.data(), cs1.p, c * 8);
+        for (int j = 0; j < c; ++j) h[j] = h[j] > 0 ? 1.0 / std::sqrt(h[j]) : 1.0;
+        h2d(ctx, cs1.p, h.data(), c * 8);
+        colscale<double>(ctx, n, c, Sp() + off, ld, cs1.as<double>());
+    }
+
+    // Rayleigh-Ritz on the first N columns of S; returns false on breakdown
+    bool rayleigh_ritz(int N) {
+        double t0 = now_us();
+        gram<double>(ctx, n, Sp(), ld, N, ASp(), ld, N, GA.as<double>(), N, wsg);
+        gram<double>(ctx, n, Sp(), ld, N, MSp(), ld, N, GM.as<double>(), N, wsg);
+        rr_solve(ctx, N, m, GA.as<double>(), GM.as<double>(), Cm.as<double>(), theta_d.as<double>(),
+                 rrwork.as<double>(), status.as<int>());
+        int s = read_status();
+        st.rr_calls++;
+        st.t_rr_us += now_us() - t0;
+        if (s) return false;
+        d2h(ctx, theta.data(), theta_d.p, m * 8);
+        return true;
+    }
+
+    void run(double* out_evals, DevBuf& out_vecs) {
+        double t_start = now_us();
+        ld = 3 * m;
+        S.alloc(n * ld * 8);
+        S2.alloc(n * ld * 8);
+        AS.alloc(n * ld * 8);
+        MS.alloc(n * ld * 8);
+        tmp.alloc(n * m * 8);
+        dinv.alloc(n * 8);
+        const int N3 = 3 * m;
+        GA.alloc(N3 * N3 * 8);
+        GM.alloc(N3 * N3 * 8);
+        Cm.alloc(N3 * std::max(N3, m) * 8);
+        Gs.alloc(m * m * 8);
+        Ts.alloc(m * m * 8);
+        theta_d.alloc(N3 * 8);
+        rrwork.alloc(rr_work_doubles(N3, m) * 8);
+        status.alloc(64);
+        cs1.alloc(N3 * 8);
+        cs2.alloc(N3 * 8);
+        theta.assign(m, 0.0);
+        res.assign(m, 0.0);
+        HIPX(hipMemsetAsync(S.p, 0, n * ld * 8, ctx->stream));
+        jacobi_diag(ctx, A, B, -sigma, 1.0, dinv.as<double>());
+
+        // start block
+        randn<double>(ctx, n, m, Sp(), ld, seed);
+        spmm_B(Sp(), MSp(), m);
+        if (!cholqr(0, m, true)) LH_THROW(LH_ERR_NUMERIC, "start block is rank deficient");
+        if (!rayleigh_ritz(m)) LH_THROW(LH_ERR_NUMERIC, "initial Rayleigh-Ritz failed");
+        gemm_nn<double>(ctx, n, m, m, Sp(), ld, Cm.as<double>(), m, tmp.as<double>(), m, 1.0, 0.0);
+        copy_block<double, double>(ctx, n, m, tmp.as<double>(), m, Sp(), ld);
+        spmm_AB(Sp(), ASp(), MSp(), m);
+        st.t_setup_us = now_us() - t_start;
+
+        bool have_p = false;
+        int it = 0;
+        double t_iter0 = now_us();
+        std::vector<double> r2(m), mx2(m);
+        for (it = 0; it < max_iter; ++it) {
+            // residual into W
+            residual<double>(ctx, n, m, ASp(), MSp(), ld, theta_d.as<double>(), Sp() + m, ld);
+            colsq<double>(ctx, n, m, Sp() + m, ld, cs1.as<double>(), wsc);
+            colsq<double>(ctx, n, m, MSp(), ld, cs2.as<double>(), wsc);
+            d2h(ctx, r2.data(), cs1.p, m * 8);
+            d2h(ctx, mx2.data(), cs2.p, m * 8);
+            const double floor = 1e-2 * std::fabs(theta[k - 1]);
+            int nconv = 0;
+            double worst = 0.0;
+            for (int j = 0; j < m; ++j) {
+                res[j] = std::sqrt(r2[j] / std::max(mx2[j], 1e-300));
+                double sc = std::max(std::fabs(theta[j]), floor);
+                double rel = res[j] / std::max(sc, 1e-300);
+                if (j < k) {
+                    worst = std::max(worst, rel);
+                    if (rel <= tol) ++nconv;
+                }
+            }
+            st.max_resid = worst;
+            st.converged = nconv;
+            if (verbose && (it % 50 == 0))
+                fprintf(stderr, "[lobpcg] it %d conv %d/%d worst %.3e theta0 %.6e theta_k %.6e\n", it, nconv, k,
+                        worst, theta[0], theta[k - 1]);
+            if (nconv == k) break;
+            // precondition
+            rowscale<double>(ctx, n, m, Sp() + m, ld, dinv.as<double>());
+            // W orthogonal to X (B inner product), twice
+            project_out(0, m, m, m);
+            project_out(0, m, m, m);
+            normalize_cols(m, m);
+            spmm_B(Sp() + m, MSp() + m, m);
+            if (!cholqr(m, m, true)) {
+                st.restarts++;
+                LH_THROW(LH_ERR_NUMERIC, "Cholesky-QR of the residual block failed at iteration %d", it);
+            }
+            int N = 2 * m;
+            if (have_p) {
+                project_out(0, m, 2 * m, m);
+                project_out(m, m, 2 * m, m);
+                normalize_cols(2 * m, m);
+                spmm_B(Sp() + 2 * m, MSp() + 2 * m, m);
+                if (cholqr(2 * m, m, true)) N = 3 * m;
+                else { have_p = false; st.restarts++; }
+            }
+            if (!rayleigh_ritz(N)) {
+                if (N == 3 * m) {
+                    st.restarts++;
+                    N = 2 * m;
+                    if (!rayleigh_ritz(N)) LH_THROW(LH_ERR_NUMERIC, "Rayleigh-Ritz failed at iteration %d", it);
+                } else {
+                    LH_THROW(LH_ERR_NUMERIC, "Rayleigh-Ritz failed at iteration %d", it);
+                }
+            }
+            // X_new = S C ; P_new = S[:, m:N] C[m:N, :]
+            double* S2p = S2.as<double>();
+            gemm_nn<double>(ctx, n, N, m, Sp(), ld, Cm.as<double>(), m, S2p, ld, 1.0, 0.0);
+            gemm_nn<double>(ctx, n, N - m, m, Sp() + m, ld, Cm.as<double>() + (size_t)m * m, m, S2p + 2 * m, ld, 1.0,
+                            0.0);
+            std::swap(S, S2);
+            spmm_AB(Sp(), ASp(), MSp(), m);
+            have_p = true;
+        }
+        HIPX(hipStreamSynchronize(ctx->stream));
+        st.iterations = it;
+        st.t_iter_us = now_us() - t_iter0;
+        for (int j = 0; j < k; ++j) out_evals[j] = theta[j];
+        out_vecs.alloc(n * k * 8);
+        copy_block<double, double>(ctx, n, k, Sp(), ld, out_vecs.as<double>(), k);
+        HIPX(hipStreamSynchronize(ctx->stream));
+        st.t_total_us = now_us() - t_start;
+        if (st.converged < k) {
+            st.iterations = it;
+        }
+    }
+};
+
+template <typename VT>
+__global__ void k_densify(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+                          const VT* __restrict__ vals, double* __restrict__ D) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int32_t s = rowptr[i]; s < rowptr[i + 1]; ++s) D[i * n + cols[s]] = (double)vals[s];
+}
+
+void densify(lh_ctx* ctx, const lh_csr* a, double* D) {
+    const int64_t n = a->n();
+    HIPX(hipMemsetAsync(D, 0, n * n * 8, ctx->stream));
+    if (a->dtype == LH_F64)
+        k_densify<double><<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, a->pat->rowptr.as<int32_t>(),
+                                                                   a->pat->cols.as<int32_t>(), a->vals.as<double>(), D);
+    else
+        k_densify<float><<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, a->pat->rowptr.as<int32_t>(),
+                                                                  a->pat->cols.as<int32_t>(), a->vals.as<float>(), D);
+    LH_CHECK_LAUNCH();
+}
+
+void dense_eigs(lh_ctx* ctx, const lh_csr* A, const lh_csr* B, int k, double* evals, DevBuf& evecs, lh_stats& st) {
+    const double t0 = now_us();
+    const int n = (int)A->n();
+    DevBuf GA((size_t)n * n * 8), GM((size_t)n * n * 8), C((size_t)n * k * 8), th((size_t)n * 8),
+        work(rr_work_doubles(n, k) * 8), status(64);
+    densify(ctx, A, GA.as<double>());
+    densify(ctx, B, GM.as<double>());
+    rr_solve(ctx, n, k, GA.as<double>(), GM.as<double>(), C.as<double>(), th.as<double>(), work.as<double>(),
+             status.as<int>());
+    int s = 0;
+    d2h(ctx, &s, status.p, 4);
+    if (s) LH_THROW(LH_ERR_NUMERIC, "mass matrix is not positive definite");
+    d2h(ctx, evals, th.p, (size_t)k * 8);
+    evecs = std::move(C);
+    st.iterations = 1;
+    st.converged = k;
+    st.rr_calls = 1;
+    st.t_total_us = now_us() - t0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lh_eigs(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const lh_eig_opts* opts, lh_eig** out,
+            lh_stats* stats) {
+    if (!ctx || !A || !B || !out) return LH_ERR_ARG;
+    *out = nullptr;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    const int64_t n = A->n();
+    if (B->n() != n) LH_THROW(LH_ERR_ARG, "A and B differ in size (%lld vs %lld)", (long long)n, (long long)B->n());
+    if (k <= 0 || k >= n) LH_THROW(LH_ERR_ARG, "k must satisfy 0 < k < n (k=%d, n=%lld)", k, (long long)n);
+    Lobpcg L{};
+    L.ctx = ctx;
+    L.A = A;
+    L.B = B;
+    L.n = n;
+    L.k = k;
+    L.sigma = sigma;
+    // block = k wanted + guard vectors, rounded to a multiple of 8
+    int m = opts && opts->block > 0 ? opts->block : ((std::max(k + 4, (k * 6 + 4) / 5) + 7) / 8) * 8;
+    m = std::max(m, k);
+    if (m > 128) m = std::max(k, 128);
+    if (3 * m > n) m = std::max(k, (int)(n / 3));
+    L.m = m;
+    L.tol = (opts && opts->tol > 0) ? opts->tol : 1e-9;
+    L.max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 20000;
+    L.seed = opts ? opts->seed : 0;
+    L.verbose = opts ? opts->verbose : 0;
+    if (opts && opts->precision != LH_PREC_FP64) LH_THROW(LH_ERR_ARG, "precision mode %d not available", opts->precision);
+    auto e = std::make_unique<lh_eig>();
+    e->ctx = ctx;
+    e->n = n;
+    e->k = k;
+    e->evals.assign(k, 0.0);
+    if (3 * m > n || n <= 384) {
+        // small problem: Rayleigh-Ritz on the whole space (S = I), i.e. the
+        // dense generalized eigensolver on the GPU
+        dense_eigs(ctx, A, B, k, e->evals.data(), e->evecs, L.st);
+        if (stats) *stats = L.st;
+        *out = e.release();
+        return LH_OK;
+    }
+    L.run(e->evals.data(), e->evecs);
+    L.st.precision_mode = LH_PREC_FP64;
+    if (stats) *stats = L.st;
+    *out = e.release();
+    if (L.st.converged < k) {
+        ctx->last_error = "LOBPCG did not converge within max_iter";
+        return LH_ERR_NOCONV;
+    }
+    LH_API_END(ctx)
+}
+
+int lh_eig_download(lh_eig* e, double* evals, double* evecs) {
+    if (!e) return LH_ERR_ARG;
+    lh_ctx* ctx = e->ctx;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    if (evals) for (int j = 0; j < e->k; ++j) evals[j] = e->evals[j];
+    if (evecs) {
+        // device row-major n x k == host column-major k x n transposed; the
+        // caller wants evecs[:, j] contiguous (Fortran order), so transpose.
+        std::vector<double> h((size_t)e->n * e->k);
+        d2h(ctx, h.data(), e->evecs.p, h.size() * 8);
+        for (int64_t i = 0; i < e->n; ++i)
+            for (int j = 0; j < e->k; ++j) evecs[(int64_t)j * e->n + i] = h[i * e->k + j];
+    }
+    LH_API_END(ctx)
+}
+
+void lh_eig_free(lh_eig* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->ctx->device);
+    (void)hipStreamSynchronize(e->ctx->stream);
+    delete e;
+}
+
+}  // extern "C"

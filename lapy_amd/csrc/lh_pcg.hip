@@ -1,0 +1,342 @@
+// Preconditioned conjugate gradients for (alpha*B + beta*A) x = b.
+//
+// Replaces the SuperLU solves of heat.diffusion (lapy/heat.py:204-227:
+// (M_lumped + t A) u = b0) and Solver.poisson (lapy/solver.py:843-877:
+// A_ff x_f = b_f after removing Dirichlet rows/columns).  The right-hand
+// sides are solved together as a row-major block (one CG recurrence per
+// column).  The operator K is formed once on the device on A's pattern
+// (B diagonal or on the same pattern); Dirichlet rows/columns are removed by
+// masking, not by slicing.  With alpha == 0 and no Dirichlet rows, A is
+// singular with the constants as null space (closed mesh / pure Neumann): the
+// right-hand side is projected onto the complement of the constants and the
+// mean-free solution is returned (SuperLU returns an arbitrary member of the
+// solution set).
+#include "lh_dense.hpp"
+
+#include <chrono>
+#include <cmath>
+
+using namespace lh;
+
+namespace {
+
+// K = beta * A + alpha * B on A's pattern
+template <typename VT>
+__global__ void k_form_k(int64_t nnz, const VT* __restrict__ av, const VT* __restrict__ bv, double alpha,
+                         double beta, double* __restrict__ kv) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nnz) return;
+    kv[s] = beta * (double)av[s] + (bv ? alpha * (double)bv[s] : 0.0);
+}
+template <typename VT>
+__global__ void k_add_diag(int64_t n, const int32_t* __restrict__ diagpos, const VT* __restrict__ d, double alpha,
+                           double* __restrict__ kv, int* __restrict__ missing) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (d[i] == VT(0)) return;
+    if (diagpos[i] < 0) { atomicOr(missing, 1); return; }
+    kv[diagpos[i]] += alpha * (double)d[i];
+}
+
+// Dirichlet-masked SpMM on K: y = mask (K (mask x))
+__global__ void k_spmv_masked(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+                              const double* __restrict__ kv, const uint8_t* __restrict__ dm, const double* __restrict__ X,
+                              double* __restrict__ Y, int nrhs) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = gid / 16;
+    const int lane = (int)(gid % 16);
+    if (row >= n) return;
+    if (dm && dm[row]) {
+        for (int c = lane; c < nrhs; c += 16) Y[row * nrhs + c] = 0.0;
+        return;
+    }
+    const int32_t r0 = rowptr[row], r1 = rowptr[row + 1];
+    for (int c = lane; c < nrhs; c += 16) {
+        double acc = 0.0;
+        for (int32_t s = r0; s < r1; ++s) {
+            int32_t j = cols[s];
+            if (dm && dm[j]) continue;
+            acc += kv[s] * X[(int64_t)j * nrhs + c];
+        }
+        Y[row * nrhs + c] = acc;
+    }
+}
+
+// per-column partial dot products of two blocks, deterministic chunks
+__global__ void k_coldot_partial(int64_t n, int m, const double* __restrict__ X, const double* __restrict__ Y,
+                                 int64_t rpc, double* __restrict__ part) {
+    const int c = threadIdx.x % 64 + 64 * blockIdx.y;
+    const int rg = threadIdx.x / 64;
+    __shared__ double s[4][64];
+    double acc = 0.0;
+    if (c < m) {
+        int64_t rb = (int64_t)blockIdx.x * rpc, re = min(rb + rpc, n);
+        for (int64_t i = rb + rg; i < re; i += 4) acc += X[i * m + c] * Y[i * m + c];
+    }
+    s[rg][threadIdx.x % 64] = acc;
+    __syncthreads();
+    if (rg == 0 && c < m)
+        part[(int64_t)blockIdx.x * m + c] = ((s[0][threadIdx.x] + s[1][threadIdx.x]) + s[2][threadIdx.x]) + s[3][threadIdx.x];
+}
+__global__ void k_reduce_cols(int nchunk, int m, const double* __restrict__ part, double* __restrict__ out) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    double s = 0.0;
+    for (int k = 0; k < nchunk; ++k) s += part[(int64_t)k * m + c];
+    out[c] = s;
+}
+
+struct ColDot {
+    int64_t n;
+    int m;
+    int64_t nchunk, rpc;
+    DevBuf part;
+    void init(int64_t n_, int m_) {
+        n = n_;
+        m = m_;
+        nchunk = std::min<int64_t>(256, std::max<int64_t>(1, n / 512));
+        rpc = (n + nchunk - 1) / nchunk;
+        nchunk = (n + rpc - 1) / rpc;
+        part.alloc(nchunk * m * 8);
+    }
+    void run(lh_ctx* ctx, const double* X, const double* Y, double* out) {
+        dim3 grid((unsigned)nchunk, (m + 63) / 64);
+        k_coldot_partial<<<grid, 256, 0, ctx->stream>>>(n, m, X, Y, rpc, part.as<double>());
+        LH_CHECK_LAUNCH();
+        k_reduce_cols<<<nblocks(m, 64), 64, 0, ctx->stream>>>((int)nchunk, m, part.as<double>(), out);
+        LH_CHECK_LAUNCH();
+    }
+};
+
+// a = rz / pq per column (0 where converged/inactive)
+__global__ void k_alpha(int m, const double* __restrict__ rz, const double* __restrict__ pq,
+                        const int* __restrict__ active, double* __restrict__ a) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    a[c] = (active[c] && pq[c] != 0.0) ? rz[c] / pq[c] : 0.0;
+}
+// x += a p ; r -= a q
+__global__ void k_update_xr(int64_t n, int m, const double* __restrict__ a, const double* __restrict__ p,
+                            const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n * m) return;
+    int c = (int)(gid % m);
+    x[gid] += a[c] * p[gid];
+    r[gid] -= a[c] * q[gid];
+}
+// z = dinv * r (masked; optional projection handled separately)
+__global__ void k_precond(int64_t n, int m, const double* __restrict__ dinv, const uint8_t* __restrict__ dm,
+                          const double* __restrict__ r, double* __restrict__ z) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n * m) return;
+    int64_t i = gid / m;
+    z[gid] = (dm && dm[i]) ? 0.0 : dinv[i] * r[gid];
+}
+// beta = rz_new / rz_old ; p = z + beta p ; rz_old = rz_new
+__global__ void k_beta(int m, const double* __restrict__ rzn, double* __restrict__ rzo, double* __restrict__ b,
+                       const int* __restrict__ active) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    b[c] = (active[c] && rzo[c] != 0.0) ? rzn[c] / rzo[c] : 0.0;
+    rzo[c] = rzn[c];
+}
+__global__ void k_update_p(int64_t n, int m, const double* __restrict__ b, const double* __restrict__ z,
+                           double* __restrict__ p) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n * m) return;
+    int c = (int)(gid % m);
+    p[gid] = z[gid] + b[c] * p[gid];
+}
+// column means over free rows -> subtract (projection off the constants)
+__global__ void k_sub_colmean(int64_t n, int m, double* __restrict__ x, const double* __restrict__ sums) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n * m) return;
+    int c = (int)(gid % m);
+    x[gid] -= sums[c] / (double)n;
+}
+__global__ void k_fill(int64_t cnt, double v, double* __restrict__ x) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < cnt) x[gid] = v;
+}
+
+double now_us() {
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const double* b, int nrhs,
+           const uint8_t* dmask, double* x, const lh_pcg_opts* opts, lh_stats* stats) {
+    if (!ctx || !A || !b || !x || nrhs <= 0) return LH_ERR_ARG;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    const double t0 = now_us();
+    const int64_t n = A->n(), nnz = A->nnz();
+    if (B && B->n() != n) LH_THROW(LH_ERR_ARG, "A and B differ in size");
+    if (alpha != 0.0 && !B) LH_THROW(LH_ERR_ARG, "alpha != 0 needs B");
+    const int m = nrhs;
+    const double rtol = (opts && opts->rtol > 0) ? opts->rtol : 1e-10;
+    const int max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : (int)std::max<int64_t>(1000, 10 * n);
+    const int verbose = opts ? opts->verbose : 0;
+    lh_stats st{};
+    // operator values on A's pattern
+    DevBuf kv(nnz * 8), missing(8);
+    memset_d(ctx, missing.p, 0, 4);
+    const bool use_b = (alpha != 0.0);
+    if (use_b && !B->is_diag && B->pat != A->pat)
+        LH_THROW(LH_ERR_ARG, "B must be diagonal or share A's sparsity pattern");
+    if (A->dtype == LH_F64) {
+        k_form_k<double><<<nblocks(nnz, 256), 256, 0, ctx->stream>>>(
+            nnz, A->vals.as<double>(), (use_b && !B->is_diag) ? B->vals.as<double>() : nullptr, alpha, beta,
+            kv.as<double>());
+        if (use_b && B->is_diag)
+            k_add_diag<double><<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, A->pat->diagpos.as<int32_t>(),
+                                                                        B->dense_diag.as<double>(), alpha,
+                                                                        kv.as<double>(), missing.as<int>());
+    } else {
+        k_form_k<float><<<nblocks(nnz, 256), 256, 0, ctx->stream>>>(
+            nnz, A->vals.as<float>(), (use_b && !B->is_diag) ? B->vals.as<float>() : nullptr, alpha, beta,
+            kv.as<double>());
+        if (use_b && B->is_diag)
+            k_add_diag<float><<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, A->pat->diagpos.as<int32_t>(),
+                                                                       B->dense_diag.as<float>(), alpha,
+                                                                       kv.as<double>(), missing.as<int>());
+    }
+    LH_CHECK_LAUNCH();
+    int miss = 0;
+    d2h(ctx, &miss, missing.p, 4);
+    if (miss) LH_THROW(LH_ERR_ARG, "B has diagonal entries outside A's pattern");
+    // Jacobi from K's diagonal
+    DevBuf dinv(n * 8);
+    {
+        lh_csr K;  // view with K values for jacobi_diag
+        K.ctx = ctx;
+        K.pat = A->pat;
+        K.dtype = LH_F64;
+        K.vals.p = kv.p;
+        jacobi_diag(ctx, &K, nullptr, 0.0, 1.0, dinv.as<double>());
+        K.vals.p = nullptr;  // not owned
+    }
+    const int precond = opts ? opts->precond : LH_PRECOND_JACOBI;
+    if (precond == LH_PRECOND_NONE) k_fill<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, 1.0, dinv.as<double>());
+    // host -> device (column-major host, row-major device)
+    std::vector<double> hb((size_t)n * m), hx((size_t)n * m);
+    for (int64_t i = 0; i < n; ++i)
+        for (int c = 0; c < m; ++c) {
+            hb[i * m + c] = b[(int64_t)c * n + i];
+            hx[i * m + c] = x[(int64_t)c * n + i];
+        }
+    DevBuf dB(n * m * 8), dX(n * m * 8), dR(n * m * 8), dZ(n * m * 8), dP(n * m * 8), dQ(n * m * 8), dm;
+    const uint8_t* dmp = nullptr;
+    bool any_dirichlet = false;
+    if (dmask) {
+        for (int64_t i = 0; i < n; ++i) if (dmask[i]) { any_dirichlet = true; break; }
+    }
+    if (any_dirichlet) {
+        dm.alloc(n);
+        h2d(ctx, dm.p, dmask, n);
+        dmp = dm.as<uint8_t>();
+    }
+    const bool singular = (alpha == 0.0) && !any_dirichlet;
+    h2d(ctx, dB.p, hb.data(), n * m * 8);
+    // r = b on free rows (x starts at 0 there)
+    HIPX(hipMemsetAsync(dX.p, 0, n * m * 8, ctx->stream));
+    {
+        // r = mask ? 0 : b   (reuse k_precond with unit scaling)
+        DevBuf ones(n * 8);
+        k_fill<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, 1.0, ones.as<double>());
+        k_precond<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, ones.as<double>(), dmp, dB.as<double>(),
+                                                              dR.as<double>());
+        LH_CHECK_LAUNCH();
+        HIPX(hipStreamSynchronize(ctx->stream));
+    }
+    ColDot cd;
+    cd.init(n, m);
+    DevBuf ones_blk, sums(m * 8);
+    if (singular) {
+        ones_blk.alloc(n * m * 8);
+        k_fill<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n * m, 1.0, ones_blk.as<double>());
+        cd.run(ctx, ones_blk.as<double>(), dR.as<double>(), sums.as<double>());
+        k_sub_colmean<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dR.as<double>(), sums.as<double>());
+    }
+    DevBuf rz(m * 8), rzn(m * 8), pq(m * 8), al(m * 8), be(m * 8), rr(m * 8), act(m * 4);
+    std::vector<double> bnorm2(m), rnorm2(m);
+    cd.run(ctx, dR.as<double>(), dR.as<double>(), rr.as<double>());
+    d2h(ctx, bnorm2.data(), rr.p, m * 8);
+    std::vector<int> active(m, 1);
+    int nconv = 0;
+    for (int c = 0; c < m; ++c)
+        if (bnorm2[c] == 0.0) { active[c] = 0; ++nconv; }
+    h2d(ctx, act.p, active.data(), m * 4);
+    k_precond<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dinv.as<double>(), dmp, dR.as<double>(),
+                                                          dZ.as<double>());
+    HIPX(hipMemcpyAsync(dP.p, dZ.p, n * m * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    cd.run(ctx, dR.as<double>(), dZ.as<double>(), rz.as<double>());
+    const int32_t* rp = A->pat->rowptr.as<int32_t>();
+    const int32_t* cl = A->pat->cols.as<int32_t>();
+    int it = 0;
+    double worst = 0.0;
+    const int check_every = 4;
+    for (it = 0; it < max_iter && nconv < m; ++it) {
+        k_spmv_masked<<<nblocks(n * 16, 256), 256, 0, ctx->stream>>>(n, rp, cl, kv.as<double>(), dmp, dP.as<double>(),
+                                                                    dQ.as<double>(), m);
+        LH_CHECK_LAUNCH();
+        st.spmm_calls++;
+        cd.run(ctx, dP.as<double>(), dQ.as<double>(), pq.as<double>());
+        k_alpha<<<nblocks(m, 64), 64, 0, ctx->stream>>>(m, rz.as<double>(), pq.as<double>(), act.as<int>(),
+                                                        al.as<double>());
+        k_update_xr<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, al.as<double>(), dP.as<double>(),
+                                                                 dQ.as<double>(), dX.as<double>(), dR.as<double>());
+        if (singular) {
+            cd.run(ctx, ones_blk.as<double>(), dR.as<double>(), sums.as<double>());
+            k_sub_colmean<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dR.as<double>(), sums.as<double>());
+        }
+        k_precond<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dinv.as<double>(), dmp, dR.as<double>(),
+                                                              dZ.as<double>());
+        cd.run(ctx, dR.as<double>(), dZ.as<double>(), rzn.as<double>());
+        k_beta<<<nblocks(m, 64), 64, 0, ctx->stream>>>(m, rzn.as<double>(), rz.as<double>(), be.as<double>(),
+                                                       act.as<int>());
+        k_update_p<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, be.as<double>(), dZ.as<double>(),
+                                                                dP.as<double>());
+        LH_CHECK_LAUNCH();
+        if ((it + 1) % check_every == 0) {
+            cd.run(ctx, dR.as<double>(), dR.as<double>(), rr.as<double>());
+            d2h(ctx, rnorm2.data(), rr.p, m * 8);
+            worst = 0.0;
+            bool changed = false;
+            for (int c = 0; c < m; ++c) {
+                if (!active[c]) continue;
+                double rel = std::sqrt(rnorm2[c] / bnorm2[c]);
+                worst = std::max(worst, rel);
+                if (rel <= rtol) { active[c] = 0; ++nconv; changed = true; }
+            }
+            if (changed) h2d(ctx, act.p, active.data(), m * 4);
+            if (verbose && ((it + 1) % 100 == 0))
+                fprintf(stderr, "[pcg] it %d conv %d/%d worst %.3e\n", it + 1, nconv, m, worst);
+        }
+    }
+    if (singular) {
+        cd.run(ctx, ones_blk.as<double>(), dX.as<double>(), sums.as<double>());
+        k_sub_colmean<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dX.as<double>(), sums.as<double>());
+    }
+    d2h(ctx, hb.data(), dX.p, n * m * 8);
+    for (int64_t i = 0; i < n; ++i) {
+        const bool dir = any_dirichlet && dmask[i];
+        for (int c = 0; c < m; ++c)
+            if (!dir) x[(int64_t)c * n + i] = hb[i * m + c];
+    }
+    st.iterations = it;
+    st.converged = nconv;
+    st.max_resid = worst;
+    st.t_total_us = now_us() - t0;
+    if (stats) *stats = st;
+    if (nconv < m) {
+        ctx->last_error = "PCG did not converge within max_iter";
+        return LH_ERR_NOCONV;
+    }
+    LH_API_END(ctx)
+}
+
+}  // extern "C"

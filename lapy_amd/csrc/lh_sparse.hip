@@ -1,0 +1,189 @@
+// CSR sparse-matrix x dense-block products (SpMM) on row-major blocks.
+//
+// Blocks of vectors are stored row-major (n x ld): the ncol values of one
+// vertex are contiguous, so the gathered rows X[col, :] of one nonzero are a
+// single coalesced load across the lanes that share a matrix row.  A group of
+// LPR lanes (16, 32 or 64) owns one matrix row; the row's column indices and
+// values are read once per group and broadcast.
+//
+// Replaces scipy's csc matvec (lapy/solver.py:844,846 and inside eigsh).
+#include "lh_internal.hpp"
+
+namespace lh {
+
+template <typename VT, typename XT, int LPR>
+__global__ void __launch_bounds__(256)
+k_spmm(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+       const VT* __restrict__ vals, const XT* __restrict__ X, XT* __restrict__ Y, int ncol, int ldx, int ldy) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = gid / LPR;
+    const int lane = (int)(gid % LPR);
+    if (row >= n) return;
+    const int32_t r0 = rowptr[row], r1 = rowptr[row + 1];
+    for (int c = lane; c < ncol; c += LPR) {
+        XT acc = XT(0);
+        for (int32_t s = r0; s < r1; ++s) acc += (XT)vals[s] * X[(int64_t)cols[s] * ldx + c];
+        Y[row * ldy + c] = acc;
+    }
+}
+
+// Y1 = A X, Y2 = B X for A, B on one pattern (consistent mass).
+template <typename VT, typename XT, int LPR>
+__global__ void __launch_bounds__(256)
+k_spmm2(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+        const VT* __restrict__ va, const VT* __restrict__ vb, const XT* __restrict__ X,
+        XT* __restrict__ Y1, XT* __restrict__ Y2, int ncol, int ldx, int ldy) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = gid / LPR;
+    const int lane = (int)(gid % LPR);
+    if (row >= n) return;
+    const int32_t r0 = rowptr[row], r1 = rowptr[row + 1];
+    for (int c = lane; c < ncol; c += LPR) {
+        XT a1 = XT(0), a2 = XT(0);
+        for (int32_t s = r0; s < r1; ++s) {
+            XT x = X[(int64_t)cols[s] * ldx + c];
+            a1 += (XT)va[s] * x;
+            a2 += (XT)vb[s] * x;
+        }
+        Y1[row * ldy + c] = a1;
+        Y2[row * ldy + c] = a2;
+    }
+}
+
+// Y = diag(d) X
+template <typename VT, typename XT>
+__global__ void k_diag_scale(int64_t n, const VT* __restrict__ d, const XT* __restrict__ X, XT* __restrict__ Y,
+                             int ncol, int ldx, int ldy) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = gid / ncol;
+    const int c = (int)(gid % ncol);
+    if (row >= n) return;
+    Y[row * ldy + c] = (XT)d[row] * X[row * ldx + c];
+}
+
+static int pick_lpr(int ncol) { return ncol > 32 ? 64 : (ncol > 16 ? 32 : 16); }
+
+template <typename VT, typename XT>
+void spmm_ld(lh_ctx* ctx, const lh_csr* a, const XT* X, XT* Y, int ncol, int ldx, int ldy) {
+    const int64_t n = a->n();
+    if (a->is_diag) {
+        k_diag_scale<VT, XT><<<nblocks(n * ncol, 256), 256, 0, ctx->stream>>>(n, a->dense_diag.as<VT>(), X, Y, ncol,
+                                                                             ldx, ldy);
+        LH_CHECK_LAUNCH();
+        return;
+    }
+    const int lpr = pick_lpr(ncol);
+    const unsigned nb = nblocks(n * lpr, 256);
+    const int32_t* rp = a->pat->rowptr.as<int32_t>();
+    const int32_t* cl = a->pat->cols.as<int32_t>();
+    const VT* v = a->vals.as<VT>();
+    if (lpr == 64) k_spmm<VT, XT, 64><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
+    else if (lpr == 32) k_spmm<VT, XT, 32><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
+    else k_spmm<VT, XT, 16><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
+    LH_CHECK_LAUNCH();
+}
+
+template <typename VT, typename XT>
+void spmm(lh_ctx* ctx, const lh_csr* a, const XT* X, XT* Y, int ncol, int ld) {
+    spmm_ld<VT, XT>(ctx, a, X, Y, ncol, ld, ld);
+}
+
+template <typename VT, typename XT>
+void spmm_pair(lh_ctx* ctx, const lh_csr* a, const lh_csr* b, const XT* X, XT* YA, XT* YB, int ncol, int ldx,
+               int ldy) {
+    if (!b->is_diag && a->pat == b->pat) {
+        const int64_t n = a->n();
+        const int lpr = pick_lpr(ncol);
+        const unsigned nb = nblocks(n * lpr, 256);
+        const int32_t* rp = a->pat->rowptr.as<int32_t>();
+        const int32_t* cl = a->pat->cols.as<int32_t>();
+        const VT* va = a->vals.as<VT>();
+        const VT* vb = b->vals.as<VT>();
+        if (lpr == 64) k_spmm2<VT, XT, 64><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, va, vb, X, YA, YB, ncol, ldx, ldy);
+        else if (lpr == 32) k_spmm2<VT, XT, 32><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, va, vb, X, YA, YB, ncol, ldx, ldy);
+        else k_spmm2<VT, XT, 16><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, va, vb, X, YA, YB, ncol, ldx, ldy);
+        LH_CHECK_LAUNCH();
+        return;
+    }
+    spmm_ld<VT, XT>(ctx, a, X, YA, ncol, ldx, ldy);
+    spmm_ld<VT, XT>(ctx, b, X, YB, ncol, ldx, ldy);
+}
+
+template void spmm<double, double>(lh_ctx*, const lh_csr*, const double*, double*, int, int);
+template void spmm<float, double>(lh_ctx*, const lh_csr*, const double*, double*, int, int);
+template void spmm<float, float>(lh_ctx*, const lh_csr*, const float*, float*, int, int);
+template void spmm<double, float>(lh_ctx*, const lh_csr*, const float*, float*, int, int);
+template void spmm_ld<double, double>(lh_ctx*, const lh_csr*, const double*, double*, int, int, int);
+template void spmm_ld<float, float>(lh_ctx*, const lh_csr*, const float*, float*, int, int, int);
+template void spmm_pair<double, double>(lh_ctx*, const lh_csr*, const lh_csr*, const double*, double*, double*, int,
+                                        int, int);
+template void spmm_pair<float, float>(lh_ctx*, const lh_csr*, const lh_csr*, const float*, float*, float*, int, int,
+                                      int);
+
+__global__ void k_f64_to_f32(const double* __restrict__ s, float* __restrict__ d, int64_t cnt) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) d[i] = (float)s[i];
+}
+void copy_block_f64_f32(lh_ctx* ctx, const double* s, float* d, int64_t cnt) {
+    if (cnt <= 0) return;
+    k_f64_to_f32<<<nblocks(cnt, 256), 256, 0, ctx->stream>>>(s, d, cnt);
+    LH_CHECK_LAUNCH();
+}
+
+template <typename T>
+__global__ void k_fill_pattern(int64_t cnt, T* __restrict__ x) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) x[i] = T(1) + T(i % 7) * T(0.125);
+}
+
+}  // namespace lh
+
+using namespace lh;
+
+extern "C" int lh_time_spmm(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int precision, int iters, double* avg_us) {
+    if (!ctx || !A || !B || !avg_us || iters <= 0 || ncol <= 0) return LH_ERR_ARG;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    const int64_t n = A->n();
+    const size_t es = precision == LH_PREC_FP64 ? 8 : 4;
+    DevBuf X(n * ncol * es), YA(n * ncol * es), YB(n * ncol * es);
+    // FP32 SpMM needs FP32 values: convert on the fly for the timing copy
+    lh_csr A32, B32;
+    const lh_csr *pa = A, *pb = B;
+    auto to32 = [&](const lh_csr* s, lh_csr& d) {
+        d.ctx = ctx;
+        d.pat = s->pat;
+        d.dtype = LH_F32;
+        d.is_diag = s->is_diag;
+        d.vals.alloc(s->nnz() * 4);
+        copy_block_f64_f32(ctx, s->vals.as<double>(), d.vals.as<float>(), s->nnz());
+        if (s->is_diag) refresh_dense_diag(ctx, &d);
+    };
+    if (precision != LH_PREC_FP64) {
+        if (A->dtype != LH_F64 || B->dtype != LH_F64) LH_THROW(LH_ERR_ARG, "expects FP64 matrices");
+        to32(A, A32);
+        to32(B, B32);
+        pa = &A32;
+        pb = &B32;
+    }
+    auto launch = [&]() {
+        if (precision == LH_PREC_FP64) spmm_pair<double, double>(ctx, pa, pb, X.as<double>(), YA.as<double>(), YB.as<double>(), ncol, ncol, ncol);
+        else spmm_pair<float, float>(ctx, pa, pb, X.as<float>(), YA.as<float>(), YB.as<float>(), ncol, ncol, ncol);
+    };
+    if (precision == LH_PREC_FP64) k_fill_pattern<double><<<nblocks(n * ncol, 256), 256, 0, ctx->stream>>>(n * ncol, X.as<double>());
+    else k_fill_pattern<float><<<nblocks(n * ncol, 256), 256, 0, ctx->stream>>>(n * ncol, X.as<float>());
+    launch();
+    hipEvent_t e0, e1;
+    HIPX(hipEventCreate(&e0));
+    HIPX(hipEventCreate(&e1));
+    HIPX(hipEventRecord(e0, ctx->stream));
+    for (int i = 0; i < iters; ++i) launch();
+    HIPX(hipEventRecord(e1, ctx->stream));
+    HIPX(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPX(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_us = 1000.0 * ms / iters;
+    LH_API_END(ctx)
+}

@@ -1,0 +1,50 @@
+"""The C-ABI library loads and exports every symbol include/lapyhip.h declares
+(no compute calls: these run without a GPU)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from lapy_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "lapyhip.h")) as f:
+        txt = f.read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(lh_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "lh_assemble" in syms and "lh_eigs" in syms and "lh_pcg" in syms
+    assert len(syms) >= 20
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="liblapyhip.so not built")
+def test_library_exports_every_declared_symbol():
+    h = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(h, s)]
+    assert not missing, missing
+
+
+def test_binding_table_matches_header():
+    assert sorted(_lib.SIGNATURES) == declared_symbols()
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="liblapyhip.so not built")
+def test_ctypes_signatures_bind():
+    L = _lib.lib()
+    for name in _lib.SIGNATURES:
+        assert getattr(L, name).argtypes is not None
+
+
+def test_stats_struct_layout():
+    # lh_stats: 4 int32, 5 double, 4 int32
+    assert ctypes.sizeof(_lib.Stats) == 4 * 4 + 5 * 8 + 4 * 4
+    assert ctypes.sizeof(_lib.EigOpts) == 4 * 4 + 8 + 8 + 4 * 2
+    assert ctypes.sizeof(_lib.PcgOpts) == 4 * 2 + 8 + 4 * 2

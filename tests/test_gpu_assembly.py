@@ -1,0 +1,192 @@
+"""GPU assembly parity against the oracle (lapy/solver.py:106-533 restated).
+
+Bar: the CSR pattern equals scipy's exactly; values equal the oracle's
+element-order sums bit for bit (the order the GPU uses) and scipy's own
+duplicate sums to a few ulps; A is bit-symmetric."""
+
+import numpy as np
+import pytest
+
+from conftest import data_path
+from lapy_amd import _lib, meshgen
+from lapy_amd.mesh import _read_off, _read_vtk
+from oracle import fem
+
+pytestmark = pytest.mark.gpu
+
+KIND = {"tria": _lib.LH_TRIA, "aniso": _lib.LH_TRIA_ANISO, "mass": _lib.LH_TRIA_MASS, "tet": _lib.LH_TET}
+
+
+def gpu_assemble(v, t, kind, lump, dtype=np.float64, u1=None, u2=None, am=None):
+    ctx = _lib.default_context()
+    dm = _lib.DeviceMesh(ctx, v, t)
+    a, b = dm.assemble(KIND[kind], lump, dtype, u1, u2, am)
+    return (a.to_scipy_csc() if a is not None else None), b.to_scipy_csc()
+
+
+def oracle_local(v, t, kind, dtype, u1=None, u2=None, am=None):
+    if kind == "tria":
+        i, j, la, lb, lbl, il = fem.tria_local(v, t, dtype)
+    elif kind == "aniso":
+        i, j, la, lb, lbl, il = fem.tria_aniso_local(v, t, u1, u2, am, dtype)
+    elif kind == "mass":
+        i, j, lb, lbl, il = fem.tria_mass_local(v, t, dtype)
+        la = None
+    else:
+        i, j, la, lb, lbl, il = fem.tet_local(v, t, dtype)
+    return i, j, la, lb, lbl, il
+
+
+def check_against_oracle(v, t, kind, lump, dtype=np.float64, u1=None, u2=None, am=None):
+    A, B = gpu_assemble(v, t, kind, lump, dtype, u1, u2, am)
+    if kind == "tria":
+        a_ref, b_ref = fem.fem_tria(v, t, lump, dtype)
+    elif kind == "aniso":
+        a_ref, b_ref = fem.fem_tria_aniso(v, t, u1, u2, am, lump, dtype)
+    elif kind == "mass":
+        a_ref, b_ref = None, fem.fem_tria_mass(v, t, lump, dtype)
+    else:
+        a_ref, b_ref = fem.fem_tetra(v, t, lump, dtype)
+    i, j, la, lb, lbl, il = oracle_local(v, t, kind, dtype, u1, u2, am)
+    for mat, ref, (ii, jj, vals) in (
+        (A, a_ref, (i, j, la)),
+        (B, b_ref, (il, il, lbl) if lump else (i, j, lb)),
+    ):
+        if ref is None:
+            continue
+        ref = ref.tocsc()
+        ref.sort_indices()
+        assert mat.shape == ref.shape
+        assert np.array_equal(mat.indptr, ref.indptr), "pattern indptr differs"
+        assert np.array_equal(mat.indices, ref.indices), "pattern indices differ"
+        assert mat.dtype == np.dtype(dtype)
+        exact = fem.element_order_values(ref, ii, jj, vals.astype(dtype))
+        assert np.array_equal(mat.data, exact), (
+            f"not bit-exact vs element-order oracle: max diff {np.abs(mat.data - exact).max()}")
+        scale = np.abs(ref.data).max()
+        tol = 1e-6 if dtype == np.float32 else 1e-14
+        np.testing.assert_allclose(mat.data, ref.data, rtol=tol, atol=tol * scale)
+    if A is not None:
+        assert (abs(A - A.T)).max() == 0.0, "A not bit-symmetric"
+    return A, B
+
+
+@pytest.mark.parametrize("lump", [False, True])
+def test_tria_sphere_f64(lump):
+    v, t = meshgen.perturbed_sphere(4, np.random.default_rng(1), axes=(1.3, 0.8, 1.1), sigma=0.02)
+    check_against_oracle(v, t, "tria", lump)
+
+
+@pytest.mark.parametrize("lump", [False, True])
+def test_tria_float32_vertices_fixture(lump):
+    v, t = _read_vtk(data_path("cubeTria.vtk"), 3)  # float32, as the reference reader
+    assert v.dtype == np.float32
+    check_against_oracle(v, t, "tria", lump)
+
+
+@pytest.mark.parametrize("lump", [False, True])
+def test_tria_output_float32(lump):
+    v, t = _read_off(data_path("square-mesh.off"))
+    check_against_oracle(v, t, "tria", lump, dtype=np.float32)
+    v2, t2 = meshgen.icosphere(3)
+    check_against_oracle(v2, t2, "tria", lump, dtype=np.float32)
+
+
+@pytest.mark.parametrize("lump", [False, True])
+def test_tria_mass_only(lump):
+    v, t = meshgen.perturbed_sphere(3, np.random.default_rng(5), sigma=0.04)
+    check_against_oracle(v, t, "mass", lump)
+    v32, t32 = _read_off(data_path("square-mesh.off"))
+    check_against_oracle(v32, t32, "mass", lump)
+
+
+@pytest.mark.parametrize("lump", [False, True])
+def test_tet_fixture_and_jittered(lump):
+    v, t = _read_vtk(data_path("cubeTetra.vtk"), 4)
+    check_against_oracle(v, t, "tet", lump)
+    v2, t2 = meshgen.tet_cube(8, jitter=0.15, seed=3)
+    check_against_oracle(v2, t2, "tet", lump)
+
+
+@pytest.mark.parametrize("lump", [False, True])
+def test_tria_aniso(lump):
+    rng = np.random.default_rng(9)
+    v, t = meshgen.perturbed_sphere(3, rng, sigma=0.03)
+    u1 = rng.standard_normal((len(t), 3))
+    u1 /= np.linalg.norm(u1, axis=1, keepdims=True)
+    u2 = np.cross(u1, rng.standard_normal((len(t), 3)))
+    u2 /= np.linalg.norm(u2, axis=1, keepdims=True)
+    am = np.exp(-rng.uniform(0, 3, size=(len(t), 2)))
+    check_against_oracle(v, t, "aniso", lump, u1=u1, u2=u2, am=am)
+
+
+@pytest.mark.parametrize("kind", ["tria", "mass", "tet"])
+def test_degenerate_elements_clamp(kind):
+    """Zero-volume elements take 1e-4 * mean(vol) (solver.py:158-159)."""
+    if kind == "tet":
+        v, t = meshgen.tet_cube(3)
+        v = v.copy()
+        t = t.copy()
+        # collapse one tet: make its 4th vertex coplanar with the first three
+        t[5] = [t[5, 0], t[5, 1], t[5, 2], t[5, 1]]
+        t[17] = [t[17, 0], t[17, 0], t[17, 2], t[17, 3]]
+    else:
+        v, t = meshgen.icosphere(2)
+        v = v.copy()
+        t = t.copy()
+        t[3] = [t[3, 0], t[3, 0], t[3, 1]]      # repeated vertex
+        t[40] = [t[40, 0], t[40, 1], t[40, 1]]
+    A, B = gpu_assemble(v, t, kind, False)
+    if kind == "tria":
+        a_ref, b_ref = fem.fem_tria(v, t)
+    elif kind == "mass":
+        a_ref, b_ref = None, fem.fem_tria_mass(v, t)
+    else:
+        a_ref, b_ref = fem.fem_tetra(v, t)
+    for mat, ref in ((A, a_ref), (B, b_ref)):
+        if ref is None:
+            continue
+        ref.sort_indices()
+        assert np.array_equal(mat.indptr, ref.indptr) and np.array_equal(mat.indices, ref.indices)
+        np.testing.assert_allclose(mat.data, ref.data, rtol=1e-12, atol=1e-14 * abs(ref.data).max())
+
+
+def test_unused_vertices_shape_and_empty_rows():
+    v, t = meshgen.icosphere(2)
+    # an unused vertex in the middle (empty row) and two trailing unused ones
+    v = np.concatenate([v, [[5.0, 5, 5], [6.0, 6, 6]]])
+    t = np.where(t >= 17, t + 1, t)
+    v = np.insert(v, 17, [[9.0, 9, 9]], axis=0)
+    A, B = gpu_assemble(v, t, "tria", False)
+    a_ref, b_ref = fem.fem_tria(v, t)
+    assert A.shape == a_ref.shape == (len(v) - 2, len(v) - 2)
+    assert A.indptr[18] == A.indptr[17]
+    assert np.array_equal(A.indptr, a_ref.indptr) and np.array_equal(A.indices, a_ref.indices)
+    _, Bl = gpu_assemble(v, t, "tria", True)
+    _, bl_ref = fem.fem_tria(v, t, lump=True)
+    assert Bl.shape == bl_ref.shape and Bl.nnz == bl_ref.nnz
+    np.testing.assert_allclose(Bl.toarray(), bl_ref.toarray(), rtol=1e-15)
+
+
+def test_deterministic_repeat():
+    v, t = meshgen.tet_cube(6, jitter=0.1, seed=8)
+    A1, B1 = gpu_assemble(v, t, "tet", False)
+    A2, B2 = gpu_assemble(v, t, "tet", False)
+    assert np.array_equal(A1.data, A2.data) and np.array_equal(B1.data, B2.data)
+
+
+def test_avg_edge_length_gpu(expected):
+    from lapy_amd import TetMesh, TriaMesh
+
+    v = np.array([[0.0, 0, 0], [0, 1, 0], [1, 1, 0], [1, 0, 0], [0, 0, 1], [0, 1, 1], [1, 1, 1], [1, 0, 1]])
+    t = np.array([[0, 1, 2], [2, 3, 0], [4, 5, 6], [6, 7, 4], [0, 4, 7], [7, 3, 0], [0, 4, 5], [5, 1, 0],
+                  [1, 5, 6], [6, 2, 1], [3, 7, 6], [6, 2, 3]])
+    assert TriaMesh(v, t).avg_edge_length() == pytest.approx(expected["test_tria_mesh"]["expected_edge_length"],
+                                                             rel=1e-14)
+    vt = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1],
+                   [0.5, 0.5, 0.5]])
+    tt = np.array([[0, 5, 8, 1], [0, 4, 5, 8], [2, 5, 6, 8], [1, 5, 2, 8], [6, 7, 3, 8], [6, 3, 2, 8], [0, 3, 4, 8],
+                   [3, 7, 4, 8], [0, 1, 2, 8], [0, 2, 3, 8], [4, 6, 5, 8], [4, 7, 6, 8]])
+    assert TetMesh(vt, tt).avg_edge_length() == pytest.approx(1.0543647924813107, rel=1e-14)
+    v2, t2 = meshgen.large_ellipsoid(level=4)
+    assert TriaMesh(v2, t2).avg_edge_length() == pytest.approx(fem.avg_edge_length(v2, t2), rel=1e-13)

@@ -1,0 +1,200 @@
+"""GPU eigensolver / PCG parity against the oracle (splu + eigsh restated) and
+the reference's own fixtures.
+
+Eigen bar (north star): eigenvalue relative error <= 1e-8 against the FP64
+oracle (a zero eigenvalue is compared against 1e-8 * lambda_max), largest
+per-cluster principal angle <= 1e-6, B-orthonormal vectors."""
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+from conftest import data_path, read_ev
+from lapy_amd import Solver, TetMesh, TriaMesh, heat, meshgen, shapedna
+from oracle import fem, metrics
+
+pytestmark = pytest.mark.gpu
+
+EIG_RTOL = 1e-8
+ANGLE_TOL = 1e-6
+
+
+def assert_eigs_match(lam, vec, lam_ref, vec_ref, mass, k_complete=True):
+    err = metrics.eig_rel_err(lam, lam_ref)
+    assert err.max() <= EIG_RTOL, f"eigenvalue rel err {err.max():.3e}: {lam} vs {lam_ref}"
+    assert metrics.m_orthonormality_err(vec, mass) < 1e-9
+    ang = metrics.subspace_angles(vec, vec_ref, mass, lam_ref, complete=k_complete)
+    worst = max(a for _, a in ang)
+    assert worst <= ANGLE_TOL, f"subspace angle {worst:.3e} ({ang})"
+
+
+def test_cubetria_fixture_k3():
+    tria = TriaMesh.read_vtk(data_path("cubeTria.vtk"))
+    fem_ = Solver(tria)
+    lam, vec = fem_.eigs(k=3)
+    ev_ref, vec_ref = read_ev(data_path("cubeTria.ev"))
+    np.testing.assert_allclose(lam, ev_ref, rtol=1e-8, atol=1e-9)
+    ang = metrics.subspace_angles(vec, vec_ref, fem_.mass, ev_ref)
+    assert max(a for _, a in ang) < 1e-5  # .ev vectors are printed to ~1e-16 but came from ARPACK
+
+
+def test_cubetetra_fixture_k3(expected):
+    tet = TetMesh.read_vtk(data_path("cubeTetra.vtk"))
+    lam, vec = Solver(tet).eigs(k=3)
+    ev_ref, _ = read_ev(data_path("cubeTetra.ev"))
+    np.testing.assert_allclose(lam, ev_ref, rtol=1e-4, atol=1e-4)
+    a, b = fem.fem_tetra(tet.v, tet.t)
+    lam_o, vec_o = fem.eigs(a, b, k=3)
+    assert_eigs_match(lam, vec, lam_o, vec_o, b)
+    e = expected["test_compute_shapedna_tet"]
+    np.testing.assert_allclose(lam, e["expected_eigen_values"], atol=e["tolerance"])
+
+
+def test_shapedna_tria(expected):
+    tria = TriaMesh.read_vtk(data_path("cubeTria.vtk"))
+    ev = shapedna.compute_shapedna(tria, k=3)
+    e = expected["test_compute_shapedna"]
+    assert np.allclose(ev["Eigenvalues"], e["expected_eigenvalues"], atol=e["tolerance"])
+    assert ev["Elements"] == 4800 and ev["DoF"] == 2402 and ev["Dimension"] == 2
+    assert ev["Eigenvectors"].shape == (2402, 3)
+
+
+def test_small_sphere_k10_vs_oracle():
+    v, t = meshgen.small_sphere(level=4)
+    fem_ = Solver(TriaMesh(v, t))
+    lam, vec = fem_.eigs(k=10)
+    a, b = fem.fem_tria(v, t)
+    lam_o, vec_o = fem.eigs(a, b, k=16)
+    # compare the first 10 against the oracle; clusters cut by k are compared
+    # only when complete inside the 16 oracle pairs
+    assert_eigs_match(lam, vec, lam_o[:10], vec_o[:, :10], b,
+                      k_complete=abs(lam_o[10] - lam_o[9]) / lam_o[9] > 1e-3)
+
+
+def test_config1_sphere_l5_k10():
+    """Icosphere level 5 with radial noise 0.02 (10,242 vertices), k=10."""
+    v, t = meshgen.small_sphere(level=5)
+    fem_ = Solver(TriaMesh(v, t), lump=False)
+    lam, vec = fem_.eigs(k=10)
+    a, b = fem.fem_tria(v, t)
+    lam_o, vec_o = fem.eigs(a, b, k=10)
+    assert_eigs_match(lam, vec, lam_o, vec_o, b, k_complete=False)
+
+
+def test_ellipsoid_k50():
+    v, t = meshgen.large_ellipsoid(level=5)
+    fem_ = Solver(TriaMesh(v, t))
+    lam, vec = fem_.eigs(k=50)
+    a, b = fem.fem_tria(v, t)
+    lam_o, vec_o = fem.eigs(a, b, k=50)
+    assert_eigs_match(lam, vec, lam_o, vec_o, b, k_complete=False)
+
+
+def test_dense_path_tiny_mesh():
+    v, t = meshgen.icosphere(1)
+    fem_ = Solver(TriaMesh(v, t))
+    lam, vec = fem_.eigs(k=6)
+    a, b = fem.fem_tria(v, t)
+    w, _ = scipy.linalg.eigh(a.toarray(), b.toarray())
+    np.testing.assert_allclose(lam, w[:6], rtol=1e-10, atol=1e-12)
+    assert metrics.m_orthonormality_err(vec, b) < 1e-10
+
+
+def test_lumped_tet_eigs_k10():
+    tet = TetMesh.read_vtk(data_path("cubeTetra.vtk"))
+    fem_ = Solver(tet, lump=True)
+    lam, vec = fem_.eigs(10)
+    assert lam.shape == (10,) and vec.shape == (9261, 10)
+    a, b = fem.fem_tetra(tet.v, tet.t, lump=True)
+    lam_o, vec_o = fem.eigs(a, b, k=10)
+    assert_eigs_match(lam, vec, lam_o, vec_o, b, k_complete=False)
+
+
+# ------------------------------------------------------------------ heat / poisson
+@pytest.fixture(scope="module")
+def square():
+    return TriaMesh.read_off(data_path("square-mesh.off"))
+
+
+def test_lumped_mass_sum_and_symmetry(square):
+    fem_ = Solver(square, lump=True)
+    assert fem_.mass.sum() == pytest.approx(1.0, rel=1e-6)
+    A = fem_.stiffness.toarray()
+    assert (A == A.T).all()
+    assert fem_.stiffness.getformat() == "csc"
+    lam, _ = fem_.eigs()
+    assert len(lam) == 10
+
+
+def test_diffusion_vs_oracle(square):
+    u = heat.diffusion(square, 0, m=1)
+    u_o = fem.diffusion(square.v, square.t, 0, m=1)
+    np.testing.assert_allclose(u, u_o, rtol=1e-9, atol=1e-12 * np.abs(u_o).max())
+
+
+def test_diffusion_multi_matches_single(square):
+    seeds = [np.array([0]), np.array([1, 2]), np.array([5, 9, 11])]
+    ub = heat.diffusion(square, seeds, m=1)
+    assert ub.shape == (len(square.v), 3)
+    for k, s in enumerate(seeds):
+        np.testing.assert_allclose(ub[:, k], heat.diffusion(square, s, m=1), rtol=1e-6, atol=1e-9)
+        np.testing.assert_allclose(ub[:, k], fem.diffusion(square.v, square.t, s, m=1), rtol=1e-9, atol=1e-12)
+
+
+def test_diffusion_tet_vs_oracle():
+    v, t = meshgen.tet_cube(10, jitter=0.15, seed=2)
+    tet = TetMesh(v, t)
+    m = 1e-3 / fem.avg_edge_length(v, t) ** 2
+    u = heat.diffusion(tet, 0, m=m)
+    u_o = fem.diffusion(v, t, 0, m=m)
+    np.testing.assert_allclose(u, u_o, rtol=1e-8, atol=1e-12 * np.abs(u_o).max())
+
+
+def test_poisson_dirichlet_vs_oracle(square):
+    fem_ = Solver(square, lump=True)
+    rng = np.random.default_rng(4)
+    h = rng.standard_normal((len(square.v), 3))
+    dtup = (np.array([0, 1, 7]), np.array([0.25, -1.0, 0.5]))
+    x = fem_.poisson(h, dtup=dtup)
+    x_o = fem.poisson(fem_.stiffness, fem_.mass, h, dtup=dtup)
+    np.testing.assert_allclose(x, x_o, rtol=1e-8, atol=1e-10 * np.abs(x_o).max())
+    assert np.array_equal(x[dtup[0], 0], dtup[1])
+
+
+def test_poisson_ntup_vs_oracle(square):
+    fem_ = Solver(square, lump=False)
+    h = np.cos(3 * square.v[:, 0])
+    dtup = (np.array([3]), np.array([0.0]))
+    ntup = (np.array([2, 4, 4]), np.array([1.0, -0.5, 0.25]))
+    x = fem_.poisson(h, dtup=dtup, ntup=ntup)
+    x_o = fem.poisson(fem_.stiffness, fem_.mass, h, dtup=dtup, ntup=ntup)
+    np.testing.assert_allclose(x, x_o, rtol=1e-8, atol=1e-10 * np.abs(x_o).max())
+
+
+def test_poisson_neumann_modulo_constant(square):
+    fem_ = Solver(square, lump=True)
+    _, evec = fem_.eigs(k=5)
+    rhs = evec[:, 1:5]
+    xb = fem_.poisson(rhs)
+    assert xb.shape == (len(square.v), 4)
+    for k in range(4):
+        x1 = fem_.poisson(rhs[:, k])
+        np.testing.assert_allclose(xb[:, k], x1, rtol=1e-6, atol=1e-9)
+        xo = fem.poisson(fem_.stiffness, fem_.mass, rhs[:, k])
+        d = (x1 - x1.mean()) - (xo - xo.mean())
+        assert np.abs(d).max() < 1e-7 * np.abs(xo - xo.mean()).max()
+    assert fem_.poisson(0.0).ndim == 1
+    assert np.abs(fem_.poisson(0.0)).max() == 0.0
+
+
+def test_mass_reassignment_is_used(square):
+    from scipy import sparse
+
+    fem_ = Solver(square, lump=True)
+    n = len(square.v)
+    fem_.mass = sparse.eye(n, format="csc")
+    h = np.sin(square.v[:, 1])
+    dtup = (np.array([0]), np.array([0.0]))
+    x = fem_.poisson(h, dtup=dtup)
+    x_o = fem.poisson(fem_.stiffness, sparse.eye(n, format="csc"), h, dtup=dtup)
+    np.testing.assert_allclose(x, x_o, rtol=1e-8, atol=1e-10 * np.abs(x_o).max())

@@ -1,0 +1,76 @@
+"""Host-side logic of the façade that runs before any device call
+(argument checks and error behaviour of lapy/solver.py, lapy/heat.py)."""
+
+import numpy as np
+import pytest
+
+from lapy_amd import Solver, TetMesh, TriaMesh, heat, meshgen
+from lapy_amd.mesh import _read_off
+from conftest import data_path
+
+
+def test_unknown_geometry_type():
+    class Polygon:
+        v = np.zeros((3, 3))
+        t = np.zeros((1, 3), int)
+
+    with pytest.raises(ValueError, match='Geometry type "Polygon" unknown'):
+        Solver(Polygon())
+
+
+def test_use_cholmod_without_scikit_sparse():
+    tria = TriaMesh(*meshgen.icosphere(1))
+    with pytest.raises(ImportError, match="Missing optional dependency 'scikit-sparse'"):
+        Solver(tria, use_cholmod=True)
+    with pytest.raises(ImportError, match="scikit-sparse"):
+        heat.diffusion(tria, 0, use_cholmod=True)
+
+
+def test_diffusion_out_of_range():
+    tria = TriaMesh.read_off(data_path("square-mesh.off"))
+    with pytest.raises(ValueError, match="out-of-range"):
+        heat.diffusion(tria, np.array([len(tria.v)]), m=1)
+    with pytest.raises(ValueError, match="out-of-range"):
+        heat.diffusion(tria, [np.array([0]), np.array([len(tria.v)])], m=1)
+    with pytest.raises(ValueError, match="out-of-range"):
+        heat.diffusion(tria, -1, m=1)
+
+
+def test_mesh_validation():
+    with pytest.raises(ValueError, match="Max index exceeds"):
+        TriaMesh(np.zeros((3, 3)), np.array([[0, 1, 3]]))
+    with pytest.raises(ValueError, match="empty mesh"):
+        TriaMesh(np.zeros((3, 3)), np.zeros((0, 3), int))
+    with pytest.raises(ValueError, match="Max index exceeds"):
+        TetMesh(np.zeros((4, 3)), np.array([[0, 1, 2, 4]]))
+    m = TriaMesh(np.zeros((3, 2)), np.array([[0, 1, 2]]))
+    assert m.v.shape == (3, 3) and m.is_2d()
+
+
+def test_readers_keep_float32():
+    v, t = _read_off(data_path("square-mesh.off"))
+    assert v.dtype == np.float32 and t.shape[1] == 3
+    tria = TriaMesh.read_vtk(data_path("cubeTria.vtk"))
+    assert tria.v.dtype == np.float32 and tria.v.shape == (2402, 3) and tria.t.shape == (4800, 3)
+    tet = TetMesh.read_vtk(data_path("cubeTetra.vtk"))
+    assert tet.v.shape == (9261, 3) and tet.t.shape == (48000, 4)
+
+
+def test_meshgen_sizes_and_fixture_split():
+    for level in range(5):
+        v, t = meshgen.icosphere(level)
+        assert v.shape == (10 * 4**level + 2, 3) and t.shape == (20 * 4**level, 3)
+    tet = TetMesh.read_vtk(data_path("cubeTetra.vtk"))
+    v, t = meshgen.tet_cube(20)
+    assert np.array_equal(t, tet.t)
+    assert np.abs(v - tet.v).max() < 1e-7
+    v, t = meshgen.volume_cube(cells=4)
+    assert v.shape == (125, 3) and t.shape == (6 * 64, 4)
+
+
+def test_meshgen_seeded():
+    a = meshgen.large_ellipsoid(level=3)
+    b = meshgen.large_ellipsoid(level=3)
+    assert np.array_equal(a[0], b[0])
+    batch = meshgen.shape_batch(count=4)
+    assert len(batch) == 4 and all(x[1].shape[1] == 3 for x in batch)

@@ -1,0 +1,248 @@
+"""Pin the CPU oracle (oracle/fem.py) against what the mathematics and the
+reference's own fixtures fix, before it is used to judge the GPU path."""
+
+import numpy as np
+import pytest
+import scipy.linalg
+from scipy import sparse
+
+from conftest import data_path, read_ev
+from lapy_amd import meshgen
+from lapy_amd.mesh import _read_off, _read_vtk
+from oracle import fem, metrics
+
+
+# ---------------------------------------------------------------- closed forms
+def right_triangle():
+    v = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    t = np.array([[0, 1, 2]])
+    return v, t
+
+
+def unit_tet():
+    v = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    t = np.array([[0, 1, 2, 3]])
+    return v, t
+
+
+def test_tria_right_triangle_closed_form():
+    v, t = right_triangle()
+    a, b = fem.fem_tria(v, t)
+    exp_a = np.array([[1, -0.5, -0.5], [-0.5, 0.5, 0], [-0.5, 0, 0.5]])
+    np.testing.assert_allclose(a.toarray(), exp_a, atol=1e-15)
+    np.testing.assert_allclose(b.toarray(), np.full((3, 3), 1 / 24) + np.eye(3) / 24, atol=1e-15)
+    _, bl = fem.fem_tria(v, t, lump=True)
+    np.testing.assert_allclose(bl.toarray(), np.eye(3) / 6, atol=1e-15)
+    m = fem.fem_tria_mass(v, t)
+    np.testing.assert_allclose(m.toarray(), b.toarray(), atol=1e-15)
+    ml = fem.fem_tria_mass(v, t, lump=True)
+    np.testing.assert_allclose(ml.toarray(), np.eye(3) / 6, atol=1e-15)
+
+
+def test_tet_unit_closed_form():
+    v, t = unit_tet()
+    a, b = fem.fem_tetra(v, t)
+    exp_a = np.array([[3, -1, -1, -1], [-1, 1, 0, 0], [-1, 0, 1, 0], [-1, 0, 0, 1]]) / 6
+    np.testing.assert_allclose(a.toarray(), exp_a, atol=1e-15)
+    np.testing.assert_allclose(b.toarray(), (np.ones((4, 4)) + np.eye(4)) / 120, atol=1e-15)
+    _, bl = fem.fem_tetra(v, t, lump=True)
+    np.testing.assert_allclose(bl.toarray(), np.eye(4) / 24, atol=1e-15)
+
+
+def test_aniso_isotropic_reduces_to_cotan():
+    # aniso weights 1 with an orthonormal in-plane basis == isotropic cotan
+    rng = np.random.default_rng(3)
+    v, t = meshgen.perturbed_sphere(2, rng, sigma=0.05)
+    e1 = v[t[:, 1]] - v[t[:, 0]]
+    e2 = v[t[:, 2]] - v[t[:, 0]]
+    u1 = e1 / np.linalg.norm(e1, axis=1, keepdims=True)
+    nrm = np.cross(e1, e2)
+    u2 = np.cross(nrm, u1)
+    u2 /= np.linalg.norm(u2, axis=1, keepdims=True)
+    am = np.ones((len(t), 2))
+    a1, b1 = fem.fem_tria_aniso(v, t, u1, u2, am)
+    a0, b0 = fem.fem_tria(v, t)
+    np.testing.assert_allclose(a1.toarray(), a0.toarray(), atol=1e-12)
+    np.testing.assert_allclose(b1.toarray(), b0.toarray(), atol=1e-15)
+
+
+# ---------------------------------------------------------------- invariants
+@pytest.mark.parametrize("lump", [False, True])
+def test_tria_invariants(lump):
+    rng = np.random.default_rng(7)
+    v, t = meshgen.perturbed_sphere(3, rng, axes=(1.5, 0.7, 1.0), sigma=0.03)
+    a, b = fem.fem_tria(v, t, lump=lump)
+    assert abs(a - a.T).max() == 0.0  # exact symmetry
+    assert np.abs(a @ np.ones(a.shape[0])).max() < 1e-12 * abs(a).max()
+    area = 0.5 * np.linalg.norm(np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]]), axis=1).sum()
+    assert b.sum() == pytest.approx(area, rel=1e-12)
+    # pattern = unique connectivity pairs
+    pairs = set()
+    for tri in t:
+        for p in tri:
+            for q in tri:
+                pairs.add((p, q))
+    coo = a.tocoo()
+    assert set(zip(coo.row.tolist(), coo.col.tolist())) == pairs
+
+
+def test_tet_invariants_and_volume():
+    v, t = meshgen.tet_cube(6, jitter=0.15, seed=1)
+    a, b = fem.fem_tetra(v, t)
+    assert abs(a - a.T).max() < 1e-15 * abs(a).max() * 10
+    assert np.abs(a @ np.ones(a.shape[0])).max() < 1e-12 * abs(a).max()
+    assert b.sum() == pytest.approx(1.0, rel=1e-12)
+    _, bl = fem.fem_tetra(v, t, lump=True)
+    assert bl.sum() == pytest.approx(1.0, rel=1e-12)
+
+
+def test_linear_patch_planar():
+    # cotan Laplacian annihilates linear functions at interior vertices
+    v, t = _read_off(data_path("square-mesh.off"))
+    v = v.astype(np.float64)
+    a, _ = fem.fem_tria(v, t)
+    bnd = np.zeros(len(v), bool)
+    bnd[(v[:, 0] < 1e-6) | (v[:, 0] > 1 - 1e-6) | (v[:, 1] < 1e-6) | (v[:, 1] > 1 - 1e-6)] = True
+    for f in (v[:, 0], v[:, 1], 2 * v[:, 0] - 3 * v[:, 1] + 1):
+        r = a @ f
+        assert np.abs(r[~bnd]).max() < 1e-12
+
+
+def test_tet_linear_patch():
+    v, t = meshgen.tet_cube(5, jitter=0.2, seed=4)
+    a, _ = fem.fem_tetra(v, t)
+    interior = np.all((v > 1e-9) & (v < 1 - 1e-9), axis=1)
+    for f in (v[:, 0], v[:, 2], v[:, 0] + 2 * v[:, 1] - v[:, 2]):
+        assert np.abs((a @ f)[interior]).max() < 1e-12
+
+
+# ---------------------------------------------------------------- fixtures
+def test_cubetria_ev_fixture():
+    v, t = _read_vtk(data_path("cubeTria.vtk"), 3)
+    a, b = fem.fem_tria(v, t)
+    ev_ref, vec_ref = read_ev(data_path("cubeTria.ev"))
+    lam, vec = fem.eigs(a, b, k=3)
+    np.testing.assert_allclose(lam, ev_ref, rtol=1e-8, atol=1e-10)
+    ang = metrics.subspace_angles(vec, vec_ref, b, ev_ref)
+    assert max(x for _, x in ang) < 1e-5
+
+
+def test_cubetetra_ev_fixture(expected):
+    v, t = _read_vtk(data_path("cubeTetra.vtk"), 4)
+    a, b = fem.fem_tetra(v, t)
+    ev_ref, _ = read_ev(data_path("cubeTetra.ev"))
+    lam, _ = fem.eigs(a, b, k=3)
+    np.testing.assert_allclose(lam, ev_ref, rtol=1e-4, atol=1e-4)
+    exp = expected["test_visualization_tetrahedral_mesh"]["expected_ev"]
+    np.testing.assert_allclose(lam, exp, rtol=1e-4, atol=1e-4)
+    # continuum: first non-zero Neumann eigenvalue of the unit cube is pi^2
+    assert abs(lam[1] / np.pi**2 - 1) < 0.01
+
+
+def test_expected_outcomes_tria(expected):
+    v, t = _read_vtk(data_path("cubeTria.vtk"), 3)
+    a, b = fem.fem_tria(v, t)
+    lam, _ = fem.eigs(a, b, k=3)
+    e = expected["test_compute_shapedna"]
+    np.testing.assert_allclose(lam, e["expected_eigenvalues"], atol=e["tolerance"])
+    e = expected["test_visualization_triangle_mesh"]
+    np.testing.assert_allclose(lam, e["expected_ev"], rtol=1e-5, atol=1e-5)
+
+
+def test_avg_edge_length_kats(expected):
+    v = np.array([[0.0, 0, 0], [0, 1, 0], [1, 1, 0], [1, 0, 0], [0, 0, 1], [0, 1, 1], [1, 1, 1], [1, 0, 1]])
+    t = np.array([[0, 1, 2], [2, 3, 0], [4, 5, 6], [6, 7, 4], [0, 4, 7], [7, 3, 0], [0, 4, 5], [5, 1, 0],
+                  [1, 5, 6], [6, 2, 1], [3, 7, 6], [6, 2, 3]])
+    assert fem.avg_edge_length(v, t) == pytest.approx(expected["test_tria_mesh"]["expected_edge_length"])
+    vt = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1],
+                   [0.5, 0.5, 0.5]])
+    tt = np.array([[0, 5, 8, 1], [0, 4, 5, 8], [2, 5, 6, 8], [1, 5, 2, 8], [6, 7, 3, 8], [6, 3, 2, 8], [0, 3, 4, 8],
+                   [3, 7, 4, 8], [0, 1, 2, 8], [0, 2, 3, 8], [4, 6, 5, 8], [4, 7, 6, 8]])
+    assert fem.avg_edge_length(vt, tt) == pytest.approx(1.0543647924813107)
+
+
+def test_lumped_mass_unit_square(expected):
+    v, t = _read_off(data_path("square-mesh.off"))
+    _, b = fem.fem_tria(v, t, lump=True)
+    assert b.sum() == pytest.approx(1.0, rel=1e-6)
+
+
+# ---------------------------------------------------------------- solves
+def test_eigs_matches_dense():
+    for v, t in (meshgen.icosphere(1), meshgen.perturbed_sphere(2, np.random.default_rng(2), sigma=0.05)):
+        a, b = fem.fem_tria(v, t)
+        lam, vec = fem.eigs(a, b, k=8)
+        w, z = scipy.linalg.eigh(a.toarray(), b.toarray())
+        np.testing.assert_allclose(lam, w[:8], rtol=1e-10, atol=1e-12)
+        assert metrics.m_orthonormality_err(vec, b) < 1e-10
+
+
+def test_poisson_matches_dense():
+    v, t = _read_off(data_path("square-mesh.off"))
+    a, b = fem.fem_tria(v, t, lump=True)
+    rng = np.random.default_rng(0)
+    h = rng.standard_normal(len(v))
+    didx = np.array([0, 1, 5])
+    dval = np.array([0.5, -1.0, 2.0])
+    x = fem.poisson(a, b, h, dtup=(didx, dval))
+    free = np.setdiff1d(np.arange(len(v)), didx)
+    ad = a.toarray()
+    rhs = (b @ h) - ad[:, didx] @ dval
+    xf = np.linalg.solve(ad[np.ix_(free, free)], rhs[free])
+    np.testing.assert_allclose(x[free], xf, rtol=1e-9, atol=1e-11)
+    assert np.array_equal(x[didx], dval)
+
+
+def test_diffusion_matches_dense():
+    v, t = _read_off(data_path("square-mesh.off"))
+    u = fem.diffusion(v, t, [0, 3], m=1.0)
+    a, b = fem.fem_tria(v, t, lump=True)
+    tt = 1.0 * fem.avg_edge_length(v, t) ** 2
+    rhs = np.zeros(len(v))
+    rhs[[0, 3]] = 1
+    np.testing.assert_allclose(u, np.linalg.solve((b + tt * a).toarray(), rhs), rtol=1e-10, atol=1e-13)
+
+
+def test_element_order_sum_matches_scipy():
+    rng = np.random.default_rng(11)
+    v, t = meshgen.perturbed_sphere(3, rng, sigma=0.02)
+    a, _ = fem.fem_tria(v, t)
+    i, j, la, *_ = fem.tria_local(v, t)
+    vals = fem.element_order_values(a, i, j, la)
+    # off-diagonals of a manifold mesh have two summands: bit-identical
+    off = np.repeat(np.arange(a.shape[0]), np.diff(a.indptr)) != a.indices
+    assert np.array_equal(vals[off], a.data[off])
+    np.testing.assert_allclose(vals, a.data, rtol=1e-13, atol=1e-15 * abs(a.data).max())
+
+
+# ---------------------------------------------------------------- planted mistakes
+def test_planted_sign_error_is_caught(monkeypatch):
+    """A plausible mistake in the element math (a dropped minus sign on one
+    cotangent term) must fail the closed-form check."""
+    orig = fem.tria_local
+
+    def broken(v, t, dtype=np.float64):
+        i, j, la, lb, lbl, il = orig(v, t, dtype)
+        la = la.copy()
+        la[0::9] *= -1.0  # a12 entry of (t1,t2) flipped
+        return i, j, la, lb, lbl, il
+
+    monkeypatch.setattr(fem, "tria_local", broken)
+    with pytest.raises(AssertionError):
+        test_tria_right_triangle_closed_form()
+
+
+def test_planted_transposed_tet_entries_are_caught(monkeypatch):
+    """Entries a13 and a14 written to each other's slots (a transposed index)
+    keep A symmetric with zero row sums; the linear-patch test must catch it."""
+    orig = fem.tet_local
+
+    def broken(v, t, dtype=np.float64):
+        i, j, la, lb, lbl, il = orig(v, t, dtype)
+        la = la.reshape(-1, 16).copy()
+        la[:, [4, 5, 6, 7]] = la[:, [6, 7, 4, 5]]
+        return i, j, la.reshape(-1), lb, lbl, il
+
+    monkeypatch.setattr(fem, "tet_local", broken)
+    with pytest.raises(AssertionError):
+        test_tet_linear_patch()
