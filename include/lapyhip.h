@@ -89,6 +89,9 @@ int lh_mesh_create(lh_ctx* ctx, const void* v, int v_dtype, int64_t nv,
  * values each, t: int32 ne x nodes row-major); they are copied. */
 int lh_mesh_create_device(lh_ctx* ctx, const void* v_soa, int v_dtype, int64_t nv,
                           const int32_t* t_dev, int64_t ne, int nodes, lh_mesh** out);
+/* Device-to-device copy of a mesh's v and t without its cached topology (the
+ * next assembly of the clone runs the symbolic phase again). */
+int lh_mesh_clone(lh_mesh* src, lh_mesh** out);
 void lh_mesh_free(lh_mesh* m);
 /* Matrix dimension n = max(t)+1 (scipy's inferred csc shape,
  * lapy/solver.py:178) and pattern size; builds the topology on first use. */

@@ -82,6 +82,7 @@ SIGNATURES = {
     "lh_sync": (C.c_int, [_vp]),
     "lh_mesh_create": (C.c_int, [_vp, _vp, C.c_int, _i64, _vp, _i64, C.c_int, C.POINTER(_vp)]),
     "lh_mesh_create_device": (C.c_int, [_vp, _vp, C.c_int, _i64, _vp, _i64, C.c_int, C.POINTER(_vp)]),
+    "lh_mesh_clone": (C.c_int, [_vp, C.POINTER(_vp)]),
     "lh_mesh_free": (None, [_vp]),
     "lh_mesh_pattern_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
     "lh_mesh_avg_edge_length": (C.c_int, [_vp, _dp]),
@@ -102,7 +103,7 @@ SIGNATURES = {
 }
 
 _lib = None
-_lock = threading.Lock()
+_lock = threading.RLock()  # default_context() -> Context() -> lib() re-enters
 
 
 def lib():
@@ -277,6 +278,20 @@ class DeviceMesh:
         self.handle = h
         self.nodes = td.shape[1]
         self.vdtype = vd.dtype
+
+    def clone(self):
+        """Device-to-device copy without the cached topology."""
+        h = _vp()
+        check(lib().lh_mesh_clone(self.handle, C.byref(h)), self.ctx, "lh_mesh_clone")
+        other = DeviceMesh.__new__(DeviceMesh)
+        other.ctx, other.handle, other.nodes, other.vdtype = self.ctx, h, self.nodes, self.vdtype
+        return other
+
+    def time_assembly(self, kind, lump, A, B, iters=20):
+        us = C.c_double()
+        check(lib().lh_time_assembly(self.ctx.handle, self.handle, kind, int(bool(lump)),
+                                     A.handle if A is not None else None, B.handle, iters, C.byref(us)), self.ctx)
+        return us.value
 
     def pattern_info(self):
         n, nnz = _i64(), _i64()

@@ -714,6 +714,32 @@ int lh_mesh_create_device(lh_ctx* ctx, const void* v_soa, int v_dtype, int64_t n
     LH_API_END(ctx)
 }
 
+int lh_mesh_clone(lh_mesh* src, lh_mesh** out) {
+    if (!src || !out) return LH_ERR_ARG;
+    *out = nullptr;
+    lh_ctx* ctx = src->ctx;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    auto m = std::make_unique<lh_mesh>();
+    m->ctx = ctx;
+    m->nv = src->nv;
+    m->ne = src->ne;
+    m->nodes = src->nodes;
+    m->vdtype = src->vdtype;
+    m->n = src->n;
+    const size_t es = src->vdtype == LH_F64 ? 8 : 4;
+    DevBuf* s[3] = {&src->x, &src->y, &src->z};
+    DevBuf* d[3] = {&m->x, &m->y, &m->z};
+    for (int c = 0; c < 3; ++c) {
+        d[c]->alloc(src->nv * es);
+        HIPX(hipMemcpyAsync(d[c]->p, s[c]->p, src->nv * es, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    m->t.alloc(src->ne * src->nodes * 4);
+    HIPX(hipMemcpyAsync(m->t.p, src->t.p, src->ne * src->nodes * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    *out = m.release();
+    LH_API_END(ctx)
+}
+
 void lh_mesh_free(lh_mesh* m) {
     if (!m) return;
     (void)hipSetDevice(m->ctx->device);

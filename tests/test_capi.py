@@ -43,6 +43,30 @@ def test_ctypes_signatures_bind():
         assert getattr(L, name).argtypes is not None
 
 
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="liblapyhip.so not built")
+def test_context_creation_without_gpu_fails_loudly():
+    """Without a HIP device the default context raises (no CPU fallback) and
+    does not dead-lock on the loader lock."""
+    import threading
+
+    L = _lib.lib()
+    if L.lh_device_count() > 0:
+        pytest.skip("a GPU is visible")
+    box = {}
+
+    def run():
+        try:
+            _lib.default_context()
+        except RuntimeError as e:
+            box["err"] = str(e)
+
+    th = threading.Thread(target=run, daemon=True)
+    th.start()
+    th.join(20)
+    assert not th.is_alive(), "default_context() hung"
+    assert "no HIP device" in box.get("err", "")
+
+
 def test_stats_struct_layout():
     # lh_stats: 4 int32, 5 double, 4 int32
     assert ctypes.sizeof(_lib.Stats) == 4 * 4 + 5 * 8 + 4 * 4
