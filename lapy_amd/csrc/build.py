@@ -34,6 +34,7 @@ SOURCES = {
     "lh_assembly.hip": ["-ffp-contract=off"],
     "lh_sparse.hip": [],
     "lh_dense.hip": [],
+    "lh_rr.hip": [],
     "lh_eigs.hip": [],
     "lh_pcg.hip": [],
 }

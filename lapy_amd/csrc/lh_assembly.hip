@@ -432,44 +432,73 @@ k_tet_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restr
 }
 
 // ---------------------------------------------------------------------------
-// average edge length over the pattern's upper triangle
+// average edge length, as lapy/tria_mesh.py:735-748 computes it: the edges
+// are the strictly upper entries (i < j) of the symmetric adjacency in its
+// CSC storage order (column j ascending, then row i), which for a symmetric
+// pattern is "row j of the CSR, columns i < j"; each length is
+// sqrt(((v_i - v_j)**2).sum()) in the vertex dtype, and the mean is numpy's
+// pairwise sum in 8192-element chunks divided by the count, also in the
+// vertex dtype.
 // ---------------------------------------------------------------------------
-template <typename VT>
-__global__ void k_edge_len_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
-                                const VT* __restrict__ x, const VT* __restrict__ y, const VT* __restrict__ z,
-                                double* __restrict__ rowsum, int32_t* __restrict__ rowcnt) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double s = 0.0;
+__global__ void k_lower_count(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+                              int32_t* __restrict__ cnt) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
     int32_t c = 0;
-    V3<double> vi{(double)x[i], (double)y[i], (double)z[i]};
-    for (int32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
-        int32_t j = cols[q];
-        if (j <= i) continue;
-        V3<double> d{(double)x[j] - vi.x, (double)y[j] - vi.y, (double)z[j] - vi.z};
-        s += sqrt(vdot(d, d));
-        ++c;
-    }
-    rowsum[i] = s;
-    rowcnt[i] = c;
+    for (int32_t q = rowptr[j]; q < rowptr[j + 1]; ++q) c += cols[q] < (int32_t)j;
+    cnt[j] = c;
 }
 
-// deterministic single-block reduction
-__global__ void k_reduce_sum_cnt(const double* __restrict__ s, const int32_t* __restrict__ c, int64_t n,
-                                 double* __restrict__ out) {
-    __shared__ double ss[256];
-    __shared__ long long cc[256];
-    double a = 0.0;
-    long long b = 0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) { a += s[i]; b += c[i]; }
-    ss[threadIdx.x] = a;
-    cc[threadIdx.x] = b;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < (unsigned)off) { ss[threadIdx.x] += ss[threadIdx.x + off]; cc[threadIdx.x] += cc[threadIdx.x + off]; }
-        __syncthreads();
+template <typename VT>
+__global__ void k_edge_lengths(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+                               const int32_t* __restrict__ offs, const VT* __restrict__ x, const VT* __restrict__ y,
+                               const VT* __restrict__ z, VT* __restrict__ elen) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int32_t o = offs[j];
+    const V3<VT> vj = load_v(x, y, z, (int32_t)j);
+    for (int32_t q = rowptr[j]; q < rowptr[j + 1]; ++q) {
+        const int32_t i = cols[q];
+        if (i >= (int32_t)j) break;
+        const V3<VT> d = vsub(load_v(x, y, z, i), vj);
+        elen[o++] = dsqrt(vdot(d, d));
     }
-    if (threadIdx.x == 0) { out[0] = ss[0]; out[1] = (double)cc[0]; }
+}
+
+template <typename T>
+__global__ void k_mean_value(const T* __restrict__ sums, int64_t n, T* __restrict__ out) {
+    int64_t nc = (n + kNpyBuf - 1) / kNpyBuf;
+    T acc = T(0);
+    for (int64_t c = 0; c < nc; ++c) acc = acc + sums[c];
+    out[0] = acc / T(n);
+}
+
+template <typename VT>
+double avg_edge_length_t(lh_ctx* ctx, lh_mesh* m) {
+    const int64_t n = m->n;
+    const Pattern& p = *m->full;
+    DevBuf cnt(n * 4), offs((n + 1) * 4);
+    k_lower_count<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, p.rowptr.as<int32_t>(), p.cols.as<int32_t>(),
+                                                           cnt.as<int32_t>());
+    LH_CHECK_LAUNCH();
+    exclusive_scan_i32(ctx, cnt.as<int32_t>(), offs.as<int32_t>(), n);
+    int32_t E = 0;
+    d2h(ctx, &E, offs.as<int32_t>() + n, 4);
+    if (E == 0) return 0.0;
+    DevBuf elen((size_t)E * sizeof(VT));
+    k_edge_lengths<VT><<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, p.rowptr.as<int32_t>(), p.cols.as<int32_t>(),
+                                                                offs.as<int32_t>(), m->x.as<VT>(), m->y.as<VT>(),
+                                                                m->z.as<VT>(), elen.as<VT>());
+    LH_CHECK_LAUNCH();
+    const int64_t nc = (E + kNpyBuf - 1) / kNpyBuf;
+    DevBuf sums(nc * sizeof(VT)), res(sizeof(VT));
+    k_chunk_sums<VT><<<nblocks(nc, 64), 64, 0, ctx->stream>>>(elen.as<VT>(), E, sums.as<VT>());
+    LH_CHECK_LAUNCH();
+    k_mean_value<VT><<<1, 1, 0, ctx->stream>>>(sums.as<VT>(), E, res.as<VT>());
+    LH_CHECK_LAUNCH();
+    VT h = 0;
+    d2h(ctx, &h, res.p, sizeof(VT));
+    return (double)h;
 }
 
 // ---------------------------------------------------------------------------
@@ -764,23 +793,7 @@ int lh_mesh_avg_edge_length(lh_mesh* m, double* out) {
     LH_API_BEGIN(ctx)
     HIPX(hipSetDevice(ctx->device));
     build_topology(ctx, m);
-    const int64_t n = m->n;
-    DevBuf rs(n * 8), rc(n * 4), res(16);
-    const Pattern& p = *m->full;
-    if (m->vdtype == LH_F64)
-        k_edge_len_rows<double><<<nblocks(n, 256), 256, 0, ctx->stream>>>(
-            n, p.rowptr.as<int32_t>(), p.cols.as<int32_t>(), m->x.as<double>(), m->y.as<double>(),
-            m->z.as<double>(), rs.as<double>(), rc.as<int32_t>());
-    else
-        k_edge_len_rows<float><<<nblocks(n, 256), 256, 0, ctx->stream>>>(
-            n, p.rowptr.as<int32_t>(), p.cols.as<int32_t>(), m->x.as<float>(), m->y.as<float>(), m->z.as<float>(),
-            rs.as<double>(), rc.as<int32_t>());
-    LH_CHECK_LAUNCH();
-    k_reduce_sum_cnt<<<1, 256, 0, ctx->stream>>>(rs.as<double>(), rc.as<int32_t>(), n, res.as<double>());
-    LH_CHECK_LAUNCH();
-    double h[2];
-    d2h(ctx, h, res.p, 16);
-    *out = h[1] > 0 ? h[0] / h[1] : 0.0;
+    *out = (m->vdtype == LH_F64) ? avg_edge_length_t<double>(ctx, m) : avg_edge_length_t<float>(ctx, m);
     LH_API_END(ctx)
 }
 

@@ -13,6 +13,11 @@ template <typename T>
 void gemm_nn(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const double* C, int ldc, T* Y, int ldy,
              double alpha, double beta);
 size_t rr_work_doubles(int N, int nev);
+// standard symmetric Rayleigh-Ritz (orthonormal basis), LDS-resident, N <= kRRStdMaxN
+constexpr int kRRStdMaxN = 192;
+size_t rr_std_work_doubles(int N, int nev);
+void rr_std(lh_ctx* ctx, int N, int nev, const double* G, int ldg, double* C, double* theta, double* work,
+            int* status);
 void rr_solve(lh_ctx* ctx, int N, int nev, double* GA, double* GM, double* C, double* theta, double* work,
               int* status);
 void cholinv(lh_ctx* ctx, int m, double* G, double* T, double shift, int* status);

@@ -100,12 +100,21 @@ struct Lobpcg {
     }
 
     // Rayleigh-Ritz on the first N columns of S; returns false on breakdown
+    // The blocks of S are B-orthonormalised (X by the previous Rayleigh-Ritz,
+    // W and P by projection + Cholesky-QR), so S^T B S = I to rounding and
+    // the small problem is a standard one; the generalized path (Cholesky of
+    // S^T B S) is kept for trial bases wider than the LDS solver takes.
     bool rayleigh_ritz(int N) {
         double t0 = now_us();
         gram<double>(ctx, n, Sp(), ld, N, ASp(), ld, N, GA.as<double>(), N, wsg);
-        gram<double>(ctx, n, Sp(), ld, N, MSp(), ld, N, GM.as<double>(), N, wsg);
-        rr_solve(ctx, N, m, GA.as<double>(), GM.as<double>(), Cm.as<double>(), theta_d.as<double>(),
-                 rrwork.as<double>(), status.as<int>());
+        if (N <= kRRStdMaxN) {
+            rr_std(ctx, N, m, GA.as<double>(), N, Cm.as<double>(), theta_d.as<double>(), rrwork.as<double>(),
+                   status.as<int>());
+        } else {
+            gram<double>(ctx, n, Sp(), ld, N, MSp(), ld, N, GM.as<double>(), N, wsg);
+            rr_solve(ctx, N, m, GA.as<double>(), GM.as<double>(), Cm.as<double>(), theta_d.as<double>(),
+                     rrwork.as<double>(), status.as<int>());
+        }
         int s = read_status();
         st.rr_calls++;
         st.t_rr_us += now_us() - t0;
@@ -130,7 +139,7 @@ struct Lobpcg {
         Gs.alloc(m * m * 8);
         Ts.alloc(m * m * 8);
         theta_d.alloc(N3 * 8);
-        rrwork.alloc(rr_work_doubles(N3, m) * 8);
+        rrwork.alloc(std::max(rr_work_doubles(N3, m), rr_std_work_doubles(N3, m)) * 8);
         status.alloc(64);
         cs1.alloc(N3 * 8);
         cs2.alloc(N3 * 8);
@@ -191,8 +200,10 @@ struct Lobpcg {
             }
             int N = 2 * m;
             if (have_p) {
-                project_out(0, m, 2 * m, m);
-                project_out(m, m, 2 * m, m);
+                for (int rep = 0; rep < 2; ++rep) {  // classical Gram-Schmidt, twice
+                    project_out(0, m, 2 * m, m);
+                    project_out(m, m, 2 * m, m);
+                }
                 normalize_cols(2 * m, m);
                 spmm_B(Sp() + 2 * m, MSp() + 2 * m, m);
                 if (cholqr(2 * m, m, true)) N = 3 * m;
@@ -219,6 +230,14 @@ struct Lobpcg {
         HIPX(hipStreamSynchronize(ctx->stream));
         st.iterations = it;
         st.t_iter_us = now_us() - t_iter0;
+        // final: B-orthonormalise X and take the Ritz pairs of span(X) alone,
+        // so that the returned vectors are B-orthonormal to rounding and the
+        // eigenvalues are Rayleigh quotients of the returned vectors
+        spmm_B(Sp(), MSp(), m);
+        if (cholqr(0, m, true) && rayleigh_ritz(m)) {
+            gemm_nn<double>(ctx, n, m, m, Sp(), ld, Cm.as<double>(), m, tmp.as<double>(), m, 1.0, 0.0);
+            copy_block<double, double>(ctx, n, m, tmp.as<double>(), m, Sp(), ld);
+        }
         for (int j = 0; j < k; ++j) out_evals[j] = theta[j];
         out_vecs.alloc(n * k * 8);
         copy_block<double, double>(ctx, n, k, Sp(), ld, out_vecs.as<double>(), k);

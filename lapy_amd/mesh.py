@@ -57,8 +57,9 @@ class TriaMesh:
         return cls(v, t)
 
     def avg_edge_length(self) -> float:
-        """Mean length of the mesh edges (lapy/tria_mesh.py:735-748), on the GPU."""
-        return device_mesh(self).avg_edge_length()
+        """Mean length of the mesh edges (lapy/tria_mesh.py:735-748), on the GPU,
+        in the vertex dtype as numpy computes it."""
+        return _vertex_scalar(self, device_mesh(self).avg_edge_length())
 
     def area(self) -> float:
         v1, v2, v3 = self.v[self.t[:, 0]], self.v[self.t[:, 1]], self.v[self.t[:, 2]]
@@ -81,8 +82,14 @@ class TetMesh:
         return cls(v, t)
 
     def avg_edge_length(self) -> float:
-        """Mean length of the mesh edges (lapy/tet_mesh.py:182-195), on the GPU."""
-        return device_mesh(self).avg_edge_length()
+        """Mean length of the mesh edges (lapy/tet_mesh.py:182-195), on the GPU,
+        in the vertex dtype as numpy computes it."""
+        return _vertex_scalar(self, device_mesh(self).avg_edge_length())
+
+
+def _vertex_scalar(mesh, value):
+    dt = np.asarray(mesh.v).dtype
+    return np.float32(value) if dt == np.float32 else np.float64(value)
 
 
 # ---------------------------------------------------------------------------

@@ -188,5 +188,13 @@ def test_avg_edge_length_gpu(expected):
     tt = np.array([[0, 5, 8, 1], [0, 4, 5, 8], [2, 5, 6, 8], [1, 5, 2, 8], [6, 7, 3, 8], [6, 3, 2, 8], [0, 3, 4, 8],
                    [3, 7, 4, 8], [0, 1, 2, 8], [0, 2, 3, 8], [4, 6, 5, 8], [4, 7, 6, 8]])
     assert TetMesh(vt, tt).avg_edge_length() == pytest.approx(1.0543647924813107, rel=1e-14)
+    # same edge order, dtype and summation order as numpy: bit-identical
     v2, t2 = meshgen.large_ellipsoid(level=4)
-    assert TriaMesh(v2, t2).avg_edge_length() == pytest.approx(fem.avg_edge_length(v2, t2), rel=1e-13)
+    assert TriaMesh(v2, t2).avg_edge_length() == fem.avg_edge_length(v2, t2)
+    tria32 = TriaMesh.read_vtk(data_path("cubeTria.vtk"))
+    got = tria32.avg_edge_length()
+    assert got.dtype == np.float32 and got == fem.avg_edge_length(tria32.v, tria32.t)
+    tet32 = TetMesh.read_vtk(data_path("cubeTetra.vtk"))
+    assert tet32.avg_edge_length() == fem.avg_edge_length(tet32.v, tet32.t)
+    big = meshgen.tet_cube(30, jitter=0.1)
+    assert TetMesh(*big).avg_edge_length() == fem.avg_edge_length(*big)
