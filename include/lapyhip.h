@@ -184,6 +184,11 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta,
 int lh_time_assembly(lh_ctx* ctx, lh_mesh* m, int kind, int lump, lh_csr* A, lh_csr* B, int iters,
                      double* avg_us);
 int lh_time_spmm(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int precision, int iters, double* avg_us);
+/* The Rayleigh-Ritz eigensolver on its own (no counterpart in the reference;
+ * exposed for its tests and timing): smallest nev eigenpairs of the symmetric
+ * N x N host matrix H (N <= 192).  evals (nev), Z (N x nev row-major); when
+ * phase_us is not NULL it receives 7 phase durations in microseconds. */
+int lh_small_symeig(lh_ctx* ctx, int N, int nev, const double* H, double* evals, double* Z, double* phase_us);
 
 #ifdef __cplusplus
 }

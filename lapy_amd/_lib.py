@@ -100,6 +100,7 @@ SIGNATURES = {
                          C.POINTER(Stats)]),
     "lh_time_assembly": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, _dp]),
     "lh_time_spmm": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _dp]),
+    "lh_small_symeig": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None
@@ -387,3 +388,15 @@ def pcg(ctx: Context, A: DeviceCSR, B, alpha: float, beta: float, b, x0=None, dm
                         _ptr(bf), bf.shape[1], _ptr(dm), _ptr(x), C.byref(opts), C.byref(st))
     check(code, ctx, "lh_pcg")
     return (x[:, 0] if squeeze else np.ascontiguousarray(x)), st
+
+
+def small_symeig(ctx: Context, H, nev: int, profile: bool = False):
+    """Smallest nev eigenpairs of a small symmetric matrix with the GPU
+    Rayleigh-Ritz solver (N <= 192); returns (evals, Z[, phase_us])."""
+    H = np.ascontiguousarray(H, dtype=np.float64)
+    N = H.shape[0]
+    ev = np.empty(nev)
+    Z = np.empty((N, nev))
+    ph = np.zeros(7) if profile else None
+    check(lib().lh_small_symeig(ctx.handle, N, nev, _ptr(H), _ptr(ev), _ptr(Z), _ptr(ph)), ctx, "lh_small_symeig")
+    return (ev, Z, ph) if profile else (ev, Z)

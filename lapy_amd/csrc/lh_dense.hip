@@ -76,27 +76,39 @@ k_gram_partial(int64_t n, int64_t rows_per_chunk, const T* __restrict__ U, int l
         for (int t = 0; t < 4; ++t) dst[(ty * 4 + s) * GT + tx * 4 + t] = acc[s][t];
 }
 
+// 64 outputs per workgroup; four lane groups sum interleaved chunks, then
+// combine in fixed order (deterministic)
 template <typename AT, typename CT>
-__global__ void k_gram_reduce(int nchunk, int p, int q, int ntz, const AT* __restrict__ part, CT* __restrict__ C,
-                              int ldc, double alpha) {
-    // one thread per output entry
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= p * q) return;
-    const int i = idx / q, j = idx % q;
-    const int ty = i / GT, tz = j / GT;
-    const int tile = ty * ntz + tz;
-    const int ntile = ((p + GT - 1) / GT) * ntz;
-    const int off = (i % GT) * GT + (j % GT);
+__global__ void __launch_bounds__(256)
+k_gram_reduce(int nchunk, int p, int q, int ntz, const AT* __restrict__ part, CT* __restrict__ C, int ldc,
+              double alpha) {
+    __shared__ AT sh[4][64];
+    const int o = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
     AT s = AT(0);
-    for (int c = 0; c < nchunk; ++c) s += part[((int64_t)c * ntile + tile) * (GT * GT) + off];
-    C[(int64_t)i * ldc + j] = (CT)(alpha * (double)s);
+    if (o < p * q) {
+        const int i = o / q, j = o - (o / q) * q;
+        const int tile = (i / GT) * ntz + (j / GT);
+        const int ntile = ((p + GT - 1) / GT) * ntz;
+        const int off = (i % GT) * GT + (j % GT);
+        const AT* src = part + (int64_t)tile * (GT * GT) + off;
+        const int64_t stride = (int64_t)ntile * (GT * GT);
+        for (int c = g; c < nchunk; c += 4) s += src[c * stride];
+    }
+    sh[g][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (g == 0 && o < p * q) {
+        const int i = o / q, j = o - (o / q) * q;
+        const int l = threadIdx.x;
+        C[(int64_t)i * ldc + j] = (CT)(alpha * (double)((sh[0][l] + sh[1][l]) + (sh[2][l] + sh[3][l])));
+    }
 }
 
 template <typename T>
 void gram(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, int ldv, int q, double* C, int ldc,
           DevBuf& ws, double alpha) {
     const int tu = (p + GT - 1) / GT, tv = (q + GT - 1) / GT;
-    int64_t nchunk = std::min<int64_t>(std::max<int64_t>(1, (int64_t)ctx->n_cu * 4 / (tu * tv)), (n + 255) / 256);
+    // about two workgroups per CU in the partial pass, at least 512 rows each
+    int64_t nchunk = std::min<int64_t>(std::max<int64_t>(1, (int64_t)ctx->n_cu * 2 / (tu * tv)), (n + 511) / 512);
     if (nchunk < 1) nchunk = 1;
     int64_t rpc = (n + nchunk - 1) / nchunk;
     rpc = ((rpc + GR - 1) / GR) * GR;
@@ -106,7 +118,7 @@ void gram(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, int ld
     dim3 grid((unsigned)nchunk, tu, tv);
     k_gram_partial<T, double><<<grid, 256, 0, ctx->stream>>>(n, rpc, U, ldu, p, V, ldv, q, ws.as<double>());
     LH_CHECK_LAUNCH();
-    k_gram_reduce<double, double><<<nblocks((int64_t)p * q, 256), 256, 0, ctx->stream>>>(
+    k_gram_reduce<double, double><<<nblocks((int64_t)p * q, 64), 256, 0, ctx->stream>>>(
         (int)nchunk, p, q, tv, ws.as<double>(), C, ldc, alpha);
     LH_CHECK_LAUNCH();
 }
@@ -566,47 +578,83 @@ void rr_solve(lh_ctx* ctx, int N, int nev, double* GA, double* GM, double* C, do
     LH_CHECK_LAUNCH();
 }
 
-// T = L^-T where G = L L^T (m x m).  status 1 on failure.  Optional diagonal
-// shift (relative to the largest diagonal) before factorising.
-__global__ void __launch_bounds__(RB) k_cholinv(int m, double* __restrict__ G, double* __restrict__ T,
+// T with T^T G T = I for a small SPD Gram G (m x m, m <= 128): G is
+// equilibrated to unit diagonal (D G D), optionally shifted, factorised
+// G_s = L L^T in LDS, and T = D L^-T.  status 1 on a non-positive pivot.
+constexpr int kCholMax = 128;
+__global__ void __launch_bounds__(RB) k_cholinv(int m, const double* __restrict__ G, double* __restrict__ T,
                                                 double shift, int* __restrict__ status) {
-    __shared__ double red[64];
+    __shared__ double L[kCholMax * kCholMax];
+    __shared__ double dsc[kCholMax];
     __shared__ int flag[1];
     const int tid = threadIdx.x;
     if (tid == 0) flag[0] = 0;
-    for (int idx = tid; idx < m * m; idx += RB) {
-        int i = idx / m, j = idx % m;
-        if (j > i) {
-            double a = 0.5 * (G[i * m + j] + G[j * m + i]);
-            G[i * m + j] = a;
-            G[j * m + i] = a;
-        }
+    for (int i = tid; i < m; i += RB) {
+        const double d = G[i * m + i];
+        dsc[i] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
+        if (!(d > 0.0)) flag[0] = 1;
     }
     __syncthreads();
-    if (shift > 0.0) {
-        if (tid == 0) {
-            double mx = 0.0;
-            for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(G[i * m + i]));
-            red[0] = mx;
-        }
-        __syncthreads();
-        const double s = shift * red[0];
-        __syncthreads();
-        for (int i = tid; i < m; i += RB) G[i * m + i] += s;
-        __syncthreads();
-    }
-    chol_inplace(G, m, red, flag);
     if (flag[0]) {
         if (tid == 0) status[0] = 1;
         return;
     }
-    for (int idx = tid; idx < m * m; idx += RB) T[idx] = ((idx / m) == (idx % m)) ? 1.0 : 0.0;
+    for (int idx = tid; idx < m * m; idx += RB) {
+        const int i = idx / m, j = idx - (idx / m) * m;
+        double v = 0.5 * (G[i * m + j] + G[j * m + i]) * dsc[i] * dsc[j];
+        if (i == j) v = 1.0 + shift;
+        L[idx] = v;
+    }
     __syncthreads();
-    trsm_lowerT_cols(G, m, T, m, m);
-    if (tid == 0) status[0] = 0;
+    // right-looking Cholesky; lanes own rows of the trailing matrix.  The
+    // smallest pivot (relative to the unit diagonal) is reported in
+    // ((double*)status)[1]; below 1e-12 the block counts as rank deficient.
+    double minpiv = 1.0;
+    for (int j = 0; j < m; ++j) {
+        const double djj = L[j * m + j];
+        minpiv = fmin(minpiv, djj);
+        if (!(djj > 1e-12)) {
+            if (tid == 0) { status[0] = 1; reinterpret_cast<double*>(status)[1] = djj; }
+            return;
+        }
+        const double ljj = sqrt(djj);
+        for (int i = j + 1 + (tid >> 4); i < m; i += RB / 16) {
+            const double lij = L[i * m + j] / ljj;
+            for (int kk = j + 1 + (tid & 15); kk <= i; kk += 16) L[i * m + kk] -= lij * (L[kk * m + j] / ljj);
+        }
+        __syncthreads();
+        for (int i = j + 1 + tid; i < m; i += RB) L[i * m + j] /= ljj;
+        if (tid == 0) L[j * m + j] = ljj;
+        __syncthreads();
+    }
+    // Tu = L^-T (upper triangular) by back substitution over rows i, all
+    // columns c >= i in parallel, the dot split over 8 lanes per column.  Tu
+    // is kept in the (unused) upper triangle of L, L's diagonal in ldg.
+    __shared__ double ldg[kCholMax];
+    for (int i = tid; i < m; i += RB) ldg[i] = L[i * m + i];
+    __syncthreads();
+    for (int i = m - 1; i >= 0; --i) {
+        const int c = i + (tid >> 3), l8 = tid & 7;
+        double s = 0.0;
+        if (c < m)
+            for (int r = i + 1 + l8; r <= c; r += 8) s += L[r * m + i] * L[r * m + c];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        s += __shfl_xor(s, 4, 64);
+        __syncthreads();  // all reads of row i's inputs done before row i is written
+        if (c < m && l8 == 0) L[i * m + c] = ((c == i ? 1.0 : 0.0) - s) / ldg[i];
+        __syncthreads();
+    }
+    // T = D Tu
+    for (int idx = tid; idx < m * m; idx += RB) {
+        const int i = idx / m, c = idx - (idx / m) * m;
+        T[idx] = (c >= i) ? dsc[i] * L[idx] : 0.0;
+    }
+    if (tid == 0) { status[0] = 0; reinterpret_cast<double*>(status)[1] = minpiv; }
 }
 
 void cholinv(lh_ctx* ctx, int m, double* G, double* T, double shift, int* status) {
+    if (m > kCholMax) LH_THROW(LH_ERR_ARG, "cholinv: block width %d exceeds %d", m, kCholMax);
     k_cholinv<<<1, RB, 0, ctx->stream>>>(m, G, T, shift, status);
     LH_CHECK_LAUNCH();
 }
@@ -651,12 +699,19 @@ __global__ void k_colsq_partial(int64_t n, int m, const T* __restrict__ X, int l
     __syncthreads();
     if (rg == 0 && c < m) part[(int64_t)blockIdx.x * m + c] = ((s[0][threadIdx.x] + s[1][threadIdx.x]) + s[2][threadIdx.x]) + s[3][threadIdx.x];
 }
-__global__ void k_colsq_reduce(int nchunk, int m, const double* __restrict__ part, double* __restrict__ out) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m) return;
+__global__ void __launch_bounds__(256) k_colsq_reduce(int nchunk, int m, const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+    __shared__ double sh[4][64];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
     double s = 0.0;
-    for (int k = 0; k < nchunk; ++k) s += part[(int64_t)k * m + c];
-    out[c] = s;
+    if (c < m)
+        for (int k = g; k < nchunk; k += 4) s += part[(int64_t)k * m + c];
+    sh[g][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (g == 0 && c < m) {
+        const int l = threadIdx.x;
+        out[c] = (sh[0][l] + sh[1][l]) + (sh[2][l] + sh[3][l]);
+    }
 }
 template <typename T>
 void colsq(lh_ctx* ctx, int64_t n, int m, const T* X, int ld, double* out, DevBuf& ws) {
@@ -667,7 +722,7 @@ void colsq(lh_ctx* ctx, int64_t n, int m, const T* X, int ld, double* out, DevBu
     dim3 grid((unsigned)nchunk, (m + 63) / 64);
     k_colsq_partial<T><<<grid, 256, 0, ctx->stream>>>(n, m, X, ld, rpc, ws.as<double>());
     LH_CHECK_LAUNCH();
-    k_colsq_reduce<<<nblocks(m, 64), 64, 0, ctx->stream>>>((int)nchunk, m, ws.as<double>(), out);
+    k_colsq_reduce<<<nblocks(m, 64), 256, 0, ctx->stream>>>((int)nchunk, m, ws.as<double>(), out);
     LH_CHECK_LAUNCH();
 }
 template void colsq<double>(lh_ctx*, int64_t, int, const double*, int, double*, DevBuf&);

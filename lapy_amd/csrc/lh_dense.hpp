@@ -17,7 +17,7 @@ size_t rr_work_doubles(int N, int nev);
 constexpr int kRRStdMaxN = 192;
 size_t rr_std_work_doubles(int N, int nev);
 void rr_std(lh_ctx* ctx, int N, int nev, const double* G, int ldg, double* C, double* theta, double* work,
-            int* status);
+            int* status, unsigned long long* prof = nullptr);
 void rr_solve(lh_ctx* ctx, int N, int nev, double* GA, double* GM, double* C, double* theta, double* work,
               int* status);
 void cholinv(lh_ctx* ctx, int m, double* G, double* T, double shift, int* status);

@@ -65,20 +65,25 @@ struct Lobpcg {
         return s;
     }
 
-    // X block (cols [off, off+c)) <- X T, T = L^-T of G = X^T (B X); returns
-    // false on breakdown.  AX/MX of the block are recomputed afterwards.
+    // X block (cols [off, off+c)) <- X T with T^T (X^T B X) T = I; B X must be
+    // current on entry.  A second pass follows when the first one met a small
+    // pivot (Cholesky-QR2).  Returns false when the block is numerically rank
+    // deficient.  A X and B X of the block are recomputed afterwards.
     bool cholqr(int off, int c, bool recompute_A) {
         double* X = Sp() + off;
         double* MX = MSp() + off;
-        gram<double>(ctx, n, X, ld, c, MX, ld, c, Gs.as<double>(), c, wsg);
-        cholinv(ctx, c, Gs.as<double>(), Ts.as<double>(), 0.0, status.as<int>());
-        if (read_status()) {
+        for (int pass = 0; pass < 2; ++pass) {
             gram<double>(ctx, n, X, ld, c, MX, ld, c, Gs.as<double>(), c, wsg);
-            cholinv(ctx, c, Gs.as<double>(), Ts.as<double>(), 1e-13, status.as<int>());
-            if (read_status()) return false;
+            cholinv(ctx, c, Gs.as<double>(), Ts.as<double>(), 0.0, status.as<int>());
+            double stv[2];
+            d2h(ctx, stv, status.p, sizeof(stv));
+            if (reinterpret_cast<int*>(stv)[0]) return false;
+            gemm_nn<double>(ctx, n, c, c, X, ld, Ts.as<double>(), c, tmp.as<double>(), c, 1.0, 0.0);
+            copy_block<double, double>(ctx, n, c, tmp.as<double>(), c, X, ld);
+            if (stv[1] > 1e-2) break;
+            gemm_nn<double>(ctx, n, c, c, MX, ld, Ts.as<double>(), c, tmp.as<double>(), c, 1.0, 0.0);
+            copy_block<double, double>(ctx, n, c, tmp.as<double>(), c, MX, ld);
         }
-        gemm_nn<double>(ctx, n, c, c, X, ld, Ts.as<double>(), c, tmp.as<double>(), c, 1.0, 0.0);
-        copy_block<double, double>(ctx, n, c, tmp.as<double>(), c, X, ld);
         if (recompute_A) spmm_AB(X, ASp() + off, MX, c);
         else spmm_B(X, MX, c);
         return true;
@@ -130,14 +135,14 @@ struct Lobpcg {
         S2.alloc(n * ld * 8);
         AS.alloc(n * ld * 8);
         MS.alloc(n * ld * 8);
-        tmp.alloc(n * m * 8);
+        tmp.alloc(n * 2 * m * 8);
         dinv.alloc(n * 8);
         const int N3 = 3 * m;
         GA.alloc(N3 * N3 * 8);
         GM.alloc(N3 * N3 * 8);
         Cm.alloc(N3 * std::max(N3, m) * 8);
-        Gs.alloc(m * m * 8);
-        Ts.alloc(m * m * 8);
+        Gs.alloc(4 * m * m * 8);
+        Ts.alloc(4 * m * m * 8);
         theta_d.alloc(N3 * 8);
         rrwork.alloc(std::max(rr_work_doubles(N3, m), rr_std_work_doubles(N3, m)) * 8);
         status.alloc(64);
@@ -163,7 +168,7 @@ struct Lobpcg {
         double t_iter0 = now_us();
         std::vector<double> r2(m), mx2(m);
         for (it = 0; it < max_iter; ++it) {
-            // residual into W
+            // residual into W and the convergence test
             residual<double>(ctx, n, m, ASp(), MSp(), ld, theta_d.as<double>(), Sp() + m, ld);
             colsq<double>(ctx, n, m, Sp() + m, ld, cs1.as<double>(), wsc);
             colsq<double>(ctx, n, m, MSp(), ld, cs2.as<double>(), wsc);
@@ -172,14 +177,11 @@ struct Lobpcg {
             const double floor = 1e-2 * std::fabs(theta[k - 1]);
             int nconv = 0;
             double worst = 0.0;
-            for (int j = 0; j < m; ++j) {
+            for (int j = 0; j < k; ++j) {
                 res[j] = std::sqrt(r2[j] / std::max(mx2[j], 1e-300));
-                double sc = std::max(std::fabs(theta[j]), floor);
-                double rel = res[j] / std::max(sc, 1e-300);
-                if (j < k) {
-                    worst = std::max(worst, rel);
-                    if (rel <= tol) ++nconv;
-                }
+                const double rel = res[j] / std::max(std::max(std::fabs(theta[j]), floor), 1e-300);
+                worst = std::max(worst, rel);
+                if (rel <= tol) ++nconv;
             }
             st.max_resid = worst;
             st.converged = nconv;
@@ -187,37 +189,26 @@ struct Lobpcg {
                 fprintf(stderr, "[lobpcg] it %d conv %d/%d worst %.3e theta0 %.6e theta_k %.6e\n", it, nconv, k,
                         worst, theta[0], theta[k - 1]);
             if (nconv == k) break;
-            // precondition
+            // precondition W
             rowscale<double>(ctx, n, m, Sp() + m, ld, dinv.as<double>());
-            // W orthogonal to X (B inner product), twice
-            project_out(0, m, m, m);
-            project_out(0, m, m, m);
-            normalize_cols(m, m);
-            spmm_B(Sp() + m, MSp() + m, m);
-            if (!cholqr(m, m, true)) {
-                st.restarts++;
-                LH_THROW(LH_ERR_NUMERIC, "Cholesky-QR of the residual block failed at iteration %d", it);
-            }
-            int N = 2 * m;
-            if (have_p) {
-                for (int rep = 0; rep < 2; ++rep) {  // classical Gram-Schmidt, twice
-                    project_out(0, m, 2 * m, m);
-                    project_out(m, m, 2 * m, m);
-                }
-                normalize_cols(2 * m, m);
-                spmm_B(Sp() + 2 * m, MSp() + 2 * m, m);
-                if (cholqr(2 * m, m, true)) N = 3 * m;
-                else { have_p = false; st.restarts++; }
-            }
-            if (!rayleigh_ritz(N)) {
-                if (N == 3 * m) {
+            // [W P] orthogonal to X in the B inner product (classical Gram-Schmidt, twice)
+            int nb = have_p ? 2 * m : m;
+            project_out(0, m, m, nb);
+            project_out(0, m, m, nb);
+            // joint Cholesky-QR of [W P]; P is dropped if [W P] is numerically rank deficient
+            spmm_B(Sp() + m, MSp() + m, nb);
+            if (!cholqr(m, nb, true)) {
+                if (nb == 2 * m) {
                     st.restarts++;
-                    N = 2 * m;
-                    if (!rayleigh_ritz(N)) LH_THROW(LH_ERR_NUMERIC, "Rayleigh-Ritz failed at iteration %d", it);
+                    nb = m;
+                    if (!cholqr(m, nb, true))
+                        LH_THROW(LH_ERR_NUMERIC, "Cholesky-QR of the residual block failed at iteration %d", it);
                 } else {
-                    LH_THROW(LH_ERR_NUMERIC, "Rayleigh-Ritz failed at iteration %d", it);
+                    LH_THROW(LH_ERR_NUMERIC, "Cholesky-QR of the residual block failed at iteration %d", it);
                 }
             }
+            const int N = m + nb;
+            if (!rayleigh_ritz(N)) LH_THROW(LH_ERR_NUMERIC, "Rayleigh-Ritz failed at iteration %d", it);
             // X_new = S C ; P_new = S[:, m:N] C[m:N, :]
             double* S2p = S2.as<double>();
             gemm_nn<double>(ctx, n, N, m, Sp(), ld, Cm.as<double>(), m, S2p, ld, 1.0, 0.0);
