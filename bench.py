@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--tol", type=float, default=0.0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--poly-degree", type=int, default=0)
+    ap.add_argument("--poly-ratio", type=float, default=0.0)
     ap.add_argument("--cpu-level", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", type=int, default=0)
@@ -91,7 +93,8 @@ def main():
         m = base.clone()
         A, B = m.assemble(_lib.LH_TRIA, False, np.float64)
         evals, eh, st = _lib.eigs(ctx, A, B, args.k, -0.01, precision=prec, tol=args.tol, block=args.block,
-                                  verbose=args.verbose, keep_device=True)
+                                  verbose=args.verbose, keep_device=True, poly_degree=args.poly_degree,
+                                  poly_ratio=args.poly_ratio)
         return evals, eh, st, (m, A, B)
 
     for w in range(args.warmup):

@@ -140,7 +140,10 @@ typedef struct lh_eig_opts {
     double  tol;           /* relative residual target (0: automatic)          */
     uint64_t seed;         /* start block seed                                 */
     int32_t verbose;
-    int32_t reserved;
+    int32_t poly_degree;   /* with LH_PRECOND_JACOBI: degree of the Chebyshev
+                              polynomial in D^-1 (A - sigma B) applied on top
+                              of Jacobi (0 or 1: plain Jacobi)                */
+    double  poly_ratio;    /* Chebyshev interval [lmax/ratio, lmax] (0: auto) */
 } lh_eig_opts;
 
 /* Replaces Solver.eigs / eigensystem (lapy/solver.py:667-716: splu of

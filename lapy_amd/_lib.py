@@ -59,7 +59,8 @@ class EigOpts(C.Structure):
         ("tol", C.c_double),
         ("seed", C.c_uint64),
         ("verbose", C.c_int32),
-        ("reserved", C.c_int32),
+        ("poly_degree", C.c_int32),
+        ("poly_ratio", C.c_double),
     ]
 
 
@@ -334,9 +335,10 @@ class DeviceMesh:
 
 
 def eigs(ctx: Context, A: DeviceCSR, B: DeviceCSR, k: int, sigma: float = -0.01, precision=LH_PREC_FP64,
-         precond=LH_PRECOND_JACOBI, block=0, max_iter=0, tol=0.0, seed=0, verbose=0, keep_device=False):
+         precond=LH_PRECOND_JACOBI, block=0, max_iter=0, tol=0.0, seed=0, verbose=0, keep_device=False,
+         poly_degree=0, poly_ratio=0.0):
     """Generalized eigenpairs on the GPU; returns (evals, evecs, stats)."""
-    opts = EigOpts(precision, precond, block, max_iter, tol, seed, verbose, 0)
+    opts = EigOpts(precision, precond, block, max_iter, tol, seed, verbose, poly_degree, poly_ratio)
     st = Stats()
     h = _vp()
     code = lib().lh_eigs(ctx.handle, A.handle, B.handle, int(k), float(sigma), C.byref(opts), C.byref(h), C.byref(st))

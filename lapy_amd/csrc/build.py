@@ -35,6 +35,7 @@ SOURCES = {
     "lh_sparse.hip": [],
     "lh_dense.hip": [],
     "lh_rr.hip": [],
+    "lh_poly.hip": [],
     "lh_eigs.hip": [],
     "lh_pcg.hip": [],
 }

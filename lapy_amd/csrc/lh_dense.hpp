@@ -36,4 +36,21 @@ void randn(lh_ctx* ctx, int64_t n, int m, T* X, int ld, uint64_t seed);
 // dinv = 1 / (alpha * diag(B) + beta * diag(A))
 void jacobi_diag(lh_ctx* ctx, const lh_csr* A, const lh_csr* B, double alpha, double beta, double* dinv);
 
+// values of K = beta A + alpha B on A's pattern (B diagonal or on that pattern)
+void form_shifted(lh_ctx* ctx, const lh_csr* A, const lh_csr* B, double alpha, double beta, DevBuf& kv);
+
+// Chebyshev-accelerated Jacobi preconditioner for K = A - sigma B (lh_poly.hip)
+struct ChebPrecond {
+    lh_ctx* ctx = nullptr;
+    std::shared_ptr<Pattern> pat;
+    int64_t n = 0;
+    int degree = 1;
+    double bmax = 2.0, amin = 0.03;
+    DevBuf kv, dinv, R, D0, D1;
+    void setup(lh_ctx* c, const lh_csr* A, const lh_csr* B, double sigma, int deg, double ratio, int64_t n_,
+               int mmax);
+    // W (n x m, leading dimension ldw) <- p(D^-1 K) D^-1 W
+    void apply(double* W, int ldw, int m);
+};
+
 }  // namespace lh

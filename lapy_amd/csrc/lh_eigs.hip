@@ -40,6 +40,9 @@ struct Lobpcg {
     int k, m, ld;
     double sigma, tol;
     int max_iter, verbose;
+    int poly_degree = 0;
+    double poly_ratio = 0.0;
+    ChebPrecond cheb;
     uint64_t seed;
     DevBuf S, S2, AS, MS, tmp, dinv;
     DevBuf GA, GM, Cm, Gs, Ts, theta_d, rrwork, status, cs1, cs2, wsg, wsc;
@@ -152,6 +155,7 @@ struct Lobpcg {
         res.assign(m, 0.0);
         HIPX(hipMemsetAsync(S.p, 0, n * ld * 8, ctx->stream));
         jacobi_diag(ctx, A, B, -sigma, 1.0, dinv.as<double>());
+        if (poly_degree > 1) cheb.setup(ctx, A, B, sigma, poly_degree, poly_ratio, n, m);
 
         // start block
         randn<double>(ctx, n, m, Sp(), ld, seed);
@@ -190,7 +194,12 @@ struct Lobpcg {
                         worst, theta[0], theta[k - 1]);
             if (nconv == k) break;
             // precondition W
-            rowscale<double>(ctx, n, m, Sp() + m, ld, dinv.as<double>());
+            if (poly_degree > 1) {
+                cheb.apply(Sp() + m, ld, m);
+                st.spmm_calls += poly_degree - 1;
+            } else {
+                rowscale<double>(ctx, n, m, Sp() + m, ld, dinv.as<double>());
+            }
             // [W P] orthogonal to X in the B inner product (classical Gram-Schmidt, twice)
             int nb = have_p ? 2 * m : m;
             project_out(0, m, m, nb);
@@ -310,6 +319,10 @@ int lh_eigs(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const lh_eig
     L.max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 20000;
     L.seed = opts ? opts->seed : 0;
     L.verbose = opts ? opts->verbose : 0;
+    if (opts && opts->precond == LH_PRECOND_JACOBI) {
+        L.poly_degree = opts->poly_degree;
+        L.poly_ratio = opts->poly_ratio;
+    }
     if (opts && opts->precision != LH_PREC_FP64) LH_THROW(LH_ERR_ARG, "precision mode %d not available", opts->precision);
     auto e = std::make_unique<lh_eig>();
     e->ctx = ctx;
