@@ -142,8 +142,10 @@ typedef struct lh_eig_opts {
     int32_t verbose;
     int32_t poly_degree;   /* with LH_PRECOND_JACOBI: degree of the Chebyshev
                               polynomial in D^-1 (A - sigma B) applied on top
-                              of Jacobi (0 or 1: plain Jacobi)                */
-    double  poly_ratio;    /* Chebyshev interval [lmax/ratio, lmax] (0: auto) */
+                              of Jacobi; 0: automatic (~sqrt(n)/12, at most
+                              96), 1: plain Jacobi                           */
+    double  poly_ratio;    /* Chebyshev interval [lmax/ratio, lmax]
+                              (0: automatic, 4 * degree^2)                   */
 } lh_eig_opts;
 
 /* Replaces Solver.eigs / eigensystem (lapy/solver.py:667-716: splu of
@@ -187,6 +189,10 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta,
 int lh_time_assembly(lh_ctx* ctx, lh_mesh* m, int kind, int lump, lh_csr* A, lh_csr* B, int iters,
                      double* avg_us);
 int lh_time_spmm(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int precision, int iters, double* avg_us);
+/* One step of the eigensolver's Chebyshev-Jacobi preconditioner (FP32, fused
+ * SpMM + update) on an n x ncol block, optionally after the solver's
+ * locality reordering (reorder = 1, as lh_eigs does). */
+int lh_time_cheb_step(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int iters, int reorder, double* avg_us);
 /* The Rayleigh-Ritz eigensolver on its own (no counterpart in the reference;
  * exposed for its tests and timing): smallest nev eigenpairs of the symmetric
  * N x N host matrix H (N <= 192).  evals (nev), Z (N x nev row-major); when

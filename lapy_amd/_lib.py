@@ -102,6 +102,7 @@ SIGNATURES = {
     "lh_time_assembly": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, _dp]),
     "lh_time_spmm": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _dp]),
     "lh_small_symeig": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "lh_time_cheb_step": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _dp]),
 }
 
 _lib = None
@@ -354,6 +355,18 @@ def eigs(ctx: Context, A: DeviceCSR, B: DeviceCSR, k: int, sigma: float = -0.01,
     finally:
         if h and not keep_device:
             lib().lh_eig_free(h)
+
+
+def time_cheb_step(ctx: Context, A: DeviceCSR, B: DeviceCSR, ncol: int, iters: int = 50, reorder: bool = True):
+    us = C.c_double()
+    check(lib().lh_time_cheb_step(ctx.handle, A.handle, B.handle, ncol, iters, int(reorder), C.byref(us)), ctx)
+    return us.value
+
+
+def time_spmm(ctx: Context, A: DeviceCSR, B: DeviceCSR, ncol: int, precision=LH_PREC_FP64, iters: int = 20):
+    us = C.c_double()
+    check(lib().lh_time_spmm(ctx.handle, A.handle, B.handle, ncol, precision, iters, C.byref(us)), ctx)
+    return us.value
 
 
 class EigHandle:

@@ -36,6 +36,7 @@ SOURCES = {
     "lh_dense.hip": [],
     "lh_rr.hip": [],
     "lh_poly.hip": [],
+    "lh_reorder.hip": [],
     "lh_eigs.hip": [],
     "lh_pcg.hip": [],
 }

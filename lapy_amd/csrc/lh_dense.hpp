@@ -46,11 +46,28 @@ struct ChebPrecond {
     int64_t n = 0;
     int degree = 1;
     double bmax = 2.0, amin = 0.03;
-    DevBuf kv, dinv, R, D0, D1;
+    DevBuf kv, dinv, kv32, dinv32, R, D0, D1;
     void setup(lh_ctx* c, const lh_csr* A, const lh_csr* B, double sigma, int deg, double ratio, int64_t n_,
                int mmax);
     // W (n x m, leading dimension ldw) <- p(D^-1 K) D^-1 W
     void apply(double* W, int ldw, int m);
+    // average duration (us) of one Chebyshev step on an n x m block
+    double time_step(int m, int iters);
+};
+
+// breadth-first level reordering of a symmetric pattern (lh_reorder.hip)
+struct Reorder {
+    DevBuf perm;   // new row -> old row
+    DevBuf iperm;  // old row -> new row
+    DevBuf src;    // new nonzero slot -> old slot
+    std::shared_ptr<Pattern> pat;  // permuted pattern
+    int nlevels = 0;
+    void build(lh_ctx* ctx, const Pattern& p);
+    // P a P^T on the permuted pattern (diagonal matrices: permuted diagonal)
+    std::unique_ptr<lh_csr> apply(lh_ctx* ctx, const lh_csr* a, std::shared_ptr<Pattern> for_pattern = nullptr,
+                                  const Reorder* diag_reorder = nullptr) const;
+    // out[i, :] = in[iperm[i], :]
+    void unpermute(lh_ctx* ctx, const double* in, int ldi, double* out, int ldo, int64_t n, int m) const;
 };
 
 }  // namespace lh

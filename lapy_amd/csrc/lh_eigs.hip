@@ -43,6 +43,10 @@ struct Lobpcg {
     int poly_degree = 0;
     double poly_ratio = 0.0;
     ChebPrecond cheb;
+    // locality reordering (solver-internal)
+    Reorder ro;
+    std::unique_ptr<lh_csr> Ap, Bp;
+    bool reordered = false;
     uint64_t seed;
     DevBuf S, S2, AS, MS, tmp, dinv;
     DevBuf GA, GM, Cm, Gs, Ts, theta_d, rrwork, status, cs1, cs2, wsg, wsc;
@@ -133,6 +137,14 @@ struct Lobpcg {
 
     void run(double* out_evals, DevBuf& out_vecs) {
         double t_start = now_us();
+        if (n >= 4096 && (B->is_diag || B->pat == A->pat)) {
+            ro.build(ctx, *A->pat);
+            Ap = ro.apply(ctx, A);
+            Bp = ro.apply(ctx, B, B->is_diag ? nullptr : ro.pat);
+            A = Ap.get();
+            B = Bp.get();
+            reordered = true;
+        }
         ld = 3 * m;
         S.alloc(n * ld * 8);
         S2.alloc(n * ld * 8);
@@ -240,7 +252,8 @@ struct Lobpcg {
         }
         for (int j = 0; j < k; ++j) out_evals[j] = theta[j];
         out_vecs.alloc(n * k * 8);
-        copy_block<double, double>(ctx, n, k, Sp(), ld, out_vecs.as<double>(), k);
+        if (reordered) ro.unpermute(ctx, Sp(), ld, out_vecs.as<double>(), k, n, k);
+        else copy_block<double, double>(ctx, n, k, Sp(), ld, out_vecs.as<double>(), k);
         HIPX(hipStreamSynchronize(ctx->stream));
         st.t_total_us = now_us() - t_start;
         if (st.converged < k) {
@@ -319,10 +332,17 @@ int lh_eigs(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const lh_eig
     L.max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 20000;
     L.seed = opts ? opts->seed : 0;
     L.verbose = opts ? opts->verbose : 0;
-    if (opts && opts->precond == LH_PRECOND_JACOBI) {
-        L.poly_degree = opts->poly_degree;
-        L.poly_ratio = opts->poly_ratio;
-    }
+    // preconditioner: Jacobi, accelerated by a Chebyshev polynomial whose
+    // degree grows with sqrt(n) (the condition number of a mesh Laplacian
+    // grows like n^(2/d)); poly_degree 1 is plain Jacobi
+    const bool jac = !opts || opts->precond == LH_PRECOND_JACOBI;
+    int deg = opts ? opts->poly_degree : 0;
+    if (!jac) deg = 1;
+    if (deg <= 0) deg = (int)std::lround(std::sqrt((double)n) / 12.0);
+    deg = std::max(1, std::min(deg, 96));
+    if (deg < 4) deg = 1;
+    L.poly_degree = deg;
+    L.poly_ratio = (opts && opts->poly_ratio > 1.0) ? opts->poly_ratio : 4.0 * deg * deg;
     if (opts && opts->precision != LH_PREC_FP64) LH_THROW(LH_ERR_ARG, "precision mode %d not available", opts->precision);
     auto e = std::make_unique<lh_eig>();
     e->ctx = ctx;

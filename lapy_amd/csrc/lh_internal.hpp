@@ -135,6 +135,18 @@ namespace lh {
 
 inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// MI355X places workgroup b of a launch on XCD (b mod 8), and each of the 8
+// XCDs has its own L2.  Row-blocked gather kernels launch a multiple of 8
+// workgroups and renumber them so that each XCD walks one contiguous eighth
+// of the rows: neighbouring rows (which gather the same block rows) then
+// share an L2.
+constexpr int kXCDs = 8;
+inline unsigned nblocks_xcd(int64_t n, int bs) { return ((nblocks(n, bs) + kXCDs - 1) / kXCDs) * kXCDs; }
+__device__ __forceinline__ int64_t xcd_block() {
+    const unsigned per = gridDim.x / kXCDs;
+    return (int64_t)(blockIdx.x % kXCDs) * per + blockIdx.x / kXCDs;
+}
+
 // exclusive scan of n int32 values in place into out (n+1 entries, out[n]=total)
 void exclusive_scan_i32(lh_ctx* ctx, const int32_t* in, int32_t* out, int64_t n);
 // host-synchronous small copy helpers
@@ -156,6 +168,69 @@ void spmm_pair(lh_ctx* ctx, const lh_csr* a, const lh_csr* b, const XT* X, XT* Y
                int ldy);
 
 void refresh_dense_diag(lh_ctx* ctx, lh_csr* a);
+
+// 16-byte vector of a block row: 4 floats or 2 doubles per lane
+template <typename T> struct Vec16;
+template <> struct Vec16<float> {
+    using type = float4;
+    static constexpr int n = 4;
+};
+template <> struct Vec16<double> {
+    using type = double2;
+    static constexpr int n = 2;
+};
+template <typename T>
+__device__ __forceinline__ void vfma(typename Vec16<T>::type& acc, T w, const typename Vec16<T>::type& x);
+template <>
+__device__ __forceinline__ void vfma<float>(float4& acc, float w, const float4& x) {
+    acc.x += w * x.x; acc.y += w * x.y; acc.z += w * x.z; acc.w += w * x.w;
+}
+template <>
+__device__ __forceinline__ void vfma<double>(double2& acc, double w, const double2& x) {
+    acc.x += w * x.x; acc.y += w * x.y;
+}
+template <typename T>
+__device__ __forceinline__ typename Vec16<T>::type vzero() {
+    typename Vec16<T>::type z;
+    z.x = T(0); z.y = T(0);
+    if constexpr (Vec16<T>::n == 4) { z.z = T(0); z.w = T(0); }
+    return z;
+}
+template <typename T>
+__device__ __forceinline__ void vadd(typename Vec16<T>::type& a, const typename Vec16<T>::type& b) {
+    a.x += b.x; a.y += b.y;
+    if constexpr (Vec16<T>::n == 4) { a.z += b.z; a.w += b.w; }
+}
+// acc = sum_s vals[s] * X[cols[s], c0 .. c0+VEC) over one CSR row, two
+// independent accumulators so that two gathers are in flight per lane
+template <typename VT, typename T>
+__device__ __forceinline__ typename Vec16<T>::type row_gather(const int32_t* __restrict__ cols,
+                                                              const VT* __restrict__ vals, int32_t r0, int32_t r1,
+                                                              const T* __restrict__ X, int ldx, int c0) {
+    using V = typename Vec16<T>::type;
+    V a0 = vzero<T>(), a1 = vzero<T>();
+    int32_t s = r0;
+    for (; s + 1 < r1; s += 2) {
+        const int32_t j0 = cols[s], j1 = cols[s + 1];
+        const T w0 = (T)vals[s], w1 = (T)vals[s + 1];
+        const V x0 = *reinterpret_cast<const V*>(X + (int64_t)j0 * ldx + c0);
+        const V x1 = *reinterpret_cast<const V*>(X + (int64_t)j1 * ldx + c0);
+        vfma<T>(a0, w0, x0);
+        vfma<T>(a1, w1, x1);
+    }
+    if (s < r1) {
+        const V x0 = *reinterpret_cast<const V*>(X + (int64_t)cols[s] * ldx + c0);
+        vfma<T>(a0, (T)vals[s], x0);
+    }
+    vadd<T>(a0, a1);
+    return a0;
+}
+// lanes per row for a vectorised block row of ncol values (power of two)
+inline int vec_lanes(int ncol, int vec) {
+    int need = (ncol + vec - 1) / vec, l = 1;
+    while (l < need) l <<= 1;
+    return l;
+}
 
 }  // namespace lh
 

@@ -44,42 +44,75 @@ __global__ void k_dinv_from(int64_t n, const int32_t* __restrict__ diagpos, cons
     dinv[i] = d != 0.0 ? 1.0 / d : 0.0;
 }
 
-// D0 = s0 * Dinv * W ; Z(W's slot) = D0 ; R = W
-template <typename T>
-__global__ void k_cheb_init(int64_t n, int m, T* __restrict__ W, int ldw, const double* __restrict__ dinv, double s0,
-                            T* __restrict__ R, T* __restrict__ D0) {
+// b = W (FP32 copy), x_1 = alpha D^-1 b, x_0 = 0
+__global__ void k_cheb_init(int64_t n, int m, const double* __restrict__ W, int ldw, const float* __restrict__ dinv,
+                            float alpha, float* __restrict__ bvec, float* __restrict__ x1, float* __restrict__ x0) {
     int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n * m) return;
     int64_t i = gid / m;
     int c = (int)(gid - i * m);
-    const double r = (double)W[i * ldw + c];
-    const double d = s0 * dinv[i] * r;
-    R[gid] = (T)r;
-    D0[gid] = (T)d;
-    W[i * ldw + c] = (T)d;
+    const float r = (float)W[i * ldw + c];
+    bvec[gid] = r;
+    x1[gid] = alpha * dinv[i] * r;
+    x0[gid] = 0.0f;
 }
 
-template <typename T, int LPR>
+// x_{k+1} = omega (x_k - x_{k-1} + alpha D^-1 (b - K x_k)) + x_{k-1}, written
+// over x_{k-1} (row-local), x_k gathered
+template <int LPR>
 __global__ void __launch_bounds__(256)
 k_cheb_step(int64_t n, int m, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
-            const double* __restrict__ kv, const double* __restrict__ dinv, const T* __restrict__ Dk,
-            T* __restrict__ Dk1, T* __restrict__ R, T* __restrict__ Z, int ldz, double c1, double c2) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+            const float* __restrict__ kv, const float* __restrict__ dinv, const float* __restrict__ bvec,
+            const float* __restrict__ xk, float* __restrict__ xprev, float alpha, float omega) {
+    const int64_t gid = xcd_block() * blockDim.x + threadIdx.x;
     const int64_t row = gid / LPR;
     const int lane = (int)(gid % LPR);
     if (row >= n) return;
     const int32_t r0 = rowptr[row], r1 = rowptr[row + 1];
-    const double di = dinv[row];
+    const float ad = alpha * dinv[row];
     for (int c = lane; c < m; c += LPR) {
-        double acc = 0.0;
-        for (int32_t s = r0; s < r1; ++s) acc += kv[s] * (double)Dk[(int64_t)cols[s] * m + c];
+        float acc = 0.0f;
+        for (int32_t s = r0; s < r1; ++s) acc += kv[s] * xk[(int64_t)cols[s] * m + c];
         const int64_t o = row * m + c;
-        const double r = (double)R[o] - acc;
-        const double dn = c1 * (double)Dk[o] + c2 * di * r;
-        R[o] = (T)r;
-        Dk1[o] = (T)dn;
-        Z[row * ldz + c] = (T)((double)Z[row * ldz + c] + dn);
+        const float xp = xprev[o];
+        xprev[o] = omega * (xk[o] - xp + ad * (bvec[o] - acc)) + xp;
     }
+}
+
+// vectorised form: 4 columns per lane, 16 lanes per row for m = 64
+template <int LPR>
+__global__ void __launch_bounds__(256)
+k_cheb_step_v(int64_t n, int m, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+              const float* __restrict__ kv, const float* __restrict__ dinv, const float* __restrict__ bvec,
+              const float* __restrict__ xk, float* __restrict__ xprev, float alpha, float omega) {
+    const int64_t gid = xcd_block() * blockDim.x + threadIdx.x;
+    const int64_t row = gid / LPR;
+    const int c0 = (int)(gid % LPR) * 4;
+    if (row >= n || c0 >= m) return;
+    const float4 acc = row_gather<float, float>(cols, kv, rowptr[row], rowptr[row + 1], xk, m, c0);
+    const float ad = alpha * dinv[row];
+    const int64_t o = row * m + c0;
+    const float4 xkv = *reinterpret_cast<const float4*>(xk + o);
+    const float4 bv = *reinterpret_cast<const float4*>(bvec + o);
+    float4 xp = *reinterpret_cast<const float4*>(xprev + o);
+    xp.x = omega * (xkv.x - xp.x + ad * (bv.x - acc.x)) + xp.x;
+    xp.y = omega * (xkv.y - xp.y + ad * (bv.y - acc.y)) + xp.y;
+    xp.z = omega * (xkv.z - xp.z + ad * (bv.z - acc.z)) + xp.z;
+    xp.w = omega * (xkv.w - xp.w + ad * (bv.w - acc.w)) + xp.w;
+    *reinterpret_cast<float4*>(xprev + o) = xp;
+}
+
+__global__ void k_cheb_out(int64_t n, int m, const float* __restrict__ x, double* __restrict__ W, int ldw) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n * m) return;
+    int64_t i = gid / m;
+    int c = (int)(gid - i * m);
+    W[i * ldw + c] = (double)x[gid];
+}
+
+__global__ void k_to_f32(int64_t cnt, const double* __restrict__ s, float* __restrict__ d) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) d[i] = (float)s[i];
 }
 
 __global__ void k_rayleigh_parts(int64_t n, const double* __restrict__ x, const double* __restrict__ kx,
@@ -194,41 +227,135 @@ void ChebPrecond::setup(lh_ctx* c, const lh_csr* A, const lh_csr* B, double sigm
     Kv.vals.p = nullptr;
     bmax = 1.1 * lmax;  // power iteration approaches lmax from below
     amin = bmax / (ratio > 1.0 ? ratio : 64.0);
-    R.alloc(n * mmax * 8);
-    D0.alloc(n * mmax * 8);
-    D1.alloc(n * mmax * 8);
+    // FP32 copies of K and D^-1 and the FP32 work blocks: the
+    // preconditioner only shapes the search directions, so its rounding does
+    // not limit the accuracy of the Ritz pairs
+    const int64_t nnz = pat->nnz;
+    kv32.alloc(nnz * 4);
+    dinv32.alloc(n * 4);
+    k_to_f32<<<nblocks(nnz, 256), 256, 0, ctx->stream>>>(nnz, kv.as<double>(), kv32.as<float>());
+    k_to_f32<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, dinv.as<double>(), dinv32.as<float>());
+    LH_CHECK_LAUNCH();
+    R.alloc(n * mmax * 4);
+    D0.alloc(n * mmax * 4);
+    D1.alloc(n * mmax * 4);
 }
 
 void ChebPrecond::apply(double* W, int ldw, int m) {
-    const double theta = 0.5 * (bmax + amin), delta = 0.5 * (bmax - amin);
-    const double sigma1 = theta / delta;
-    double rho = 1.0 / sigma1;
-    k_cheb_init<double><<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, W, ldw, dinv.as<double>(), 1.0 / theta,
-                                                                     R.as<double>(), D0.as<double>());
+    // Chebyshev semi-iteration (Golub-Varga two-step form) for K x = w,
+    // Jacobi-scaled, on [amin, bmax]
+    const double alpha = 2.0 / (bmax + amin);
+    const double mu = (bmax - amin) / (bmax + amin);
+    float* bvec = R.as<float>();
+    float* xk = D0.as<float>();
+    float* xp = D1.as<float>();
+    k_cheb_init<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, W, ldw, dinv32.as<float>(), (float)alpha, bvec,
+                                                              xk, xp);
     LH_CHECK_LAUNCH();
-    double* dk = D0.as<double>();
-    double* dk1 = D1.as<double>();
     const int lpr = m > 32 ? 64 : (m > 16 ? 32 : 16);
-    const unsigned nb = nblocks(n * lpr, 256);
+    const unsigned nb = nblocks_xcd(n * lpr, 256);
+    double omega = 1.0;
+    const bool vec = (m % 4 == 0) && vec_lanes(m, 4) >= 4 && vec_lanes(m, 4) <= 64;
+    const int vl = vec ? vec_lanes(m, 4) : 0;
+    const unsigned nbv = vec ? nblocks_xcd(n * vl, 256) : 0;
     for (int k = 1; k < degree; ++k) {
-        const double rho_new = 1.0 / (2.0 * sigma1 - rho);
-        const double c1 = rho_new * rho, c2 = 2.0 * rho_new / delta;
+        omega = (k == 1) ? 1.0 / (1.0 - 0.5 * mu * mu) : 1.0 / (1.0 - 0.25 * mu * mu * omega);
+        const int32_t* rp = pat->rowptr.as<int32_t>();
+        const int32_t* cl = pat->cols.as<int32_t>();
+        if (vec) {
+            const float* kk = kv32.as<float>();
+            const float* dd = dinv32.as<float>();
+            const float fa = (float)alpha, fo = (float)omega;
+            switch (vl) {
+                case 4: k_cheb_step_v<4><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, bvec, xk, xp, fa, fo); break;
+                case 8: k_cheb_step_v<8><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, bvec, xk, xp, fa, fo); break;
+                case 16: k_cheb_step_v<16><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, bvec, xk, xp, fa, fo); break;
+                case 32: k_cheb_step_v<32><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, bvec, xk, xp, fa, fo); break;
+                default: k_cheb_step_v<64><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, bvec, xk, xp, fa, fo); break;
+            }
+            LH_CHECK_LAUNCH();
+            std::swap(xk, xp);
+            continue;
+        }
         if (lpr == 64)
-            k_cheb_step<double, 64><<<nb, 256, 0, ctx->stream>>>(n, m, pat->rowptr.as<int32_t>(), pat->cols.as<int32_t>(),
-                                                                kv.as<double>(), dinv.as<double>(), dk, dk1,
-                                                                R.as<double>(), W, ldw, c1, c2);
+            k_cheb_step<64><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk,
+                                                        xp, (float)alpha, (float)omega);
         else if (lpr == 32)
-            k_cheb_step<double, 32><<<nb, 256, 0, ctx->stream>>>(n, m, pat->rowptr.as<int32_t>(), pat->cols.as<int32_t>(),
-                                                                kv.as<double>(), dinv.as<double>(), dk, dk1,
-                                                                R.as<double>(), W, ldw, c1, c2);
+            k_cheb_step<32><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk,
+                                                        xp, (float)alpha, (float)omega);
         else
-            k_cheb_step<double, 16><<<nb, 256, 0, ctx->stream>>>(n, m, pat->rowptr.as<int32_t>(), pat->cols.as<int32_t>(),
-                                                                kv.as<double>(), dinv.as<double>(), dk, dk1,
-                                                                R.as<double>(), W, ldw, c1, c2);
+            k_cheb_step<16><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk,
+                                                        xp, (float)alpha, (float)omega);
         LH_CHECK_LAUNCH();
-        std::swap(dk, dk1);
-        rho = rho_new;
+        std::swap(xk, xp);
     }
+    k_cheb_out<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, xk, W, ldw);
+    LH_CHECK_LAUNCH();
+}
+
+double ChebPrecond::time_step(int m, int iters) {
+    // random block, then `iters` back-to-back steps between HIP events
+    DevBuf w(n * m * 8);
+    randn<double>(ctx, n, m, w.as<double>(), m, 99);
+    const int saved = degree;
+    degree = 2;
+    apply(w.as<double>(), m, m);  // initialises b, x_1, x_0
+    degree = saved;
+    const double alpha = 2.0 / (bmax + amin), mu = (bmax - amin) / (bmax + amin);
+    const float omega = (float)(1.0 / (1.0 - 0.5 * mu * mu));
+    float* bvec = R.as<float>();
+    float* xk = D0.as<float>();
+    float* xp = D1.as<float>();
+    const int vl = vec_lanes(m, 4);
+    if (m % 4 != 0 || vl < 4 || vl > 64) LH_THROW(LH_ERR_ARG, "time_step: block width %d not vectorisable", m);
+    const unsigned nbv = nblocks_xcd(n * vl, 256);
+    hipEvent_t e0, e1;
+    HIPX(hipEventCreate(&e0));
+    HIPX(hipEventCreate(&e1));
+    HIPX(hipEventRecord(e0, ctx->stream));
+    for (int it = 0; it < iters; ++it) {
+        const int32_t* rp = pat->rowptr.as<int32_t>();
+        const int32_t* cl = pat->cols.as<int32_t>();
+        switch (vl) {
+            case 4: k_cheb_step_v<4><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk, xp, (float)alpha, omega); break;
+            case 8: k_cheb_step_v<8><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk, xp, (float)alpha, omega); break;
+            case 16: k_cheb_step_v<16><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk, xp, (float)alpha, omega); break;
+            case 32: k_cheb_step_v<32><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk, xp, (float)alpha, omega); break;
+            default: k_cheb_step_v<64><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk, xp, (float)alpha, omega); break;
+        }
+        std::swap(xk, xp);
+    }
+    LH_CHECK_LAUNCH();
+    HIPX(hipEventRecord(e1, ctx->stream));
+    HIPX(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPX(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 1000.0 * ms / iters;
 }
 
 }  // namespace lh
+
+using namespace lh;
+
+extern "C" int lh_time_cheb_step(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int iters, int reorder,
+                                 double* avg_us) {
+    if (!ctx || !A || !B || !avg_us || iters <= 0 || ncol <= 0) return LH_ERR_ARG;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    const lh_csr *a = A, *b = B;
+    Reorder ro;
+    std::unique_ptr<lh_csr> Ap, Bp;
+    if (reorder && (B->is_diag || B->pat == A->pat)) {
+        ro.build(ctx, *A->pat);
+        Ap = ro.apply(ctx, A);
+        Bp = ro.apply(ctx, B, B->is_diag ? nullptr : ro.pat);
+        a = Ap.get();
+        b = Bp.get();
+    }
+    ChebPrecond cp;
+    cp.setup(ctx, a, b, -0.01, 2, 64.0, a->n(), ncol);
+    *avg_us = cp.time_step(ncol, iters);
+    LH_API_END(ctx)
+}

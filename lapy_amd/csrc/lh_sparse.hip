@@ -15,7 +15,7 @@ template <typename VT, typename XT, int LPR>
 __global__ void __launch_bounds__(256)
 k_spmm(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
        const VT* __restrict__ vals, const XT* __restrict__ X, XT* __restrict__ Y, int ncol, int ldx, int ldy) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gid = xcd_block() * blockDim.x + threadIdx.x;
     const int64_t row = gid / LPR;
     const int lane = (int)(gid % LPR);
     if (row >= n) return;
@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(256)
 k_spmm2(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
         const VT* __restrict__ va, const VT* __restrict__ vb, const XT* __restrict__ X,
         XT* __restrict__ Y1, XT* __restrict__ Y2, int ncol, int ldx, int ldy) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t gid = xcd_block() * blockDim.x + threadIdx.x;
     const int64_t row = gid / LPR;
     const int lane = (int)(gid % LPR);
     if (row >= n) return;
@@ -49,6 +49,75 @@ k_spmm2(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict
         Y2[row * ldy + c] = a2;
     }
 }
+
+// Vectorised forms: each lane owns 16 bytes of the block row (4 floats or 2
+// doubles), LPR lanes per matrix row, 64 / LPR rows per wavefront, and two
+// gathers in flight per lane.  Used when the widths and offsets allow 16-byte
+// aligned accesses.
+template <typename VT, typename XT, int LPR>
+__global__ void __launch_bounds__(256)
+k_spmm_v(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+         const VT* __restrict__ vals, const XT* __restrict__ X, XT* __restrict__ Y, int ncol, int ldx, int ldy) {
+    using V = typename Vec16<XT>::type;
+    const int64_t gid = xcd_block() * blockDim.x + threadIdx.x;
+    const int64_t row = gid / LPR;
+    const int c0 = (int)(gid % LPR) * Vec16<XT>::n;
+    if (row >= n || c0 >= ncol) return;
+    const V acc = row_gather<VT, XT>(cols, vals, rowptr[row], rowptr[row + 1], X, ldx, c0);
+    *reinterpret_cast<V*>(Y + row * ldy + c0) = acc;
+}
+
+template <typename VT, typename XT, int LPR>
+__global__ void __launch_bounds__(256)
+k_spmm2_v(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+          const VT* __restrict__ va, const VT* __restrict__ vb, const XT* __restrict__ X, XT* __restrict__ Y1,
+          XT* __restrict__ Y2, int ncol, int ldx, int ldy) {
+    using V = typename Vec16<XT>::type;
+    const int64_t gid = xcd_block() * blockDim.x + threadIdx.x;
+    const int64_t row = gid / LPR;
+    const int c0 = (int)(gid % LPR) * Vec16<XT>::n;
+    if (row >= n || c0 >= ncol) return;
+    V a1 = vzero<XT>(), a2 = vzero<XT>(), b1 = vzero<XT>(), b2 = vzero<XT>();
+    const int32_t r1 = rowptr[row + 1];
+    int32_t s = rowptr[row];
+    for (; s + 1 < r1; s += 2) {
+        const int32_t j0 = cols[s], j1 = cols[s + 1];
+        const V x0 = *reinterpret_cast<const V*>(X + (int64_t)j0 * ldx + c0);
+        const V x1 = *reinterpret_cast<const V*>(X + (int64_t)j1 * ldx + c0);
+        vfma<XT>(a1, (XT)va[s], x0);
+        vfma<XT>(b1, (XT)vb[s], x0);
+        vfma<XT>(a2, (XT)va[s + 1], x1);
+        vfma<XT>(b2, (XT)vb[s + 1], x1);
+    }
+    if (s < r1) {
+        const V x0 = *reinterpret_cast<const V*>(X + (int64_t)cols[s] * ldx + c0);
+        vfma<XT>(a1, (XT)va[s], x0);
+        vfma<XT>(b1, (XT)vb[s], x0);
+    }
+    vadd<XT>(a1, a2);
+    vadd<XT>(b1, b2);
+    *reinterpret_cast<V*>(Y1 + row * ldy + c0) = a1;
+    *reinterpret_cast<V*>(Y2 + row * ldy + c0) = b1;
+}
+
+template <typename XT>
+static bool vec_ok(const XT* X, const XT* Y, int ncol, int ldx, int ldy) {
+    constexpr int v = Vec16<XT>::n;
+    return ncol % v == 0 && ldx % v == 0 && ldy % v == 0 && vec_lanes(ncol, v) >= 4 &&
+           (reinterpret_cast<uintptr_t>(X) % 16) == 0 && (reinterpret_cast<uintptr_t>(Y) % 16) == 0;
+}
+
+#define LH_VEC_DISPATCH(KERNEL, LANES, ...)                                                          \
+    do {                                                                                             \
+        const unsigned nbv = nblocks_xcd(n * (LANES), 256);                                          \
+        switch (LANES) {                                                                             \
+            case 4: KERNEL<VT, XT, 4><<<nbv, 256, 0, ctx->stream>>>(__VA_ARGS__); break;             \
+            case 8: KERNEL<VT, XT, 8><<<nbv, 256, 0, ctx->stream>>>(__VA_ARGS__); break;             \
+            case 16: KERNEL<VT, XT, 16><<<nbv, 256, 0, ctx->stream>>>(__VA_ARGS__); break;           \
+            case 32: KERNEL<VT, XT, 32><<<nbv, 256, 0, ctx->stream>>>(__VA_ARGS__); break;           \
+            default: KERNEL<VT, XT, 64><<<nbv, 256, 0, ctx->stream>>>(__VA_ARGS__); break;           \
+        }                                                                                            \
+    } while (0)
 
 // Y = diag(d) X
 template <typename VT, typename XT>
@@ -73,10 +142,16 @@ void spmm_ld(lh_ctx* ctx, const lh_csr* a, const XT* X, XT* Y, int ncol, int ldx
         return;
     }
     const int lpr = pick_lpr(ncol);
-    const unsigned nb = nblocks(n * lpr, 256);
+    const unsigned nb = nblocks_xcd(n * lpr, 256);
     const int32_t* rp = a->pat->rowptr.as<int32_t>();
     const int32_t* cl = a->pat->cols.as<int32_t>();
     const VT* v = a->vals.as<VT>();
+    if (vec_ok<XT>(X, Y, ncol, ldx, ldy) && vec_lanes(ncol, Vec16<XT>::n) <= 64) {
+        const int lanes = vec_lanes(ncol, Vec16<XT>::n);
+        LH_VEC_DISPATCH(k_spmm_v, lanes, n, rp, cl, v, X, Y, ncol, ldx, ldy);
+        LH_CHECK_LAUNCH();
+        return;
+    }
     if (lpr == 64) k_spmm<VT, XT, 64><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
     else if (lpr == 32) k_spmm<VT, XT, 32><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
     else k_spmm<VT, XT, 16><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
@@ -94,11 +169,18 @@ void spmm_pair(lh_ctx* ctx, const lh_csr* a, const lh_csr* b, const XT* X, XT* Y
     if (!b->is_diag && a->pat == b->pat) {
         const int64_t n = a->n();
         const int lpr = pick_lpr(ncol);
-        const unsigned nb = nblocks(n * lpr, 256);
+        const unsigned nb = nblocks_xcd(n * lpr, 256);
         const int32_t* rp = a->pat->rowptr.as<int32_t>();
         const int32_t* cl = a->pat->cols.as<int32_t>();
         const VT* va = a->vals.as<VT>();
         const VT* vb = b->vals.as<VT>();
+        if (vec_ok<XT>(X, YA, ncol, ldx, ldy) && vec_ok<XT>(X, YB, ncol, ldx, ldy) &&
+            vec_lanes(ncol, Vec16<XT>::n) <= 64) {
+            const int lanes = vec_lanes(ncol, Vec16<XT>::n);
+            LH_VEC_DISPATCH(k_spmm2_v, lanes, n, rp, cl, va, vb, X, YA, YB, ncol, ldx, ldy);
+            LH_CHECK_LAUNCH();
+            return;
+        }
         if (lpr == 64) k_spmm2<VT, XT, 64><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, va, vb, X, YA, YB, ncol, ldx, ldy);
         else if (lpr == 32) k_spmm2<VT, XT, 32><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, va, vb, X, YA, YB, ncol, ldx, ldy);
         else k_spmm2<VT, XT, 16><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, va, vb, X, YA, YB, ncol, ldx, ldy);

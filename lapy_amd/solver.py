@@ -139,7 +139,7 @@ class Solver:
 
     # -- eigensolve -----------------------------------------------------------
     def eigs(self, k: int = 10, sigma: float = -0.01, *, precision: str = "fp64", precond: str = "jacobi",
-             tol: float = 0.0, block: int = 0, max_iter: int = 0, seed: int = 0):
+             tol: float = 0.0, block: int = 0, max_iter: int = 0, seed: int = 0, poly_degree: int = 0):
         """k eigenpairs of ``A x = lambda B x`` nearest ``sigma`` (lapy/solver.py:667-716).
 
         Returns ``(eigenvalues (k,), eigenvectors (n, k))``, ascending, vectors
@@ -163,7 +163,7 @@ class Solver:
         kk = k
         while True:
             evals, evecs, st = _lib.eigs(self._ctx, a, b, kk, sigma, precision=prec, precond=pc, block=block,
-                                         max_iter=max_iter, tol=tol, seed=seed)
+                                         max_iter=max_iter, tol=tol, seed=seed, poly_degree=poly_degree)
             self.last_stats = st
             if sigma <= evals[0] or kk >= n - 1:
                 break
