@@ -247,30 +247,67 @@ __global__ void k_clamp_value(const T* __restrict__ sums, int64_t n, T* __restri
 // ---------------------------------------------------------------------------
 // numeric kernels (one thread per row)
 // ---------------------------------------------------------------------------
+// Each workgroup owns the rows [i0, i0 + blockDim.x); their values are the
+// contiguous range [rowptr[i0], rowptr[i0 + blockDim.x]) of A and B.  The
+// rows are accumulated in an LDS window over that range (zeroed, summed per
+// slot in ascending element order, written out coalesced once), so each value
+// leaves the CU as one full-line store instead of one read-modify-write per
+// incident element.  A workgroup whose range exceeds the window accumulates in
+// global memory directly (same summation order, same results).
+template <typename OT>
+struct RowAcc {
+    OT* a;       // A values, indexed by (slot - off)
+    OT* b;       // B values (consistent mass), indexed by (slot - off)
+    int32_t off;
+};
+
 template <typename OT>
 __device__ __forceinline__ void zero_row(OT* __restrict__ v, int32_t r0, int32_t r1) {
     for (int32_t s = r0; s < r1; ++s) v[s] = OT(0);
 }
 
-// triangles: KIND in {LH_TRIA, LH_TRIA_ANISO, LH_TRIA_MASS}
+// stage the workgroup's value range in LDS when it fits; returns false (and
+// zeroes nothing) otherwise.  Called by every thread of the workgroup.
+template <typename OT, bool HAS_A, bool HAS_B>
+__device__ __forceinline__ bool lds_window_begin(OT* sA, OT* sB, int32_t len, int cap) {
+    if (len > cap) return false;
+    for (int32_t s = threadIdx.x; s < len; s += blockDim.x) {
+        if (HAS_A) sA[s] = OT(0);
+        if (HAS_B) sB[s] = OT(0);
+    }
+    __syncthreads();
+    return true;
+}
+
+template <typename OT, bool HAS_A, bool HAS_B>
+__device__ __forceinline__ void lds_window_end(const OT* sA, const OT* sB, OT* __restrict__ Aval,
+                                               OT* __restrict__ Bval, int32_t base, int32_t len) {
+    __syncthreads();
+    for (int32_t s = threadIdx.x; s < len; s += blockDim.x) {
+        if (HAS_A) Aval[base + s] = sA[s];
+        if (HAS_B) Bval[base + s] = sB[s];
+    }
+}
+
+// triangles: KIND in {LH_TRIA, LH_TRIA_ANISO, LH_TRIA_MASS}; one matrix row
 template <typename VT, typename OT, int KIND, bool LUMP>
-__global__ void __launch_bounds__(256)
-k_tria_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
-            const int32_t* __restrict__ diagpos, const int32_t* __restrict__ lrowptr,
-            const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
-            const int32_t* __restrict__ t, const VT* __restrict__ x, const VT* __restrict__ y,
-            const VT* __restrict__ z, const double* __restrict__ u1, const double* __restrict__ u2,
-            const double* __restrict__ am, int clamp, const VT* __restrict__ vol_mean,
-            OT* __restrict__ Aval, OT* __restrict__ Bval, int32_t* __restrict__ degen) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ rowptr,
+                                         const int32_t* __restrict__ cols, const int32_t* __restrict__ diagpos,
+                                         const int32_t* __restrict__ lrowptr, const int32_t* __restrict__ inc_ptr,
+                                         const int32_t* __restrict__ inc, const int32_t* __restrict__ t,
+                                         const VT* __restrict__ x, const VT* __restrict__ y,
+                                         const VT* __restrict__ z, const double* __restrict__ u1,
+                                         const double* __restrict__ u2, const double* __restrict__ am, int clamp,
+                                         const VT* __restrict__ vol_mean, RowAcc<OT> acc, OT* __restrict__ Bval,
+                                         int32_t* __restrict__ degen) {
     const int32_t r0 = rowptr[i], r1 = rowptr[i + 1];
     const int32_t pa = inc_ptr[i], pb = inc_ptr[i + 1];
     if (pb == pa) return;  // unused vertex: empty row
     constexpr bool HAS_A = (KIND != LH_TRIA_MASS);
-    if (HAS_A) zero_row(Aval, r0, r1);
+    OT* const Aacc = acc.a;
+    OT* const Bacc = acc.b;
+    const int32_t off = acc.off;
     OT bl = OT(0);
-    if (!LUMP) zero_row(Bval, r0, r1);
     const int32_t dslot = diagpos[i];
     const VT vm = clamp ? vol_mean[0] : VT(0);
     for (int32_t p = pa; p < pb; ++p) {
@@ -326,9 +363,9 @@ k_tria_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __rest
                 else if (l == 1) { oa = (OT)a12; ob = (OT)a23; od = (OT)(-a12 - a23); }
                 else { oa = (OT)a23; ob = (OT)a31; od = (OT)(-a31 - a23); }
             }
-            Aval[sa] += oa;
-            Aval[sb] += ob;
-            Aval[dslot] += od;
+            Aacc[sa - off] += oa;
+            Aacc[sb - off] += ob;
+            Aacc[dslot - off] += od;
         }
         if constexpr (LUMP) {
             if constexpr (KIND == LH_TRIA_MASS) bl += (OT)(vol / VT(3));
@@ -337,30 +374,61 @@ k_tria_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __rest
             OT bij, bii;
             if constexpr (KIND == LH_TRIA_MASS) { bij = (OT)(vol / VT(12)); bii = (OT)(vol / VT(6)); }
             else { bij = (OT)(vol / VT(48)); bii = (OT)(vol / VT(24)); }
-            Bval[sa] += bij;
-            Bval[sb] += bij;
-            Bval[dslot] += bii;
+            Bacc[sa - off] += bij;
+            Bacc[sb - off] += bij;
+            Bacc[dslot - off] += bii;
         }
     }
     if constexpr (LUMP) Bval[lrowptr[i]] = bl;
 }
 
+template <typename VT, typename OT, int KIND, bool LUMP>
+__global__ void __launch_bounds__(256)
+k_tria_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+            const int32_t* __restrict__ diagpos, const int32_t* __restrict__ lrowptr,
+            const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
+            const int32_t* __restrict__ t, const VT* __restrict__ x, const VT* __restrict__ y,
+            const VT* __restrict__ z, const double* __restrict__ u1, const double* __restrict__ u2,
+            const double* __restrict__ am, int clamp, const VT* __restrict__ vol_mean,
+            OT* __restrict__ Aval, OT* __restrict__ Bval, int32_t* __restrict__ degen, int cap) {
+    constexpr bool HAS_A = (KIND != LH_TRIA_MASS);
+    constexpr bool HAS_B = !LUMP;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    OT* sA = reinterpret_cast<OT*>(lds_raw);
+    OT* sB = sA + cap;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t i = i0 + threadIdx.x;
+    const int64_t iend = (i0 + blockDim.x < n) ? i0 + blockDim.x : n;
+    const int32_t base = rowptr[i0], len = rowptr[iend] - base;
+    if (lds_window_begin<OT, HAS_A, HAS_B>(sA, sB, len, cap)) {
+        if (i < n)
+            tria_row<VT, OT, KIND, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, t, x, y, z, u1, u2, am,
+                                         clamp, vol_mean, RowAcc<OT>{sA, sB, base}, Bval, degen);
+        lds_window_end<OT, HAS_A, HAS_B>(sA, sB, Aval, Bval, base, len);
+    } else if (i < n) {
+        const int32_t r0 = rowptr[i], r1 = rowptr[i + 1];
+        if (HAS_A) zero_row(Aval, r0, r1);
+        if (HAS_B) zero_row(Bval, r0, r1);
+        tria_row<VT, OT, KIND, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, t, x, y, z, u1, u2, am,
+                                     clamp, vol_mean, RowAcc<OT>{Aval, Bval, 0}, Bval, degen);
+    }
+}
+
 // tetrahedra: lapy/solver.py:414-532
 template <typename VT, typename OT, bool LUMP>
-__global__ void __launch_bounds__(256)
-k_tet_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
-           const int32_t* __restrict__ diagpos, const int32_t* __restrict__ lrowptr,
-           const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
-           const int32_t* __restrict__ t, const VT* __restrict__ x, const VT* __restrict__ y,
-           const VT* __restrict__ z, int clamp, const VT* __restrict__ vol_mean,
-           OT* __restrict__ Aval, OT* __restrict__ Bval, int32_t* __restrict__ degen) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void tet_row(int64_t i, const int32_t* __restrict__ rowptr,
+                                        const int32_t* __restrict__ cols, const int32_t* __restrict__ diagpos,
+                                        const int32_t* __restrict__ lrowptr, const int32_t* __restrict__ inc_ptr,
+                                        const int32_t* __restrict__ inc, const int32_t* __restrict__ t,
+                                        const VT* __restrict__ x, const VT* __restrict__ y,
+                                        const VT* __restrict__ z, int clamp, const VT* __restrict__ vol_mean,
+                                        RowAcc<OT> acc, OT* __restrict__ Bval, int32_t* __restrict__ degen) {
     const int32_t r0 = rowptr[i], r1 = rowptr[i + 1];
     const int32_t pa = inc_ptr[i], pb = inc_ptr[i + 1];
     if (pb == pa) return;
-    zero_row(Aval, r0, r1);
-    if (!LUMP) zero_row(Bval, r0, r1);
+    OT* const Aacc = acc.a;
+    OT* const Bacc = acc.b;
+    const int32_t off = acc.off;
     OT bl = OT(0);
     const int32_t dslot = diagpos[i];
     const VT vm = clamp ? vol_mean[0] : VT(0);
@@ -414,21 +482,51 @@ k_tet_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restr
         const int32_t s0 = find_slot(cols, r0, r1, c0);
         const int32_t s1 = find_slot(cols, r0, r1, c1);
         const int32_t s2 = find_slot(cols, r0, r1, c2);
-        Aval[s0] += (OT)(o0 / VT(6.0));
-        Aval[s1] += (OT)(o1 / VT(6.0));
-        Aval[s2] += (OT)(o2 / VT(6.0));
-        Aval[dslot] += (OT)(dg / VT(6.0));
+        Aacc[s0 - off] += (OT)(o0 / VT(6.0));
+        Aacc[s1 - off] += (OT)(o1 / VT(6.0));
+        Aacc[s2 - off] += (OT)(o2 / VT(6.0));
+        Aacc[dslot - off] += (OT)(dg / VT(6.0));
         if constexpr (LUMP) {
             bl += (OT)(vol / VT(24.0));
         } else {
             OT bij = (OT)(vol / VT(120.0)), bii = (OT)(vol / VT(60.0));
-            Bval[s0] += bij;
-            Bval[s1] += bij;
-            Bval[s2] += bij;
-            Bval[dslot] += bii;
+            Bacc[s0 - off] += bij;
+            Bacc[s1 - off] += bij;
+            Bacc[s2 - off] += bij;
+            Bacc[dslot - off] += bii;
         }
     }
     if constexpr (LUMP) Bval[lrowptr[i]] = bl;
+}
+
+template <typename VT, typename OT, bool LUMP>
+__global__ void __launch_bounds__(256)
+k_tet_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+           const int32_t* __restrict__ diagpos, const int32_t* __restrict__ lrowptr,
+           const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
+           const int32_t* __restrict__ t, const VT* __restrict__ x, const VT* __restrict__ y,
+           const VT* __restrict__ z, int clamp, const VT* __restrict__ vol_mean,
+           OT* __restrict__ Aval, OT* __restrict__ Bval, int32_t* __restrict__ degen, int cap) {
+    constexpr bool HAS_B = !LUMP;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    OT* sA = reinterpret_cast<OT*>(lds_raw);
+    OT* sB = sA + cap;
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t i = i0 + threadIdx.x;
+    const int64_t iend = (i0 + blockDim.x < n) ? i0 + blockDim.x : n;
+    const int32_t base = rowptr[i0], len = rowptr[iend] - base;
+    if (lds_window_begin<OT, true, HAS_B>(sA, sB, len, cap)) {
+        if (i < n)
+            tet_row<VT, OT, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, t, x, y, z, clamp, vol_mean,
+                                  RowAcc<OT>{sA, sB, base}, Bval, degen);
+        lds_window_end<OT, true, HAS_B>(sA, sB, Aval, Bval, base, len);
+    } else if (i < n) {
+        const int32_t r0 = rowptr[i], r1 = rowptr[i + 1];
+        zero_row(Aval, r0, r1);
+        if (HAS_B) zero_row(Bval, r0, r1);
+        tet_row<VT, OT, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, t, x, y, z, clamp, vol_mean,
+                              RowAcc<OT>{Aval, Bval, 0}, Bval, degen);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -596,20 +694,25 @@ void launch_numeric(lh_ctx* ctx, lh_mesh* m, int kind, int lump, const double* u
     const int32_t* lrp = m->lumped->rowptr.as<int32_t>();
     const int bs = 128;
     const unsigned nb = nblocks(n, bs);
+    // LDS window per matrix: 1.5x the average row length times the rows of a
+    // workgroup, at most 2048 entries (32 KB for A and B in FP64)
+    const int64_t avg_row = (p.nnz + n - 1) / n;
+    const int cap = (int)std::min<int64_t>(2048, (avg_row * bs * 3 / 2 + 63) / 64 * 64);
+    const size_t shm = 2 * (size_t)cap * sizeof(OT);
     auto* x = m->x.as<VT>();
     auto* y = m->y.as<VT>();
     auto* z = m->z.as<VT>();
     const int32_t *rp = p.rowptr.as<int32_t>(), *cl = p.cols.as<int32_t>(), *dp = p.diagpos.as<int32_t>();
     const int32_t *ip = m->inc_ptr.as<int32_t>(), *ic = m->inc.as<int32_t>(), *t = m->t.as<int32_t>();
 #define TRIA_LAUNCH(K, L)                                                                          \
-    k_tria_rows<VT, OT, K, L><<<nb, bs, 0, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, t, x, y, z, u1, u2, am, \
-                                                         clamp, vm, Aval, Bval, degen)
+    k_tria_rows<VT, OT, K, L><<<nb, bs, shm, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, t, x, y, z, u1, u2, am, \
+                                                           clamp, vm, Aval, Bval, degen, cap)
     if (kind == LH_TRIA) { if (lump) TRIA_LAUNCH(LH_TRIA, true); else TRIA_LAUNCH(LH_TRIA, false); }
     else if (kind == LH_TRIA_ANISO) { if (lump) TRIA_LAUNCH(LH_TRIA_ANISO, true); else TRIA_LAUNCH(LH_TRIA_ANISO, false); }
     else if (kind == LH_TRIA_MASS) { if (lump) TRIA_LAUNCH(LH_TRIA_MASS, true); else TRIA_LAUNCH(LH_TRIA_MASS, false); }
     else {
-        if (lump) k_tet_rows<VT, OT, true><<<nb, bs, 0, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, t, x, y, z, clamp, vm, Aval, Bval, degen);
-        else k_tet_rows<VT, OT, false><<<nb, bs, 0, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, t, x, y, z, clamp, vm, Aval, Bval, degen);
+        if (lump) k_tet_rows<VT, OT, true><<<nb, bs, shm, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, t, x, y, z, clamp, vm, Aval, Bval, degen, cap);
+        else k_tet_rows<VT, OT, false><<<nb, bs, shm, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, t, x, y, z, clamp, vm, Aval, Bval, degen, cap);
     }
 #undef TRIA_LAUNCH
     LH_CHECK_LAUNCH();

@@ -151,6 +151,29 @@ def test_degenerate_elements_clamp(kind):
         np.testing.assert_allclose(mat.data, ref.data, rtol=1e-12, atol=1e-14 * abs(ref.data).max())
 
 
+@pytest.mark.parametrize("kind,lump", [("tria", False), ("tria", True), ("mass", False), ("tet", False),
+                                       ("tet", True)])
+def test_high_valence_hub_rows(kind, lump):
+    """A hub vertex shared by 3000 elements: its workgroup's value range is
+    larger than the kernel's LDS window, so those rows take the direct global
+    accumulation path while the other workgroups of the launch stage in LDS.
+    Both must give the element-order sums bit for bit."""
+    rng = np.random.default_rng(7)
+    nr = 3000
+    ang = np.sort(rng.uniform(0, 2 * np.pi, nr))
+    rim = np.stack([np.cos(ang), np.sin(ang), 0.05 * rng.standard_normal(nr)], axis=1)
+    if kind == "tet":
+        v = np.vstack([[0.0, 0.0, 0.0], [0.0, 0.0, 1.0], rim])
+        k = np.arange(nr)
+        t = np.stack([np.zeros(nr, np.int64), np.ones(nr, np.int64), 2 + k, 2 + (k + 1) % nr], axis=1)
+    else:
+        v = np.vstack([[0.0, 0.0, 0.3], rim])
+        k = np.arange(nr)
+        t = np.stack([np.zeros(nr, np.int64), 1 + k, 1 + (k + 1) % nr], axis=1)
+    A, B = check_against_oracle(v, t, kind, lump)
+    assert (B if A is None else A).indptr[1] - (B if A is None else A).indptr[0] == nr + 1 + (kind == "tet")
+
+
 def test_unused_vertices_shape_and_empty_rows():
     v, t = meshgen.icosphere(2)
     # an unused vertex in the middle (empty row) and two trailing unused ones
