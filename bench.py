@@ -72,9 +72,12 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--poly-degree", type=int, default=0)
     ap.add_argument("--poly-ratio", type=float, default=0.0)
+    ap.add_argument("--poly-storage", default="auto", choices=["auto", "f32", "f16"])
     ap.add_argument("--cpu-level", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run one device-to-device copy of 3 x 22,369,056 bytes (for PMC scale factors)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -83,6 +86,16 @@ def main():
     from lapy_amd import _lib, meshgen
 
     ctx = _lib.default_context()
+    if args.copy_calibration:
+        # a device mesh clone is three plain device-to-device copies of the
+        # vertex planes: 3 x 2,796,132 doubles = 3 x 22,369,056 bytes
+        nvc = 2796132
+        cm = _lib.DeviceMesh(ctx, np.random.default_rng(0).random((nvc, 3)), np.zeros((1, 3), np.int64))
+        cm.clone()
+        ctx.sync()
+        print(json.dumps({"copy_calibration_bytes_per_copy": nvc * 8, "copies": 3}), flush=True)
+        return
+    storage = {"auto": _lib.LH_STORE_AUTO, "f32": _lib.LH_STORE_F32, "f16": _lib.LH_STORE_F16}[args.poly_storage]
     v, t = meshgen.large_ellipsoid(level=args.level)
     nv, nf = len(v), len(t)
     log(f"[bench] mesh level {args.level}: V={nv} F={nf}")
@@ -94,7 +107,7 @@ def main():
         A, B = m.assemble(_lib.LH_TRIA, False, np.float64)
         evals, eh, st = _lib.eigs(ctx, A, B, args.k, -0.01, precision=prec, tol=args.tol, block=args.block,
                                   verbose=args.verbose, keep_device=True, poly_degree=args.poly_degree,
-                                  poly_ratio=args.poly_ratio)
+                                  poly_ratio=args.poly_ratio, poly_storage=storage)
         return evals, eh, st, (m, A, B)
 
     for w in range(args.warmup):
@@ -137,16 +150,20 @@ def main():
     m = base.clone()
     A, B = m.assemble(_lib.LH_TRIA, False, np.float64)
     nnz = A.nnz
-    # dominant kernel: one Chebyshev step of the preconditioner (FP32, block of m = 64 columns)
+    # dominant kernel: one Chebyshev step of the preconditioner (FP32 arithmetic, block of m = 64 columns)
     mcols = args.block if args.block else 64
-    us_c = _lib.time_cheb_step(ctx, A, B, mcols, iters=50, reorder=True)
-    # x_k (gathered, counted once), b, x_{k-1} read + x_{k+1} write: 16 n m; K values + cols 8 nnz;
-    # rowptr + D^-1: 8 n
-    cheb_bytes = 16 * nv * mcols + 8 * nnz + 8 * nv
+    half = storage == _lib.LH_STORE_F16 or (storage == _lib.LH_STORE_AUTO and mcols % 8 == 0)
+    us_c = _lib.time_cheb_step(ctx, A, B, mcols, iters=50, reorder=True,
+                               storage=_lib.LH_STORE_F16 if half else _lib.LH_STORE_F32)
+    # x_k (gathered, counted once), b, x_{k-1} read + x_{k+1} write: 4 values per entry of the block
+    # (2 bytes each in FP16 storage, 4 in FP32); K values (FP32) + cols (int32): 8 nnz; rowptr + D^-1: 8 n
+    es = 2 if half else 4
+    cheb_bytes = 4 * es * nv * mcols + 8 * nnz + 8 * nv
     ach_c = cheb_bytes / (us_c * 1e-6) / 1e9
+    kname = "k_cheb_step_h (FP16 storage)" if half else "k_cheb_step_v (FP32 storage)"
     roofline = {"bound": "hbm", "achieved": round(ach_c, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach_c / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_cheb_step_v (Chebyshev-Jacobi preconditioner step, FP32, m=%d)" % mcols,
+                "kernel": f"{kname}, Chebyshev-Jacobi preconditioner step, m={mcols}",
                 "avg_us": round(us_c, 2), "algorithmic_bytes": int(cheb_bytes)}
     # the metric's second figure: numeric assembly
     us = m.time_assembly(_lib.LH_TRIA, False, A, B, iters=20)

@@ -68,6 +68,11 @@ typedef struct lh_stats {
 #define LH_PRECOND_LEARNED 1
 #define LH_PRECOND_NONE    2
 
+/* storage of the Chebyshev preconditioner's work blocks (arithmetic is FP32) */
+#define LH_STORE_AUTO 0
+#define LH_STORE_F32  1
+#define LH_STORE_F16  2   /* columns scaled to a fixed RMS, saturating stores */
+
 /* ---- context ------------------------------------------------------------- */
 /* New; the reference runs on the host and has no device context. */
 int  lh_ctx_create(int device, lh_ctx** out);
@@ -142,10 +147,12 @@ typedef struct lh_eig_opts {
     int32_t verbose;
     int32_t poly_degree;   /* with LH_PRECOND_JACOBI: degree of the Chebyshev
                               polynomial in D^-1 (A - sigma B) applied on top
-                              of Jacobi; 0: automatic (~sqrt(n)/12, at most
-                              96), 1: plain Jacobi                           */
+                              of Jacobi; 0: automatic (~sqrt(n)/8.5, at most
+                              128), 1: plain Jacobi                          */
     double  poly_ratio;    /* Chebyshev interval [lmax/ratio, lmax]
                               (0: automatic, 4 * degree^2)                   */
+    int32_t poly_storage;  /* LH_STORE_* for the Chebyshev work blocks        */
+    int32_t reserved;      /* must be 0                                       */
 } lh_eig_opts;
 
 /* Replaces Solver.eigs / eigensystem (lapy/solver.py:667-716: splu of
@@ -189,10 +196,24 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta,
 int lh_time_assembly(lh_ctx* ctx, lh_mesh* m, int kind, int lump, lh_csr* A, lh_csr* B, int iters,
                      double* avg_us);
 int lh_time_spmm(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int precision, int iters, double* avg_us);
-/* One step of the eigensolver's Chebyshev-Jacobi preconditioner (FP32, fused
- * SpMM + update) on an n x ncol block, optionally after the solver's
- * locality reordering (reorder = 1, as lh_eigs does). */
-int lh_time_cheb_step(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int iters, int reorder, double* avg_us);
+/* One step of the eigensolver's Chebyshev-Jacobi preconditioner (FP32
+ * arithmetic, fused SpMM + update, work blocks stored per `storage` =
+ * LH_STORE_F32 or LH_STORE_F16) on an n x ncol block, optionally after the
+ * solver's locality reordering (reorder = 1, as lh_eigs does). */
+int lh_time_cheb_step(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int iters, int reorder, int storage,
+                      double* avg_us);
+/* The eigensolver's tall-skinny block products on their own (no counterpart
+ * in the reference, whose eigsh does its dense work inside ARPACK; exposed
+ * for their tests and timing), on host row-major buffers:
+ * lh_block_gram: C (p x q, row-major) = U^T V over n rows (sym != 0: the
+ *   product is known to be symmetric, p == q; only upper tiles are formed);
+ * lh_block_gemm: Y = alpha U C + beta Y with C p x q row-major.
+ * iters > 0 and avg_us non-NULL: also the average kernel time of `iters`
+ * further runs (lh_block_gemm times them with beta = 0, after Y is returned). */
+int lh_block_gram(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q, int sym,
+                  double* C, int iters, double* avg_us);
+int lh_block_gemm(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* Cs, double* Y,
+                  int ldy, double alpha, double beta, int iters, double* avg_us);
 /* The Rayleigh-Ritz eigensolver on its own (no counterpart in the reference;
  * exposed for its tests and timing): smallest nev eigenpairs of the symmetric
  * N x N host matrix H (N <= 192).  evals (nev), Z (N x nev row-major); when

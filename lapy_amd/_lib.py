@@ -22,6 +22,7 @@ LH_F64, LH_F32 = 0, 1
 LH_TRIA, LH_TRIA_ANISO, LH_TRIA_MASS, LH_TET = 0, 1, 2, 3
 LH_PREC_FP64, LH_PREC_MIXED = 0, 1
 LH_PRECOND_JACOBI, LH_PRECOND_LEARNED, LH_PRECOND_NONE = 0, 1, 2
+LH_STORE_AUTO, LH_STORE_F32, LH_STORE_F16 = 0, 1, 2
 
 _vp = C.c_void_p
 _i32 = C.c_int32
@@ -61,6 +62,8 @@ class EigOpts(C.Structure):
         ("verbose", C.c_int32),
         ("poly_degree", C.c_int32),
         ("poly_ratio", C.c_double),
+        ("poly_storage", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -102,7 +105,11 @@ SIGNATURES = {
     "lh_time_assembly": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, _dp]),
     "lh_time_spmm": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _dp]),
     "lh_small_symeig": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
-    "lh_time_cheb_step": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _dp]),
+    "lh_time_cheb_step": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
+    "lh_block_gram": (C.c_int, [_vp, C.c_int64, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
+                                _dp]),
+    "lh_block_gemm": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.c_double,
+                                C.c_double, C.c_int, _dp]),
 }
 
 _lib = None
@@ -337,9 +344,9 @@ class DeviceMesh:
 
 def eigs(ctx: Context, A: DeviceCSR, B: DeviceCSR, k: int, sigma: float = -0.01, precision=LH_PREC_FP64,
          precond=LH_PRECOND_JACOBI, block=0, max_iter=0, tol=0.0, seed=0, verbose=0, keep_device=False,
-         poly_degree=0, poly_ratio=0.0):
+         poly_degree=0, poly_ratio=0.0, poly_storage=LH_STORE_AUTO):
     """Generalized eigenpairs on the GPU; returns (evals, evecs, stats)."""
-    opts = EigOpts(precision, precond, block, max_iter, tol, seed, verbose, poly_degree, poly_ratio)
+    opts = EigOpts(precision, precond, block, max_iter, tol, seed, verbose, poly_degree, poly_ratio, poly_storage, 0)
     st = Stats()
     h = _vp()
     code = lib().lh_eigs(ctx.handle, A.handle, B.handle, int(k), float(sigma), C.byref(opts), C.byref(h), C.byref(st))
@@ -357,10 +364,42 @@ def eigs(ctx: Context, A: DeviceCSR, B: DeviceCSR, k: int, sigma: float = -0.01,
             lib().lh_eig_free(h)
 
 
-def time_cheb_step(ctx: Context, A: DeviceCSR, B: DeviceCSR, ncol: int, iters: int = 50, reorder: bool = True):
+def time_cheb_step(ctx: Context, A: DeviceCSR, B: DeviceCSR, ncol: int, iters: int = 50, reorder: bool = True,
+                   storage=LH_STORE_F16):
     us = C.c_double()
-    check(lib().lh_time_cheb_step(ctx.handle, A.handle, B.handle, ncol, iters, int(reorder), C.byref(us)), ctx)
+    check(lib().lh_time_cheb_step(ctx.handle, A.handle, B.handle, ncol, iters, int(reorder), int(storage),
+                                  C.byref(us)), ctx)
     return us.value
+
+
+def block_gram(ctx: Context, U: np.ndarray, V: np.ndarray, p: int, q: int, sym: bool = False, iters: int = 0):
+    """C = U[:, :p]^T V[:, :q] on the GPU (FP64 matrix cores); U, V row-major
+    float64 (n x ldu), (n x ldv).  Returns (C, average microseconds or None)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    if U.shape[0] != V.shape[0]:
+        raise ValueError("U and V must have the same number of rows")
+    out = np.empty((p, q), dtype=np.float64)
+    us = C.c_double(0.0)
+    check(lib().lh_block_gram(ctx.handle, U.shape[0], _ptr(U), U.shape[1], p, _ptr(V), V.shape[1], q, int(sym),
+                              _ptr(out), iters, C.byref(us) if iters else None), ctx)
+    return out, (us.value if iters else None)
+
+
+def block_gemm(ctx: Context, U: np.ndarray, p: int, Cm: np.ndarray, Y: np.ndarray, alpha=1.0, beta=0.0,
+               iters: int = 0):
+    """Y[:, :q] = alpha U[:, :p] Cm + beta Y[:, :q] on the GPU (FP64 matrix
+    cores).  Returns (Y, average microseconds or None)."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    Cm = np.ascontiguousarray(Cm, dtype=np.float64)
+    Y = np.array(Y, dtype=np.float64, order="C", copy=True)
+    q = Cm.shape[1]
+    if Cm.shape[0] != p or U.shape[0] != Y.shape[0]:
+        raise ValueError("shape mismatch")
+    us = C.c_double(0.0)
+    check(lib().lh_block_gemm(ctx.handle, U.shape[0], p, q, _ptr(U), U.shape[1], _ptr(Cm), _ptr(Y), Y.shape[1],
+                              float(alpha), float(beta), iters, C.byref(us) if iters else None), ctx)
+    return Y, (us.value if iters else None)
 
 
 def time_spmm(ctx: Context, A: DeviceCSR, B: DeviceCSR, ncol: int, precision=LH_PREC_FP64, iters: int = 20):

@@ -34,6 +34,7 @@ SOURCES = {
     "lh_assembly.hip": ["-ffp-contract=off"],
     "lh_sparse.hip": [],
     "lh_dense.hip": [],
+    "lh_mfma.hip": [],
     "lh_rr.hip": [],
     "lh_poly.hip": [],
     "lh_reorder.hip": [],

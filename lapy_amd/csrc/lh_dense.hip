@@ -16,6 +16,7 @@
 #include "lh_dense.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace lh {
 
@@ -105,7 +106,12 @@ k_gram_reduce(int nchunk, int p, int q, int ntz, const AT* __restrict__ part, CT
 
 template <typename T>
 void gram(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, int ldv, int q, double* C, int ldc,
-          DevBuf& ws, double alpha) {
+          DevBuf& ws, double alpha, bool sym) {
+    if constexpr (std::is_same<T, double>::value) {
+        gram_mfma(ctx, n, U, ldu, p, V, ldv, q, C, ldc, ws, alpha, sym);
+        return;
+    }
+    (void)sym;
     const int tu = (p + GT - 1) / GT, tv = (q + GT - 1) / GT;
     // about two workgroups per CU in the partial pass, at least 512 rows each
     int64_t nchunk = std::min<int64_t>(std::max<int64_t>(1, (int64_t)ctx->n_cu * 2 / (tu * tv)), (n + 511) / 512);
@@ -123,9 +129,9 @@ void gram(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, int ld
     LH_CHECK_LAUNCH();
 }
 template void gram<double>(lh_ctx*, int64_t, const double*, int, int, const double*, int, int, double*, int, DevBuf&,
-                           double);
+                           double, bool);
 template void gram<float>(lh_ctx*, int64_t, const float*, int, int, const float*, int, int, double*, int, DevBuf&,
-                          double);
+                          double, bool);
 
 // ---------------------------------------------------------------------------
 // Y = alpha U C + beta Y     U: n x p (ldu), C: p x q (ldc, FP64), Y: n x q (ldy)
@@ -188,6 +194,10 @@ k_gemm_nn(int64_t n, int p, int q, const T* __restrict__ U, int ldu, const doubl
 template <typename T>
 void gemm_nn(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const double* C, int ldc, T* Y, int ldy,
              double alpha, double beta) {
+    if constexpr (std::is_same<T, double>::value) {
+        gemm_mfma(ctx, n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta);
+        return;
+    }
     if (n == 0 || q == 0) return;
     dim3 grid(nblocks(n, MT), (q + NT - 1) / NT);
     k_gemm_nn<T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta);

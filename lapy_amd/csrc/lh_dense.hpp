@@ -4,14 +4,21 @@
 
 namespace lh {
 
-// C (p x q, ldc, FP64) = alpha * U^T V over n rows
+// C (p x q, ldc, FP64) = alpha * U^T V over n rows; sym (p == q): the
+// product is known to be symmetric, only its upper 64 x 64 tiles are formed
+// and the lower ones mirror them
 template <typename T>
 void gram(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, int ldv, int q, double* C, int ldc,
-          DevBuf& ws, double alpha = 1.0);
+          DevBuf& ws, double alpha = 1.0, bool sym = false);
 // Y = alpha * U C + beta * Y
 template <typename T>
 void gemm_nn(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const double* C, int ldc, T* Y, int ldy,
              double alpha, double beta);
+// FP64 matrix-core forms of the two (lh_mfma.hip)
+void gram_mfma(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q, double* C,
+               int ldc, DevBuf& ws, double alpha, bool sym);
+void gemm_mfma(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* C, int ldc, double* Y,
+               int ldy, double alpha, double beta);
 size_t rr_work_doubles(int N, int nev);
 // standard symmetric Rayleigh-Ritz (orthonormal basis), LDS-resident, N <= kRRStdMaxN
 constexpr int kRRStdMaxN = 192;
@@ -45,14 +52,21 @@ struct ChebPrecond {
     std::shared_ptr<Pattern> pat;
     int64_t n = 0;
     int degree = 1;
+    int mmax = 0;
+    bool half = false;  // FP16 storage of the work blocks (FP32 arithmetic)
     double bmax = 2.0, amin = 0.03;
-    DevBuf kv, dinv, kv32, dinv32, R, D0, D1;
+    DevBuf kv, dinv, kv32, dinv32, R, D0, D1, sc;
     void setup(lh_ctx* c, const lh_csr* A, const lh_csr* B, double sigma, int deg, double ratio, int64_t n_,
-               int mmax);
-    // W (n x m, leading dimension ldw) <- p(D^-1 K) D^-1 W
-    void apply(double* W, int ldw, int m);
+               int mmax_, bool half_storage);
+    // W (n x m, leading dimension ldw) <- p(D^-1 K) D^-1 W; colnorm2 (device,
+    // m doubles) holds the squared column norms of W (needed for FP16 storage)
+    void apply(double* W, int ldw, int m, const double* colnorm2);
     // average duration (us) of one Chebyshev step on an n x m block
     double time_step(int m, int iters);
+
+   private:
+    void init(const double* W, int ldw, int m, const double* colnorm2, void** xk, void** xp);
+    void step(int m, const void* bvec, const void* xk, void* xp, double omega);
 };
 
 // breadth-first level reordering of a symmetric pattern (lh_reorder.hip)

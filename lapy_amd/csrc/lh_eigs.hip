@@ -42,6 +42,7 @@ struct Lobpcg {
     int max_iter, verbose;
     int poly_degree = 0;
     double poly_ratio = 0.0;
+    bool poly_half = false;
     ChebPrecond cheb;
     // locality reordering (solver-internal)
     Reorder ro;
@@ -80,7 +81,7 @@ struct Lobpcg {
         double* X = Sp() + off;
         double* MX = MSp() + off;
         for (int pass = 0; pass < 2; ++pass) {
-            gram<double>(ctx, n, X, ld, c, MX, ld, c, Gs.as<double>(), c, wsg);
+            gram<double>(ctx, n, X, ld, c, MX, ld, c, Gs.as<double>(), c, wsg, 1.0, true);
             cholinv(ctx, c, Gs.as<double>(), Ts.as<double>(), 0.0, status.as<int>());
             double stv[2];
             d2h(ctx, stv, status.p, sizeof(stv));
@@ -118,12 +119,12 @@ struct Lobpcg {
     // S^T B S) is kept for trial bases wider than the LDS solver takes.
     bool rayleigh_ritz(int N) {
         double t0 = now_us();
-        gram<double>(ctx, n, Sp(), ld, N, ASp(), ld, N, GA.as<double>(), N, wsg);
+        gram<double>(ctx, n, Sp(), ld, N, ASp(), ld, N, GA.as<double>(), N, wsg, 1.0, true);
         if (N <= kRRStdMaxN) {
             rr_std(ctx, N, m, GA.as<double>(), N, Cm.as<double>(), theta_d.as<double>(), rrwork.as<double>(),
                    status.as<int>());
         } else {
-            gram<double>(ctx, n, Sp(), ld, N, MSp(), ld, N, GM.as<double>(), N, wsg);
+            gram<double>(ctx, n, Sp(), ld, N, MSp(), ld, N, GM.as<double>(), N, wsg, 1.0, true);
             rr_solve(ctx, N, m, GA.as<double>(), GM.as<double>(), Cm.as<double>(), theta_d.as<double>(),
                      rrwork.as<double>(), status.as<int>());
         }
@@ -167,7 +168,7 @@ struct Lobpcg {
         res.assign(m, 0.0);
         HIPX(hipMemsetAsync(S.p, 0, n * ld * 8, ctx->stream));
         jacobi_diag(ctx, A, B, -sigma, 1.0, dinv.as<double>());
-        if (poly_degree > 1) cheb.setup(ctx, A, B, sigma, poly_degree, poly_ratio, n, m);
+        if (poly_degree > 1) cheb.setup(ctx, A, B, sigma, poly_degree, poly_ratio, n, m, poly_half);
 
         // start block
         randn<double>(ctx, n, m, Sp(), ld, seed);
@@ -207,7 +208,7 @@ struct Lobpcg {
             if (nconv == k) break;
             // precondition W
             if (poly_degree > 1) {
-                cheb.apply(Sp() + m, ld, m);
+                cheb.apply(Sp() + m, ld, m, cs1.as<double>());  // cs1: ||w_c||^2 from the test above
                 st.spmm_calls += poly_degree - 1;
             } else {
                 rowscale<double>(ctx, n, m, Sp() + m, ld, dinv.as<double>());
@@ -328,20 +329,31 @@ int lh_eigs(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const lh_eig
     if (m > 128) m = std::max(k, 128);
     if (3 * m > n) m = std::max(k, (int)(n / 3));
     L.m = m;
+    L.poly_half = (opts && opts->poly_storage == LH_STORE_F16) ||
+                  ((!opts || opts->poly_storage == LH_STORE_AUTO) && m % 8 == 0);
+    if (L.poly_half && m % 8 != 0) LH_THROW(LH_ERR_ARG, "FP16 Chebyshev storage needs a block size divisible by 8");
     L.tol = (opts && opts->tol > 0) ? opts->tol : 1e-9;
     L.max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 20000;
     L.seed = opts ? opts->seed : 0;
     L.verbose = opts ? opts->verbose : 0;
     // preconditioner: Jacobi, accelerated by a Chebyshev polynomial whose
     // degree grows with sqrt(n) (the condition number of a mesh Laplacian
-    // grows like n^(2/d)); poly_degree 1 is plain Jacobi
+    // grows like n^(2/d)) until the iteration count stops falling (~100 at
+    // 655k vertices: beyond it the block's own gap ratio limits LOBPCG);
+    // poly_degree 1 is plain Jacobi
     const bool jac = !opts || opts->precond == LH_PRECOND_JACOBI;
     int deg = opts ? opts->poly_degree : 0;
     if (!jac) deg = 1;
-    if (deg <= 0) deg = (int)std::lround(std::sqrt((double)n) / 12.0);
-    deg = std::max(1, std::min(deg, 96));
-    if (deg < 4) deg = 1;
+    if (deg <= 0) {
+        deg = (int)std::lround(std::sqrt((double)n) / 8.5);
+        deg = std::min(deg, 128);
+        if (deg < 4) deg = 1;
+    }
+    deg = std::max(1, std::min(deg, 1024));
     L.poly_degree = deg;
+    const int store = opts ? opts->poly_storage : LH_STORE_AUTO;
+    if (store != LH_STORE_AUTO && store != LH_STORE_F32 && store != LH_STORE_F16)
+        LH_THROW(LH_ERR_ARG, "poly_storage %d unknown", store);
     L.poly_ratio = (opts && opts->poly_ratio > 1.0) ? opts->poly_ratio : 4.0 * deg * deg;
     if (opts && opts->precision != LH_PREC_FP64) LH_THROW(LH_ERR_ARG, "precision mode %d not available", opts->precision);
     auto e = std::make_unique<lh_eig>();

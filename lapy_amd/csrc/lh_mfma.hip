@@ -1,0 +1,310 @@
+// FP64 matrix-core (MFMA) kernels for the tall-skinny products of the block
+// eigensolver: the Gram matrix C = U^T V and the update Y = alpha U C + beta Y,
+// with U, V, Y row-major n x (p | q) blocks (n ~ 10^6, p, q <= 192).
+//
+// Both use v_mfma_f64_16x16x4f64 (one wave: D[16x16] += A[16x4] B[4x16]);
+// lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16], and holds
+// D[l / 16 + 4 r][l % 16] in accumulator r (r = 0..3).
+//
+// Gram: the reduction runs over the rows.  A wave owns a 64 x 64 output tile
+// and a range of rows; per group of 4 rows each lane loads 4 consecutive
+// columns of U and of V (one 32-byte load each, 512 contiguous bytes per 16
+// lanes), which feed the 4 x 4 MFMA blocks of the tile: block (bi, bj) uses
+// the columns 4 i + bi of U and 4 j + bj of V.  That permutation of the tile's
+// rows and columns is undone when the wave writes its partial tile.  The
+// partials are summed in fixed order by a second kernel (deterministic).
+//
+// Update: a wave owns 64 rows x 64 columns of Y; per group of 16 columns of U
+// (the summed index) each lane loads 4 consecutive values of its row (one
+// 32-byte load), which serve as the A operand of 4 successive MFMAs, and the
+// matching 4 rows of C (L2-resident) as B.
+#include "lh_dense.hpp"
+
+namespace lh {
+
+namespace {
+
+typedef double d2_t __attribute__((ext_vector_type(2)));
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4_t mfma16(double a, double b, d4_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// four consecutive values row[c .. c+3] of a row-major block, zero outside
+// [0, ncols); VEC: the 32 bytes are 16-byte aligned and in bounds as a whole
+template <bool VEC>
+__device__ __forceinline__ d4_t load4(const double* __restrict__ row, int c, int ncols) {
+    d4_t v = {0.0, 0.0, 0.0, 0.0};
+    if (VEC && c + 3 < ncols) {
+        const d2_t lo = *reinterpret_cast<const d2_t*>(row + c);
+        const d2_t hi = *reinterpret_cast<const d2_t*>(row + c + 2);
+        v[0] = lo[0]; v[1] = lo[1]; v[2] = hi[0]; v[3] = hi[1];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (c + q < ncols) v[q] = row[c + q];
+    }
+    return v;
+}
+
+// tile index -> (ti, tj); sym: upper triangle ti <= tj in row order
+__device__ __forceinline__ void tile_of(int t, int tv, int sym, int& ti, int& tj) {
+    if (!sym) { ti = t / tv; tj = t - ti * tv; return; }
+    ti = 0;
+    int rowlen = tv;
+    while (t >= rowlen) { t -= rowlen; ++ti; --rowlen; }
+    tj = ti + t;
+}
+
+template <bool VEC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_gram_mfma(int64_t n, int64_t rows_per_wave, int nwaves, const double* __restrict__ U, int ldu, int p,
+            const double* __restrict__ V, int ldv, int q, int tv, int sym, double* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= nwaves) return;  // whole wave exits together
+    const int tile = blockIdx.y;
+    int ti, tj;
+    tile_of(tile, tv, sym, ti, tj);
+    const int u0 = ti * 64, v0 = tj * 64;
+    const int il = lane & 15, kg = lane >> 4;
+    const int64_t rb = (int64_t)w * rows_per_wave;
+    const int64_t re = (rb + rows_per_wave < n) ? rb + rows_per_wave : n;
+    d4_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = (d4_t){0.0, 0.0, 0.0, 0.0};
+    const int cu = u0 + 4 * il, cv = v0 + 4 * il;
+    for (int64_t r0 = rb; r0 < re; r0 += 16) {
+        d4_t a[4], b[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int64_t r = r0 + 4 * s + kg;
+            if (r < re) {
+                a[s] = load4<VEC>(U + r * ldu, cu, p);
+                b[s] = load4<VEC>(V + r * ldv, cv, q);
+            } else {
+                a[s] = (d4_t){0.0, 0.0, 0.0, 0.0};
+                b[s] = (d4_t){0.0, 0.0, 0.0, 0.0};
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = mfma16(a[s][bi], b[s][bj], acc[bi][bj]);
+    }
+    // partial tile [w][tile][64 x 64] in natural order
+    double* dst = part + ((int64_t)w * gridDim.y + tile) * 4096;
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 4 * (kg + 4 * r) + bi;  // tile-local column of U
+                const int j = 4 * il + bj;            // tile-local column of V
+                dst[i * 64 + j] = acc[bi][bj][r];
+            }
+}
+
+// C[i][j] = alpha * sum over waves (fixed order: 4 interleaved lane groups,
+// combined pairwise); sym: the lower triangle mirrors the upper tiles
+__global__ void __launch_bounds__(256)
+k_gram_mfma_reduce(int nwaves, int ntile, int p, int q, int tv, int sym, const double* __restrict__ part,
+                   double* __restrict__ C, int ldc, double alpha) {
+    __shared__ double sh[4][64];
+    const int o = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+    double s = 0.0;
+    int i = 0, j = 0;
+    if (o < p * q) {
+        i = o / q;
+        j = o - i * q;
+        int ti = i >> 6, tj = j >> 6, li = i & 63, lj = j & 63;
+        if (sym && ti > tj) {
+            const int t = ti; ti = tj; tj = t;
+            const int l = li; li = lj; lj = l;
+        }
+        int tile;
+        if (sym) tile = ti * tv - (ti * (ti - 1)) / 2 + (tj - ti);
+        else tile = ti * tv + tj;
+        const double* src = part + (int64_t)tile * 4096 + li * 64 + lj;
+        const int64_t stride = (int64_t)ntile * 4096;
+        for (int c = g; c < nwaves; c += 4) s += src[c * stride];
+    }
+    sh[g][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (g == 0 && o < p * q) {
+        const int l = threadIdx.x;
+        C[(int64_t)i * ldc + j] = alpha * ((sh[0][l] + sh[1][l]) + (sh[2][l] + sh[3][l]));
+    }
+}
+
+template <bool VEC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_gemm_mfma(int64_t n, int p, int q, const double* __restrict__ U, int ldu, const double* __restrict__ Cm, int ldc,
+            double* __restrict__ Y, int ldy, double alpha, double beta) {
+    const int lane = threadIdx.x & 63;
+    const int il = lane & 15, kg = lane >> 4;
+    const int64_t rw = (int64_t)xcd_block() * 256 + (threadIdx.x >> 6) * 64;  // first row of this wave
+    if (rw >= n) return;
+    const int n0 = blockIdx.y * 64;
+    d4_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = (d4_t){0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < p; k0 += 16) {
+        const int kc = k0 + 4 * kg;  // this lane's 4 summed indices
+        d4_t a[4];
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi) {
+            const int64_t r = rw + 16 * bi + il;
+            a[bi] = (r < n) ? load4<VEC>(U + r * ldu, kc, p) : (d4_t){0.0, 0.0, 0.0, 0.0};
+        }
+        double b[4][4];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj) {
+                const int col = n0 + 16 * bj + il;
+                b[jj][bj] = (kc + jj < p && col < q) ? Cm[(int64_t)(kc + jj) * ldc + col] : 0.0;
+            }
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+            for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = mfma16(a[bi][jj], b[jj][bj], acc[bi][bj]);
+    }
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t row = rw + 16 * bi + kg + 4 * r;
+            if (row >= n) continue;
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj) {
+                const int col = n0 + 16 * bj + il;
+                if (col >= q) continue;
+                double v = alpha * acc[bi][bj][r];
+                if (beta != 0.0) v += beta * Y[row * ldy + col];
+                Y[row * ldy + col] = v;
+            }
+        }
+}
+
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+}  // namespace
+
+void gram_mfma(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q, double* C,
+               int ldc, DevBuf& ws, double alpha, bool sym) {
+    if (sym && p != q) LH_THROW(LH_ERR_ARG, "symmetric Gram needs p == q");
+    const int tu = (p + 63) / 64, tv = (q + 63) / 64;
+    const int ntile = sym ? tu * (tu + 1) / 2 : tu * tv;
+    // about two waves per SIMD over all tiles, at least 64 rows per wave
+    const int64_t target = (int64_t)ctx->n_cu * 8;
+    int64_t nw = std::max<int64_t>(1, target / ntile);
+    nw = std::min<int64_t>(nw, (n + 63) / 64);
+    int64_t rpw = (n + nw - 1) / nw;
+    rpw = ((rpw + 15) / 16) * 16;
+    nw = (n + rpw - 1) / rpw;
+    ws.ensure((size_t)nw * ntile * 4096 * sizeof(double));
+    const bool vec = aligned16(U) && aligned16(V) && ldu % 2 == 0 && ldv % 2 == 0;
+    dim3 grid((unsigned)((nw + 3) / 4), (unsigned)ntile);
+    if (vec)
+        k_gram_mfma<true><<<grid, 256, 0, ctx->stream>>>(n, rpw, (int)nw, U, ldu, p, V, ldv, q, tv, sym ? 1 : 0,
+                                                         ws.as<double>());
+    else
+        k_gram_mfma<false><<<grid, 256, 0, ctx->stream>>>(n, rpw, (int)nw, U, ldu, p, V, ldv, q, tv, sym ? 1 : 0,
+                                                          ws.as<double>());
+    LH_CHECK_LAUNCH();
+    k_gram_mfma_reduce<<<nblocks((int64_t)p * q, 64), 256, 0, ctx->stream>>>((int)nw, ntile, p, q, tv, sym ? 1 : 0,
+                                                                             ws.as<double>(), C, ldc, alpha);
+    LH_CHECK_LAUNCH();
+}
+
+void gemm_mfma(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* C, int ldc, double* Y,
+               int ldy, double alpha, double beta) {
+    if (n == 0 || q == 0) return;
+    const bool vec = aligned16(U) && ldu % 2 == 0;
+    dim3 grid(nblocks_xcd(n, 256), (unsigned)((q + 63) / 64));
+    if (vec)
+        k_gemm_mfma<true><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta);
+    else
+        k_gemm_mfma<false><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta);
+    LH_CHECK_LAUNCH();
+}
+
+}  // namespace lh
+
+using namespace lh;
+
+extern "C" {
+
+int lh_block_gram(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q, int sym,
+                  double* C, int iters, double* avg_us) {
+    if (!ctx || !U || !V || !C || n <= 0 || p <= 0 || q <= 0 || ldu < p || ldv < q) return LH_ERR_ARG;
+    if (p > 1024 || q > 1024) return LH_ERR_ARG;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    if (sym && p != q) LH_THROW(LH_ERR_ARG, "symmetric Gram needs p == q");
+    DevBuf du(n * ldu * 8), dv(n * ldv * 8), dc((size_t)p * q * 8), ws;
+    h2d(ctx, du.p, U, n * ldu * 8);
+    h2d(ctx, dv.p, V, n * ldv * 8);
+    gram_mfma(ctx, n, du.as<double>(), ldu, p, dv.as<double>(), ldv, q, dc.as<double>(), q, ws, 1.0, sym != 0);
+    if (avg_us && iters > 0) {
+        hipEvent_t e0, e1;
+        HIPX(hipEventCreate(&e0));
+        HIPX(hipEventCreate(&e1));
+        HIPX(hipEventRecord(e0, ctx->stream));
+        for (int it = 0; it < iters; ++it)
+            gram_mfma(ctx, n, du.as<double>(), ldu, p, dv.as<double>(), ldv, q, dc.as<double>(), q, ws, 1.0,
+                      sym != 0);
+        HIPX(hipEventRecord(e1, ctx->stream));
+        HIPX(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPX(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_us = 1000.0 * ms / iters;
+    }
+    d2h(ctx, C, dc.p, (size_t)p * q * 8);
+    LH_API_END(ctx)
+}
+
+int lh_block_gemm(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* Cs, double* Y,
+                  int ldy, double alpha, double beta, int iters, double* avg_us) {
+    if (!ctx || !U || !Cs || !Y || n <= 0 || p <= 0 || q <= 0 || ldu < p || ldy < q) return LH_ERR_ARG;
+    if (p > 1024 || q > 1024) return LH_ERR_ARG;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    DevBuf du(n * ldu * 8), dy(n * ldy * 8), dc((size_t)p * q * 8);
+    h2d(ctx, du.p, U, n * ldu * 8);
+    h2d(ctx, dy.p, Y, n * ldy * 8);
+    h2d(ctx, dc.p, Cs, (size_t)p * q * 8);
+    gemm_mfma(ctx, n, p, q, du.as<double>(), ldu, dc.as<double>(), q, dy.as<double>(), ldy, alpha, beta);
+    d2h(ctx, Y, dy.p, n * ldy * 8);
+    if (avg_us && iters > 0) {
+        hipEvent_t e0, e1;
+        HIPX(hipEventCreate(&e0));
+        HIPX(hipEventCreate(&e1));
+        HIPX(hipEventRecord(e0, ctx->stream));
+        for (int it = 0; it < iters; ++it)
+            gemm_mfma(ctx, n, p, q, du.as<double>(), ldu, dc.as<double>(), q, dy.as<double>(), ldy, alpha, 0.0);
+        HIPX(hipEventRecord(e1, ctx->stream));
+        HIPX(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPX(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *avg_us = 1000.0 * ms / iters;
+    }
+    LH_API_END(ctx)
+}
+
+}  // extern "C"

@@ -102,6 +102,91 @@ k_cheb_step_v(int64_t n, int m, const int32_t* __restrict__ rowptr, const int32_
     *reinterpret_cast<float4*>(xprev + o) = xp;
 }
 
+// ---- FP16 storage, FP32 arithmetic --------------------------------------
+// The block is scaled per column to RMS 2^-4 before the first step, so that
+// the smooth components, which the polynomial amplifies at most ~2 d^2 /
+// lmax times (times D^-1), stay far below the FP16 limit; stores saturate
+// at +-65504 in any case.  The scales are undone on output.
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float float8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ float8 h2f(half8 h) { return __builtin_convertvector(h, float8); }
+__device__ __forceinline__ _Float16 f2h_sat(float v) {
+    return (_Float16)fminf(fmaxf(v, -65504.0f), 65504.0f);
+}
+__device__ __forceinline__ half8 f2h_sat(float8 v) {
+    half8 h;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) h[q] = f2h_sat(v[q]);
+    return h;
+}
+
+// scale[c] = target / ||w_c|| (1 for a zero column), inv[c] = 1 / scale[c]
+__global__ void k_col_scales(int m, const double* __restrict__ colnorm2, double target, float* __restrict__ scale,
+                             double* __restrict__ inv) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m) return;
+    const double q = colnorm2[c];
+    const double sc = q > 0.0 ? target / sqrt(q) : 1.0;
+    scale[c] = (float)sc;
+    inv[c] = 1.0 / (double)(float)sc;
+}
+
+// b = s_c W, x_1 = alpha D^-1 b, x_0 = 0 (FP16)
+__global__ void k_cheb_init_h(int64_t n, int m, const double* __restrict__ W, int ldw, const float* __restrict__ dinv,
+                              const float* __restrict__ scale, float alpha, _Float16* __restrict__ bvec,
+                              _Float16* __restrict__ x1, _Float16* __restrict__ x0) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n * m) return;
+    int64_t i = gid / m;
+    int c = (int)(gid - i * m);
+    const float r = (float)W[i * ldw + c] * scale[c];
+    bvec[gid] = f2h_sat(r);
+    x1[gid] = f2h_sat(alpha * dinv[i] * r);
+    x0[gid] = (_Float16)0.0f;
+}
+
+// one step, 8 columns (16 bytes) per lane; requires m % 8 == 0
+template <int LPR>
+__global__ void __launch_bounds__(256)
+k_cheb_step_h(int64_t n, int m, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+              const float* __restrict__ kv, const float* __restrict__ dinv, const _Float16* __restrict__ bvec,
+              const _Float16* __restrict__ xk, _Float16* __restrict__ xprev, float alpha, float omega) {
+    const int64_t gid = xcd_block() * blockDim.x + threadIdx.x;
+    const int64_t row = gid / LPR;
+    const int c0 = (int)(gid % LPR) * 8;
+    if (row >= n || c0 >= m) return;
+    const int32_t r0 = rowptr[row], r1 = rowptr[row + 1];
+    float8 a0 = 0.0f, a1 = 0.0f;
+    int32_t s = r0;
+    for (; s + 1 < r1; s += 2) {
+        const int32_t j0 = cols[s], j1 = cols[s + 1];
+        const float w0 = kv[s], w1 = kv[s + 1];
+        const half8 x0 = *reinterpret_cast<const half8*>(xk + (int64_t)j0 * m + c0);
+        const half8 x1 = *reinterpret_cast<const half8*>(xk + (int64_t)j1 * m + c0);
+        a0 += w0 * h2f(x0);
+        a1 += w1 * h2f(x1);
+    }
+    if (s < r1) a0 += kv[s] * h2f(*reinterpret_cast<const half8*>(xk + (int64_t)cols[s] * m + c0));
+    a0 += a1;
+    const float ad = alpha * dinv[row];
+    const int64_t o = row * m + c0;
+    const float8 xkv = h2f(*reinterpret_cast<const half8*>(xk + o));
+    const float8 bv = h2f(*reinterpret_cast<const half8*>(bvec + o));
+    const float8 xp = h2f(*reinterpret_cast<const half8*>(xprev + o));
+    const float8 r = omega * (xkv - xp + ad * (bv - a0)) + xp;
+    *reinterpret_cast<half8*>(xprev + o) = f2h_sat(r);
+}
+
+__global__ void k_cheb_out_h(int64_t n, int m, const _Float16* __restrict__ x, const double* __restrict__ inv,
+                             double* __restrict__ W, int ldw) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n * m) return;
+    int64_t i = gid / m;
+    int c = (int)(gid - i * m);
+    W[i * ldw + c] = (double)(float)x[gid] * inv[c];
+}
+
 __global__ void k_cheb_out(int64_t n, int m, const float* __restrict__ x, double* __restrict__ W, int ldw) {
     int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n * m) return;
@@ -183,8 +268,10 @@ void form_shifted(lh_ctx* ctx, const lh_csr* A, const lh_csr* B, double alpha, d
 }
 
 void ChebPrecond::setup(lh_ctx* c, const lh_csr* A, const lh_csr* B, double sigma, int deg, double ratio,
-                        int64_t n_, int mmax) {
+                        int64_t n_, int mmax_, bool half_storage) {
     ctx = c;
+    mmax = mmax_;
+    half = half_storage;
     pat = A->pat;
     n = n_;
     degree = deg;
@@ -236,96 +323,126 @@ void ChebPrecond::setup(lh_ctx* c, const lh_csr* A, const lh_csr* B, double sigm
     k_to_f32<<<nblocks(nnz, 256), 256, 0, ctx->stream>>>(nnz, kv.as<double>(), kv32.as<float>());
     k_to_f32<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, dinv.as<double>(), dinv32.as<float>());
     LH_CHECK_LAUNCH();
-    R.alloc(n * mmax * 4);
-    D0.alloc(n * mmax * 4);
-    D1.alloc(n * mmax * 4);
+    const size_t es = half ? 2 : 4;
+    R.alloc(n * mmax * es);
+    D0.alloc(n * mmax * es);
+    D1.alloc(n * mmax * es);
 }
 
-void ChebPrecond::apply(double* W, int ldw, int m) {
+// one Chebyshev step x_prev <- step(x_k, x_prev) on an n x m block
+void ChebPrecond::step(int m, const void* bvec, const void* xk, void* xp, double omega) {
+    const int32_t* rp = pat->rowptr.as<int32_t>();
+    const int32_t* cl = pat->cols.as<int32_t>();
+    const float* kk = kv32.as<float>();
+    const float* dd = dinv32.as<float>();
+    const float fa = (float)(2.0 / (bmax + amin)), fo = (float)omega;
+    if (half) {
+        auto* b = static_cast<const _Float16*>(bvec);
+        auto* x = static_cast<const _Float16*>(xk);
+        auto* y = static_cast<_Float16*>(xp);
+        const int vl = vec_lanes(m, 8);
+        const unsigned nb = nblocks_xcd(n * vl, 256);
+        switch (vl) {
+            case 1: k_cheb_step_h<1><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+            case 2: k_cheb_step_h<2><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+            case 4: k_cheb_step_h<4><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+            case 8: k_cheb_step_h<8><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+            default: k_cheb_step_h<16><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+        }
+        LH_CHECK_LAUNCH();
+        return;
+    }
+    auto* b = static_cast<const float*>(bvec);
+    auto* x = static_cast<const float*>(xk);
+    auto* y = static_cast<float*>(xp);
+    const bool vec = (m % 4 == 0) && vec_lanes(m, 4) >= 4 && vec_lanes(m, 4) <= 64;
+    if (vec) {
+        const int vl = vec_lanes(m, 4);
+        const unsigned nbv = nblocks_xcd(n * vl, 256);
+        switch (vl) {
+            case 4: k_cheb_step_v<4><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+            case 8: k_cheb_step_v<8><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+            case 16: k_cheb_step_v<16><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+            case 32: k_cheb_step_v<32><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+            default: k_cheb_step_v<64><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+        }
+    } else {
+        const int lpr = m > 32 ? 64 : (m > 16 ? 32 : 16);
+        const unsigned nb = nblocks_xcd(n * lpr, 256);
+        if (lpr == 64) k_cheb_step<64><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo);
+        else if (lpr == 32) k_cheb_step<32><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo);
+        else k_cheb_step<16><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo);
+    }
+    LH_CHECK_LAUNCH();
+}
+
+// b, x_1, x_0 from W; returns the (x_k, x_prev) buffers
+void ChebPrecond::init(const double* W, int ldw, int m, const double* colnorm2, void** xk, void** xp) {
+    const float fa = (float)(2.0 / (bmax + amin));
+    if (half) {
+        if (!colnorm2) LH_THROW(LH_ERR_ARG, "FP16 Chebyshev storage needs the column norms of the block");
+        sc.ensure(m * 16);
+        float* scale = sc.as<float>();
+        double* inv = reinterpret_cast<double*>(sc.as<char>() + 8 * m);
+        k_col_scales<<<1, 256, 0, ctx->stream>>>(m, colnorm2, std::sqrt((double)n) / 16.0, scale, inv);
+        k_cheb_init_h<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, W, ldw, dinv32.as<float>(), scale, fa,
+                                                                    R.as<_Float16>(), D0.as<_Float16>(),
+                                                                    D1.as<_Float16>());
+    } else {
+        k_cheb_init<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, W, ldw, dinv32.as<float>(), fa, R.as<float>(),
+                                                                  D0.as<float>(), D1.as<float>());
+    }
+    LH_CHECK_LAUNCH();
+    *xk = D0.p;
+    *xp = D1.p;
+}
+
+void ChebPrecond::apply(double* W, int ldw, int m, const double* colnorm2) {
     // Chebyshev semi-iteration (Golub-Varga two-step form) for K x = w,
     // Jacobi-scaled, on [amin, bmax]
-    const double alpha = 2.0 / (bmax + amin);
+    if (m > mmax) LH_THROW(LH_ERR_ARG, "Chebyshev block width %d exceeds %d", m, mmax);
+    if (half && m % 8 != 0) LH_THROW(LH_ERR_ARG, "FP16 Chebyshev storage needs a block width divisible by 8");
     const double mu = (bmax - amin) / (bmax + amin);
-    float* bvec = R.as<float>();
-    float* xk = D0.as<float>();
-    float* xp = D1.as<float>();
-    k_cheb_init<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, W, ldw, dinv32.as<float>(), (float)alpha, bvec,
-                                                              xk, xp);
-    LH_CHECK_LAUNCH();
-    const int lpr = m > 32 ? 64 : (m > 16 ? 32 : 16);
-    const unsigned nb = nblocks_xcd(n * lpr, 256);
+    void *xk, *xp;
+    init(W, ldw, m, colnorm2, &xk, &xp);
     double omega = 1.0;
-    const bool vec = (m % 4 == 0) && vec_lanes(m, 4) >= 4 && vec_lanes(m, 4) <= 64;
-    const int vl = vec ? vec_lanes(m, 4) : 0;
-    const unsigned nbv = vec ? nblocks_xcd(n * vl, 256) : 0;
     for (int k = 1; k < degree; ++k) {
         omega = (k == 1) ? 1.0 / (1.0 - 0.5 * mu * mu) : 1.0 / (1.0 - 0.25 * mu * mu * omega);
-        const int32_t* rp = pat->rowptr.as<int32_t>();
-        const int32_t* cl = pat->cols.as<int32_t>();
-        if (vec) {
-            const float* kk = kv32.as<float>();
-            const float* dd = dinv32.as<float>();
-            const float fa = (float)alpha, fo = (float)omega;
-            switch (vl) {
-                case 4: k_cheb_step_v<4><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, bvec, xk, xp, fa, fo); break;
-                case 8: k_cheb_step_v<8><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, bvec, xk, xp, fa, fo); break;
-                case 16: k_cheb_step_v<16><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, bvec, xk, xp, fa, fo); break;
-                case 32: k_cheb_step_v<32><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, bvec, xk, xp, fa, fo); break;
-                default: k_cheb_step_v<64><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, bvec, xk, xp, fa, fo); break;
-            }
-            LH_CHECK_LAUNCH();
-            std::swap(xk, xp);
-            continue;
-        }
-        if (lpr == 64)
-            k_cheb_step<64><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk,
-                                                        xp, (float)alpha, (float)omega);
-        else if (lpr == 32)
-            k_cheb_step<32><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk,
-                                                        xp, (float)alpha, (float)omega);
-        else
-            k_cheb_step<16><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk,
-                                                        xp, (float)alpha, (float)omega);
-        LH_CHECK_LAUNCH();
+        step(m, R.p, xk, xp, omega);
         std::swap(xk, xp);
     }
-    k_cheb_out<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, xk, W, ldw);
+    if (half) {
+        const double* inv = reinterpret_cast<const double*>(sc.as<char>() + 8 * m);
+        k_cheb_out_h<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, static_cast<const _Float16*>(xk), inv, W,
+                                                                   ldw);
+    } else {
+        k_cheb_out<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, static_cast<const float*>(xk), W, ldw);
+    }
     LH_CHECK_LAUNCH();
 }
 
 double ChebPrecond::time_step(int m, int iters) {
     // random block, then `iters` back-to-back steps between HIP events
-    DevBuf w(n * m * 8);
+    if (m > mmax) LH_THROW(LH_ERR_ARG, "Chebyshev block width %d exceeds %d", m, mmax);
+    if (half && m % 8 != 0) LH_THROW(LH_ERR_ARG, "FP16 Chebyshev storage needs a block width divisible by 8");
+    if (!half && (m % 4 != 0 || vec_lanes(m, 4) < 4 || vec_lanes(m, 4) > 64))
+        LH_THROW(LH_ERR_ARG, "time_step: block width %d not vectorisable", m);
+    DevBuf w(n * m * 8), nrm(m * 8);
     randn<double>(ctx, n, m, w.as<double>(), m, 99);
-    const int saved = degree;
-    degree = 2;
-    apply(w.as<double>(), m, m);  // initialises b, x_1, x_0
-    degree = saved;
-    const double alpha = 2.0 / (bmax + amin), mu = (bmax - amin) / (bmax + amin);
-    const float omega = (float)(1.0 / (1.0 - 0.5 * mu * mu));
-    float* bvec = R.as<float>();
-    float* xk = D0.as<float>();
-    float* xp = D1.as<float>();
-    const int vl = vec_lanes(m, 4);
-    if (m % 4 != 0 || vl < 4 || vl > 64) LH_THROW(LH_ERR_ARG, "time_step: block width %d not vectorisable", m);
-    const unsigned nbv = nblocks_xcd(n * vl, 256);
+    std::vector<double> hn(m, (double)n);  // unit-variance entries: ||w_c||^2 ~ n
+    h2d(ctx, nrm.p, hn.data(), m * 8);
+    void *xk, *xp;
+    init(w.as<double>(), m, m, nrm.as<double>(), &xk, &xp);
+    const double mu = (bmax - amin) / (bmax + amin);
+    const double omega = 1.0 / (1.0 - 0.5 * mu * mu);
     hipEvent_t e0, e1;
     HIPX(hipEventCreate(&e0));
     HIPX(hipEventCreate(&e1));
     HIPX(hipEventRecord(e0, ctx->stream));
     for (int it = 0; it < iters; ++it) {
-        const int32_t* rp = pat->rowptr.as<int32_t>();
-        const int32_t* cl = pat->cols.as<int32_t>();
-        switch (vl) {
-            case 4: k_cheb_step_v<4><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk, xp, (float)alpha, omega); break;
-            case 8: k_cheb_step_v<8><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk, xp, (float)alpha, omega); break;
-            case 16: k_cheb_step_v<16><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk, xp, (float)alpha, omega); break;
-            case 32: k_cheb_step_v<32><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk, xp, (float)alpha, omega); break;
-            default: k_cheb_step_v<64><<<nbv, 256, 0, ctx->stream>>>(n, m, rp, cl, kv32.as<float>(), dinv32.as<float>(), bvec, xk, xp, (float)alpha, omega); break;
-        }
+        step(m, R.p, xk, xp, omega);
         std::swap(xk, xp);
     }
-    LH_CHECK_LAUNCH();
     HIPX(hipEventRecord(e1, ctx->stream));
     HIPX(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -339,9 +456,10 @@ double ChebPrecond::time_step(int m, int iters) {
 
 using namespace lh;
 
-extern "C" int lh_time_cheb_step(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int iters, int reorder,
+extern "C" int lh_time_cheb_step(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int iters, int reorder, int storage,
                                  double* avg_us) {
     if (!ctx || !A || !B || !avg_us || iters <= 0 || ncol <= 0) return LH_ERR_ARG;
+    if (storage != LH_STORE_F32 && storage != LH_STORE_F16) return LH_ERR_ARG;
     LH_API_BEGIN(ctx)
     HIPX(hipSetDevice(ctx->device));
     const lh_csr *a = A, *b = B;
@@ -355,7 +473,7 @@ extern "C" int lh_time_cheb_step(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, in
         b = Bp.get();
     }
     ChebPrecond cp;
-    cp.setup(ctx, a, b, -0.01, 2, 64.0, a->n(), ncol);
+    cp.setup(ctx, a, b, -0.01, 2, 64.0, a->n(), ncol, storage == LH_STORE_F16);
     *avg_us = cp.time_step(ncol, iters);
     LH_API_END(ctx)
 }

@@ -139,7 +139,8 @@ class Solver:
 
     # -- eigensolve -----------------------------------------------------------
     def eigs(self, k: int = 10, sigma: float = -0.01, *, precision: str = "fp64", precond: str = "jacobi",
-             tol: float = 0.0, block: int = 0, max_iter: int = 0, seed: int = 0, poly_degree: int = 0):
+             tol: float = 0.0, block: int = 0, max_iter: int = 0, seed: int = 0, poly_degree: int = 0,
+             poly_storage: str = "auto"):
         """k eigenpairs of ``A x = lambda B x`` nearest ``sigma`` (lapy/solver.py:667-716).
 
         Returns ``(eigenvalues (k,), eigenvectors (n, k))``, ascending, vectors
@@ -160,10 +161,12 @@ class Solver:
         prec = {"fp64": _lib.LH_PREC_FP64, "mixed": _lib.LH_PREC_MIXED}[precision]
         pc = {"jacobi": _lib.LH_PRECOND_JACOBI, "learned": _lib.LH_PRECOND_LEARNED,
               "none": _lib.LH_PRECOND_NONE}[precond]
+        store = {"auto": _lib.LH_STORE_AUTO, "f32": _lib.LH_STORE_F32, "f16": _lib.LH_STORE_F16}[poly_storage]
         kk = k
         while True:
             evals, evecs, st = _lib.eigs(self._ctx, a, b, kk, sigma, precision=prec, precond=pc, block=block,
-                                         max_iter=max_iter, tol=tol, seed=seed, poly_degree=poly_degree)
+                                         max_iter=max_iter, tol=tol, seed=seed, poly_degree=poly_degree,
+                                         poly_storage=store)
             self.last_stats = st
             if sigma <= evals[0] or kk >= n - 1:
                 break

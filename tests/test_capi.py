@@ -70,5 +70,33 @@ def test_context_creation_without_gpu_fails_loudly():
 def test_stats_struct_layout():
     # lh_stats: 4 int32, 5 double, 4 int32
     assert ctypes.sizeof(_lib.Stats) == 4 * 4 + 5 * 8 + 4 * 4
-    assert ctypes.sizeof(_lib.EigOpts) == 4 * 4 + 8 + 8 + 4 * 2 + 8
+    assert ctypes.sizeof(_lib.EigOpts) == 4 * 4 + 8 + 8 + 4 * 2 + 8 + 4 * 2
     assert ctypes.sizeof(_lib.PcgOpts) == 4 * 2 + 8 + 4 * 2
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """Compile the header with the host C compiler and compare every struct's
+    size and field offsets with the ctypes mirror."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    structs = {"lh_stats": _lib.Stats, "lh_eig_opts": _lib.EigOpts, "lh_pcg_opts": _lib.PcgOpts}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "lapyhip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {(a, b): int(c) for a, b, c in (ln.split() for ln in out if ln)}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

@@ -1,0 +1,78 @@
+"""The eigensolver's tall-skinny FP64 block products (matrix-core kernels)
+against numpy: Gram C = U^T V (full and symmetric-tile forms) and the update
+Y = alpha U C + beta Y, over widths that are and are not multiples of the
+64-wide tiles, leading dimensions larger than the width, odd leading
+dimensions (unaligned rows: the element-wise load path) and row counts that
+are not multiples of the 16-row groups."""
+
+import numpy as np
+import pytest
+
+from lapy_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n,p,q,ldu,ldv", [
+    (10000, 64, 64, 192, 192),
+    (10007, 64, 128, 192, 192),
+    (4099, 192, 192, 192, 192),
+    (3001, 56, 112, 168, 168),
+    (2000, 100, 37, 101, 37),
+    (17, 5, 3, 7, 3),
+])
+def test_gram_matches_numpy(n, p, q, ldu, ldv):
+    rng = np.random.default_rng(n + p + q)
+    U = rng.standard_normal((n, ldu))
+    V = rng.standard_normal((n, ldv))
+    C, _ = _lib.block_gram(_lib.default_context(), U, V, p, q)
+    ref = U[:, :p].T @ V[:, :q]
+    scale = np.sqrt(n)
+    np.testing.assert_allclose(C, ref, rtol=0, atol=1e-13 * scale * 8)
+
+
+@pytest.mark.parametrize("n,p,ld", [(10000, 64, 192), (5003, 128, 192), (4099, 192, 192), (1001, 72, 73)])
+def test_gram_symmetric_tiles(n, p, ld):
+    """S^T (A S) with A SPD: the symmetric form fills the lower tiles from the
+    upper ones, so tiles off the diagonal are exactly mirrored."""
+    rng = np.random.default_rng(p)
+    S = rng.standard_normal((n, ld))
+    d = rng.uniform(0.5, 2.0, n)
+    AS = S * d[:, None]
+    C, _ = _lib.block_gram(_lib.default_context(), S, AS, p, p, sym=True)
+    ref = S[:, :p].T @ AS[:, :p]
+    np.testing.assert_allclose(C, ref, rtol=0, atol=1e-12 * np.sqrt(n) * 8)
+    for i in range(p):
+        for j in range(p):
+            if i // 64 != j // 64:
+                assert C[i, j] == C[j, i]
+
+
+def test_gram_deterministic():
+    rng = np.random.default_rng(3)
+    U = rng.standard_normal((50000, 192))
+    ctx = _lib.default_context()
+    c1, _ = _lib.block_gram(ctx, U, U, 192, 192, sym=True)
+    c2, _ = _lib.block_gram(ctx, U, U, 192, 192, sym=True)
+    assert np.array_equal(c1, c2)
+
+
+@pytest.mark.parametrize("n,p,q,ldu,ldy,alpha,beta", [
+    (10000, 192, 64, 192, 192, 1.0, 0.0),
+    (10007, 128, 128, 192, 128, 1.0, 0.0),
+    (4099, 64, 128, 192, 192, -1.0, 1.0),
+    (3001, 56, 112, 168, 168, 0.5, -2.0),
+    (2000, 37, 100, 37, 101, 1.0, 0.0),
+    (17, 3, 5, 3, 5, 2.0, 1.0),
+])
+def test_gemm_matches_numpy(n, p, q, ldu, ldy, alpha, beta):
+    rng = np.random.default_rng(n * 7 + p)
+    U = rng.standard_normal((n, ldu))
+    Cm = rng.standard_normal((p, q))
+    Y0 = rng.standard_normal((n, ldy))
+    Y, _ = _lib.block_gemm(_lib.default_context(), U, p, Cm, Y0, alpha, beta)
+    ref = Y0.copy()
+    ref[:, :q] = alpha * (U[:, :p] @ Cm) + beta * Y0[:, :q]
+    np.testing.assert_allclose(Y, ref, rtol=0, atol=1e-13 * np.sqrt(p) * 8 * (1 + abs(beta)))
+    # columns beyond q are untouched
+    assert np.array_equal(Y[:, q:], Y0[:, q:])

@@ -90,6 +90,19 @@ def test_ellipsoid_k50():
     assert_eigs_match(lam, vec, lam_o, vec_o, b, k_complete=False)
 
 
+@pytest.mark.parametrize("storage,degree", [("f32", 0), ("f16", 0), ("f16", 40), ("f32", 1)])
+def test_ellipsoid_l6_preconditioner_storage(storage, degree):
+    """Icosphere level 6 ellipsoid (40,962 vertices), k=50: the eigenpairs do
+    not depend on the preconditioner's storage precision or degree (plain
+    Jacobi included), only the iteration count does."""
+    v, t = meshgen.large_ellipsoid(level=6)
+    fem_ = Solver(TriaMesh(v, t))
+    lam, vec = fem_.eigs(k=50, poly_storage=storage, poly_degree=degree)
+    a, b = fem.fem_tria(v, t)
+    lam_o, vec_o = fem.eigs(a, b, k=50)
+    assert_eigs_match(lam, vec, lam_o, vec_o, b, k_complete=False)
+
+
 def test_dense_path_tiny_mesh():
     v, t = meshgen.icosphere(1)
     fem_ = Solver(TriaMesh(v, t))
