@@ -65,6 +65,16 @@ __device__ __forceinline__ V3<T> load_v(const T* __restrict__ x, const T* __rest
     return {x[i], y[i], z[i]};
 }
 
+__device__ __forceinline__ V3<double> load_xyz(const double* __restrict__ xyz, int32_t i) {
+    const double2* q = reinterpret_cast<const double2*>(xyz + 4 * (int64_t)i);
+    const double2 a = q[0], b = q[1];
+    return {a.x, a.y, b.x};
+}
+__device__ __forceinline__ V3<float> load_xyz(const float* __restrict__ xyz, int32_t i) {
+    const float4 a = *reinterpret_cast<const float4*>(xyz + 4 * (int64_t)i);
+    return {a.x, a.y, a.z};
+}
+
 __device__ __forceinline__ int32_t find_slot(const int32_t* __restrict__ cols, int32_t r0, int32_t r1,
                                              int32_t j) {
     // rows are sorted; binary search
@@ -123,6 +133,36 @@ __global__ void k_inc_sort(int64_t n, const int32_t* __restrict__ inc_ptr, int32
         }
         inc[q + 1] = key;
     }
+}
+
+// per incidence entry, the element's vertices in local order (triangles: the
+// local index of the row's vertex in the fourth slot), so that the numeric
+// rows read one 16-byte record per incident element instead of gathering
+// the connectivity
+__global__ void k_inc_verts(int64_t cnt, int nodes, const int32_t* __restrict__ inc, const int32_t* __restrict__ t,
+                            int4* __restrict__ incv) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= cnt) return;
+    const int32_t code = inc[p];
+    const int32_t* te = t + (int64_t)(code >> 2) * nodes;
+    int4 r;
+    r.x = te[0];
+    r.y = te[1];
+    r.z = te[2];
+    r.w = nodes == 4 ? te[3] : (code & 3);
+    incv[p] = r;
+}
+
+// AoS copy of the coordinates, one 4-wide record per vertex
+template <typename VT>
+__global__ void k_aos(int64_t nv, const VT* __restrict__ x, const VT* __restrict__ y, const VT* __restrict__ z,
+                      VT* __restrict__ xyz) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    xyz[4 * i] = x[i];
+    xyz[4 * i + 1] = y[i];
+    xyz[4 * i + 2] = z[i];
+    xyz[4 * i + 3] = VT(0);
 }
 
 // Row i's columns: the sorted unique vertices of all elements incident to i
@@ -261,19 +301,52 @@ struct RowAcc {
     int32_t off;
 };
 
+// the row block's vertex coordinates staged in LDS by CSR slot (x == nullptr:
+// gather them from the AoS copy instead)
+template <typename VT>
+struct RowVerts {
+    const VT* x;
+    const VT* y;
+    const VT* z;
+    const int32_t* c;  // the slots' column indices
+};
+
+// slot of column j in row [r0, r1): from the LDS copy of the columns when the
+// row block is staged (indices shifted by off), else from global memory
+template <typename VT>
+__device__ __forceinline__ int32_t row_slot(const RowVerts<VT>& rv, const int32_t* __restrict__ cols, int32_t r0,
+                                            int32_t r1, int32_t off, int32_t j) {
+    if (rv.c) return find_slot(rv.c, r0 - off, r1 - off, j) + off;
+    return find_slot(cols, r0, r1, j);
+}
+
+template <typename VT>
+__device__ __forceinline__ V3<VT> slot_xyz(const RowVerts<VT>& rv, int32_t s) {
+    return {rv.x[s], rv.y[s], rv.z[s]};
+}
+
 template <typename OT>
 __device__ __forceinline__ void zero_row(OT* __restrict__ v, int32_t r0, int32_t r1) {
     for (int32_t s = r0; s < r1; ++s) v[s] = OT(0);
 }
 
-// stage the workgroup's value range in LDS when it fits; returns false (and
-// zeroes nothing) otherwise.  Called by every thread of the workgroup.
-template <typename OT, bool HAS_A, bool HAS_B>
-__device__ __forceinline__ bool lds_window_begin(OT* sA, OT* sB, int32_t len, int cap) {
+// stage the workgroup's value range in LDS when it fits (zeroed A and B
+// windows, and the coordinates of every slot's column vertex); returns false
+// (and stages nothing) otherwise.  Called by every thread of the workgroup.
+template <typename VT, typename OT, bool HAS_A, bool HAS_B>
+__device__ __forceinline__ bool lds_window_begin(OT* sA, OT* sB, VT* sx, VT* sy, VT* sz, int32_t* sc,
+                                                 const int32_t* __restrict__ cols, const VT* __restrict__ xyz,
+                                                 int32_t base, int32_t len, int cap) {
     if (len > cap) return false;
     for (int32_t s = threadIdx.x; s < len; s += blockDim.x) {
         if (HAS_A) sA[s] = OT(0);
         if (HAS_B) sB[s] = OT(0);
+        const int32_t c = cols[base + s];
+        sc[s] = c;
+        const V3<VT> v = load_xyz(xyz, c);
+        sx[s] = v.x;
+        sy[s] = v.y;
+        sz[s] = v.z;
     }
     __syncthreads();
     return true;
@@ -294,12 +367,11 @@ template <typename VT, typename OT, int KIND, bool LUMP>
 __device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ rowptr,
                                          const int32_t* __restrict__ cols, const int32_t* __restrict__ diagpos,
                                          const int32_t* __restrict__ lrowptr, const int32_t* __restrict__ inc_ptr,
-                                         const int32_t* __restrict__ inc, const int32_t* __restrict__ t,
-                                         const VT* __restrict__ x, const VT* __restrict__ y,
-                                         const VT* __restrict__ z, const double* __restrict__ u1,
+                                         const int32_t* __restrict__ inc, const int4* __restrict__ incv,
+                                         const VT* __restrict__ xyz, const double* __restrict__ u1,
                                          const double* __restrict__ u2, const double* __restrict__ am, int clamp,
-                                         const VT* __restrict__ vol_mean, RowAcc<OT> acc, OT* __restrict__ Bval,
-                                         int32_t* __restrict__ degen) {
+                                         const VT* __restrict__ vol_mean, RowAcc<OT> acc, RowVerts<VT> rv,
+                                         OT* __restrict__ Bval, int32_t* __restrict__ degen) {
     const int32_t r0 = rowptr[i], r1 = rowptr[i + 1];
     const int32_t pa = inc_ptr[i], pb = inc_ptr[i + 1];
     if (pb == pa) return;  // unused vertex: empty row
@@ -310,12 +382,30 @@ __device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ 
     OT bl = OT(0);
     const int32_t dslot = diagpos[i];
     const VT vm = clamp ? vol_mean[0] : VT(0);
+    int4 ev_next = incv[pa];
     for (int32_t p = pa; p < pb; ++p) {
-        const int32_t code = inc[p];
-        const int64_t e = code >> 2;
-        const int l = code & 3;
-        const int32_t i1 = t[3 * e], i2 = t[3 * e + 1], i3 = t[3 * e + 2];
-        V3<VT> v1 = load_v(x, y, z, i1), v2 = load_v(x, y, z, i2), v3 = load_v(x, y, z, i3);
+        const int4 ev = ev_next;
+        if (p + 1 < pb) ev_next = incv[p + 1];  // next record's load overlaps this element
+        const int l = ev.w;
+        const int32_t i1 = ev.x, i2 = ev.y, i3 = ev.z;
+        // columns of this row's three entries: the other two vertices + diag
+        int32_t ja, jb;  // ja: "next" column, jb: "previous" column
+        if (l == 0) { ja = i2; jb = i3; }
+        else if (l == 1) { ja = i1; jb = i3; }
+        else { ja = i2; jb = i1; }
+        const int32_t sa = row_slot(rv, cols, r0, r1, off, ja);
+        const int32_t sb = row_slot(rv, cols, r0, r1, off, jb);
+        V3<VT> v1, v2, v3;
+        if (rv.x) {
+            const V3<VT> vi = slot_xyz(rv, dslot - off), va = slot_xyz(rv, sa - off), vb = slot_xyz(rv, sb - off);
+            if (l == 0) { v1 = vi; v2 = va; v3 = vb; }
+            else if (l == 1) { v1 = va; v2 = vi; v3 = vb; }
+            else { v1 = vb; v2 = va; v3 = vi; }
+        } else {
+            v1 = load_xyz(xyz, i1);
+            v2 = load_xyz(xyz, i2);
+            v3 = load_xyz(xyz, i3);
+        }
         V3<VT> v2mv1 = vsub(v2, v1), v3mv2 = vsub(v3, v2), v1mv3 = vsub(v1, v3);
         V3<VT> cr = vcross(v3mv2, v1mv3);
         VT vol;
@@ -325,13 +415,6 @@ __device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ 
             if (clamp) vol = vm;
             else atomicOr(degen, 1);
         }
-        // columns of this row's three entries: the other two vertices + diag
-        int32_t ja, jb;  // ja: "next" column, jb: "previous" column
-        if (l == 0) { ja = i2; jb = i3; }
-        else if (l == 1) { ja = i1; jb = i3; }
-        else { ja = i2; jb = i1; }
-        const int32_t sa = find_slot(cols, r0, r1, ja);
-        const int32_t sb = find_slot(cols, r0, r1, jb);
         if constexpr (HAS_A) {
             // a12 (t1,t2), a23 (t2,t3), a31 (t3,t1); row t1: a12 -> t2, a31 -> t3
             // row t2: a12 -> t1, a23 -> t3; row t3: a23 -> t2, a31 -> t1
@@ -345,6 +428,7 @@ __device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ 
                 else { oa = (OT)a23; ob = (OT)a31; od = (OT)(-a31 - a23); }
             } else {
                 // lapy/solver.py:259-280: edges projected on (u1, u2), weighted
+                const int64_t e = inc[p] >> 2;
                 const double* pu1 = u1 + 3 * e;
                 const double* pu2 = u2 + 3 * e;
                 V3<double> U1{pu1[0], pu1[1], pu1[2]}, U2{pu2[0], pu2[1], pu2[2]};
@@ -387,30 +471,35 @@ __global__ void __launch_bounds__(256)
 k_tria_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
             const int32_t* __restrict__ diagpos, const int32_t* __restrict__ lrowptr,
             const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
-            const int32_t* __restrict__ t, const VT* __restrict__ x, const VT* __restrict__ y,
-            const VT* __restrict__ z, const double* __restrict__ u1, const double* __restrict__ u2,
-            const double* __restrict__ am, int clamp, const VT* __restrict__ vol_mean,
+            const int4* __restrict__ incv, const VT* __restrict__ xyz, const double* __restrict__ u1,
+            const double* __restrict__ u2, const double* __restrict__ am, int clamp, const VT* __restrict__ vol_mean,
             OT* __restrict__ Aval, OT* __restrict__ Bval, int32_t* __restrict__ degen, int cap) {
     constexpr bool HAS_A = (KIND != LH_TRIA_MASS);
     constexpr bool HAS_B = !LUMP;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     OT* sA = reinterpret_cast<OT*>(lds_raw);
     OT* sB = sA + cap;
+    VT* sx = reinterpret_cast<VT*>(sB + cap);
+    VT* sy = sx + cap;
+    VT* sz = sy + cap;
+    int32_t* sc = reinterpret_cast<int32_t*>(sz + cap);
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
     const int64_t i = i0 + threadIdx.x;
     const int64_t iend = (i0 + blockDim.x < n) ? i0 + blockDim.x : n;
     const int32_t base = rowptr[i0], len = rowptr[iend] - base;
-    if (lds_window_begin<OT, HAS_A, HAS_B>(sA, sB, len, cap)) {
+    if (lds_window_begin<VT, OT, HAS_A, HAS_B>(sA, sB, sx, sy, sz, sc, cols, xyz, base, len, cap)) {
         if (i < n)
-            tria_row<VT, OT, KIND, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, t, x, y, z, u1, u2, am,
-                                         clamp, vol_mean, RowAcc<OT>{sA, sB, base}, Bval, degen);
+            tria_row<VT, OT, KIND, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, incv, xyz, u1, u2, am,
+                                         clamp, vol_mean, RowAcc<OT>{sA, sB, base}, RowVerts<VT>{sx, sy, sz, sc}, Bval,
+                                         degen);
         lds_window_end<OT, HAS_A, HAS_B>(sA, sB, Aval, Bval, base, len);
     } else if (i < n) {
         const int32_t r0 = rowptr[i], r1 = rowptr[i + 1];
         if (HAS_A) zero_row(Aval, r0, r1);
         if (HAS_B) zero_row(Bval, r0, r1);
-        tria_row<VT, OT, KIND, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, t, x, y, z, u1, u2, am,
-                                     clamp, vol_mean, RowAcc<OT>{Aval, Bval, 0}, Bval, degen);
+        tria_row<VT, OT, KIND, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, incv, xyz, u1, u2, am,
+                                     clamp, vol_mean, RowAcc<OT>{Aval, Bval, 0},
+                                     RowVerts<VT>{nullptr, nullptr, nullptr, nullptr}, Bval, degen);
     }
 }
 
@@ -419,10 +508,10 @@ template <typename VT, typename OT, bool LUMP>
 __device__ __forceinline__ void tet_row(int64_t i, const int32_t* __restrict__ rowptr,
                                         const int32_t* __restrict__ cols, const int32_t* __restrict__ diagpos,
                                         const int32_t* __restrict__ lrowptr, const int32_t* __restrict__ inc_ptr,
-                                        const int32_t* __restrict__ inc, const int32_t* __restrict__ t,
-                                        const VT* __restrict__ x, const VT* __restrict__ y,
-                                        const VT* __restrict__ z, int clamp, const VT* __restrict__ vol_mean,
-                                        RowAcc<OT> acc, OT* __restrict__ Bval, int32_t* __restrict__ degen) {
+                                        const int32_t* __restrict__ inc, const int4* __restrict__ incv,
+                                        const VT* __restrict__ xyz, int clamp, const VT* __restrict__ vol_mean,
+                                        RowAcc<OT> acc, RowVerts<VT> rv, OT* __restrict__ Bval,
+                                        int32_t* __restrict__ degen) {
     const int32_t r0 = rowptr[i], r1 = rowptr[i + 1];
     const int32_t pa = inc_ptr[i], pb = inc_ptr[i + 1];
     if (pb == pa) return;
@@ -433,13 +522,27 @@ __device__ __forceinline__ void tet_row(int64_t i, const int32_t* __restrict__ r
     const int32_t dslot = diagpos[i];
     const VT vm = clamp ? vol_mean[0] : VT(0);
     for (int32_t p = pa; p < pb; ++p) {
-        const int32_t code = inc[p];
-        const int64_t e = code >> 2;
-        const int l = code & 3;
-        const int32_t* te = t + 4 * e;
-        const int32_t t1 = te[0], t2 = te[1], t3 = te[2], t4 = te[3];
-        V3<VT> v1 = load_v(x, y, z, t1), v2 = load_v(x, y, z, t2);
-        V3<VT> v3 = load_v(x, y, z, t3), v4 = load_v(x, y, z, t4);
+        const int l = inc[p] & 3;
+        const int4 ev = incv[p];
+        const int32_t t1 = ev.x, t2 = ev.y, t3 = ev.z, t4 = ev.w;
+        // the three off-diagonal columns of this row: the other vertices in
+        // local order, and their slots
+        const int32_t c0 = (l == 0) ? t2 : t1, c1 = (l <= 1) ? t3 : t2, c2 = (l <= 2) ? t4 : t3;
+        const int32_t s0 = row_slot(rv, cols, r0, r1, off, c0);
+        const int32_t s1 = row_slot(rv, cols, r0, r1, off, c1);
+        const int32_t s2 = row_slot(rv, cols, r0, r1, off, c2);
+        V3<VT> v1, v2, v3, v4;
+        if (rv.x) {
+            v1 = slot_xyz(rv, (l == 0 ? dslot : s0) - off);
+            v2 = slot_xyz(rv, (l == 1 ? dslot : (l == 0 ? s0 : s1)) - off);
+            v3 = slot_xyz(rv, (l == 2 ? dslot : (l < 2 ? s1 : s2)) - off);
+            v4 = slot_xyz(rv, (l == 3 ? dslot : s2) - off);
+        } else {
+            v1 = load_xyz(xyz, t1);
+            v2 = load_xyz(xyz, t2);
+            v3 = load_xyz(xyz, t3);
+            v4 = load_xyz(xyz, t4);
+        }
         V3<VT> e1 = vsub(v2, v1), e2 = vsub(v3, v2), e3 = vsub(v1, v3);
         V3<VT> e4 = vsub(v4, v1), e5 = vsub(v4, v2), e6 = vsub(v4, v3);
         V3<VT> cr = vcross(e1, e3);
@@ -453,25 +556,24 @@ __device__ __forceinline__ void tet_row(int64_t i, const int32_t* __restrict__ r
         VT e12 = vdot(e1, e2), e13 = vdot(e1, e3), e14 = vdot(e1, e4), e15 = vdot(e1, e5);
         VT e23 = vdot(e2, e3), e25 = vdot(e2, e5), e26 = vdot(e2, e6);
         VT e34 = vdot(e3, e4), e36 = vdot(e3, e6);
-        // the three off-diagonals touching local vertex l, with their columns
+        // the three off-diagonals touching local vertex l (columns c0, c1, c2)
         VT o0, o1, o2;
-        int32_t c0, c1, c2;
         if (l == 0) {
-            o0 = ((-e36) * e26 + e23 * e66) / vol; c0 = t2;  // a12
-            o1 = ((-e15) * e25 + e12 * e55) / vol; c1 = t3;  // a13
-            o2 = (e23 * e26 - e36 * e22) / vol;    c2 = t4;  // a14
+            o0 = ((-e36) * e26 + e23 * e66) / vol;  // a12
+            o1 = ((-e15) * e25 + e12 * e55) / vol;  // a13
+            o2 = (e23 * e26 - e36 * e22) / vol;     // a14
         } else if (l == 1) {
-            o0 = ((-e36) * e26 + e23 * e66) / vol; c0 = t1;  // a12
-            o1 = ((-e14) * e34 + e13 * e44) / vol; c1 = t3;  // a23
-            o2 = (e13 * e34 - e14 * e33) / vol;    c2 = t4;  // a24
+            o0 = ((-e36) * e26 + e23 * e66) / vol;  // a12
+            o1 = ((-e14) * e34 + e13 * e44) / vol;  // a23
+            o2 = (e13 * e34 - e14 * e33) / vol;     // a24
         } else if (l == 2) {
-            o0 = ((-e15) * e25 + e12 * e55) / vol; c0 = t1;  // a13
-            o1 = ((-e14) * e34 + e13 * e44) / vol; c1 = t2;  // a23
-            o2 = ((-e14) * e13 + e11 * e34) / vol; c2 = t4;  // a34
+            o0 = ((-e15) * e25 + e12 * e55) / vol;  // a13
+            o1 = ((-e14) * e34 + e13 * e44) / vol;  // a23
+            o2 = ((-e14) * e13 + e11 * e34) / vol;  // a34
         } else {
-            o0 = (e23 * e26 - e36 * e22) / vol;    c0 = t1;  // a14
-            o1 = (e13 * e34 - e14 * e33) / vol;    c1 = t2;  // a24
-            o2 = ((-e14) * e13 + e11 * e34) / vol; c2 = t3;  // a34
+            o0 = (e23 * e26 - e36 * e22) / vol;     // a14
+            o1 = (e13 * e34 - e14 * e33) / vol;     // a24
+            o2 = ((-e14) * e13 + e11 * e34) / vol;  // a34
         }
         // diagonal from the row sum, in the reference's term order
         VT dg;
@@ -479,9 +581,6 @@ __device__ __forceinline__ void tet_row(int64_t i, const int32_t* __restrict__ r
         else if (l == 1) dg = (-o0 - o1) - o2;   // a22 = -a12 - a23 - a24
         else if (l == 2) dg = (-o0 - o1) - o2;   // a33 = -a13 - a23 - a34
         else dg = (-o0 - o1) - o2;               // a44 = -a14 - a24 - a34
-        const int32_t s0 = find_slot(cols, r0, r1, c0);
-        const int32_t s1 = find_slot(cols, r0, r1, c1);
-        const int32_t s2 = find_slot(cols, r0, r1, c2);
         Aacc[s0 - off] += (OT)(o0 / VT(6.0));
         Aacc[s1 - off] += (OT)(o1 / VT(6.0));
         Aacc[s2 - off] += (OT)(o2 / VT(6.0));
@@ -504,28 +603,31 @@ __global__ void __launch_bounds__(256)
 k_tet_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
            const int32_t* __restrict__ diagpos, const int32_t* __restrict__ lrowptr,
            const int32_t* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
-           const int32_t* __restrict__ t, const VT* __restrict__ x, const VT* __restrict__ y,
-           const VT* __restrict__ z, int clamp, const VT* __restrict__ vol_mean,
+           const int4* __restrict__ incv, const VT* __restrict__ xyz, int clamp, const VT* __restrict__ vol_mean,
            OT* __restrict__ Aval, OT* __restrict__ Bval, int32_t* __restrict__ degen, int cap) {
     constexpr bool HAS_B = !LUMP;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     OT* sA = reinterpret_cast<OT*>(lds_raw);
     OT* sB = sA + cap;
+    VT* sx = reinterpret_cast<VT*>(sB + cap);
+    VT* sy = sx + cap;
+    VT* sz = sy + cap;
+    int32_t* sc = reinterpret_cast<int32_t*>(sz + cap);
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
     const int64_t i = i0 + threadIdx.x;
     const int64_t iend = (i0 + blockDim.x < n) ? i0 + blockDim.x : n;
     const int32_t base = rowptr[i0], len = rowptr[iend] - base;
-    if (lds_window_begin<OT, true, HAS_B>(sA, sB, len, cap)) {
+    if (lds_window_begin<VT, OT, true, HAS_B>(sA, sB, sx, sy, sz, sc, cols, xyz, base, len, cap)) {
         if (i < n)
-            tet_row<VT, OT, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, t, x, y, z, clamp, vol_mean,
-                                  RowAcc<OT>{sA, sB, base}, Bval, degen);
+            tet_row<VT, OT, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, incv, xyz, clamp, vol_mean,
+                                  RowAcc<OT>{sA, sB, base}, RowVerts<VT>{sx, sy, sz, sc}, Bval, degen);
         lds_window_end<OT, true, HAS_B>(sA, sB, Aval, Bval, base, len);
     } else if (i < n) {
         const int32_t r0 = rowptr[i], r1 = rowptr[i + 1];
         zero_row(Aval, r0, r1);
         if (HAS_B) zero_row(Bval, r0, r1);
-        tet_row<VT, OT, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, t, x, y, z, clamp, vol_mean,
-                              RowAcc<OT>{Aval, Bval, 0}, Bval, degen);
+        tet_row<VT, OT, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, incv, xyz, clamp, vol_mean,
+                              RowAcc<OT>{Aval, Bval, 0}, RowVerts<VT>{nullptr, nullptr, nullptr, nullptr}, Bval, degen);
     }
 }
 
@@ -638,6 +740,19 @@ void build_topology(lh_ctx* ctx, lh_mesh* m) {
     LH_CHECK_LAUNCH();
     k_inc_sort<<<nblocks(n, 128), 128, 0, ctx->stream>>>(n, m->inc_ptr.as<int32_t>(), m->inc.as<int32_t>());
     LH_CHECK_LAUNCH();
+    m->incv.alloc(cnt * sizeof(int4));
+    k_inc_verts<<<nblocks(cnt, 256), 256, 0, ctx->stream>>>(cnt, nodes, m->inc.as<int32_t>(), t, m->incv.as<int4>());
+    LH_CHECK_LAUNCH();
+    if (m->vdtype == LH_F64) {
+        m->xyz.alloc(m->nv * 4 * sizeof(double));
+        k_aos<double><<<nblocks(m->nv, 256), 256, 0, ctx->stream>>>(m->nv, m->x.as<double>(), m->y.as<double>(),
+                                                                    m->z.as<double>(), m->xyz.as<double>());
+    } else {
+        m->xyz.alloc(m->nv * 4 * sizeof(float));
+        k_aos<float><<<nblocks(m->nv, 256), 256, 0, ctx->stream>>>(m->nv, m->x.as<float>(), m->y.as<float>(),
+                                                                  m->z.as<float>(), m->xyz.as<float>());
+    }
+    LH_CHECK_LAUNCH();
     // rows
     auto p = std::make_shared<Pattern>();
     p->n = n;
@@ -692,27 +807,29 @@ void launch_numeric(lh_ctx* ctx, lh_mesh* m, int kind, int lump, const double* u
     const Pattern& p = *m->full;
     const int64_t n = m->n;
     const int32_t* lrp = m->lumped->rowptr.as<int32_t>();
-    const int bs = 128;
+    const int bs = kind == LH_TET ? 64 : 128;
     const unsigned nb = nblocks(n, bs);
-    // LDS window per matrix: 1.5x the average row length times the rows of a
-    // workgroup, at most 2048 entries (32 KB for A and B in FP64)
+    // LDS window: 1.25x the average row length times the rows of a
+    // workgroup, at most 64 KB (A and B values, three coordinates and the
+    // column index per slot)
     const int64_t avg_row = (p.nnz + n - 1) / n;
-    const int cap = (int)std::min<int64_t>(2048, (avg_row * bs * 3 / 2 + 63) / 64 * 64);
-    const size_t shm = 2 * (size_t)cap * sizeof(OT);
-    auto* x = m->x.as<VT>();
-    auto* y = m->y.as<VT>();
-    auto* z = m->z.as<VT>();
+    const int64_t per_slot = 2 * sizeof(OT) + 3 * sizeof(VT) + sizeof(int32_t);
+    const int64_t cap_max = (65536 / per_slot) / 64 * 64;
+    const int cap = (int)std::min<int64_t>(cap_max, (avg_row * bs * 5 / 4 + 63) / 64 * 64);
+    const size_t shm = (size_t)cap * per_slot;
+    const VT* xyz = m->xyz.as<VT>();
+    const int4* iv = m->incv.as<int4>();
     const int32_t *rp = p.rowptr.as<int32_t>(), *cl = p.cols.as<int32_t>(), *dp = p.diagpos.as<int32_t>();
-    const int32_t *ip = m->inc_ptr.as<int32_t>(), *ic = m->inc.as<int32_t>(), *t = m->t.as<int32_t>();
+    const int32_t *ip = m->inc_ptr.as<int32_t>(), *ic = m->inc.as<int32_t>();
 #define TRIA_LAUNCH(K, L)                                                                          \
-    k_tria_rows<VT, OT, K, L><<<nb, bs, shm, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, t, x, y, z, u1, u2, am, \
-                                                           clamp, vm, Aval, Bval, degen, cap)
+    k_tria_rows<VT, OT, K, L><<<nb, bs, shm, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, iv, xyz, u1, u2, am, clamp, \
+                                                           vm, Aval, Bval, degen, cap)
     if (kind == LH_TRIA) { if (lump) TRIA_LAUNCH(LH_TRIA, true); else TRIA_LAUNCH(LH_TRIA, false); }
     else if (kind == LH_TRIA_ANISO) { if (lump) TRIA_LAUNCH(LH_TRIA_ANISO, true); else TRIA_LAUNCH(LH_TRIA_ANISO, false); }
     else if (kind == LH_TRIA_MASS) { if (lump) TRIA_LAUNCH(LH_TRIA_MASS, true); else TRIA_LAUNCH(LH_TRIA_MASS, false); }
     else {
-        if (lump) k_tet_rows<VT, OT, true><<<nb, bs, shm, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, t, x, y, z, clamp, vm, Aval, Bval, degen, cap);
-        else k_tet_rows<VT, OT, false><<<nb, bs, shm, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, t, x, y, z, clamp, vm, Aval, Bval, degen, cap);
+        if (lump) k_tet_rows<VT, OT, true><<<nb, bs, shm, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, iv, xyz, clamp, vm, Aval, Bval, degen, cap);
+        else k_tet_rows<VT, OT, false><<<nb, bs, shm, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, iv, xyz, clamp, vm, Aval, Bval, degen, cap);
     }
 #undef TRIA_LAUNCH
     LH_CHECK_LAUNCH();

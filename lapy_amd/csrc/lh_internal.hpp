@@ -116,6 +116,9 @@ struct lh_mesh {
     // topology (built lazily)
     int64_t n = -1;                      // max(t)+1
     DevBuf inc_ptr, inc;                 // vertex -> (e<<2 | local), ascending
+    DevBuf incv;                         // per incidence entry: int4 of the element's
+                                         // vertices (triangles: 3 + local index)
+    DevBuf xyz;                          // AoS copy (x, y, z, 0) in the vertex dtype
     std::shared_ptr<Pattern> full;       // consistent pattern
     std::shared_ptr<Pattern> lumped;     // diagonal-of-used-vertices pattern
 };
