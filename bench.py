@@ -60,6 +60,19 @@ def cpu_baseline(level: int, k: int):
     }
 
 
+def profiled_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 summary (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,8 +178,11 @@ def main():
     cheb_bytes = 4 * es * nv * mcols + 8 * nnz + 8 * nv
     ach_c = cheb_bytes / (us_c * 1e-6) / 1e9
     kname = "k_cheb_step_h (FP16 storage)" if half else "k_cheb_step_v (FP32 storage)"
+    # traffic: measured by rocprofv3 on exactly this configuration (L8, m = 64, FP16 storage)
+    profiled = args.level == 8 and mcols == 64 and half
     roofline = {"bound": "hbm", "achieved": round(ach_c, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach_c / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach_c / HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic("cheb_step_h_l8_m64_traffic.txt") if profiled else None,
                 "kernel": f"{kname}, Chebyshev-Jacobi preconditioner step, m={mcols}",
                 "avg_us": round(us_c, 2), "algorithmic_bytes": int(cheb_bytes)}
     # the metric's second figure: numeric assembly
@@ -174,7 +190,8 @@ def main():
     alg_bytes = 24 * nv + 12 * nf + 16 * nnz  # v (FP64 x3), t (int32 x3), A and B values (FP64)
     achieved = alg_bytes / (us * 1e-6) / 1e9
     roofline_asm = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": profiled_traffic("tria_rows_l8_traffic.txt") if args.level == 8 else None,
                     "kernel": "k_tria_rows (numeric assembly, FP64)", "avg_us": round(us, 2),
                     "algorithmic_bytes": int(alg_bytes)}
 

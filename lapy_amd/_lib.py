@@ -109,7 +109,7 @@ SIGNATURES = {
     "lh_block_gram": (C.c_int, [_vp, C.c_int64, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
                                 _dp]),
     "lh_block_gemm": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.c_double,
-                                C.c_double, C.c_int, _dp]),
+                                C.c_double, C.c_int, C.c_int, _dp]),
 }
 
 _lib = None
@@ -386,19 +386,32 @@ def block_gram(ctx: Context, U: np.ndarray, V: np.ndarray, p: int, q: int, sym: 
     return out, (us.value if iters else None)
 
 
-def block_gemm(ctx: Context, U: np.ndarray, p: int, Cm: np.ndarray, Y: np.ndarray, alpha=1.0, beta=0.0,
-               iters: int = 0):
+LH_GEMM_UPPER, LH_GEMM_INPLACE = 1, 2
+
+
+def block_gemm(ctx: Context, U, p: int, Cm: np.ndarray, Y: np.ndarray, alpha=1.0, beta=0.0, iters: int = 0,
+               upper: bool = False, inplace: bool = False):
     """Y[:, :q] = alpha U[:, :p] Cm + beta Y[:, :q] on the GPU (FP64 matrix
-    cores).  Returns (Y, average microseconds or None)."""
-    U = np.ascontiguousarray(U, dtype=np.float64)
+    cores); inplace: Y[:, :p] = alpha Y[:, :p] Cm (U is ignored); upper: Cm is
+    upper triangular.  Returns (Y, average microseconds or None)."""
     Cm = np.ascontiguousarray(Cm, dtype=np.float64)
     Y = np.array(Y, dtype=np.float64, order="C", copy=True)
     q = Cm.shape[1]
-    if Cm.shape[0] != p or U.shape[0] != Y.shape[0]:
+    if inplace:
+        U = None
+        if p != q:
+            raise ValueError("in-place update needs a square C")
+    else:
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape[0] != Y.shape[0]:
+            raise ValueError("shape mismatch")
+    if Cm.shape[0] != p:
         raise ValueError("shape mismatch")
+    flags = (LH_GEMM_UPPER if upper else 0) | (LH_GEMM_INPLACE if inplace else 0)
     us = C.c_double(0.0)
-    check(lib().lh_block_gemm(ctx.handle, U.shape[0], p, q, _ptr(U), U.shape[1], _ptr(Cm), _ptr(Y), Y.shape[1],
-                              float(alpha), float(beta), iters, C.byref(us) if iters else None), ctx)
+    check(lib().lh_block_gemm(ctx.handle, Y.shape[0], p, q, _ptr(U), U.shape[1] if U is not None else 0, _ptr(Cm),
+                              _ptr(Y), Y.shape[1], float(alpha), float(beta), flags, iters,
+                              C.byref(us) if iters else None), ctx)
     return Y, (us.value if iters else None)
 
 

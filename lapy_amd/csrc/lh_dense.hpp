@@ -18,7 +18,10 @@ void gemm_nn(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const do
 void gram_mfma(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q, double* C,
                int ldc, DevBuf& ws, double alpha, bool sym);
 void gemm_mfma(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* C, int ldc, double* Y,
-               int ldy, double alpha, double beta);
+               int ldy, double alpha, double beta, bool upper = false);
+// X (n x p, p <= 256) <- alpha X C in place; upper: C upper triangular
+void gemm_mfma_inplace(lh_ctx* ctx, int64_t n, int p, double* X, int ldx, const double* C, int ldc, double alpha,
+                       bool upper);
 size_t rr_work_doubles(int N, int nev);
 // standard symmetric Rayleigh-Ritz (orthonormal basis), LDS-resident, N <= kRRStdMaxN
 constexpr int kRRStdMaxN = 192;

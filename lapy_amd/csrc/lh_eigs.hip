@@ -49,7 +49,7 @@ struct Lobpcg {
     std::unique_ptr<lh_csr> Ap, Bp;
     bool reordered = false;
     uint64_t seed;
-    DevBuf S, S2, AS, MS, tmp, dinv;
+    DevBuf S, S2, AS, MS, dinv;
     DevBuf GA, GM, Cm, Gs, Ts, theta_d, rrwork, status, cs1, cs2, wsg, wsc;
     std::vector<double> theta, res;
     lh_stats st{};
@@ -86,11 +86,9 @@ struct Lobpcg {
             double stv[2];
             d2h(ctx, stv, status.p, sizeof(stv));
             if (reinterpret_cast<int*>(stv)[0]) return false;
-            gemm_nn<double>(ctx, n, c, c, X, ld, Ts.as<double>(), c, tmp.as<double>(), c, 1.0, 0.0);
-            copy_block<double, double>(ctx, n, c, tmp.as<double>(), c, X, ld);
+            gemm_mfma_inplace(ctx, n, c, X, ld, Ts.as<double>(), c, 1.0, true);  // T is upper triangular
             if (stv[1] > 1e-2) break;
-            gemm_nn<double>(ctx, n, c, c, MX, ld, Ts.as<double>(), c, tmp.as<double>(), c, 1.0, 0.0);
-            copy_block<double, double>(ctx, n, c, tmp.as<double>(), c, MX, ld);
+            gemm_mfma_inplace(ctx, n, c, MX, ld, Ts.as<double>(), c, 1.0, true);
         }
         if (recompute_A) spmm_AB(X, ASp() + off, MX, c);
         else spmm_B(X, MX, c);
@@ -151,7 +149,6 @@ struct Lobpcg {
         S2.alloc(n * ld * 8);
         AS.alloc(n * ld * 8);
         MS.alloc(n * ld * 8);
-        tmp.alloc(n * 2 * m * 8);
         dinv.alloc(n * 8);
         const int N3 = 3 * m;
         GA.alloc(N3 * N3 * 8);
@@ -175,8 +172,7 @@ struct Lobpcg {
         spmm_B(Sp(), MSp(), m);
         if (!cholqr(0, m, true)) LH_THROW(LH_ERR_NUMERIC, "start block is rank deficient");
         if (!rayleigh_ritz(m)) LH_THROW(LH_ERR_NUMERIC, "initial Rayleigh-Ritz failed");
-        gemm_nn<double>(ctx, n, m, m, Sp(), ld, Cm.as<double>(), m, tmp.as<double>(), m, 1.0, 0.0);
-        copy_block<double, double>(ctx, n, m, tmp.as<double>(), m, Sp(), ld);
+        gemm_mfma_inplace(ctx, n, m, Sp(), ld, Cm.as<double>(), m, 1.0, false);
         spmm_AB(Sp(), ASp(), MSp(), m);
         st.t_setup_us = now_us() - t_start;
 
@@ -202,7 +198,7 @@ struct Lobpcg {
             }
             st.max_resid = worst;
             st.converged = nconv;
-            if (verbose && (it % 50 == 0))
+            if (verbose > 1 || (verbose && it % 50 == 0))
                 fprintf(stderr, "[lobpcg] it %d conv %d/%d worst %.3e theta0 %.6e theta_k %.6e\n", it, nconv, k,
                         worst, theta[0], theta[k - 1]);
             if (nconv == k) break;
@@ -248,8 +244,7 @@ struct Lobpcg {
         // eigenvalues are Rayleigh quotients of the returned vectors
         spmm_B(Sp(), MSp(), m);
         if (cholqr(0, m, true) && rayleigh_ritz(m)) {
-            gemm_nn<double>(ctx, n, m, m, Sp(), ld, Cm.as<double>(), m, tmp.as<double>(), m, 1.0, 0.0);
-            copy_block<double, double>(ctx, n, m, tmp.as<double>(), m, Sp(), ld);
+            gemm_mfma_inplace(ctx, n, m, Sp(), ld, Cm.as<double>(), m, 1.0, false);
         }
         for (int j = 0; j < k; ++j) out_evals[j] = theta[j];
         out_vecs.alloc(n * k * 8);

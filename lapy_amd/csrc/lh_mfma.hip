@@ -143,27 +143,40 @@ k_gram_mfma_reduce(int nwaves, int ntile, int p, int q, int tv, int sym, const d
     }
 }
 
-template <bool VEC>
+// Y = alpha U C + beta Y.  upper: C is upper triangular (rows k > j of column
+// j are zero), so a 64-column block only sums k < its last column + 1.
+// INPLACE (Y = U, p = q <= 256): a workgroup owns all column blocks of its
+// rows, and writes only after every wave of it has read its rows of U.
+template <bool VEC, bool INPLACE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-k_gemm_mfma(int64_t n, int p, int q, const double* __restrict__ U, int ldu, const double* __restrict__ Cm, int ldc,
-            double* __restrict__ Y, int ldy, double alpha, double beta) {
-    const int lane = threadIdx.x & 63;
+k_gemm_mfma(int64_t n, int p, int q, const double* U, int ldu, const double* __restrict__ Cm, int ldc, double* Y,
+            int ldy, double alpha, double beta, int upper, int nq) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int il = lane & 15, kg = lane >> 4;
-    const int64_t rw = (int64_t)xcd_block() * 256 + (threadIdx.x >> 6) * 64;  // first row of this wave
-    if (rw >= n) return;
-    const int n0 = blockIdx.y * 64;
+    int64_t rw;  // first row of this wave
+    int n0;      // first column of this wave
+    if (INPLACE) {
+        const int rsub = (int)(blockDim.x >> 6) / nq;
+        rw = xcd_block() * 64 * rsub + (int64_t)(wv / nq) * 64;
+        n0 = (wv % nq) * 64;
+    } else {
+        rw = xcd_block() * 256 + (int64_t)wv * 64;
+        n0 = blockIdx.y * 64;
+        if (rw >= n) return;  // whole wave
+    }
+    const int kend = upper ? min(p, n0 + 64) : p;
     d4_t acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = (d4_t){0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < p; k0 += 16) {
+    for (int k0 = 0; k0 < kend; k0 += 16) {
         const int kc = k0 + 4 * kg;  // this lane's 4 summed indices
         d4_t a[4];
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi) {
             const int64_t r = rw + 16 * bi + il;
-            a[bi] = (r < n) ? load4<VEC>(U + r * ldu, kc, p) : (d4_t){0.0, 0.0, 0.0, 0.0};
+            a[bi] = (r < n) ? load4<VEC>(U + r * ldu, kc, kend) : (d4_t){0.0, 0.0, 0.0, 0.0};
         }
         double b[4][4];
 #pragma unroll
@@ -171,7 +184,7 @@ k_gemm_mfma(int64_t n, int p, int q, const double* __restrict__ U, int ldu, cons
 #pragma unroll
             for (int bj = 0; bj < 4; ++bj) {
                 const int col = n0 + 16 * bj + il;
-                b[jj][bj] = (kc + jj < p && col < q) ? Cm[(int64_t)(kc + jj) * ldc + col] : 0.0;
+                b[jj][bj] = (kc + jj < kend && col < q) ? Cm[(int64_t)(kc + jj) * ldc + col] : 0.0;
             }
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj)
@@ -180,6 +193,7 @@ k_gemm_mfma(int64_t n, int p, int q, const double* __restrict__ U, int ldu, cons
 #pragma unroll
                 for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = mfma16(a[bi][jj], b[jj][bj], acc[bi][bj]);
     }
+    if (INPLACE) __syncthreads();
 #pragma unroll
     for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
@@ -229,14 +243,33 @@ void gram_mfma(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const do
 }
 
 void gemm_mfma(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* C, int ldc, double* Y,
-               int ldy, double alpha, double beta) {
+               int ldy, double alpha, double beta, bool upper) {
     if (n == 0 || q == 0) return;
     const bool vec = aligned16(U) && ldu % 2 == 0;
-    dim3 grid(nblocks_xcd(n, 256), (unsigned)((q + 63) / 64));
+    const int nq = (q + 63) / 64;
+    dim3 grid(nblocks_xcd(n, 256), (unsigned)nq);
+    const int up = upper ? 1 : 0;
     if (vec)
-        k_gemm_mfma<true><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta);
+        k_gemm_mfma<true, false><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta, up, nq);
     else
-        k_gemm_mfma<false><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta);
+        k_gemm_mfma<false, false><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta, up, nq);
+    LH_CHECK_LAUNCH();
+}
+
+void gemm_mfma_inplace(lh_ctx* ctx, int64_t n, int p, double* X, int ldx, const double* C, int ldc, double alpha,
+                       bool upper) {
+    if (n == 0 || p == 0) return;
+    if (p > 256) LH_THROW(LH_ERR_ARG, "in-place block update needs width <= 256, got %d", p);
+    const bool vec = aligned16(X) && ldx % 2 == 0;
+    const int nq = (p + 63) / 64;
+    const int rsub = nq == 1 ? 4 : (nq == 2 ? 2 : 1);
+    const int bs = 64 * nq * rsub;
+    const unsigned nb = nblocks_xcd(n, 64 * rsub);
+    const int up = upper ? 1 : 0;
+    if (vec)
+        k_gemm_mfma<true, true><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0, up, nq);
+    else
+        k_gemm_mfma<false, true><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0, up, nq);
     LH_CHECK_LAUNCH();
 }
 
@@ -278,24 +311,31 @@ int lh_block_gram(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const
 }
 
 int lh_block_gemm(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* Cs, double* Y,
-                  int ldy, double alpha, double beta, int iters, double* avg_us) {
-    if (!ctx || !U || !Cs || !Y || n <= 0 || p <= 0 || q <= 0 || ldu < p || ldy < q) return LH_ERR_ARG;
+                  int ldy, double alpha, double beta, int flags, int iters, double* avg_us) {
+    if (!ctx || !Cs || !Y || n <= 0 || p <= 0 || q <= 0 || ldy < q) return LH_ERR_ARG;
     if (p > 1024 || q > 1024) return LH_ERR_ARG;
+    const bool upper = (flags & LH_GEMM_UPPER) != 0, inplace = (flags & LH_GEMM_INPLACE) != 0;
+    if (inplace && (p != q || p > 256 || beta != 0.0)) return LH_ERR_ARG;
+    if (!inplace && (!U || ldu < p)) return LH_ERR_ARG;
     LH_API_BEGIN(ctx)
     HIPX(hipSetDevice(ctx->device));
-    DevBuf du(n * ldu * 8), dy(n * ldy * 8), dc((size_t)p * q * 8);
-    h2d(ctx, du.p, U, n * ldu * 8);
+    DevBuf du(inplace ? 8 : n * ldu * 8), dy(n * ldy * 8), dc((size_t)p * q * 8);
+    if (!inplace) h2d(ctx, du.p, U, n * ldu * 8);
     h2d(ctx, dy.p, Y, n * ldy * 8);
     h2d(ctx, dc.p, Cs, (size_t)p * q * 8);
-    gemm_mfma(ctx, n, p, q, du.as<double>(), ldu, dc.as<double>(), q, dy.as<double>(), ldy, alpha, beta);
+    if (inplace) gemm_mfma_inplace(ctx, n, p, dy.as<double>(), ldy, dc.as<double>(), q, alpha, upper);
+    else gemm_mfma(ctx, n, p, q, du.as<double>(), ldu, dc.as<double>(), q, dy.as<double>(), ldy, alpha, beta, upper);
     d2h(ctx, Y, dy.p, n * ldy * 8);
     if (avg_us && iters > 0) {
         hipEvent_t e0, e1;
         HIPX(hipEventCreate(&e0));
         HIPX(hipEventCreate(&e1));
         HIPX(hipEventRecord(e0, ctx->stream));
-        for (int it = 0; it < iters; ++it)
-            gemm_mfma(ctx, n, p, q, du.as<double>(), ldu, dc.as<double>(), q, dy.as<double>(), ldy, alpha, 0.0);
+        for (int it = 0; it < iters; ++it) {
+            if (inplace) gemm_mfma_inplace(ctx, n, p, dy.as<double>(), ldy, dc.as<double>(), q, alpha, upper);
+            else gemm_mfma(ctx, n, p, q, du.as<double>(), ldu, dc.as<double>(), q, dy.as<double>(), ldy, alpha, 0.0,
+                           upper);
+        }
         HIPX(hipEventRecord(e1, ctx->stream));
         HIPX(hipEventSynchronize(e1));
         float ms = 0.f;

@@ -1,23 +1,24 @@
 // Rayleigh-Ritz eigensolver for an orthonormal trial basis: the smallest nev
-// eigenpairs of a small dense symmetric H (N <= 192), FP64, one workgroup of
-// 1024 threads, working set in LDS.
+// eigenpairs of a small dense symmetric H (N <= 192), FP64.
 //
-//  1. the lower triangle of H is held packed in LDS (N(N+1)/2 doubles,
-//     148 KB at N = 192) and reduced to tridiagonal form by Householder
-//     reflections; every row of the trailing matrix is owned by four lanes,
-//     so the symmetric matvec and the rank-2 update walk contiguous packed
-//     rows without index arithmetic in the inner loops;
-//  2. the reflectors are parked in global memory, freeing LDS for Z;
-//  3. the wanted eigenvalues of the tridiagonal T are found by multisection:
-//     16 lanes per eigenvalue evaluate Sturm counts at 16 interior points,
-//     shrinking the bracket 17-fold per round;
-//  4. their eigenvectors by inverse iteration (LU with partial pivoting of
-//     T - lambda I, per-thread scratch in global memory); numerically equal
-//     eigenvalues are handled by one thread with Gram-Schmidt inside the group;
-//  5. Z := Q Z applies the reflectors in reverse order with Z in LDS, the next
-//     reflector prefetched into registers while the current one is applied;
-//  6. a Cholesky-QR of Z in the small space restores orthonormality to
-//     rounding level (inverse iteration leaves errors ~ eps ||T|| / gap).
+// Five kernels on the library stream:
+//  1. k_rr_tri (one workgroup, 1024 threads): the lower triangle of H is held
+//     packed in LDS (N(N+1)/2 doubles, 148 KB at N = 192) and reduced to
+//     tridiagonal form by Householder reflections, three barriers per step.
+//     The reflectors, the tridiagonal T and its Gershgorin bounds are parked
+//     in global memory.
+//  2. k_rr_bisect (one wave per wanted eigenvalue): multisection on T, the
+//     64 lanes evaluating Sturm counts at 64 interior points (65-fold
+//     bracket reduction per round).
+//  3. k_rr_invit (one wave per eigenvalue; only the first of a group of
+//     numerically equal eigenvalues works): inverse iteration with the LU
+//     factors of T - lambda I in LDS, Gram-Schmidt inside the group.
+//  4. k_rr_back (one wave per eigenvector): Z := Q Y, the reflectors applied
+//     in reverse order, the next one loaded while the current is applied.
+//  5. k_rr_polish (one workgroup): Newton-Schulz orthonormalisation of Z in
+//     the small space, Z := Z (3 I - Z^T Z) / 2 (inverse iteration leaves
+//     orthogonality errors ~ eps ||T|| / gap; each step squares them; up to
+//     three steps, none for a basis with ||Z^T Z - I|| >= 1/2).
 //
 // Replaces the dense Rayleigh-Ritz step ARPACK performs inside eigsh
 // (lapy/solver.py:713-715) for the block solver.
@@ -51,8 +52,25 @@ __device__ double bsum(double v, double* red) {
 }
 
 __device__ __forceinline__ void stamp(unsigned long long* prof, int slot) {
-    if (prof && threadIdx.x == 0) prof[slot] = wall_clock64();
+    if (prof && threadIdx.x == 0 && blockIdx.x == 0) prof[slot] = wall_clock64();
 }
+
+// work layout (doubles): reflectors R[k][i] (N x N), d (N), e (N), tau (N),
+// bounds (4: lo, hi, ||T||, -), lam (nev), Y (nev x N, column g contiguous),
+// Z (N x nev row-major)
+struct RRWork {
+    double *R, *d, *e, *tau, *bnd, *lam, *Y, *Z;
+    __host__ __device__ RRWork(double* w, int N, int nev) {
+        R = w;
+        d = R + (size_t)N * N;
+        e = d + N;
+        tau = e + N;
+        bnd = tau + N;
+        lam = bnd + 4;
+        Y = lam + nev;
+        Z = Y + (size_t)N * nev;
+    }
+};
 
 __device__ int sturm(const double* d, const double* e2, int N, double x, double pivmin) {
     int cnt = 0;
@@ -96,8 +114,8 @@ __device__ void tri_factor(const double* d, const double* e, int N, double lam, 
     u0[N - 1] = (fabs(c0) > tiny) ? c0 : tiny;
 }
 
-// solve with the factorisation; y (stride ys) overwritten
-__device__ void tri_lu_solve(const double* ws, int N, double* y, int ys) {
+// solve with the factorisation; y overwritten
+__device__ void tri_lu_solve(const double* ws, int N, double* y) {
     const double* u0 = ws;
     const double* u1 = ws + N;
     const double* u2 = ws + 2 * N;
@@ -105,93 +123,129 @@ __device__ void tri_lu_solve(const double* ws, int N, double* y, int ys) {
     const double* pv = ws + 4 * N;
     double a = y[0];
     for (int i = 0; i < N - 1; ++i) {
-        double b = y[(i + 1) * ys];
+        double b = y[i + 1];
         if (pv[i] != 0.0) { double t = a; a = b; b = t; }
-        y[i * ys] = a;
+        y[i] = a;
         a = b - lm[i] * a;
     }
     double x2 = a / u0[N - 1];
-    y[(N - 1) * ys] = x2;
+    y[N - 1] = x2;
     if (N >= 2) {
-        double x1 = (y[(N - 2) * ys] - u1[N - 2] * x2) / u0[N - 2];
-        y[(N - 2) * ys] = x1;
+        double x1 = (y[N - 2] - u1[N - 2] * x2) / u0[N - 2];
+        y[N - 2] = x1;
         for (int i = N - 3; i >= 0; --i) {
-            double x0 = (y[i * ys] - u1[i] * x1 - u2[i] * x2) / u0[i];
-            y[i * ys] = x0;
+            double x0 = (y[i] - u1[i] * x1 - u2[i] * x2) / u0[i];
+            y[i] = x0;
             x2 = x1;
             x1 = x0;
         }
     }
 }
 
-__global__ void __launch_bounds__(NB)
-k_rr_std(int N, int nev, const double* __restrict__ G, int ldg, double* __restrict__ C, double* __restrict__ theta,
-         double* __restrict__ work, int* __restrict__ status, unsigned long long* __restrict__ prof) {
-    __shared__ double H[PK];  // packed lower triangle of H; later Z (N x nev) and the small Gram
-    __shared__ double sv[NMAX], sp[NMAX], sd[NMAX], se[NMAX], se2[NMAX], stau[NMAX];
-    __shared__ double red[32];
-    const int tid = threadIdx.x;
-    const int rl = tid >> 2, gl = tid & 3;  // row / lane-in-row for the 4-lanes-per-row loops
-    stamp(prof, 0);
+// ---- 1. tridiagonalisation ------------------------------------------------
+// The packed lower triangle of H lives in LDS.  Step k needs three barriers:
+// every wave reads column k (below the diagonal) straight from the packed
+// matrix and forms the reflector's norm, alpha, v0 and tau itself (the same
+// summation order in every wave, so all agree) while wave 0 publishes v
+// contiguously and parks it in global memory (barrier 1); the matvec
+// p = tau H22 v is split over row groups (4 lanes per trailing row while the
+// trailing block is long, up to 32 when it is short) and published
+// (barrier 2); every wave forms K = tau (p . v) / 2 itself, and the rank-2
+// update H22 -= v w^T + w v^T with w = p - K v runs on the packed rows
+// (barrier 3).
+__device__ __forceinline__ int lanes_per_row(int L) { return L > 128 ? 4 : (L > 64 ? 8 : (L > 32 ? 16 : 32)); }
 
-    // 1. load (symmetrised) packed lower triangle, 4 lanes per row
-    for (int i = rl; i < N; i += NB / 4) {
-        const int b = rowbase(i);
-        for (int j = gl; j <= i; j += 4) H[b + j] = 0.5 * (G[(int64_t)i * ldg + j] + G[(int64_t)j * ldg + i]);
+__global__ void __launch_bounds__(NB)
+k_rr_tri(int N, int nev, const double* __restrict__ G, int ldg, double* __restrict__ work,
+         unsigned long long* __restrict__ prof) {
+    __shared__ double H[PK];  // packed lower triangle of H
+    __shared__ double sv[NMAX], sp[NMAX], sd[NMAX], se[NMAX], stau[NMAX];
+    RRWork W(work, N, nev);
+    const int tid = threadIdx.x, lane = tid & 63;
+    stamp(prof, 0);
+    {
+        const int rl = tid >> 2, gl = tid & 3;
+        for (int i = rl; i < N; i += NB / 4) {
+            const int b = rowbase(i);
+            for (int j = gl; j <= i; j += 4) H[b + j] = 0.5 * (G[(int64_t)i * ldg + j] + G[(int64_t)j * ldg + i]);
+        }
     }
     __syncthreads();
     stamp(prof, 1);
-
-    // 2. Householder tridiagonalisation
     for (int k = 0; k < N - 2; ++k) {
         const int L = N - k - 1;
+        // column k below the diagonal: x_c = H[k+1+c][k]
+        auto colx = [&](int c) -> double { return H[rowbase(k + 1 + c) + k]; };
         double part = 0.0;
-        for (int i = tid; i < L; i += NB) {
-            const double x = H[rowbase(k + 1 + i) + k];
-            sv[i] = x;
+        for (int c = lane; c < L; c += 64) {
+            const double x = colx(c);
             part += x * x;
         }
-        const double xn2 = bsum(part, red);
-        const double x0 = sv[0];
+        const double xn2 = wsum(part);
+        const double x0 = colx(0);
+        if (tid == 0) sd[k] = H[rowbase(k) + k];
         if (xn2 == 0.0) {
-            if (tid == 0) { sd[k] = H[rowbase(k) + k]; se[k] = 0.0; stau[k] = 0.0; }
-            __syncthreads();
-            continue;
+            if (tid == 0) { se[k] = 0.0; stau[k] = 0.0; }
+            continue;  // H22 untouched: no barrier needed
         }
         const double alpha = (x0 >= 0.0) ? -sqrt(xn2) : sqrt(xn2);
         const double v0 = x0 - alpha;
         const double t = 2.0 / (xn2 - x0 * x0 + v0 * v0);
+        if (tid == 0) { se[k] = alpha; stau[k] = t; }
+        // wave 0 publishes v contiguously (and parks it in global memory)
+        if (tid < 64)
+            for (int c = lane; c < L; c += 64) {
+                const double v = (c == 0) ? v0 : colx(c);
+                sv[c] = v;
+                W.R[(int64_t)k * N + c] = v;
+            }
         __syncthreads();
-        if (tid == 0) {
-            sv[0] = v0;
-            H[rowbase(k + 1) + k] = v0;
-            sd[k] = H[rowbase(k) + k];
-            se[k] = alpha;
-            stau[k] = t;
-        }
-        __syncthreads();
+        auto vof = [&](int c) -> double { return sv[c]; };
+        const int lpr = lanes_per_row(L);
+        const int r = tid / lpr, gl = tid % lpr;  // NB / lpr >= L row groups
         // p = t * H22 v ; row r of H22 = packed row i = k+1+r (columns <= i
         // contiguous) plus column i below it (entries j > i, stride j)
-        for (int r = rl; r < L; r += NB / 4) {
-            const int i = k + 1 + r;
-            const double* hr = H + rowbase(i) + (k + 1);
-            double s = 0.0;
-            for (int c = gl; c <= r; c += 4) s += hr[c] * sv[c];
-            for (int c = r + 1 + gl; c < L; c += 4) s += H[rowbase(k + 1 + c) + i] * sv[c];
-            s += __shfl_xor(s, 1, 64);
-            s += __shfl_xor(s, 2, 64);
-            if (gl == 0) sp[r] = t * s;
+        {
+            double s0 = 0.0, s1 = 0.0;
+            if (r < L) {
+                const int i = k + 1 + r;
+                const double* hr = H + rowbase(i) + (k + 1);
+                int c = gl;
+                for (; c + lpr <= r; c += 2 * lpr) {
+                    s0 += hr[c] * vof(c);
+                    s1 += hr[c + lpr] * vof(c + lpr);
+                }
+                if (c <= r) s0 += hr[c] * vof(c);
+                c = r + 1 + gl;
+                for (; c + lpr < L; c += 2 * lpr) {
+                    s0 += H[rowbase(k + 1 + c) + i] * vof(c);
+                    s1 += H[rowbase(k + 1 + c + lpr) + i] * vof(c + lpr);
+                }
+                if (c < L) s0 += H[rowbase(k + 1 + c) + i] * vof(c);
+            }
+            double s = s0 + s1;
+            for (int off = lpr >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            if (r < L && gl == 0) sp[r] = t * s;
         }
         __syncthreads();
-        double pp = 0.0;
-        for (int i = tid; i < L; i += NB) pp += sp[i] * sv[i];
-        const double K = 0.5 * t * bsum(pp, red);
-        for (int i = tid; i < L; i += NB) sp[i] -= K * sv[i];
-        __syncthreads();
-        for (int r = rl; r < L; r += NB / 4) {
+        double pv = 0.0;
+        for (int c = lane; c < L; c += 64) pv += sp[c] * vof(c);
+        const double K = 0.5 * t * wsum(pv);
+        if (r < L) {
             double* hr = H + rowbase(k + 1 + r) + (k + 1);
-            const double vr = sv[r], wr = sp[r];
-            for (int c = gl; c <= r; c += 4) hr[c] -= vr * sp[c] + wr * sv[c];
+            const double vr = vof(r), wr = sp[r] - K * vr;
+            int c = gl;
+            for (; c + lpr <= r; c += 2 * lpr) {
+                const double va = vof(c), vb = vof(c + lpr);
+                const double wa = sp[c] - K * va, wb = sp[c + lpr] - K * vb;
+                const double h0 = hr[c], h1 = hr[c + lpr];
+                hr[c] = h0 - (vr * wa + wr * va);
+                hr[c + lpr] = h1 - (vr * wb + wr * vb);
+            }
+            if (c <= r) {
+                const double va = vof(c), wa = sp[c] - K * va;
+                hr[c] -= vr * wa + wr * va;
+            }
         }
         __syncthreads();
     }
@@ -203,14 +257,14 @@ k_rr_std(int N, int nev, const double* __restrict__ G, int ldg, double* __restri
         }
         sd[N - 1] = H[rowbase(N - 1) + N - 1];
         stau[N - 1] = 0.0;
+        se[N - 1] = 0.0;
     }
-    stamp(prof, 2);
-    // 3. park reflectors in global memory: R[k][i] = v_k[i]
-    double* R = work;
-    for (int k = rl; k < N - 2; k += NB / 4)
-        for (int i = gl; i < N - k - 1; i += 4) R[(int64_t)k * N + i] = H[rowbase(k + 1 + i) + k];
     __syncthreads();
-    for (int i = tid; i < N - 1; i += NB) se2[i] = se[i] * se[i];
+    for (int i = tid; i < N; i += NB) {
+        W.d[i] = sd[i];
+        W.e[i] = se[i];
+        W.tau[i] = stau[i];
+    }
     if (tid == 0) {
         double lo = sd[0], hi = sd[0], nrm = 0.0;
         for (int i = 0; i < N; ++i) {
@@ -219,170 +273,219 @@ k_rr_std(int N, int nev, const double* __restrict__ G, int ldg, double* __restri
             hi = fmax(hi, sd[i] + rr);
             nrm = fmax(nrm, fabs(sd[i]) + rr);
         }
-        red[0] = lo;
-        red[1] = hi;
-        red[2] = fmax(nrm, 1e-300);
+        W.bnd[0] = lo;
+        W.bnd[1] = hi;
+        W.bnd[2] = fmax(nrm, 1e-300);
     }
-    __syncthreads();
-    const double glo = red[0], ghi = red[1], tnorm = red[2];
-    const double pivmin = 1e-290 + 1e-300 * tnorm;
-    // 4. multisection: 16 lanes per eigenvalue, 64 eigenvalues per pass
-    double* lam = sp;  // reuse
-    for (int base = 0; base < nev; base += NB / 16) {
-        const int j = base + tid / 16, q = tid % 16;
-        double a = glo - 1e-14 * tnorm, b = ghi + 1e-14 * tnorm;
-        for (int it = 0; it < 40; ++it) {
-            const double h = (b - a) / 17.0;
-            const double x = a + (q + 1) * h;
-            const bool pred = (j < nev) && (sturm(sd, se2, N, x, pivmin) > j);
-            const unsigned long long bal = __ballot(pred);
-            const unsigned grp = (unsigned)((bal >> (16 * ((tid & 63) >> 4))) & 0xFFFFull);
-            const int first = grp ? (__ffs(grp) - 1) : 16;
-            const double nb = (first < 16) ? a + (first + 1) * h : b;
-            const double na = (first > 0) ? a + first * h : a;
-            a = na;
-            b = nb;
-            if (b - a <= 2.2e-16 * (fabs(a) + fabs(b)) + 1e-17 * tnorm) break;
-        }
-        if (j < nev && q == 0) lam[j] = 0.5 * (a + b);
-    }
-    __syncthreads();
+    stamp(prof, 2);
+}
+
+// ---- 2. eigenvalues by multisection ----------------------------------------
+__global__ void __launch_bounds__(64)
+k_rr_bisect(int N, int nev, double* __restrict__ work, double* __restrict__ theta,
+            unsigned long long* __restrict__ prof) {
+    __shared__ double sd[NMAX], se2[NMAX];
     stamp(prof, 3);
-    // 5. inverse iteration; Z (N x nev) lives in LDS (H is free now).  The
-    // eigenvalue threads are spread over the waves (4 per wave) so that the
-    // latency of the scalar recurrences is hidden across waves.
-    double* Z = H;
-    const double tiny = 2.2e-16 * tnorm;
-    {
-        const int w = tid >> 6, l = tid & 63;
-        const int j = (l < 4) ? (w * 4 + l) : -1;  // 64 eigenvalue slots per pass
-        for (int jb = 0; jb < nev; jb += 64) {
-            const int jj = (j >= 0) ? jb + j : nev;
-            if (jj < nev) {
-                const double scale_j = fmax(fabs(lam[jj]), 1e-6 * tnorm);
-                const bool leader = (jj == 0) || (lam[jj] - lam[jj - 1] > 1e-7 * scale_j);
-                if (leader) {
-                    int gend = jj + 1;
-                    while (gend < nev && lam[gend] - lam[gend - 1] <= 1e-7 * fmax(fabs(lam[gend]), 1e-6 * tnorm))
-                        ++gend;
-                    double* ws = work + (int64_t)N * N + (int64_t)jj * 5 * N;
-                    for (int g = jj; g < gend; ++g) {
-                        double* y = Z + g;  // column g, stride nev
-                        for (int i = 0; i < N; ++i) y[i * nev] = 1.0 + 0.1 * (double)((7 * i + 3 * g) % 11);
-                        tri_factor(sd, se, N, lam[g] + (double)(g - jj) * 8.0 * tiny, tiny, ws);
-                        for (int itr = 0; itr < 2; ++itr) {
-                            tri_lu_solve(ws, N, y, nev);
-                            for (int h = jj; h < g; ++h) {
-                                double s = 0.0;
-                                for (int i = 0; i < N; ++i) s += y[i * nev] * Z[i * nev + h];
-                                for (int i = 0; i < N; ++i) y[i * nev] -= s * Z[i * nev + h];
-                            }
-                            double nn = 0.0;
-                            for (int i = 0; i < N; ++i) nn += y[i * nev] * y[i * nev];
-                            nn = 1.0 / sqrt(fmax(nn, 1e-300));
-                            for (int i = 0; i < N; ++i) y[i * nev] *= nn;
-                        }
-                    }
-                }
-            }
-        }
+    RRWork W(work, N, nev);
+    const int j = blockIdx.x, q = threadIdx.x;
+    for (int i = q; i < N; i += 64) {
+        sd[i] = W.d[i];
+        se2[i] = (i < N - 1) ? W.e[i] * W.e[i] : 0.0;
     }
     __syncthreads();
-    for (int j = tid; j < nev; j += NB) theta[j] = lam[j];
-    stamp(prof, 4);
-    // 6. Z := Q Z, reflectors k = N-3 .. 0; sv holds the current reflector,
-    // the next one is loaded into a register before the current is applied
-    {
-        int k = N - 3;
-        double nxt = (k >= 0 && tid < N - k - 1) ? R[(int64_t)k * N + tid] : 0.0;
-        for (; k >= 0; --k) {
-            const int L = N - k - 1;
-            if (tid < L) sv[tid] = nxt;
-            __syncthreads();
-            nxt = (k - 1 >= 0 && tid < N - k) ? R[(int64_t)(k - 1) * N + tid] : 0.0;
-            const double t = stau[k];
-            if (t != 0.0) {
-                for (int c0 = 0; c0 < nev; c0 += NB / 16) {
-                    const int c = c0 + tid / 16, g16 = tid % 16;
-                    double s = 0.0;
-                    if (c < nev)
-                        for (int i = g16; i < L; i += 16) s += sv[i] * Z[(k + 1 + i) * nev + c];
-                    for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-                    if (c < nev && g16 == 0) se2[c] = t * s;  // se2 reused as dots
-                }
-                __syncthreads();
-                for (int i = rl; i < L; i += NB / 4) {
-                    double* zr = Z + (k + 1 + i) * nev;
-                    const double vi = sv[i];
-                    for (int c = gl; c < nev; c += 4) zr[c] -= vi * se2[c];
-                }
-            }
-            __syncthreads();
-        }
+    const double glo = W.bnd[0], ghi = W.bnd[1], tnorm = W.bnd[2];
+    const double pivmin = 1e-290 + 1e-300 * tnorm;
+    double a = glo - 1e-14 * tnorm, b = ghi + 1e-14 * tnorm;
+    for (int it = 0; it < 24; ++it) {
+        const double h = (b - a) / 65.0;
+        const double x = a + (q + 1) * h;
+        const bool pred = sturm(sd, se2, N, x, pivmin) > j;
+        const unsigned long long bal = __ballot(pred);
+        const int first = bal ? (__ffsll((long long)bal) - 1) : 64;
+        const double nb = (first < 64) ? a + (first + 1) * h : b;
+        const double na = (first > 0) ? a + first * h : a;
+        a = na;
+        b = nb;
+        if (b - a <= 2.2e-16 * (fabs(a) + fabs(b)) + 1e-17 * tnorm) break;
     }
-    stamp(prof, 5);
-    // 7. Cholesky-QR of Z in the small space (when the Gram fits in LDS)
-    if (N * nev + nev * nev <= PK) {
-        double* Gs = H + N * nev;
-        for (int idx = tid; idx < nev * nev; idx += NB) {
-            const int a = idx / nev, b = idx - a * nev;
-            double s = 0.0;
-            if (b <= a)
-                for (int i = 0; i < N; ++i) s += Z[i * nev + a] * Z[i * nev + b];
-            Gs[idx] = s;
-        }
+    if (q == 0) {
+        W.lam[j] = 0.5 * (a + b);
+        theta[j] = 0.5 * (a + b);
+    }
+}
+
+// ---- 3. eigenvectors of T by inverse iteration ------------------------------
+// lane q owns the entries i = q, q + 64, q + 128 of every vector it touches
+__device__ __forceinline__ double wave_dot3(const double* y, const double* z, int N) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < N; i += 64) s += y[i] * z[i];
+    return wsum(s);
+}
+
+__global__ void __launch_bounds__(64)
+k_rr_invit(int N, int nev, double* __restrict__ work) {
+    __shared__ double sd[NMAX], se[NMAX], ws[5 * NMAX], y[NMAX];
+    RRWork W(work, N, nev);
+    const int j = blockIdx.x, q = threadIdx.x;
+    const double tnorm = W.bnd[2];
+    const double* lam = W.lam;
+    const double scale_j = fmax(fabs(lam[j]), 1e-6 * tnorm);
+    const bool leader = (j == 0) || (lam[j] - lam[j - 1] > 1e-7 * scale_j);
+    if (!leader) return;  // whole wave
+    int gend = j + 1;
+    while (gend < nev && lam[gend] - lam[gend - 1] <= 1e-7 * fmax(fabs(lam[gend]), 1e-6 * tnorm)) ++gend;
+    for (int i = q; i < N; i += 64) {
+        sd[i] = W.d[i];
+        se[i] = W.e[i];
+    }
+    const double tiny = 2.2e-16 * tnorm;
+    for (int g = j; g < gend; ++g) {
+        for (int i = q; i < N; i += 64) y[i] = 1.0 + 0.1 * (double)((7 * i + 3 * g) % 11);
         __syncthreads();
-        bool ok = true;
-        for (int jj = 0; jj < nev; ++jj) {
-            if (tid == 0) {
-                const double dd = Gs[jj * nev + jj];
-                Gs[jj * nev + jj] = dd > 0.5 ? sqrt(dd) : -1.0;
+        if (q == 0) tri_factor(sd, se, N, lam[g] + (double)(g - j) * 8.0 * tiny, tiny, ws);
+        for (int itr = 0; itr < 2; ++itr) {
+            __syncthreads();
+            if (q == 0) tri_lu_solve(ws, N, y);
+            __syncthreads();
+            // Gram-Schmidt against the group's earlier vectors (Y columns
+            // written by this same lane mapping), then normalise
+            for (int h = j; h < g; ++h) {
+                const double* yh = W.Y + (int64_t)h * N;
+                const double s = wave_dot3(y, yh, N);
+                for (int i = q; i < N; i += 64) y[i] -= s * yh[i];
             }
-            __syncthreads();
-            const double ljj = Gs[jj * nev + jj];
-            if (ljj < 0.0) { ok = false; break; }
-            const int Lr = nev - jj - 1;
-            // column scale and trailing update fused: lanes own rows
-            for (int i = jj + 1 + (tid >> 4); i < nev; i += NB / 16) {
-                const double lij = Gs[i * nev + jj] / ljj;
-                // row i's trailing entries kk in (jj, i]; column jj of row kk
-                // is scaled by the lane owning row kk in this same pass, so
-                // read the unscaled value and divide here
-                for (int kk = jj + 1 + (tid & 15); kk <= i; kk += 16)
-                    Gs[i * nev + kk] -= lij * (Gs[kk * nev + jj] / ljj);
-            }
-            __syncthreads();
-            for (int i = jj + 1 + tid; i < nev; i += NB) Gs[i * nev + jj] /= ljj;
-            __syncthreads();
-            (void)Lr;
+            const double nn = 1.0 / sqrt(fmax(wave_dot3(y, y, N), 1e-300));
+            for (int i = q; i < N; i += 64) y[i] *= nn;
         }
-        if (ok) {
-            // each row z of Z: solve z_new L^T = z
-            for (int i = tid; i < N; i += NB) {
-                double* z = Z + i * nev;
-                for (int c = 0; c < nev; ++c) {
-                    double s = z[c];
-                    for (int h = 0; h < c; ++h) s -= Gs[c * nev + h] * z[h];
-                    z[c] = s / Gs[c * nev + c];
+        for (int i = q; i < N; i += 64) W.Y[(int64_t)g * N + i] = y[i];
+    }
+}
+
+// ---- 4. back-transformation Z = Q Y ----------------------------------------
+__global__ void __launch_bounds__(64)
+k_rr_back(int N, int nev, double* __restrict__ work, unsigned long long* __restrict__ prof) {
+    stamp(prof, 4);
+    RRWork W(work, N, nev);
+    const int g = blockIdx.x, q = threadIdx.x;
+    // rows q, q + 64, q + 128 of the vector
+    double z0 = 0.0, z1 = 0.0, z2 = 0.0;
+    const double* yg = W.Y + (int64_t)g * N;
+    if (q < N) z0 = yg[q];
+    if (q + 64 < N) z1 = yg[q + 64];
+    if (q + 128 < N) z2 = yg[q + 128];
+    // reflector k acts on rows k+1 .. N-1 with v_k[r - k - 1]
+    auto vload = [&](int k, int row) -> double {
+        return (k >= 0 && row >= k + 1 && row < N) ? W.R[(int64_t)k * N + (row - k - 1)] : 0.0;
+    };
+    int k = N - 3;
+    double n0 = vload(k, q), n1 = vload(k, q + 64), n2 = vload(k, q + 128);
+    for (; k >= 0; --k) {
+        const double v0 = n0, v1 = n1, v2 = n2;
+        n0 = vload(k - 1, q);
+        n1 = vload(k - 1, q + 64);
+        n2 = vload(k - 1, q + 128);
+        const double t = W.tau[k];
+        if (t == 0.0) continue;
+        const double s = t * wsum(v0 * z0 + v1 * z1 + v2 * z2);
+        z0 -= s * v0;
+        z1 -= s * v1;
+        z2 -= s * v2;
+    }
+    if (q < N) W.Z[(int64_t)q * nev + g] = z0;
+    if (q + 64 < N) W.Z[(int64_t)(q + 64) * nev + g] = z1;
+    if (q + 128 < N) W.Z[(int64_t)(q + 128) * nev + g] = z2;
+}
+
+// ---- 5. orthonormalisation in the small space and output --------------------
+__global__ void __launch_bounds__(NB)
+k_rr_polish(int N, int nev, double* __restrict__ work, double* __restrict__ C, int* __restrict__ status,
+            unsigned long long* __restrict__ prof) {
+    __shared__ double Zs[PK];  // Z (N x nev), then E (nev x nev) behind it
+    __shared__ double red[32];
+    stamp(prof, 5);
+    RRWork W(work, N, nev);
+    const int tid = threadIdx.x;
+    const int nz = N * nev;
+    if (nz + nev * nev <= PK) {
+        double* E = Zs + nz;
+        for (int idx = tid; idx < nz; idx += NB) Zs[idx] = W.Z[idx];
+        __syncthreads();
+        for (int pass = 0; pass < 3; ++pass) {
+            // E = Z^T Z - I (symmetric: lower entries computed, mirrored)
+            double emax = 0.0;
+            for (int idx = tid; idx < nev * nev; idx += NB) {
+                const int a = idx / nev, b = idx - a * nev;
+                if (b > a) continue;
+                double s0 = 0.0, s1 = 0.0;
+                int i = 0;
+                for (; i + 1 < N; i += 2) {
+                    s0 += Zs[i * nev + a] * Zs[i * nev + b];
+                    s1 += Zs[(i + 1) * nev + a] * Zs[(i + 1) * nev + b];
+                }
+                if (i < N) s0 += Zs[i * nev + a] * Zs[i * nev + b];
+                const double ev = (s0 + s1) - (a == b ? 1.0 : 0.0);
+                E[a * nev + b] = ev;
+                E[b * nev + a] = ev;
+                emax = fmax(emax, fabs(ev));
+            }
+            // block max of |E|
+            for (int off = 32; off > 0; off >>= 1) emax = fmax(emax, __shfl_xor(emax, off, 64));
+            __syncthreads();
+            if ((tid & 63) == 0) red[tid >> 6] = emax;
+            __syncthreads();
+            double em = 0.0;
+#pragma unroll
+            for (int w = 0; w < NB / 64; ++w) em = fmax(em, red[w]);
+            // Newton-Schulz needs ||E|| < 1; a basis that far off is left as it is
+            if (em >= 0.5 || (pass > 0 && em <= 1e-8)) break;
+            // Z <- Z - Z E / 2 ; each thread owns entries idx = tid + NB * u
+            constexpr int U = (PK + NB - 1) / NB;
+            double nz_[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = tid + NB * u;
+                nz_[u] = 0.0;
+                if (idx < nz) {
+                    const int i = idx / nev, c = idx - i * nev;
+                    double s = 0.0;
+                    for (int h = 0; h < nev; ++h) s += Zs[i * nev + h] * E[h * nev + c];
+                    nz_[u] = Zs[idx] - 0.5 * s;
                 }
             }
             __syncthreads();
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = tid + NB * u;
+                if (idx < nz) Zs[idx] = nz_[u];
+            }
+            __syncthreads();
+            if (em <= 1e-8) break;  // one step leaves ~ em^2: rounding level
         }
+        for (int idx = tid; idx < nz; idx += NB) C[idx] = Zs[idx];
+    } else {
+        for (int idx = tid; idx < nz; idx += NB) C[idx] = W.Z[idx];
     }
-    stamp(prof, 6);
-    for (int idx = tid; idx < N * nev; idx += NB) C[idx] = Z[idx];
     if (tid == 0) status[0] = 0;
-    stamp(prof, 7);
+    stamp(prof, 6);
 }
 
 }  // namespace
 
-size_t rr_std_work_doubles(int N, int nev) { return (size_t)N * N + (size_t)nev * 5 * N + 64; }
+size_t rr_std_work_doubles(int N, int nev) { return (size_t)N * N + 3 * (size_t)N + 4 + nev + 2 * (size_t)N * nev + 64; }
 
 void rr_std(lh_ctx* ctx, int N, int nev, const double* G, int ldg, double* C, double* theta, double* work,
             int* status, unsigned long long* prof) {
-    if (N > kRRStdMaxN || nev > N || N < 1) LH_THROW(LH_ERR_ARG, "rr_std: N=%d nev=%d out of range", N, nev);
-    k_rr_std<<<1, NB, 0, ctx->stream>>>(N, nev, G, ldg, C, theta, work, status, prof);
+    if (N > kRRStdMaxN || nev > N || N < 1 || nev < 1)
+        LH_THROW(LH_ERR_ARG, "rr_std: N=%d nev=%d out of range", N, nev);
+    k_rr_tri<<<1, NB, 0, ctx->stream>>>(N, nev, G, ldg, work, prof);
+    LH_CHECK_LAUNCH();
+    k_rr_bisect<<<nev, 64, 0, ctx->stream>>>(N, nev, work, theta, prof);
+    LH_CHECK_LAUNCH();
+    k_rr_invit<<<nev, 64, 0, ctx->stream>>>(N, nev, work);
+    LH_CHECK_LAUNCH();
+    k_rr_back<<<nev, 64, 0, ctx->stream>>>(N, nev, work, prof);
+    LH_CHECK_LAUNCH();
+    k_rr_polish<<<1, NB, 0, ctx->stream>>>(N, nev, work, C, status, prof);
     LH_CHECK_LAUNCH();
 }
 
@@ -404,11 +507,14 @@ extern "C" int lh_small_symeig(lh_ctx* ctx, int N, int nev, const double* H, dou
     if (evals) d2h(ctx, evals, dth.p, (size_t)nev * 8);
     if (Z) d2h(ctx, Z, dC.p, (size_t)N * nev * 8);  // row-major N x nev
     if (phase_us) {
+        // stamps: tri start, after load, tri end, bisect start, back start,
+        // polish start, polish end (kernel starts on the in-order stream)
         unsigned long long p[8];
         d2h(ctx, p, prof.p, sizeof(p));
         int khz = 100000;
         (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device);
-        for (int i = 0; i < 7; ++i) phase_us[i] = (double)(p[i + 1] - p[i]) * 1000.0 / (double)khz;
+        for (int i = 0; i < 6; ++i) phase_us[i] = (double)(p[i + 1] - p[i]) * 1000.0 / (double)khz;
+        phase_us[6] = 0.0;
     }
     HIPX(hipStreamSynchronize(ctx->stream));
     LH_API_END(ctx)

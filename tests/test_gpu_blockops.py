@@ -76,3 +76,31 @@ def test_gemm_matches_numpy(n, p, q, ldu, ldy, alpha, beta):
     np.testing.assert_allclose(Y, ref, rtol=0, atol=1e-13 * np.sqrt(p) * 8 * (1 + abs(beta)))
     # columns beyond q are untouched
     assert np.array_equal(Y[:, q:], Y0[:, q:])
+
+
+@pytest.mark.parametrize("n,p,ld,upper", [(10000, 64, 192, True), (10007, 128, 192, True), (4099, 64, 192, False),
+                                          (3001, 192, 192, False), (1001, 72, 73, True), (17, 5, 5, False)])
+def test_gemm_in_place(n, p, ld, upper):
+    """X <- X C in place (the Cholesky-QR and Ritz rotations): every
+    workgroup reads all columns of its rows before any of them is written."""
+    rng = np.random.default_rng(n + p)
+    X0 = rng.standard_normal((n, ld))
+    Cm = rng.standard_normal((p, p))
+    if upper:
+        Cm = np.triu(Cm)
+    X, _ = _lib.block_gemm(_lib.default_context(), None, p, Cm, X0, 1.0, 0.0, upper=upper, inplace=True)
+    ref = X0.copy()
+    ref[:, :p] = X0[:, :p] @ Cm
+    np.testing.assert_allclose(X, ref, rtol=0, atol=1e-13 * np.sqrt(p) * 8)
+    assert np.array_equal(X[:, p:], X0[:, p:])
+
+
+def test_gemm_upper_skips_only_zero_blocks():
+    rng = np.random.default_rng(11)
+    U = rng.standard_normal((5000, 128))
+    Cm = np.triu(rng.standard_normal((128, 128)))
+    Y0 = np.zeros((5000, 128))
+    ctx = _lib.default_context()
+    y_full, _ = _lib.block_gemm(ctx, U, 128, Cm, Y0)
+    y_up, _ = _lib.block_gemm(ctx, U, 128, Cm, Y0, upper=True)
+    np.testing.assert_allclose(y_up, y_full, rtol=0, atol=1e-12)
