@@ -73,6 +73,148 @@ def profiled_traffic(name):
     return None
 
 
+def golden(name):
+    path = os.path.join(ROOT, "tests", "golden", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def rel_err(evals, gold_evals):
+    gold = np.asarray(gold_evals)
+    scale = np.abs(gold).max()
+    den = np.where(np.abs(gold) < 1e-6 * scale, scale, np.abs(gold))
+    return float(np.max(np.abs(np.asarray(evals) - gold) / den))
+
+
+def run_workload(args, ctx, _lib, meshgen):
+    """The other workloads of the benchmark plan (one JSON line each; the
+    default L8 line is the contract line):
+      small: icosphere L5 sphere (10,242 vertices), radial noise 0.02, k = 10;
+      tet:   64^3-cell tetrahedral cube (274,625 vertices), k = 30, then heat
+             diffusion from vertex 0 with t = 1e-3 (m = 1e-3 / avg_edge^2),
+             PCG to ||r|| / ||b|| <= 1e-10;
+      pcie:  the L8 workload from mesh arrays in host memory to eigenvalues
+             and eigenvectors in host memory (upload, assembly, eigensolve,
+             download);
+      batch: 256 seeded ellipsoids of icosphere levels 3-6, k = 50 each,
+             assembled per mesh and solved by the batched eigensolver with
+             the autotuned worker count."""
+    from lapy_amd import TetMesh, heat
+
+    out = {"unit": "eigenpairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+    if args.workload in ("small", "tet"):
+        if args.workload == "small":
+            v, t = meshgen.small_sphere(level=5)
+            kind, k, gold = _lib.LH_TRIA, 10, golden("sphere_l5_k10.json")
+            name = "icosphere L5 sphere (radial noise 0.02), FP64 assembly + k=10 eigenpairs"
+        else:
+            v, t = meshgen.volume_cube(cells=64)
+            kind, k, gold = _lib.LH_TET, 30, golden("volume_c64_k30.json")
+            name = "64^3-cell tetrahedral cube, FP64 assembly + k=30 eigenpairs"
+        base = _lib.DeviceMesh(ctx, v, t)
+
+        def step():
+            m = base.clone()
+            A, B = m.assemble(kind, False, np.float64)
+            ev, eh, st = _lib.eigs(ctx, A, B, k, -0.01, keep_device=True)
+            return ev, eh, st, (m, A, B)
+
+        for _ in range(args.warmup):
+            ev, eh, st, keep = step()
+            ctx.sync()
+            del eh, keep
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ev, eh, st, keep = step()
+            ctx.sync()
+            del eh, keep
+        total = time.perf_counter() - t0
+        out.update({"metric": f"eigenpairs/sec (k={k})", "value": round(k * args.steps / total, 4),
+                    "ms_per_step": round(1000 * total / args.steps, 2),
+                    "config": {"workload": name, "vertices": len(v), "elements": len(t), "k": k,
+                               "lobpcg_iterations": st.iterations, "max_rel_resid": st.max_resid,
+                               "eig_rel_err_vs_oracle": rel_err(ev, gold["evals"]) if gold else None}})
+        if args.workload == "tet":
+            mesh = TetMesh(v, t)
+            ael = float(mesh.avg_edge_length())
+            mh = 1e-3 / (ael * ael)
+            heat.diffusion(mesh, 0, m=mh, rtol=1e-10)  # warm the device mesh
+            t1 = time.perf_counter()
+            u = heat.diffusion(mesh, 0, m=mh, rtol=1e-10)
+            th = time.perf_counter() - t1
+            herr = None
+            if gold:
+                idx = np.asarray(gold["probe_index"])
+                ref = np.asarray(gold["heat_probe_values"])
+                herr = float(np.max(np.abs(u[idx] - ref)) / np.max(np.abs(ref)))
+            out["config"].update({"heat_seconds": round(th, 4), "heat_rtol": 1e-10, "heat_probe_rel_err": herr})
+    elif args.workload == "batch":
+        meshes = meshgen.shape_batch(count=256)
+        k = 50
+        gold = golden("batch_256_k50.json")
+        base = [_lib.DeviceMesh(ctx, v, t) for v, t in meshes]
+
+        def assemble_all():
+            ms = [b.clone() for b in base]
+            ab = [m.assemble(_lib.LH_TRIA, False, np.float64) for m in ms]
+            return ms, [a for a, _ in ab], [b for _, b in ab]
+
+        # autotune the worker count on a sample of 32 meshes of every size
+        ms, As, Bs = assemble_all()
+        sample = list(range(0, 256, 8))
+        t0 = time.perf_counter()
+        best, secs = _lib.eigs_autotune(ctx, [As[i] for i in sample], [Bs[i] for i in sample], [k] * len(sample))
+        log(f"[bench] autotune: best workers {best}, sample seconds {secs} ({time.perf_counter() - t0:.1f} s)")
+        del ms, As, Bs
+        for _ in range(args.warmup):
+            ms, As, Bs = assemble_all()
+            res = _lib.eigs_batched(ctx, As, Bs, [k] * 256, keep_device=True)
+            ctx.sync()
+            del res, ms, As, Bs
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ms, As, Bs = assemble_all()
+            res = _lib.eigs_batched(ctx, As, Bs, [k] * 256, keep_device=True)
+            ctx.sync()
+        total = time.perf_counter() - t0
+        err = None
+        if gold:
+            err = max(rel_err(r[0], g) for r, g in zip(res, gold["evals"]))
+        its = [r[2].iterations for r in res]
+        out.update({"metric": "eigenpairs/sec (k=50), 256-mesh ShapeDNA batch",
+                    "value": round(256 * k * args.steps / total, 4), "ms_per_step": round(1000 * total / args.steps, 2),
+                    "config": {"workload": "256 ellipsoids, icosphere levels 3-6, FP64 assembly + k=50 eigenpairs each",
+                               "vertices_total": int(sum(len(v) for v, _ in meshes)), "k": k, "workers": best,
+                               "autotune_seconds": secs, "lobpcg_iterations_min_max": [min(its), max(its)],
+                               "eig_rel_err_vs_oracle": err}})
+    else:  # pcie
+        v, t = meshgen.large_ellipsoid(level=8)
+        k = 50
+        gold = golden("ellipsoid_l8_k50.json")
+
+        def step():
+            m = _lib.DeviceMesh(ctx, v, t)  # host -> device
+            A, B = m.assemble(_lib.LH_TRIA, False, np.float64)
+            return _lib.eigs(ctx, A, B, k, -0.01)  # eigenvalues and eigenvectors downloaded
+
+        for _ in range(args.warmup):
+            step()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ev, vec, st = step()
+        total = time.perf_counter() - t0
+        out.update({"metric": "eigenpairs/sec (k=50), host arrays to host eigenpairs",
+                    "value": round(k * args.steps / total, 4), "ms_per_step": round(1000 * total / args.steps, 2),
+                    "config": {"workload": "icosphere L8 ellipsoid, host mesh -> host eigenpairs (PCIe inclusive)",
+                               "vertices": len(v), "k": k, "lobpcg_iterations": st.iterations,
+                               "eigenvector_bytes_downloaded": int(vec.nbytes),
+                               "eig_rel_err_vs_oracle": rel_err(ev, gold["evals"]) if gold else None}})
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +231,8 @@ def main():
     ap.add_argument("--cpu-level", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", type=int, default=0)
+    ap.add_argument("--workload", default="l8", choices=["l8", "small", "tet", "pcie", "batch"],
+                    help="l8: the benchmark line (default); small / tet / pcie / batch: the other workloads")
     ap.add_argument("--kernels-only", action="store_true",
                     help="skip the eigensolves; only the roofline timings of the two measured kernels")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -98,6 +242,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
+    # the batched eigensolver runs one stream per worker; let HIP map up to 16
+    # of them to hardware queues (an explicit setting in the environment wins)
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
     from lapy_amd import _lib, meshgen
 
     ctx = _lib.default_context()
@@ -109,6 +256,9 @@ def main():
         cm.clone()
         ctx.sync()
         print(json.dumps({"copy_calibration_bytes_per_copy": nvc * 8, "copies": 3}), flush=True)
+        return
+    if args.workload != "l8":
+        run_workload(args, ctx, _lib, meshgen)
         return
     storage = {"auto": _lib.LH_STORE_AUTO, "f32": _lib.LH_STORE_F32, "f16": _lib.LH_STORE_F16}[args.poly_storage]
     v, t = meshgen.large_ellipsoid(level=args.level)

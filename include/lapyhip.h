@@ -161,6 +161,25 @@ typedef struct lh_eig_opts {
  * B-orthonormal.  Results stay on the device in *out. */
 int lh_eigs(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma,
             const lh_eig_opts* opts, lh_eig** out, lh_stats* stats);
+/* Many independent problems A_i x = lambda B_i x, k_i eigenpairs each (the
+ * reference computes a shape collection's ShapeDNA one mesh at a time,
+ * lapy/shapedna.py:96-166).  The problems are taken largest first by
+ * `workers` concurrent solvers, each with its own stream and a device arena
+ * for its buffers (so no allocation synchronises the device between
+ * problems); workers = 0 uses the count lh_eigs_autotune chose for this
+ * context (8 when it was not run).  out[i] receives problem i's result
+ * (free with lh_eig_free); status[i] (nullable) its status; stats[i]
+ * (nullable) its statistics.  Returns LH_OK, or the first failure's code
+ * (LH_ERR_NOCONV: some problem stopped at max_iter; its result is still in
+ * out[i]). */
+int lh_eigs_batched(lh_ctx* ctx, int nmesh, lh_csr* const* A, lh_csr* const* B, const int* k, double sigma,
+                    const lh_eig_opts* opts, int workers, lh_eig** out, int* status, lh_stats* stats);
+/* Autotuner of the batched solver: solves the given problems with 1, 2, 4, 8
+ * and 16 workers, keeps the fastest count in the context (used by
+ * lh_eigs_batched with workers = 0) and reports it and the five wall times
+ * (seconds, nullable). */
+int lh_eigs_autotune(lh_ctx* ctx, int nmesh, lh_csr* const* A, lh_csr* const* B, const int* k, double sigma,
+                     const lh_eig_opts* opts, int* best_workers, double* seconds);
 /* Copy results to host: evals (k), evecs (n x k, column-major = numpy
  * Fortran order, i.e. evecs[:, j] is contiguous). Either may be NULL. */
 int  lh_eig_download(lh_eig* e, double* evals, double* evecs);
@@ -207,13 +226,18 @@ int lh_time_cheb_step(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int iters, in
  * for their tests and timing), on host row-major buffers:
  * lh_block_gram: C (p x q, row-major) = U^T V over n rows (sym != 0: the
  *   product is known to be symmetric, p == q; only upper tiles are formed);
- * lh_block_gemm: Y = alpha U C + beta Y with C p x q row-major.
+ * lh_block_gemm: Y = alpha U C + beta Y with C p x q row-major; flags:
+ *   LH_GEMM_UPPER: C is upper triangular (its zero blocks are skipped),
+ *   LH_GEMM_INPLACE: Y <- alpha Y C in place (U unused, p == q <= 256,
+ *   beta == 0), the eigensolver's Cholesky-QR and Ritz rotations.
  * iters > 0 and avg_us non-NULL: also the average kernel time of `iters`
  * further runs (lh_block_gemm times them with beta = 0, after Y is returned). */
+#define LH_GEMM_UPPER   1
+#define LH_GEMM_INPLACE 2
 int lh_block_gram(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q, int sym,
                   double* C, int iters, double* avg_us);
 int lh_block_gemm(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* Cs, double* Y,
-                  int ldy, double alpha, double beta, int iters, double* avg_us);
+                  int ldy, double alpha, double beta, int flags, int iters, double* avg_us);
 /* The Rayleigh-Ritz eigensolver on its own (no counterpart in the reference;
  * exposed for its tests and timing): smallest nev eigenpairs of the symmetric
  * N x N host matrix H (N <= 192).  evals (nev), Z (N x nev row-major); when

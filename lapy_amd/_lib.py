@@ -106,6 +106,8 @@ SIGNATURES = {
     "lh_time_spmm": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _dp]),
     "lh_small_symeig": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     "lh_time_cheb_step": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
+    "lh_eigs_batched": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_double, _vp, C.c_int, _vp, _vp, _vp]),
+    "lh_eigs_autotune": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_double, _vp, _vp, _vp]),
     "lh_block_gram": (C.c_int, [_vp, C.c_int64, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
                                 _dp]),
     "lh_block_gemm": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.c_double,
@@ -362,6 +364,75 @@ def eigs(ctx: Context, A: DeviceCSR, B: DeviceCSR, k: int, sigma: float = -0.01,
     finally:
         if h and not keep_device:
             lib().lh_eig_free(h)
+
+
+def _eig_opts(precision=LH_PREC_FP64, precond=LH_PRECOND_JACOBI, block=0, max_iter=0, tol=0.0, seed=0, verbose=0,
+              poly_degree=0, poly_ratio=0.0, poly_storage=LH_STORE_AUTO):
+    return EigOpts(precision, precond, block, max_iter, tol, seed, verbose, poly_degree, poly_ratio, poly_storage, 0)
+
+
+def _problem_arrays(As, Bs, ks):
+    if not (len(As) == len(Bs) == len(ks)):
+        raise ValueError("As, Bs and ks must have the same length")
+    nm = len(As)
+    pa = (C.c_void_p * max(nm, 1))(*[a.handle for a in As])
+    pb = (C.c_void_p * max(nm, 1))(*[b.handle for b in Bs])
+    pk = (C.c_int * max(nm, 1))(*[int(x) for x in ks])
+    return nm, pa, pb, pk
+
+
+def eigs_batched(ctx: Context, As, Bs, ks, sigma: float = -0.01, workers: int = 0, keep_device: bool = False,
+                 **opts):
+    """Many independent generalized eigenproblems on the GPU, solved by
+    concurrent solver streams (lh_eigs_batched).  Returns a list of
+    (evals, evecs | EigHandle, stats, status) in the input order; raises on
+    argument errors, and NoConvergence when some problem stopped at max_iter
+    (after all results were gathered, with the list as its .results)."""
+    nm, pa, pb, pk = _problem_arrays(As, Bs, ks)
+    o = _eig_opts(**opts)
+    out = (C.c_void_p * max(nm, 1))()
+    status = (C.c_int * max(nm, 1))()
+    stats = (Stats * max(nm, 1))()
+    code = lib().lh_eigs_batched(ctx.handle, nm, pa, pb, pk, float(sigma), C.byref(o), int(workers), out, status,
+                                 stats)
+    results = []
+    for i in range(nm):
+        h = out[i]
+        if not h:
+            results.append(None)
+            continue
+        k = int(ks[i])
+        if keep_device:
+            evals = np.empty(k, dtype=np.float64)
+            check(lib().lh_eig_download(h, _ptr(evals), None), ctx)
+            results.append((evals, EigHandle(ctx, h), stats[i], status[i]))
+        else:
+            evals = np.empty(k, dtype=np.float64)
+            evecs_t = np.empty((k, As[i].n), dtype=np.float64)
+            try:
+                check(lib().lh_eig_download(h, _ptr(evals), _ptr(evecs_t)), ctx)
+            finally:
+                lib().lh_eig_free(h)
+            results.append((evals, evecs_t.T, stats[i], status[i]))
+    if code == LH_ERR_NOCONV:
+        err = NoConvergence(code, lib().lh_last_error(ctx.handle).decode())
+        err.results = results
+        raise err
+    check(code, ctx, "lh_eigs_batched")
+    return results
+
+
+def eigs_autotune(ctx: Context, As, Bs, ks, sigma: float = -0.01, **opts):
+    """Time the batched solver with 1, 2, 4, 8 and 16 workers on these
+    problems; the fastest count is kept in the context.  Returns
+    (best_workers, {workers: seconds})."""
+    nm, pa, pb, pk = _problem_arrays(As, Bs, ks)
+    o = _eig_opts(**opts)
+    best = C.c_int(0)
+    secs = (C.c_double * 5)()
+    check(lib().lh_eigs_autotune(ctx.handle, nm, pa, pb, pk, float(sigma), C.byref(o), C.byref(best), secs), ctx,
+          "lh_eigs_autotune")
+    return best.value, dict(zip((1, 2, 4, 8, 16), list(secs)))
 
 
 def time_cheb_step(ctx: Context, A: DeviceCSR, B: DeviceCSR, ncol: int, iters: int = 50, reorder: bool = True,

@@ -163,6 +163,8 @@ void refresh_dense_diag(lh_ctx* ctx, lh_csr* a) {
 
 using namespace lh;
 
+thread_local Arena* lh_thread_arena = nullptr;
+
 extern "C" {
 
 int lh_device_count(void) {
@@ -175,6 +177,8 @@ int lh_ctx_create(int device, lh_ctx** out) {
     if (!out) return LH_ERR_ARG;
     *out = nullptr;
     lh_ctx* ctx = new lh_ctx();
+    ctx->scratch.persistent = ctx->small.persistent = ctx->pinned_dummy.persistent = true;
+    ctx->arena_mem.persistent = true;
     try {
         int nd = 0;
         HIPX(hipGetDeviceCount(&nd));
@@ -201,6 +205,9 @@ void lh_ctx_destroy(lh_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (lh_ctx* w : ctx->workers) lh_ctx_destroy(w);
+    ctx->workers.clear();
+    ctx->arena_mem.release();
     ctx->scratch.release();
     ctx->small.release();
     if (ctx->host_small) (void)hipHostFree(ctx->host_small);

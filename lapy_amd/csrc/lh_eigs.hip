@@ -21,6 +21,11 @@
 #include "lh_dense.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <thread>
+
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 
@@ -247,7 +252,10 @@ struct Lobpcg {
             gemm_mfma_inplace(ctx, n, m, Sp(), ld, Cm.as<double>(), m, 1.0, false);
         }
         for (int j = 0; j < k; ++j) out_evals[j] = theta[j];
-        out_vecs.alloc(n * k * 8);
+        {
+            ArenaOff keep;  // the result outlives the solver's buffers
+            out_vecs.alloc(n * k * 8);
+        }
         if (reordered) ro.unpermute(ctx, Sp(), ld, out_vecs.as<double>(), k, n, k);
         else copy_block<double, double>(ctx, n, k, Sp(), ld, out_vecs.as<double>(), k);
         HIPX(hipStreamSynchronize(ctx->stream));
@@ -291,23 +299,23 @@ void dense_eigs(lh_ctx* ctx, const lh_csr* A, const lh_csr* B, int k, double* ev
     d2h(ctx, &s, status.p, 4);
     if (s) LH_THROW(LH_ERR_NUMERIC, "mass matrix is not positive definite");
     d2h(ctx, evals, th.p, (size_t)k * 8);
-    evecs = std::move(C);
+    {
+        ArenaOff keep;  // the result outlives the solver's buffers
+        evecs.alloc((size_t)n * k * 8);
+    }
+    HIPX(hipMemcpyAsync(evecs.p, C.p, (size_t)n * k * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPX(hipStreamSynchronize(ctx->stream));
     st.iterations = 1;
     st.converged = k;
     st.rr_calls = 1;
     st.t_total_us = now_us() - t0;
 }
 
-}  // namespace
-
-extern "C" {
-
-int lh_eigs(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const lh_eig_opts* opts, lh_eig** out,
-            lh_stats* stats) {
-    if (!ctx || !A || !B || !out) return LH_ERR_ARG;
-    *out = nullptr;
-    LH_API_BEGIN(ctx)
-    HIPX(hipSetDevice(ctx->device));
+// one generalized eigenproblem on ctx's stream; throws lh_error on bad
+// arguments or failures, returns the result with code LH_OK or
+// LH_ERR_NOCONV (results of the last iteration)
+lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const lh_eig_opts* opts, lh_stats& st,
+                  int& code) {
     const int64_t n = A->n();
     if (B->n() != n) LH_THROW(LH_ERR_ARG, "A and B differ in size (%lld vs %lld)", (long long)n, (long long)B->n());
     if (k <= 0 || k >= n) LH_THROW(LH_ERR_ARG, "k must satisfy 0 < k < n (k=%d, n=%lld)", k, (long long)n);
@@ -351,7 +359,7 @@ int lh_eigs(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const lh_eig
         LH_THROW(LH_ERR_ARG, "poly_storage %d unknown", store);
     L.poly_ratio = (opts && opts->poly_ratio > 1.0) ? opts->poly_ratio : 4.0 * deg * deg;
     if (opts && opts->precision != LH_PREC_FP64) LH_THROW(LH_ERR_ARG, "precision mode %d not available", opts->precision);
-    auto e = std::make_unique<lh_eig>();
+    std::unique_ptr<lh_eig> e(new lh_eig());
     e->ctx = ctx;
     e->n = n;
     e->k = k;
@@ -360,18 +368,173 @@ int lh_eigs(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const lh_eig
         // small problem: Rayleigh-Ritz on the whole space (S = I), i.e. the
         // dense generalized eigensolver on the GPU
         dense_eigs(ctx, A, B, k, e->evals.data(), e->evecs, L.st);
-        if (stats) *stats = L.st;
-        *out = e.release();
-        return LH_OK;
+        st = L.st;
+        code = LH_OK;
+        return e.release();
     }
     L.run(e->evals.data(), e->evecs);
     L.st.precision_mode = LH_PREC_FP64;
-    if (stats) *stats = L.st;
-    *out = e.release();
-    if (L.st.converged < k) {
+    st = L.st;
+    code = (L.st.converged < k) ? LH_ERR_NOCONV : LH_OK;
+    return e.release();
+}
+
+// the batched solver's workers: their own contexts (streams) and arenas
+void ensure_workers(lh_ctx* ctx, int nw) {
+    while ((int)ctx->workers.size() < nw) {
+        lh_ctx* w = nullptr;
+        const int c = lh_ctx_create(ctx->device, &w);
+        if (c != LH_OK) LH_THROW(c, "could not create a worker context");
+        ctx->workers.push_back(w);
+    }
+}
+
+// worker arena: large enough for the biggest problem's solver buffers (the
+// S, A S, B S and next-S blocks dominate: 4 n (3m) doubles), plus slack for
+// the preconditioner, reordering and reductions; allocations beyond it fall
+// back to hipMalloc
+size_t arena_bytes(int64_t nmax, int m) { return (size_t)nmax * 3 * m * 8 * 4 + (size_t)nmax * m * 8 + (256u << 20); }
+
+struct BatchResult {
+    int code = LH_OK;
+    std::string err;
+};
+
+void solve_batch(lh_ctx* ctx, int nmesh, lh_csr* const* A, lh_csr* const* B, const int* k, double sigma,
+                 const lh_eig_opts* opts, int nw, lh_eig** out, int* status, lh_stats* stats, BatchResult& res) {
+    HIPX(hipStreamSynchronize(ctx->stream));  // inputs were made on ctx's stream
+    std::vector<int> order(nmesh);
+    for (int i = 0; i < nmesh; ++i) order[i] = i;
+    // largest first (LPT): the long solves start early, the short ones fill in
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return A[a]->n() > A[b]->n(); });
+    int64_t nmax = 0;
+    int kmax = 1;
+    for (int i = 0; i < nmesh; ++i) {
+        nmax = std::max<int64_t>(nmax, A[i]->n());
+        kmax = std::max(kmax, k[i]);
+    }
+    const int mmax = std::min(128, ((std::max(kmax + 4, (kmax * 6 + 4) / 5) + 7) / 8) * 8);
+    nw = std::max(1, std::min(nw, nmesh));
+    ensure_workers(ctx, nw);
+    const size_t abytes = arena_bytes(nmax, mmax);
+    for (int w = 0; w < nw; ++w) {
+        lh_ctx* wc = ctx->workers[w];
+        if (wc->arena_mem.bytes < abytes) {
+            wc->arena_mem.alloc(abytes);
+            wc->arena.base = wc->arena_mem.as<char>();
+            wc->arena.cap = abytes;
+            wc->arena.top = 0;
+        }
+    }
+    std::atomic<int> next{0};
+    std::mutex mu;
+    auto work = [&](int w) {
+        lh_ctx* wc = ctx->workers[w];
+        (void)hipSetDevice(wc->device);
+        lh_thread_arena = &wc->arena;
+        for (;;) {
+            const int q = next.fetch_add(1);
+            if (q >= nmesh) break;
+            const int i = order[q];
+            const size_t mark = wc->arena.top;
+            lh_stats sti{};
+            int code = LH_OK;
+            lh_eig* e = nullptr;
+            try {
+                e = eigs_impl(wc, A[i], B[i], k[i], sigma, opts, sti, code);
+                if (e) e->ctx = ctx;
+            } catch (const lh_error& ex) {
+                code = ex.code;
+                std::lock_guard<std::mutex> g(mu);
+                if (res.code == LH_OK) { res.code = ex.code; res.err = "problem " + std::to_string(i) + ": " + ex.msg; }
+            } catch (const std::exception& ex) {
+                code = LH_ERR_HIP;
+                std::lock_guard<std::mutex> g(mu);
+                if (res.code == LH_OK) { res.code = LH_ERR_HIP; res.err = ex.what(); }
+            }
+            (void)hipStreamSynchronize(wc->stream);
+            wc->arena.top = mark;  // every buffer of the problem is released
+            out[i] = e;
+            if (status) status[i] = code;
+            if (stats) stats[i] = sti;
+            if (code == LH_ERR_NOCONV) {
+                std::lock_guard<std::mutex> g(mu);
+                if (res.code == LH_OK) { res.code = LH_ERR_NOCONV; res.err = "problem " + std::to_string(i) + " did not converge"; }
+            }
+        }
+        lh_thread_arena = nullptr;
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nw; ++w) th.emplace_back(work, w);
+    work(0);
+    for (auto& t : th) t.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+int lh_eigs(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const lh_eig_opts* opts, lh_eig** out,
+            lh_stats* stats) {
+    if (!ctx || !A || !B || !out) return LH_ERR_ARG;
+    *out = nullptr;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    lh_stats st{};
+    int code = LH_OK;
+    *out = eigs_impl(ctx, A, B, k, sigma, opts, st, code);
+    if (stats) *stats = st;
+    if (code == LH_ERR_NOCONV) {
         ctx->last_error = "LOBPCG did not converge within max_iter";
         return LH_ERR_NOCONV;
     }
+    LH_API_END(ctx)
+}
+
+int lh_eigs_batched(lh_ctx* ctx, int nmesh, lh_csr* const* A, lh_csr* const* B, const int* k, double sigma,
+                    const lh_eig_opts* opts, int workers, lh_eig** out, int* status, lh_stats* stats) {
+    if (!ctx || nmesh < 0 || (nmesh > 0 && (!A || !B || !k || !out))) return LH_ERR_ARG;
+    for (int i = 0; i < nmesh; ++i) out[i] = nullptr;
+    if (nmesh == 0) return LH_OK;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    for (int i = 0; i < nmesh; ++i)
+        if (!A[i] || !B[i]) LH_THROW(LH_ERR_ARG, "problem %d has no matrix", i);
+    if (workers < 0 || workers > 64) LH_THROW(LH_ERR_ARG, "workers must be in [0, 64], got %d", workers);
+    const int nw = workers > 0 ? workers : (ctx->tuned_workers > 0 ? ctx->tuned_workers : 8);
+    BatchResult res;
+    solve_batch(ctx, nmesh, A, B, k, sigma, opts, nw, out, status, stats, res);
+    if (res.code != LH_OK) {
+        ctx->last_error = res.err;
+        return res.code;
+    }
+    LH_API_END(ctx)
+}
+
+int lh_eigs_autotune(lh_ctx* ctx, int nmesh, lh_csr* const* A, lh_csr* const* B, const int* k, double sigma,
+                     const lh_eig_opts* opts, int* best_workers, double* seconds) {
+    if (!ctx || nmesh <= 0 || !A || !B || !k) return LH_ERR_ARG;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    static const int cand[5] = {1, 2, 4, 8, 16};
+    double best = 1e300;
+    int bw = 1;
+    std::vector<lh_eig*> out(nmesh, nullptr);
+    for (int c = 0; c < 5; ++c) {
+        BatchResult res;
+        const double t0 = now_us();
+        solve_batch(ctx, nmesh, A, B, k, sigma, opts, cand[c], out.data(), nullptr, nullptr, res);
+        const double dt = (now_us() - t0) * 1e-6;
+        for (auto& e : out) { lh_eig_free(e); e = nullptr; }
+        if (res.code != LH_OK && res.code != LH_ERR_NOCONV) {
+            ctx->last_error = res.err;
+            return res.code;
+        }
+        if (seconds) seconds[c] = dt;
+        if (dt < best) { best = dt; bw = cand[c]; }
+    }
+    ctx->tuned_workers = bw;
+    if (best_workers) *best_workers = bw;
     LH_API_END(ctx)
 }
 

@@ -41,22 +41,57 @@ struct lh_error {
 // --------------------------------------------------------------------------
 // device buffers (RAII)
 // --------------------------------------------------------------------------
+// A bump allocator over one device allocation.  A thread that sets
+// lh_thread_arena takes its solver buffers from it (no hipMalloc/hipFree,
+// which synchronise the device, between the problems of a batch) and resets
+// `top` to a mark once every buffer of a problem is gone.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, top = 0;
+    void* take(size_t n) {
+        n = (n + 255) & ~size_t(255);
+        if (!base || top + n > cap) return nullptr;
+        void* q = base + top;
+        top += n;
+        return q;
+    }
+};
+extern thread_local Arena* lh_thread_arena;
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool pooled = false;      // taken from the thread's arena (not freed here)
+    bool persistent = false;  // never taken from an arena (outlives a problem)
     DevBuf() = default;
     explicit DevBuf(size_t n) { alloc(n); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), pooled(o.pooled), persistent(o.persistent) {
+        o.p = nullptr;
+        o.bytes = 0;
+        o.pooled = false;
+    }
     DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; bytes = o.bytes; pooled = o.pooled;
+            o.p = nullptr; o.bytes = 0; o.pooled = false;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
     void alloc(size_t n) {
         release();
         if (n == 0) n = 8;
+        if (!persistent && lh_thread_arena) {
+            if (void* q = lh_thread_arena->take(n)) {
+                p = q;
+                bytes = n;
+                pooled = true;
+                return;
+            }
+        }
         hipError_t e = hipMalloc(&p, n);
         if (e != hipSuccess) {
             p = nullptr;
@@ -67,9 +102,20 @@ struct DevBuf {
     // grow-only reallocation (contents not preserved)
     void ensure(size_t n) { if (n > bytes) alloc(n); }
     void release() {
-        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        if (p && !pooled) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        pooled = false;
     }
     template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// allocations in this scope bypass the thread's arena (results that outlive
+// the problem)
+struct ArenaOff {
+    Arena* saved;
+    ArenaOff() : saved(lh_thread_arena) { lh_thread_arena = nullptr; }
+    ~ArenaOff() { lh_thread_arena = saved; }
 };
 
 // --------------------------------------------------------------------------
@@ -83,6 +129,11 @@ struct lh_ctx {
     DevBuf scratch;      // general scratch (grow-only)
     DevBuf small;        // small reduction scratch
     DevBuf pinned_dummy;
+    // batched eigensolver: worker contexts (own streams) and their arenas
+    std::vector<lh_ctx*> workers;
+    DevBuf arena_mem;
+    Arena arena;
+    int tuned_workers = 0;  // 0: not tuned
     void* host_small = nullptr;   // pinned host staging (hipHostMalloc)
     size_t host_small_bytes = 0;
 };

@@ -50,3 +50,13 @@ def test_eigs_and_timing_argument_types(fake):
     _lib.eigs(ctx, A, A, 3, keep_device=True)
     _lib.time_cheb_step(ctx, A, A, 64, iters=5, reorder=False, storage=_lib.LH_STORE_F32)
     assert calls.count("lh_eigs") == 2 and "lh_time_cheb_step" in calls
+
+
+def test_batched_argument_types(fake):
+    ctx, calls = fake
+    A = types.SimpleNamespace(handle=None, n=10)
+    res = _lib.eigs_batched(ctx, [A, A], [A, A], [3, 4], workers=2, poly_degree=8)
+    assert res == [None, None]
+    best, secs = _lib.eigs_autotune(ctx, [A], [A], [3])
+    assert set(secs) == {1, 2, 4, 8, 16}
+    assert calls == ["lh_eigs_batched", "lh_eigs_autotune"]
