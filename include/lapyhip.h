@@ -43,6 +43,7 @@ typedef struct lh_ctx  lh_ctx;   /* device, stream, scratch, last error        *
 typedef struct lh_mesh lh_mesh;  /* device-resident mesh + cached topology     */
 typedef struct lh_csr  lh_csr;   /* device-resident CSR matrix (n x n)         */
 typedef struct lh_eig  lh_eig;   /* device-resident eigen result               */
+typedef struct lh_mpnn lh_mpnn;  /* learned preconditioner (weights on device) */
 
 /* Per-solve statistics returned by the solvers. */
 typedef struct lh_stats {
@@ -152,7 +153,10 @@ typedef struct lh_eig_opts {
     double  poly_ratio;    /* Chebyshev interval [lmax/ratio, lmax]
                               (0: automatic, 4 * degree^2)                   */
     int32_t poly_storage;  /* LH_STORE_* for the Chebyshev work blocks        */
-    int32_t reserved;      /* must be 0                                       */
+    int32_t no_fallback;   /* LH_PRECOND_LEARNED: 1 keeps the network even when
+                              the worst residual falls by less than half per
+                              iteration three iterations running (0: switch to
+                              Jacobi then, stats.fallback_used = 1)           */
 } lh_eig_opts;
 
 /* Replaces Solver.eigs / eigensystem (lapy/solver.py:667-716: splu of
@@ -199,7 +203,10 @@ typedef struct lh_pcg_opts {
     int32_t max_iter;      /* 0: automatic                                      */
     double  rtol;          /* ||r|| / ||b||                                     */
     int32_t verbose;
-    int32_t reserved;
+    int32_t no_fallback;   /* LH_PRECOND_LEARNED: as lh_eig_opts.no_fallback,
+                              judged on the per-iteration residual reduction
+                              over the 4-iteration convergence checks; the
+                              fallback restarts CG with Jacobi               */
 } lh_pcg_opts;
 int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta,
            const double* b, int nrhs, const uint8_t* dmask, double* x,
@@ -221,6 +228,31 @@ int lh_time_spmm(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int precision, int
  * solver's locality reordering (reorder = 1, as lh_eigs does). */
 int lh_time_cheb_step(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int iters, int reorder, int storage,
                       double* avg_us);
+/* ---- learned preconditioner (no counterpart in the reference) -------------
+ * A 3-layer message-passing network of width 32 on the operator's graph,
+ * linear in the residual: for K = beta A + alpha B, s = diag(K)^-1/2 and the
+ * normalised off-diagonal weights W^ = S (K - diag K) S,
+ *   H0 = (s r) u^T,  H_{l+1} = H_l Ws_l^T + (W^ H_l) Wn_l^T,  z = s (s r + H3 v)
+ * (Jacobi plus a learned correction; v = 0 is exactly Jacobi).  Weights are
+ * bfloat16 on the device, arithmetic FP32.  Parameters (lh_mpnn_param_count()
+ * floats): u (32), then Ws_l, Wn_l (32 x 32 row-major each) for l = 0, 1, 2,
+ * then v (32).  params NULL: seeded small random weights with v = 0.
+ * lh_ctx_set_mpnn selects the network LH_PRECOND_LEARNED uses in lh_eigs
+ * (K = A - sigma B) and lh_pcg (K = alpha B + beta A); NULL clears it.
+ * lh_mpnn_apply: Z = P R for host R, Z (n x nc row-major).
+ * lh_mpnn_loss_grad: training loss L = (1/nc) sum_c ||S(K z_c - r_c)||^2 /
+ * ||S r_c||^2 and dL/dparams (host, FP64). */
+int  lh_mpnn_param_count(void);
+int  lh_mpnn_create(lh_ctx* ctx, const float* params, lh_mpnn** out);
+void lh_mpnn_free(lh_mpnn* net);
+int  lh_mpnn_set_params(lh_mpnn* net, const float* params);
+int  lh_mpnn_get_params(lh_mpnn* net, float* params);
+int  lh_ctx_set_mpnn(lh_ctx* ctx, lh_mpnn* net);
+int  lh_mpnn_apply(lh_mpnn* net, lh_csr* A, lh_csr* B, double alpha, double beta, const double* R, int nc,
+                   double* Z);
+int  lh_mpnn_loss_grad(lh_mpnn* net, lh_csr* A, lh_csr* B, double alpha, double beta, const double* R, int nc,
+                       double* loss, double* grad);
+
 /* The eigensolver's tall-skinny block products on their own (no counterpart
  * in the reference, whose eigsh does its dense work inside ARPACK; exposed
  * for their tests and timing), on host row-major buffers:

@@ -63,7 +63,7 @@ class EigOpts(C.Structure):
         ("poly_degree", C.c_int32),
         ("poly_ratio", C.c_double),
         ("poly_storage", C.c_int32),
-        ("reserved", C.c_int32),
+        ("no_fallback", C.c_int32),
     ]
 
 
@@ -73,7 +73,7 @@ class PcgOpts(C.Structure):
         ("max_iter", C.c_int32),
         ("rtol", C.c_double),
         ("verbose", C.c_int32),
-        ("reserved", C.c_int32),
+        ("no_fallback", C.c_int32),
     ]
 
 
@@ -108,6 +108,14 @@ SIGNATURES = {
     "lh_time_cheb_step": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     "lh_eigs_batched": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_double, _vp, C.c_int, _vp, _vp, _vp]),
     "lh_eigs_autotune": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_double, _vp, _vp, _vp]),
+    "lh_mpnn_param_count": (C.c_int, []),
+    "lh_mpnn_create": (C.c_int, [_vp, _vp, _vp]),
+    "lh_mpnn_free": (None, [_vp]),
+    "lh_mpnn_set_params": (C.c_int, [_vp, _vp]),
+    "lh_mpnn_get_params": (C.c_int, [_vp, _vp]),
+    "lh_ctx_set_mpnn": (C.c_int, [_vp, _vp]),
+    "lh_mpnn_apply": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp, C.c_int, _vp]),
+    "lh_mpnn_loss_grad": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp, C.c_int, _dp, _vp]),
     "lh_block_gram": (C.c_int, [_vp, C.c_int64, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
                                 _dp]),
     "lh_block_gemm": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.c_double,
@@ -346,9 +354,10 @@ class DeviceMesh:
 
 def eigs(ctx: Context, A: DeviceCSR, B: DeviceCSR, k: int, sigma: float = -0.01, precision=LH_PREC_FP64,
          precond=LH_PRECOND_JACOBI, block=0, max_iter=0, tol=0.0, seed=0, verbose=0, keep_device=False,
-         poly_degree=0, poly_ratio=0.0, poly_storage=LH_STORE_AUTO):
+         poly_degree=0, poly_ratio=0.0, poly_storage=LH_STORE_AUTO, no_fallback=False):
     """Generalized eigenpairs on the GPU; returns (evals, evecs, stats)."""
-    opts = EigOpts(precision, precond, block, max_iter, tol, seed, verbose, poly_degree, poly_ratio, poly_storage, 0)
+    opts = EigOpts(precision, precond, block, max_iter, tol, seed, verbose, poly_degree, poly_ratio, poly_storage,
+                   int(bool(no_fallback)))
     st = Stats()
     h = _vp()
     code = lib().lh_eigs(ctx.handle, A.handle, B.handle, int(k), float(sigma), C.byref(opts), C.byref(h), C.byref(st))
@@ -367,8 +376,9 @@ def eigs(ctx: Context, A: DeviceCSR, B: DeviceCSR, k: int, sigma: float = -0.01,
 
 
 def _eig_opts(precision=LH_PREC_FP64, precond=LH_PRECOND_JACOBI, block=0, max_iter=0, tol=0.0, seed=0, verbose=0,
-              poly_degree=0, poly_ratio=0.0, poly_storage=LH_STORE_AUTO):
-    return EigOpts(precision, precond, block, max_iter, tol, seed, verbose, poly_degree, poly_ratio, poly_storage, 0)
+              poly_degree=0, poly_ratio=0.0, poly_storage=LH_STORE_AUTO, no_fallback=False):
+    return EigOpts(precision, precond, block, max_iter, tol, seed, verbose, poly_degree, poly_ratio, poly_storage,
+                   int(bool(no_fallback)))
 
 
 def _problem_arrays(As, Bs, ks):
@@ -433,6 +443,65 @@ def eigs_autotune(ctx: Context, As, Bs, ks, sigma: float = -0.01, **opts):
     check(lib().lh_eigs_autotune(ctx.handle, nm, pa, pb, pk, float(sigma), C.byref(o), C.byref(best), secs), ctx,
           "lh_eigs_autotune")
     return best.value, dict(zip((1, 2, 4, 8, 16), list(secs)))
+
+
+class Mpnn:
+    """Device handle of the learned preconditioner's weights (lh_mpnn)."""
+
+    def __init__(self, ctx: Context, params=None):
+        self.ctx = ctx
+        self.handle = None
+        h = _vp()
+        p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        if p is not None and p.size != mpnn_param_count():
+            raise ValueError(f"expected {mpnn_param_count()} parameters, got {p.size}")
+        check(lib().lh_mpnn_create(ctx.handle, _ptr(p), C.byref(h)), ctx, "lh_mpnn_create")
+        self.handle = h
+
+    @property
+    def params(self):
+        out = np.empty(mpnn_param_count(), dtype=np.float32)
+        check(lib().lh_mpnn_get_params(self.handle, _ptr(out)), self.ctx)
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != mpnn_param_count():
+            raise ValueError(f"expected {mpnn_param_count()} parameters, got {p.size}")
+        check(lib().lh_mpnn_set_params(self.handle, _ptr(p)), self.ctx)
+
+    def apply(self, A: DeviceCSR, B, alpha: float, beta: float, R):
+        R = np.ascontiguousarray(np.asarray(R, dtype=np.float64).reshape(A.n, -1))
+        Z = np.empty_like(R)
+        check(lib().lh_mpnn_apply(self.handle, A.handle, B.handle if B is not None else None, float(alpha),
+                                  float(beta), _ptr(R), R.shape[1], _ptr(Z)), self.ctx, "lh_mpnn_apply")
+        return Z
+
+    def loss_grad(self, A: DeviceCSR, B, alpha: float, beta: float, R):
+        R = np.ascontiguousarray(np.asarray(R, dtype=np.float64).reshape(A.n, -1))
+        loss = C.c_double()
+        g = np.empty(mpnn_param_count(), dtype=np.float64)
+        check(lib().lh_mpnn_loss_grad(self.handle, A.handle, B.handle if B is not None else None, float(alpha),
+                                      float(beta), _ptr(R), R.shape[1], C.byref(loss), _ptr(g)), self.ctx,
+              "lh_mpnn_loss_grad")
+        return loss.value, g
+
+    def use(self, on: bool = True):
+        """Make this network the context's LH_PRECOND_LEARNED (or clear it)."""
+        check(lib().lh_ctx_set_mpnn(self.ctx.handle, self.handle if on else None), self.ctx)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().lh_mpnn_free(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+def mpnn_param_count():
+    return int(lib().lh_mpnn_param_count())
 
 
 def time_cheb_step(ctx: Context, A: DeviceCSR, B: DeviceCSR, ncol: int, iters: int = 50, reorder: bool = True,
@@ -512,7 +581,7 @@ class EigHandle:
 
 
 def pcg(ctx: Context, A: DeviceCSR, B, alpha: float, beta: float, b, x0=None, dmask=None, rtol=1e-12,
-        max_iter=0, precond=LH_PRECOND_JACOBI, verbose=0):
+        max_iter=0, precond=LH_PRECOND_JACOBI, verbose=0, no_fallback=False):
     """Solve (alpha*B + beta*A) x = b (b: (n,) or (n, nrhs)); returns (x, stats)."""
     b = np.asarray(b, dtype=np.float64)
     squeeze = b.ndim == 1
@@ -520,7 +589,7 @@ def pcg(ctx: Context, A: DeviceCSR, B, alpha: float, beta: float, b, x0=None, dm
     bf = np.asfortranarray(b2)
     x = np.asfortranarray(np.zeros_like(b2) if x0 is None else np.asarray(x0, dtype=np.float64).reshape(b2.shape))
     dm = None if dmask is None else np.ascontiguousarray(dmask, dtype=np.uint8)
-    opts = PcgOpts(precond, max_iter, rtol, verbose, 0)
+    opts = PcgOpts(precond, max_iter, rtol, verbose, int(bool(no_fallback)))
     st = Stats()
     code = lib().lh_pcg(ctx.handle, A.handle, B.handle if B is not None else None, float(alpha), float(beta),
                         _ptr(bf), bf.shape[1], _ptr(dm), _ptr(x), C.byref(opts), C.byref(st))

@@ -40,6 +40,7 @@ SOURCES = {
     "lh_reorder.hip": [],
     "lh_eigs.hip": [],
     "lh_pcg.hip": [],
+    "lh_mpnn.hip": [],
 }
 
 

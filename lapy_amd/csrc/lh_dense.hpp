@@ -72,6 +72,18 @@ struct ChebPrecond {
     void step(int m, const void* bvec, const void* xk, void* xp, double omega);
 };
 
+// learned message-passing preconditioner (lh_mpnn.hip) for K = beta A + alpha B
+struct MpnnRun;
+struct LearnedPrecond {
+    lh_mpnn* net = nullptr;
+    std::unique_ptr<MpnnRun> run;
+    LearnedPrecond();
+    ~LearnedPrecond();
+    void setup(lh_ctx* c, lh_mpnn* net, const lh_csr* A, const lh_csr* B, double alpha, double beta);
+    // W (n x m, leading dimension ldw) <- P W
+    void apply(double* W, int ldw, int m);
+};
+
 // breadth-first level reordering of a symmetric pattern (lh_reorder.hip)
 struct Reorder {
     DevBuf perm;   // new row -> old row

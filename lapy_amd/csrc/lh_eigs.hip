@@ -48,6 +48,9 @@ struct Lobpcg {
     int poly_degree = 0;
     double poly_ratio = 0.0;
     bool poly_half = false;
+    int precond = LH_PRECOND_JACOBI;
+    bool no_fallback = false;
+    LearnedPrecond learned;
     ChebPrecond cheb;
     // locality reordering (solver-internal)
     Reorder ro;
@@ -171,6 +174,13 @@ struct Lobpcg {
         HIPX(hipMemsetAsync(S.p, 0, n * ld * 8, ctx->stream));
         jacobi_diag(ctx, A, B, -sigma, 1.0, dinv.as<double>());
         if (poly_degree > 1) cheb.setup(ctx, A, B, sigma, poly_degree, poly_ratio, n, m, poly_half);
+        bool learned_on = false;
+        if (precond == LH_PRECOND_LEARNED) {
+            learned.setup(ctx, ctx->mpnn, A, B, -sigma, 1.0);  // K = A - sigma B
+            learned_on = true;
+        }
+        double prev_worst = 0.0;
+        int slow = 0;
 
         // start block
         randn<double>(ctx, n, m, Sp(), ld, seed);
@@ -203,15 +213,28 @@ struct Lobpcg {
             }
             st.max_resid = worst;
             st.converged = nconv;
+            // learned preconditioner: back to Jacobi when the worst residual
+            // falls by less than half per iteration, three iterations running
+            if (learned_on && !no_fallback) {
+                slow = (it > 0 && worst > 0.5 * prev_worst) ? slow + 1 : 0;
+                if (slow >= 3) {
+                    learned_on = false;
+                    st.fallback_used = 1;
+                    if (verbose) fprintf(stderr, "[lobpcg] it %d: learned preconditioner -> Jacobi\n", it);
+                }
+            }
+            prev_worst = worst;
             if (verbose > 1 || (verbose && it % 50 == 0))
                 fprintf(stderr, "[lobpcg] it %d conv %d/%d worst %.3e theta0 %.6e theta_k %.6e\n", it, nconv, k,
                         worst, theta[0], theta[k - 1]);
             if (nconv == k) break;
             // precondition W
-            if (poly_degree > 1) {
+            if (learned_on) {
+                learned.apply(Sp() + m, ld, m);
+            } else if (poly_degree > 1) {
                 cheb.apply(Sp() + m, ld, m, cs1.as<double>());  // cs1: ||w_c||^2 from the test above
                 st.spmm_calls += poly_degree - 1;
-            } else {
+            } else if (precond != LH_PRECOND_NONE) {
                 rowscale<double>(ctx, n, m, Sp() + m, ld, dinv.as<double>());
             }
             // [W P] orthogonal to X in the B inner product (classical Gram-Schmidt, twice)
@@ -344,7 +367,14 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
     // grows like n^(2/d)) until the iteration count stops falling (~100 at
     // 655k vertices: beyond it the block's own gap ratio limits LOBPCG);
     // poly_degree 1 is plain Jacobi
-    const bool jac = !opts || opts->precond == LH_PRECOND_JACOBI;
+    const int pc = opts ? opts->precond : LH_PRECOND_JACOBI;
+    if (pc != LH_PRECOND_JACOBI && pc != LH_PRECOND_LEARNED && pc != LH_PRECOND_NONE)
+        LH_THROW(LH_ERR_ARG, "preconditioner %d unknown", pc);
+    if (pc == LH_PRECOND_LEARNED && !ctx->mpnn)
+        LH_THROW(LH_ERR_ARG, "LH_PRECOND_LEARNED needs a network on the context (lh_ctx_set_mpnn)");
+    L.precond = pc;
+    L.no_fallback = opts && opts->no_fallback;
+    const bool jac = pc == LH_PRECOND_JACOBI;
     int deg = opts ? opts->poly_degree : 0;
     if (!jac) deg = 1;
     if (deg <= 0) {

@@ -134,6 +134,7 @@ struct lh_ctx {
     DevBuf arena_mem;
     Arena arena;
     int tuned_workers = 0;  // 0: not tuned
+    lh_mpnn* mpnn = nullptr;  // network used by LH_PRECOND_LEARNED (not owned)
     void* host_small = nullptr;   // pinned host staging (hipHostMalloc)
     size_t host_small_bytes = 0;
 };

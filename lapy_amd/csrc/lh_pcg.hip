@@ -133,6 +133,13 @@ __global__ void k_precond(int64_t n, int m, const double* __restrict__ dinv, con
     z[gid] = (dm && dm[i]) ? 0.0 : dinv[i] * r[gid];
 }
 // beta = rz_new / rz_old ; p = z + beta p ; rz_old = rz_new
+// z = 0 on Dirichlet rows
+__global__ void k_mask_rows(int64_t n, int m, const uint8_t* __restrict__ dm, double* __restrict__ z) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n * m) return;
+    if (dm[gid / m]) z[gid] = 0.0;
+}
+
 __global__ void k_beta(int m, const double* __restrict__ rzn, double* __restrict__ rzo, double* __restrict__ b,
                        const int* __restrict__ active) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -220,7 +227,17 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
         K.vals.p = nullptr;  // not owned
     }
     const int precond = opts ? opts->precond : LH_PRECOND_JACOBI;
+    if (precond != LH_PRECOND_JACOBI && precond != LH_PRECOND_LEARNED && precond != LH_PRECOND_NONE)
+        LH_THROW(LH_ERR_ARG, "preconditioner %d unknown", precond);
     if (precond == LH_PRECOND_NONE) k_fill<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, 1.0, dinv.as<double>());
+    bool learned_on = false;
+    LearnedPrecond learned;
+    if (precond == LH_PRECOND_LEARNED) {
+        if (!ctx->mpnn) LH_THROW(LH_ERR_ARG, "LH_PRECOND_LEARNED needs a network on the context (lh_ctx_set_mpnn)");
+        learned.setup(ctx, ctx->mpnn, A, B, alpha, beta);
+        learned_on = true;
+    }
+    const bool no_fallback = opts && opts->no_fallback;
     // host -> device (column-major host, row-major device)
     std::vector<double> hb((size_t)n * m), hx((size_t)n * m);
     for (int64_t i = 0; i < n; ++i)
@@ -270,14 +287,25 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
     for (int c = 0; c < m; ++c)
         if (bnorm2[c] == 0.0) { active[c] = 0; ++nconv; }
     h2d(ctx, act.p, active.data(), m * 4);
-    k_precond<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dinv.as<double>(), dmp, dR.as<double>(),
-                                                          dZ.as<double>());
+    auto precondition = [&]() {
+        if (learned_on) {
+            HIPX(hipMemcpyAsync(dZ.p, dR.p, n * m * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            learned.apply(dZ.as<double>(), m, m);
+            if (dmp) k_mask_rows<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dmp, dZ.as<double>());
+        } else {
+            k_precond<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dinv.as<double>(), dmp, dR.as<double>(),
+                                                                  dZ.as<double>());
+        }
+        LH_CHECK_LAUNCH();
+    };
+    precondition();
     HIPX(hipMemcpyAsync(dP.p, dZ.p, n * m * 8, hipMemcpyDeviceToDevice, ctx->stream));
     cd.run(ctx, dR.as<double>(), dZ.as<double>(), rz.as<double>());
     const int32_t* rp = A->pat->rowptr.as<int32_t>();
     const int32_t* cl = A->pat->cols.as<int32_t>();
     int it = 0;
-    double worst = 0.0;
+    double worst = 0.0, prev_worst = 1.0;
+    int slow = 0;
     const int check_every = 4;
     for (it = 0; it < max_iter && nconv < m; ++it) {
         k_spmv_masked<<<nblocks(n * 16, 256), 256, 0, ctx->stream>>>(n, rp, cl, kv.as<double>(), dmp, dP.as<double>(),
@@ -293,8 +321,7 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
             cd.run(ctx, ones_blk.as<double>(), dR.as<double>(), sums.as<double>());
             k_sub_colmean<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dR.as<double>(), sums.as<double>());
         }
-        k_precond<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dinv.as<double>(), dmp, dR.as<double>(),
-                                                              dZ.as<double>());
+        precondition();
         cd.run(ctx, dR.as<double>(), dZ.as<double>(), rzn.as<double>());
         k_beta<<<nblocks(m, 64), 64, 0, ctx->stream>>>(m, rzn.as<double>(), rz.as<double>(), be.as<double>(),
                                                        act.as<int>());
@@ -313,6 +340,21 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
                 if (rel <= rtol) { active[c] = 0; ++nconv; changed = true; }
             }
             if (changed) h2d(ctx, act.p, active.data(), m * 4);
+            // learned preconditioner: back to Jacobi (and a CG restart) when
+            // the residual falls by less than half per iteration over three
+            // checks running
+            if (learned_on && !no_fallback) {
+                const double per_it = prev_worst > 0.0 ? std::pow(worst / prev_worst, 1.0 / check_every) : 0.0;
+                slow = (per_it > 0.5) ? slow + 1 : 0;
+                if (slow >= 3) {
+                    learned_on = false;
+                    st.fallback_used = 1;
+                    precondition();
+                    cd.run(ctx, dR.as<double>(), dZ.as<double>(), rz.as<double>());
+                    HIPX(hipMemcpyAsync(dP.p, dZ.p, n * m * 8, hipMemcpyDeviceToDevice, ctx->stream));
+                }
+            }
+            prev_worst = worst;
             if (verbose && ((it + 1) % 100 == 0))
                 fprintf(stderr, "[pcg] it %d conv %d/%d worst %.3e\n", it + 1, nconv, m, worst);
         }

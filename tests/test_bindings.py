@@ -60,3 +60,15 @@ def test_batched_argument_types(fake):
     best, secs = _lib.eigs_autotune(ctx, [A], [A], [3])
     assert set(secs) == {1, 2, 4, 8, 16}
     assert calls == ["lh_eigs_batched", "lh_eigs_autotune"]
+
+
+def test_mpnn_argument_types(fake):
+    ctx, calls = fake
+    A = types.SimpleNamespace(handle=None, n=6)
+    net = _lib.Mpnn.__new__(_lib.Mpnn)
+    net.ctx, net.handle = ctx, None
+    net.apply(A, None, 0.01, 1.0, np.zeros((6, 2)))
+    net.loss_grad(A, A, 0.01, 1.0, np.zeros((6, 2)))
+    net.use()
+    called = [c for c in calls if c != "lh_mpnn_param_count"]
+    assert called[-3:] == ["lh_mpnn_apply", "lh_mpnn_loss_grad", "lh_ctx_set_mpnn"]

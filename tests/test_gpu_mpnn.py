@@ -1,0 +1,138 @@
+"""Learned message-passing preconditioner on the GPU: forward pass, loss
+and weight gradients against the CPU restatement (oracle/mpnn.py, itself
+pinned by finite differences), training, the ablation against Jacobi in the
+eigensolver and the heat solver, and the Jacobi fallback."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from lapy_amd import TriaMesh, _lib, heat, learned, meshgen
+from lapy_amd.solver import Solver
+from oracle import fem, mpnn
+
+pytestmark = pytest.mark.gpu
+
+
+def mesh(level, seed):
+    rng = np.random.default_rng(seed)
+    return meshgen.perturbed_sphere(level, rng, axes=rng.uniform(0.6, 1.6, 3), sigma=0.02)
+
+
+def device_ops(ctx, v, t):
+    dm = _lib.DeviceMesh(ctx, v, t)
+    A, B = dm.assemble(_lib.LH_TRIA, False, np.float64)
+    K = (A.to_scipy_csc() + 0.01 * B.to_scipy_csc()).tocsr()
+    return dm, A, B, K
+
+
+def random_params(seed, scale=0.15):
+    return (scale * np.random.default_rng(seed).standard_normal(mpnn.n_params())).astype(np.float32)
+
+
+def test_forward_matches_oracle_and_is_jacobi_at_zero_readout():
+    ctx = _lib.default_context()
+    v, t = mesh(2, 1)
+    dm, A, B, K = device_ops(ctx, v, t)
+    p = random_params(3)
+    net = _lib.Mpnn(ctx, p)
+    R = np.random.default_rng(4).standard_normal((A.n, 5))
+    Z = net.apply(A, B, 0.01, 1.0, R)
+    Zo = mpnn.forward(K, R, p.astype(np.float64))
+    assert np.abs(Z - Zo).max() <= 2e-5 * np.abs(Zo).max()
+    # linear in R (FP32 arithmetic)
+    Z2 = net.apply(A, B, 0.01, 1.0, 2.0 * R[:, :1] - R[:, 1:2])
+    np.testing.assert_allclose(Z2[:, 0], 2.0 * Z[:, 0] - Z[:, 1], rtol=0, atol=1e-5 * np.abs(Z).max())
+    p0 = p.copy()
+    p0[-mpnn.F:] = 0.0
+    net.params = p0
+    Z0 = net.apply(A, B, 0.01, 1.0, R)
+    np.testing.assert_allclose(Z0, R / K.diagonal()[:, None], rtol=1e-6)
+
+
+def test_loss_and_gradient_match_oracle():
+    ctx = _lib.default_context()
+    v, t = mesh(2, 5)
+    dm, A, B, K = device_ops(ctx, v, t)
+    p = random_params(6, scale=0.2)
+    net = _lib.Mpnn(ctx, p)
+    R = np.random.default_rng(7).standard_normal((A.n, 4))
+    loss, g = net.loss_grad(A, B, 0.01, 1.0, R)
+    loss_o, g_o = mpnn.loss_and_grad(K, R, p.astype(np.float64))
+    assert abs(loss - loss_o) <= 1e-5 * abs(loss_o)
+    assert np.linalg.norm(g - g_o) <= 1e-4 * np.linalg.norm(g_o), (np.linalg.norm(g - g_o), np.linalg.norm(g_o))
+    # each parameter block
+    for sl in (slice(0, mpnn.F), slice(mpnn.F, mpnn.F + mpnn.F * mpnn.F), slice(-mpnn.F, None)):
+        assert np.linalg.norm(g[sl] - g_o[sl]) <= 1e-4 * max(np.linalg.norm(g_o[sl]), 1e-12)
+    # deterministic
+    loss2, g2 = net.loss_grad(A, B, 0.01, 1.0, R)
+    assert loss2 == loss and np.array_equal(g2, g)
+
+
+@pytest.fixture(scope="module")
+def trained():
+    meshes = [mesh(lev, 100 + i) for i, lev in enumerate((3, 4, 3, 4, 3, 4))]
+    net = learned.LearnedPreconditioner()
+    hist = net.train(meshes, steps=300, ncol=8, lr=3e-3, seed=1)
+    return net, hist
+
+
+def test_training_lowers_the_loss(trained):
+    net, hist = trained
+    assert np.mean(hist[-30:]) < 0.8 * np.mean(hist[:10]), (hist[:10], hist[-30:])
+
+
+def test_ablation_eigensolver_and_heat(trained):
+    """Held-out mesh: the learned preconditioner needs fewer LOBPCG
+    iterations than plain Jacobi (same solver, Chebyshev acceleration off),
+    and no more PCG iterations in the heat solve; results agree."""
+    net, _ = trained
+    v, t = mesh(4, 999)
+    ctx = _lib.default_context()
+    dm, A, B, K = device_ops(ctx, v, t)
+    net.use()
+    try:
+        ev_j, _, st_j = _lib.eigs(ctx, A, B, 10, precond=_lib.LH_PRECOND_JACOBI, poly_degree=1)
+        ev_l, _, st_l = _lib.eigs(ctx, A, B, 10, precond=_lib.LH_PRECOND_LEARNED, no_fallback=True)
+        ev_n, _, st_n = _lib.eigs(ctx, A, B, 10, precond=_lib.LH_PRECOND_NONE, max_iter=20000)
+        print("LOBPCG iterations: none", st_n.iterations, "jacobi", st_j.iterations, "learned", st_l.iterations)
+        np.testing.assert_allclose(ev_l, ev_j, rtol=1e-8, atol=1e-10)
+        assert st_l.iterations < st_j.iterations
+        assert st_l.fallback_used == 0
+        # heat: (B_lumped + t A) x = e_0
+        dl = _lib.DeviceMesh(ctx, v, t)
+        Al, Bl = dl.assemble(_lib.LH_TRIA, True, np.float64)
+        tt = TriaMesh(v, t).avg_edge_length() ** 2
+        b = np.zeros(Al.n)
+        b[0] = 1.0
+        xj, sj = _lib.pcg(ctx, Al, Bl, 1.0, tt, b, rtol=1e-10)
+        xl, sl = _lib.pcg(ctx, Al, Bl, 1.0, tt, b, rtol=1e-10, precond=_lib.LH_PRECOND_LEARNED, no_fallback=True)
+        print("heat PCG iterations: jacobi", sj.iterations, "learned", sl.iterations)
+        np.testing.assert_allclose(xl, xj, rtol=0, atol=1e-8 * np.abs(xj).max())
+        assert sl.iterations <= sj.iterations
+    finally:
+        net.use(False)
+
+
+def test_fallback_to_jacobi_with_a_bad_network():
+    ctx = _lib.default_context()
+    v, t = mesh(3, 42)
+    dm, A, B, K = device_ops(ctx, v, t)
+    bad = _lib.Mpnn(ctx, random_params(9, scale=1.5))
+    bad.use()
+    try:
+        ev, _, st = _lib.eigs(ctx, A, B, 6, precond=_lib.LH_PRECOND_LEARNED, max_iter=20000)
+        assert st.fallback_used == 1
+        a, b = fem.fem_tria(v, t)
+        lam_o, _ = fem.eigs(a, b, k=6)
+        np.testing.assert_allclose(ev, lam_o, rtol=1e-8, atol=1e-10)
+    finally:
+        bad.use(False)
+
+
+def test_learned_needs_a_network():
+    ctx = _lib.default_context()
+    v, t = mesh(2, 3)
+    dm, A, B, K = device_ops(ctx, v, t)
+    with pytest.raises(ValueError):
+        _lib.eigs(ctx, A, B, 4, precond=_lib.LH_PRECOND_LEARNED)
