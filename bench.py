@@ -156,6 +156,22 @@ def run_workload(args, ctx, _lib, meshgen):
         k = 50
         gold = golden("batch_256_k50.json")
         base = [_lib.DeviceMesh(ctx, v, t) for v, t in meshes]
+        # preconditioner: Chebyshev-Jacobi (default), plain Jacobi, or the
+        # learned network, trained first (untimed) on six other meshes of
+        # the same family (levels 3 and 4, seeds outside the batch)
+        popts = {"cheb": {}, "jacobi": {"poly_degree": 1},
+                 "learned": {"precond": _lib.LH_PRECOND_LEARNED}}[args.precond]
+        net = None
+        if args.precond == "learned":
+            from lapy_amd import learned
+
+            t0 = time.perf_counter()
+            net = learned.LearnedPreconditioner(ctx=ctx)
+            train = meshgen.shape_batch(count=6, seed=50_000, levels=(3, 4))
+            hist = net.train(train, steps=300, ncol=8, lr=3e-3, seed=1)
+            net.use()
+            log(f"[bench] learned preconditioner trained: loss {hist[:10].mean():.3f} -> {hist[-30:].mean():.3f} "
+                f"({time.perf_counter() - t0:.1f} s)")
 
         def assemble_all():
             ms = [b.clone() for b in base]
@@ -166,18 +182,19 @@ def run_workload(args, ctx, _lib, meshgen):
         ms, As, Bs = assemble_all()
         sample = list(range(0, 256, 8))
         t0 = time.perf_counter()
-        best, secs = _lib.eigs_autotune(ctx, [As[i] for i in sample], [Bs[i] for i in sample], [k] * len(sample))
+        best, secs = _lib.eigs_autotune(ctx, [As[i] for i in sample], [Bs[i] for i in sample], [k] * len(sample),
+                                        **popts)
         log(f"[bench] autotune: best workers {best}, sample seconds {secs} ({time.perf_counter() - t0:.1f} s)")
         del ms, As, Bs
         for _ in range(args.warmup):
             ms, As, Bs = assemble_all()
-            res = _lib.eigs_batched(ctx, As, Bs, [k] * 256, keep_device=True)
+            res = _lib.eigs_batched(ctx, As, Bs, [k] * 256, keep_device=True, **popts)
             ctx.sync()
             del res, ms, As, Bs
         t0 = time.perf_counter()
         for _ in range(args.steps):
             ms, As, Bs = assemble_all()
-            res = _lib.eigs_batched(ctx, As, Bs, [k] * 256, keep_device=True)
+            res = _lib.eigs_batched(ctx, As, Bs, [k] * 256, keep_device=True, **popts)
             ctx.sync()
         total = time.perf_counter() - t0
         err = None
@@ -188,6 +205,8 @@ def run_workload(args, ctx, _lib, meshgen):
                     "value": round(256 * k * args.steps / total, 4), "ms_per_step": round(1000 * total / args.steps, 2),
                     "config": {"workload": "256 ellipsoids, icosphere levels 3-6, FP64 assembly + k=50 eigenpairs each",
                                "vertices_total": int(sum(len(v) for v, _ in meshes)), "k": k, "workers": best,
+                               "precond": args.precond,
+                               "fallbacks": int(sum(r[2].fallback_used for r in res)),
                                "autotune_seconds": secs, "lobpcg_iterations_min_max": [min(its), max(its)],
                                "eig_rel_err_vs_oracle": err}})
     else:  # pcie
@@ -233,6 +252,8 @@ def main():
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--workload", default="l8", choices=["l8", "small", "tet", "pcie", "batch"],
                     help="l8: the benchmark line (default); small / tet / pcie / batch: the other workloads")
+    ap.add_argument("--precond", default="cheb", choices=["cheb", "jacobi", "learned"],
+                    help="batch workload: Chebyshev-Jacobi (default), plain Jacobi or the learned network")
     ap.add_argument("--kernels-only", action="store_true",
                     help="skip the eigensolves; only the roofline timings of the two measured kernels")
     ap.add_argument("--copy-calibration", action="store_true",

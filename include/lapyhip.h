@@ -154,9 +154,9 @@ typedef struct lh_eig_opts {
                               (0: automatic, 4 * degree^2)                   */
     int32_t poly_storage;  /* LH_STORE_* for the Chebyshev work blocks        */
     int32_t no_fallback;   /* LH_PRECOND_LEARNED: 1 keeps the network even when
-                              the worst residual falls by less than half per
-                              iteration three iterations running (0: switch to
-                              Jacobi then, stats.fallback_used = 1)           */
+                              the worst residual has not halved over the last
+                              8 iterations (0: switch to plain Jacobi then,
+                              stats.fallback_used = 1)                        */
 } lh_eig_opts;
 
 /* Replaces Solver.eigs / eigensystem (lapy/solver.py:667-716: splu of
@@ -203,10 +203,10 @@ typedef struct lh_pcg_opts {
     int32_t max_iter;      /* 0: automatic                                      */
     double  rtol;          /* ||r|| / ||b||                                     */
     int32_t verbose;
-    int32_t no_fallback;   /* LH_PRECOND_LEARNED: as lh_eig_opts.no_fallback,
-                              judged on the per-iteration residual reduction
-                              over the 4-iteration convergence checks; the
-                              fallback restarts CG with Jacobi               */
+    int32_t no_fallback;   /* LH_PRECOND_LEARNED: 1 keeps the network; 0 falls
+                              back to Jacobi (restarting CG) when the residual
+                              falls by less than half per iteration over three
+                              4-iteration convergence checks running         */
 } lh_pcg_opts;
 int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta,
            const double* b, int nrhs, const uint8_t* dmask, double* x,

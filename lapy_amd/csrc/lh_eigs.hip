@@ -179,8 +179,8 @@ struct Lobpcg {
             learned.setup(ctx, ctx->mpnn, A, B, -sigma, 1.0);  // K = A - sigma B
             learned_on = true;
         }
-        double prev_worst = 0.0;
-        int slow = 0;
+        constexpr int kStallWindow = 8;
+        double hist[kStallWindow] = {};  // worst residual of the last iterations
 
         // start block
         randn<double>(ctx, n, m, Sp(), ld, seed);
@@ -214,16 +214,19 @@ struct Lobpcg {
             st.max_resid = worst;
             st.converged = nconv;
             // learned preconditioner: back to Jacobi when the worst residual
-            // falls by less than half per iteration, three iterations running
+            // has not halved over the last kStallWindow iterations.  (A test
+            // per iteration would reject every preconditioner of Jacobi's
+            // class: LOBPCG reduces the residual by 0.5 - 0.97 per iteration
+            // with them.)
             if (learned_on && !no_fallback) {
-                slow = (it > 0 && worst > 0.5 * prev_worst) ? slow + 1 : 0;
-                if (slow >= 3) {
+                const double before = hist[it % kStallWindow];  // iteration it - kStallWindow
+                hist[it % kStallWindow] = worst;
+                if (it >= kStallWindow && worst > 0.5 * before) {
                     learned_on = false;
                     st.fallback_used = 1;
                     if (verbose) fprintf(stderr, "[lobpcg] it %d: learned preconditioner -> Jacobi\n", it);
                 }
             }
-            prev_worst = worst;
             if (verbose > 1 || (verbose && it % 50 == 0))
                 fprintf(stderr, "[lobpcg] it %d conv %d/%d worst %.3e theta0 %.6e theta_k %.6e\n", it, nconv, k,
                         worst, theta[0], theta[k - 1]);
@@ -449,6 +452,7 @@ void solve_batch(lh_ctx* ctx, int nmesh, lh_csr* const* A, lh_csr* const* B, con
     const size_t abytes = arena_bytes(nmax, mmax);
     for (int w = 0; w < nw; ++w) {
         lh_ctx* wc = ctx->workers[w];
+        wc->mpnn = ctx->mpnn;  // the learned preconditioner's weights are shared read-only
         if (wc->arena_mem.bytes < abytes) {
             wc->arena_mem.alloc(abytes);
             wc->arena.base = wc->arena_mem.as<char>();

@@ -537,9 +537,13 @@ int lh_mpnn_create(lh_ctx* ctx, const float* params, lh_mpnn** out) {
 
 void lh_mpnn_free(lh_mpnn* net) {
     if (!net) return;
-    lh_ctx* ctx = net->ctx;
-    if (ctx && ctx->mpnn == net) ctx->mpnn = nullptr;
+    lh_ctx* ctx = net->ctx;  // set at creation
     (void)hipSetDevice(ctx->device);
+    if (ctx->mpnn == net) ctx->mpnn = nullptr;
+    for (lh_ctx* w : ctx->workers) {
+        if (w->mpnn == net) w->mpnn = nullptr;
+        (void)hipStreamSynchronize(w->stream);
+    }
     (void)hipStreamSynchronize(ctx->stream);
     delete net;
 }

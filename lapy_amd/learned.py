@@ -17,8 +17,9 @@ eigensolver's shift, sigma = -0.01).
 
 Use: ``net.use()`` makes it the context's LH_PRECOND_LEARNED for
 ``Solver.eigs(precond="learned")`` and ``heat.diffusion``'s PCG; both fall
-back to Jacobi when it reduces the residual by less than half per iteration
-three times running.
+back to plain Jacobi when convergence stalls: LOBPCG when the worst residual
+has not halved over the last 8 iterations, PCG when it falls by less than half
+per iteration over three 4-iteration checks running.
 """
 
 from __future__ import annotations

@@ -95,3 +95,27 @@ def test_batched_empty_and_bad_arguments():
     As, Bs, keep = assemble_all(ctx, meshes)
     with pytest.raises(ValueError):
         _lib.eigs_batched(ctx, As, Bs, [As[0].n + 5])
+
+
+def test_batched_with_the_learned_preconditioner():
+    """The network set on the context reaches every worker stream; results
+    match the single-problem solves with the same preconditioner."""
+    ctx = _lib.default_context()
+    meshes = problems()[:4]
+    As, Bs, keep = assemble_all(ctx, meshes)
+    ks = [8, 6, 8, 6]
+    net = _lib.Mpnn(ctx)
+    net.use()
+    try:
+        res = _lib.eigs_batched(ctx, As, Bs, ks, workers=3, precond=_lib.LH_PRECOND_LEARNED)
+        for i, (a, b, k) in enumerate(zip(As, Bs, ks)):
+            ev1, _, st1 = _lib.eigs(ctx, a, b, k, precond=_lib.LH_PRECOND_LEARNED)
+            evb, _, st, status = res[i]
+            assert status == 0
+            assert st.fallback_used == st1.fallback_used
+            np.testing.assert_allclose(evb, ev1, rtol=1e-10, atol=1e-12 * abs(ev1).max())
+    finally:
+        net.use(False)
+    # cleared on the context: the workers no longer see it
+    with pytest.raises(ValueError):
+        _lib.eigs_batched(ctx, As, Bs, ks, workers=2, precond=_lib.LH_PRECOND_LEARNED)
