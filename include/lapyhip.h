@@ -59,11 +59,16 @@ typedef struct lh_stats {
     int32_t spmm_calls;
     int32_t rr_calls;
     int32_t restarts;
-    int32_t reserved;
+    int32_t iterations_fp32; /* LH_PREC_MIXED: iterations of the FP32 phase
+                                (included in `iterations`)                    */
 } lh_stats;
 
 #define LH_PREC_FP64   0   /* FP64 everywhere                                */
-#define LH_PREC_MIXED  1   /* FP32 SpMM + BF16-MFMA Gram, FP64 RR / QR / check */
+#define LH_PREC_MIXED  1   /* an FP32 phase (FP32 blocks and SpMM, FP32 matrix-
+                              core Grams and updates, FP64 Rayleigh-Ritz and
+                              Cholesky factors) until its residuals stop
+                              falling or reach 1e-5, then FP64 iterations from
+                              its X block to the requested tolerance        */
 
 #define LH_PRECOND_JACOBI  0
 #define LH_PRECOND_LEARNED 1
@@ -256,18 +261,24 @@ int  lh_mpnn_loss_grad(lh_mpnn* net, lh_csr* A, lh_csr* B, double alpha, double 
 /* The eigensolver's tall-skinny block products on their own (no counterpart
  * in the reference, whose eigsh does its dense work inside ARPACK; exposed
  * for their tests and timing), on host row-major buffers:
- * lh_block_gram: C (p x q, row-major) = U^T V over n rows (sym != 0: the
- *   product is known to be symmetric, p == q; only upper tiles are formed);
+ * lh_block_gram: C (p x q, row-major) = U^T V over n rows; flags:
+ *   LH_GRAM_SYM: the product is known to be symmetric, p == q; only upper
+ *   tiles are formed;
  * lh_block_gemm: Y = alpha U C + beta Y with C p x q row-major; flags:
  *   LH_GEMM_UPPER: C is upper triangular (its zero blocks are skipped),
  *   LH_GEMM_INPLACE: Y <- alpha Y C in place (U unused, p == q <= 256,
- *   beta == 0), the eigensolver's Cholesky-QR and Ritz rotations.
+ *   beta == 0), the eigensolver's Cholesky-QR and Ritz rotations;
+ * both: LH_BLOCK_F32: the blocks are rounded to FP32 on the device and the
+ *   FP32 matrix-core kernels run (the mixed-precision eigensolver's), C of
+ *   the Gram summed in FP64, Y returned widened to FP64.
  * iters > 0 and avg_us non-NULL: also the average kernel time of `iters`
  * further runs (lh_block_gemm times them with beta = 0, after Y is returned). */
+#define LH_GRAM_SYM     1
 #define LH_GEMM_UPPER   1
 #define LH_GEMM_INPLACE 2
-int lh_block_gram(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q, int sym,
-                  double* C, int iters, double* avg_us);
+#define LH_BLOCK_F32    4
+int lh_block_gram(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q,
+                  int flags, double* C, int iters, double* avg_us);
 int lh_block_gemm(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* Cs, double* Y,
                   int ldy, double alpha, double beta, int flags, int iters, double* avg_us);
 /* The Rayleigh-Ritz eigensolver on its own (no counterpart in the reference;

@@ -44,11 +44,11 @@ class Stats(C.Structure):
         ("spmm_calls", C.c_int32),
         ("rr_calls", C.c_int32),
         ("restarts", C.c_int32),
-        ("reserved", C.c_int32),
+        ("iterations_fp32", C.c_int32),
     ]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class EigOpts(C.Structure):
@@ -512,16 +512,22 @@ def time_cheb_step(ctx: Context, A: DeviceCSR, B: DeviceCSR, ncol: int, iters: i
     return us.value
 
 
-def block_gram(ctx: Context, U: np.ndarray, V: np.ndarray, p: int, q: int, sym: bool = False, iters: int = 0):
-    """C = U[:, :p]^T V[:, :q] on the GPU (FP64 matrix cores); U, V row-major
-    float64 (n x ldu), (n x ldv).  Returns (C, average microseconds or None)."""
+LH_GRAM_SYM, LH_BLOCK_F32 = 1, 4
+
+
+def block_gram(ctx: Context, U: np.ndarray, V: np.ndarray, p: int, q: int, sym: bool = False, iters: int = 0,
+               f32: bool = False):
+    """C = U[:, :p]^T V[:, :q] on the GPU (FP64 matrix cores; f32: the blocks
+    rounded to FP32 and the FP32 matrix cores); U, V row-major float64
+    (n x ldu), (n x ldv).  Returns (C, average microseconds or None)."""
     U = np.ascontiguousarray(U, dtype=np.float64)
     V = np.ascontiguousarray(V, dtype=np.float64)
     if U.shape[0] != V.shape[0]:
         raise ValueError("U and V must have the same number of rows")
     out = np.empty((p, q), dtype=np.float64)
     us = C.c_double(0.0)
-    check(lib().lh_block_gram(ctx.handle, U.shape[0], _ptr(U), U.shape[1], p, _ptr(V), V.shape[1], q, int(sym),
+    flags = (LH_GRAM_SYM if sym else 0) | (LH_BLOCK_F32 if f32 else 0)
+    check(lib().lh_block_gram(ctx.handle, U.shape[0], _ptr(U), U.shape[1], p, _ptr(V), V.shape[1], q, flags,
                               _ptr(out), iters, C.byref(us) if iters else None), ctx)
     return out, (us.value if iters else None)
 
@@ -530,9 +536,10 @@ LH_GEMM_UPPER, LH_GEMM_INPLACE = 1, 2
 
 
 def block_gemm(ctx: Context, U, p: int, Cm: np.ndarray, Y: np.ndarray, alpha=1.0, beta=0.0, iters: int = 0,
-               upper: bool = False, inplace: bool = False):
+               upper: bool = False, inplace: bool = False, f32: bool = False):
     """Y[:, :q] = alpha U[:, :p] Cm + beta Y[:, :q] on the GPU (FP64 matrix
-    cores); inplace: Y[:, :p] = alpha Y[:, :p] Cm (U is ignored); upper: Cm is
+    cores; f32: U and Y rounded to FP32, the FP32 matrix cores, Y widened
+    back); inplace: Y[:, :p] = alpha Y[:, :p] Cm (U is ignored); upper: Cm is
     upper triangular.  Returns (Y, average microseconds or None)."""
     Cm = np.ascontiguousarray(Cm, dtype=np.float64)
     Y = np.array(Y, dtype=np.float64, order="C", copy=True)
@@ -547,7 +554,7 @@ def block_gemm(ctx: Context, U, p: int, Cm: np.ndarray, Y: np.ndarray, alpha=1.0
             raise ValueError("shape mismatch")
     if Cm.shape[0] != p:
         raise ValueError("shape mismatch")
-    flags = (LH_GEMM_UPPER if upper else 0) | (LH_GEMM_INPLACE if inplace else 0)
+    flags = (LH_GEMM_UPPER if upper else 0) | (LH_GEMM_INPLACE if inplace else 0) | (LH_BLOCK_F32 if f32 else 0)
     us = C.c_double(0.0)
     check(lib().lh_block_gemm(ctx.handle, Y.shape[0], p, q, _ptr(U), U.shape[1] if U is not None else 0, _ptr(Cm),
                               _ptr(Y), Y.shape[1], float(alpha), float(beta), flags, iters,

@@ -21,187 +21,23 @@
 namespace lh {
 
 // ---------------------------------------------------------------------------
-// Gram C = U^T V
+// Gram C = U^T V and the update Y = alpha U C + beta Y: matrix-core kernels
+// of lh_mfma.hip for both FP64 and FP32 blocks
 // ---------------------------------------------------------------------------
-constexpr int GT = 64;   // output tile
-constexpr int GR = 32;   // rows per LDS stage
-
-template <typename T, typename AT>
-__global__ void __launch_bounds__(256)
-k_gram_partial(int64_t n, int64_t rows_per_chunk, const T* __restrict__ U, int ldu, int p,
-               const T* __restrict__ V, int ldv, int q, AT* __restrict__ part) {
-    __shared__ AT su[GR][GT + 1];
-    __shared__ AT sv[GR][GT + 1];
-    const int tid = threadIdx.x;
-    const int tx = tid % 16, ty = tid / 16;
-    const int u0 = blockIdx.y * GT, v0 = blockIdx.z * GT;
-    const int64_t rb = (int64_t)blockIdx.x * rows_per_chunk;
-    const int64_t re = (rb + rows_per_chunk < n) ? rb + rows_per_chunk : n;
-    AT acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = AT(0);
-    for (int64_t r0 = rb; r0 < re; r0 += GR) {
-        // load GR x 64 of U and V (zero beyond bounds)
-        for (int idx = tid; idx < GR * GT; idx += 256) {
-            int rr = idx / GT, cc = idx % GT;
-            int64_t r = r0 + rr;
-            AT uu = AT(0), vv = AT(0);
-            if (r < re) {
-                if (u0 + cc < p) uu = (AT)U[r * ldu + u0 + cc];
-                if (v0 + cc < q) vv = (AT)V[r * ldv + v0 + cc];
-            }
-            su[rr][cc] = uu;
-            sv[rr][cc] = vv;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int rr = 0; rr < GR; ++rr) {
-            AT a[4], b[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) { a[s] = su[rr][ty * 4 + s]; b[s] = sv[rr][tx * 4 + s]; }
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
-#pragma unroll
-                for (int t = 0; t < 4; ++t) acc[s][t] += a[s] * b[t];
-        }
-        __syncthreads();
-    }
-    const int ntile = gridDim.y * gridDim.z;
-    const int tile = blockIdx.y * gridDim.z + blockIdx.z;
-    AT* dst = part + ((int64_t)blockIdx.x * ntile + tile) * (GT * GT);
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) dst[(ty * 4 + s) * GT + tx * 4 + t] = acc[s][t];
-}
-
-// 64 outputs per workgroup; four lane groups sum interleaved chunks, then
-// combine in fixed order (deterministic)
-template <typename AT, typename CT>
-__global__ void __launch_bounds__(256)
-k_gram_reduce(int nchunk, int p, int q, int ntz, const AT* __restrict__ part, CT* __restrict__ C, int ldc,
-              double alpha) {
-    __shared__ AT sh[4][64];
-    const int o = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
-    AT s = AT(0);
-    if (o < p * q) {
-        const int i = o / q, j = o - (o / q) * q;
-        const int tile = (i / GT) * ntz + (j / GT);
-        const int ntile = ((p + GT - 1) / GT) * ntz;
-        const int off = (i % GT) * GT + (j % GT);
-        const AT* src = part + (int64_t)tile * (GT * GT) + off;
-        const int64_t stride = (int64_t)ntile * (GT * GT);
-        for (int c = g; c < nchunk; c += 4) s += src[c * stride];
-    }
-    sh[g][threadIdx.x & 63] = s;
-    __syncthreads();
-    if (g == 0 && o < p * q) {
-        const int i = o / q, j = o - (o / q) * q;
-        const int l = threadIdx.x;
-        C[(int64_t)i * ldc + j] = (CT)(alpha * (double)((sh[0][l] + sh[1][l]) + (sh[2][l] + sh[3][l])));
-    }
-}
-
 template <typename T>
 void gram(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, int ldv, int q, double* C, int ldc,
           DevBuf& ws, double alpha, bool sym) {
-    if constexpr (std::is_same<T, double>::value) {
-        gram_mfma(ctx, n, U, ldu, p, V, ldv, q, C, ldc, ws, alpha, sym);
-        return;
-    }
-    (void)sym;
-    const int tu = (p + GT - 1) / GT, tv = (q + GT - 1) / GT;
-    // about two workgroups per CU in the partial pass, at least 512 rows each
-    int64_t nchunk = std::min<int64_t>(std::max<int64_t>(1, (int64_t)ctx->n_cu * 2 / (tu * tv)), (n + 511) / 512);
-    if (nchunk < 1) nchunk = 1;
-    int64_t rpc = (n + nchunk - 1) / nchunk;
-    rpc = ((rpc + GR - 1) / GR) * GR;
-    nchunk = (n + rpc - 1) / rpc;
-    if (nchunk < 1) nchunk = 1;
-    ws.ensure((size_t)nchunk * tu * tv * GT * GT * sizeof(double));
-    dim3 grid((unsigned)nchunk, tu, tv);
-    k_gram_partial<T, double><<<grid, 256, 0, ctx->stream>>>(n, rpc, U, ldu, p, V, ldv, q, ws.as<double>());
-    LH_CHECK_LAUNCH();
-    k_gram_reduce<double, double><<<nblocks((int64_t)p * q, 64), 256, 0, ctx->stream>>>(
-        (int)nchunk, p, q, tv, ws.as<double>(), C, ldc, alpha);
-    LH_CHECK_LAUNCH();
+    gram_mfma<T>(ctx, n, U, ldu, p, V, ldv, q, C, ldc, ws, alpha, sym);
 }
 template void gram<double>(lh_ctx*, int64_t, const double*, int, int, const double*, int, int, double*, int, DevBuf&,
                            double, bool);
 template void gram<float>(lh_ctx*, int64_t, const float*, int, int, const float*, int, int, double*, int, DevBuf&,
                           double, bool);
 
-// ---------------------------------------------------------------------------
-// Y = alpha U C + beta Y     U: n x p (ldu), C: p x q (ldc, FP64), Y: n x q (ldy)
-// ---------------------------------------------------------------------------
-constexpr int MT = 64, NT = 64, KT = 32;
-
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_gemm_nn(int64_t n, int p, int q, const T* __restrict__ U, int ldu, const double* __restrict__ Cm, int ldc,
-          T* __restrict__ Y, int ldy, double alpha, double beta) {
-    __shared__ double sU[MT][KT + 1];
-    __shared__ double sC[KT][NT + 1];
-    const int tid = threadIdx.x;
-    const int tx = tid % 16, ty = tid / 16;
-    const int64_t row0 = (int64_t)blockIdx.x * MT;
-    const int col0 = blockIdx.y * NT;
-    double acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-    for (int k0 = 0; k0 < p; k0 += KT) {
-        for (int idx = tid; idx < MT * KT; idx += 256) {
-            int r = idx / KT, kk = idx % KT;
-            int64_t gr = row0 + r;
-            sU[r][kk] = (gr < n && k0 + kk < p) ? (double)U[gr * ldu + k0 + kk] : 0.0;
-        }
-        for (int idx = tid; idx < KT * NT; idx += 256) {
-            int kk = idx / NT, c = idx % NT;
-            sC[kk][c] = (k0 + kk < p && col0 + c < q) ? Cm[(int64_t)(k0 + kk) * ldc + col0 + c] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < KT; ++kk) {
-            double a[4], b[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) { a[s] = sU[ty * 4 + s][kk]; b[s] = sC[kk][tx * 4 + s]; }
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
-#pragma unroll
-                for (int t = 0; t < 4; ++t) acc[s][t] += a[s] * b[t];
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        int64_t gr = row0 + ty * 4 + s;
-        if (gr >= n) continue;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            int gc = col0 + tx * 4 + t;
-            if (gc >= q) continue;
-            double v = alpha * acc[s][t];
-            if (beta != 0.0) v += beta * (double)Y[gr * ldy + gc];
-            Y[gr * ldy + gc] = (T)v;
-        }
-    }
-}
-
 template <typename T>
 void gemm_nn(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const double* C, int ldc, T* Y, int ldy,
              double alpha, double beta) {
-    if constexpr (std::is_same<T, double>::value) {
-        gemm_mfma(ctx, n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta);
-        return;
-    }
-    if (n == 0 || q == 0) return;
-    dim3 grid(nblocks(n, MT), (q + NT - 1) / NT);
-    k_gemm_nn<T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta);
-    LH_CHECK_LAUNCH();
+    gemm_mfma<T>(ctx, n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta);
 }
 template void gemm_nn<double>(lh_ctx*, int64_t, int, int, const double*, int, const double*, int, double*, int, double,
                               double);

@@ -14,13 +14,16 @@ void gram(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, int ld
 template <typename T>
 void gemm_nn(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const double* C, int ldc, T* Y, int ldy,
              double alpha, double beta);
-// FP64 matrix-core forms of the two (lh_mfma.hip)
-void gram_mfma(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q, double* C,
-               int ldc, DevBuf& ws, double alpha, bool sym);
-void gemm_mfma(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* C, int ldc, double* Y,
-               int ldy, double alpha, double beta, bool upper = false);
+// matrix-core forms of the two (lh_mfma.hip; T = double or float)
+template <typename T>
+void gram_mfma(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, int ldv, int q, double* C, int ldc,
+               DevBuf& ws, double alpha, bool sym);
+template <typename T>
+void gemm_mfma(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const double* C, int ldc, T* Y, int ldy,
+               double alpha, double beta, bool upper = false);
 // X (n x p, p <= 256) <- alpha X C in place; upper: C upper triangular
-void gemm_mfma_inplace(lh_ctx* ctx, int64_t n, int p, double* X, int ldx, const double* C, int ldc, double alpha,
+template <typename T>
+void gemm_mfma_inplace(lh_ctx* ctx, int64_t n, int p, T* X, int ldx, const double* C, int ldc, double alpha,
                        bool upper);
 size_t rr_work_doubles(int N, int nev);
 // standard symmetric Rayleigh-Ritz (orthonormal basis), LDS-resident, N <= kRRStdMaxN
@@ -63,12 +66,14 @@ struct ChebPrecond {
                int mmax_, bool half_storage);
     // W (n x m, leading dimension ldw) <- p(D^-1 K) D^-1 W; colnorm2 (device,
     // m doubles) holds the squared column norms of W (needed for FP16 storage)
-    void apply(double* W, int ldw, int m, const double* colnorm2);
+    template <typename WT>  // double or float blocks
+    void apply(WT* W, int ldw, int m, const double* colnorm2);
     // average duration (us) of one Chebyshev step on an n x m block
     double time_step(int m, int iters);
 
    private:
-    void init(const double* W, int ldw, int m, const double* colnorm2, void** xk, void** xp);
+    template <typename WT>
+    void init(const WT* W, int ldw, int m, const double* colnorm2, void** xk, void** xp);
     void step(int m, const void* bvec, const void* xk, void* xp, double omega);
 };
 

@@ -37,10 +37,18 @@ double now_us() {
     return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// T: the block type.  double for the FP64 solve; float for the first phase
+// of the mixed-precision solve (FP32 blocks, FP32 SpMM, FP32 matrix-core
+// Grams and updates, FP64 Rayleigh-Ritz and Cholesky factors), which stops
+// at a looser tolerance and hands its X block to an FP64 solve.
+template <typename T>
 struct Lobpcg {
     lh_ctx* ctx;
-    const lh_csr* A;
+    const lh_csr* A;  // FP64 matrices (reordered when the solver reorders)
     const lh_csr* B;
+    const lh_csr* Am = nullptr;  // the matrices of the block products: A, B or their FP32 copies
+    const lh_csr* Bm = nullptr;
+    std::unique_ptr<lh_csr> A32, B32;
     int64_t n;
     int k, m, ld;
     double sigma, tol;
@@ -50,10 +58,18 @@ struct Lobpcg {
     bool poly_half = false;
     int precond = LH_PRECOND_JACOBI;
     bool no_fallback = false;
+    // first phase of a mixed-precision solve: also stop when the worst
+    // residual has stopped falling (FP32 rounding floor)
+    bool phase_one = false;
+    const T* x0 = nullptr;  // start block of a second phase (n x m, leading dimension x0_ld), else random
+    int x0_ld = 0;
     LearnedPrecond learned;
-    ChebPrecond cheb;
+    ChebPrecond own_cheb;
+    ChebPrecond* cheb = &own_cheb;  // a second phase shares the first one's
+    bool cheb_ready = false;
     // locality reordering (solver-internal)
     Reorder ro;
+    const Reorder* ext_ro = nullptr;  // a second phase's: the first phase's order
     std::unique_ptr<lh_csr> Ap, Bp;
     bool reordered = false;
     uint64_t seed;
@@ -62,17 +78,16 @@ struct Lobpcg {
     std::vector<double> theta, res;
     lh_stats st{};
 
-    double* Sp() { return S.as<double>(); }
-    double* ASp() { return AS.as<double>(); }
-    double* MSp() { return MS.as<double>(); }
+    T* Sp() { return S.as<T>(); }
+    T* ASp() { return AS.as<T>(); }
+    T* MSp() { return MS.as<T>(); }
 
-    void spmm_AB(const double* X, double* AX, double* MX, int ncol) {
-        if (A->dtype == LH_F64) spmm_pair<double, double>(ctx, A, B, X, AX, MX, ncol, ld, ld);
-        else LH_THROW(LH_ERR_ARG, "FP64 eigensolver needs FP64 matrices");
+    void spmm_AB(const T* X, T* AX, T* MX, int ncol) {
+        spmm_pair<T, T>(ctx, Am, Bm, X, AX, MX, ncol, ld, ld);
         st.spmm_calls++;
     }
-    void spmm_B(const double* X, double* MX, int ncol) {
-        spmm_ld<double, double>(ctx, B, X, MX, ncol, ld, ld);
+    void spmm_B(const T* X, T* MX, int ncol) {
+        spmm_ld<T, T>(ctx, Bm, X, MX, ncol, ld, ld);
         st.spmm_calls++;
     }
     int read_status() {
@@ -86,10 +101,10 @@ struct Lobpcg {
     // pivot (Cholesky-QR2).  Returns false when the block is numerically rank
     // deficient.  A X and B X of the block are recomputed afterwards.
     bool cholqr(int off, int c, bool recompute_A) {
-        double* X = Sp() + off;
-        double* MX = MSp() + off;
+        T* X = Sp() + off;
+        T* MX = MSp() + off;
         for (int pass = 0; pass < 2; ++pass) {
-            gram<double>(ctx, n, X, ld, c, MX, ld, c, Gs.as<double>(), c, wsg, 1.0, true);
+            gram<T>(ctx, n, X, ld, c, MX, ld, c, Gs.as<double>(), c, wsg, 1.0, true);
             cholinv(ctx, c, Gs.as<double>(), Ts.as<double>(), 0.0, status.as<int>());
             double stv[2];
             d2h(ctx, stv, status.p, sizeof(stv));
@@ -105,17 +120,17 @@ struct Lobpcg {
 
     // Y -= X C with C = (B X)^T Y  (X block at xoff, Y block at yoff, widths cx, cy)
     void project_out(int xoff, int cx, int yoff, int cy) {
-        gram<double>(ctx, n, MSp() + xoff, ld, cx, Sp() + yoff, ld, cy, Cm.as<double>(), cy, wsg);
-        gemm_nn<double>(ctx, n, cx, cy, Sp() + xoff, ld, Cm.as<double>(), cy, Sp() + yoff, ld, -1.0, 1.0);
+        gram<T>(ctx, n, MSp() + xoff, ld, cx, Sp() + yoff, ld, cy, Cm.as<double>(), cy, wsg);
+        gemm_nn<T>(ctx, n, cx, cy, Sp() + xoff, ld, Cm.as<double>(), cy, Sp() + yoff, ld, -1.0, 1.0);
     }
 
     void normalize_cols(int off, int c) {
-        colsq<double>(ctx, n, c, Sp() + off, ld, cs1.as<double>(), wsc);
+        colsq<T>(ctx, n, c, Sp() + off, ld, cs1.as<double>(), wsc);
         std::vector<double> h(c);
         d2h(ctx, h.data(), cs1.p, c * 8);
         for (int j = 0; j < c; ++j) h[j] = h[j] > 0 ? 1.0 / std::sqrt(h[j]) : 1.0;
         h2d(ctx, cs1.p, h.data(), c * 8);
-        colscale<double>(ctx, n, c, Sp() + off, ld, cs1.as<double>());
+        colscale<T>(ctx, n, c, Sp() + off, ld, cs1.as<double>());
     }
 
     // Rayleigh-Ritz on the first N columns of S; returns false on breakdown
@@ -125,12 +140,12 @@ struct Lobpcg {
     // S^T B S) is kept for trial bases wider than the LDS solver takes.
     bool rayleigh_ritz(int N) {
         double t0 = now_us();
-        gram<double>(ctx, n, Sp(), ld, N, ASp(), ld, N, GA.as<double>(), N, wsg, 1.0, true);
+        gram<T>(ctx, n, Sp(), ld, N, ASp(), ld, N, GA.as<double>(), N, wsg, 1.0, true);
         if (N <= kRRStdMaxN) {
             rr_std(ctx, N, m, GA.as<double>(), N, Cm.as<double>(), theta_d.as<double>(), rrwork.as<double>(),
                    status.as<int>());
         } else {
-            gram<double>(ctx, n, Sp(), ld, N, MSp(), ld, N, GM.as<double>(), N, wsg, 1.0, true);
+            gram<T>(ctx, n, Sp(), ld, N, MSp(), ld, N, GM.as<double>(), N, wsg, 1.0, true);
             rr_solve(ctx, N, m, GA.as<double>(), GM.as<double>(), Cm.as<double>(), theta_d.as<double>(),
                      rrwork.as<double>(), status.as<int>());
         }
@@ -142,9 +157,11 @@ struct Lobpcg {
         return true;
     }
 
-    void run(double* out_evals, DevBuf& out_vecs) {
+    // out_evals / out_vecs: the result (a second phase or an FP64 solve);
+    // a first phase leaves its X block in S
+    void run(double* out_evals, DevBuf* out_vecs) {
         double t_start = now_us();
-        if (n >= 4096 && (B->is_diag || B->pat == A->pat)) {
+        if (!ext_ro && n >= 4096 && (B->is_diag || B->pat == A->pat)) {
             ro.build(ctx, *A->pat);
             Ap = ro.apply(ctx, A);
             Bp = ro.apply(ctx, B, B->is_diag ? nullptr : ro.pat);
@@ -152,11 +169,21 @@ struct Lobpcg {
             B = Bp.get();
             reordered = true;
         }
+        if constexpr (sizeof(T) == 8) {
+            if (A->dtype != LH_F64 || B->dtype != LH_F64) LH_THROW(LH_ERR_ARG, "the eigensolver needs FP64 matrices");
+            Am = A;
+            Bm = B;
+        } else {
+            A32 = csr_to_f32(ctx, A);
+            B32 = csr_to_f32(ctx, B);
+            Am = A32.get();
+            Bm = B32.get();
+        }
         ld = 3 * m;
-        S.alloc(n * ld * 8);
-        S2.alloc(n * ld * 8);
-        AS.alloc(n * ld * 8);
-        MS.alloc(n * ld * 8);
+        S.alloc(n * ld * sizeof(T));
+        S2.alloc(n * ld * sizeof(T));
+        AS.alloc(n * ld * sizeof(T));
+        MS.alloc(n * ld * sizeof(T));
         dinv.alloc(n * 8);
         const int N3 = 3 * m;
         GA.alloc(N3 * N3 * 8);
@@ -171,9 +198,9 @@ struct Lobpcg {
         cs2.alloc(N3 * 8);
         theta.assign(m, 0.0);
         res.assign(m, 0.0);
-        HIPX(hipMemsetAsync(S.p, 0, n * ld * 8, ctx->stream));
+        HIPX(hipMemsetAsync(S.p, 0, n * ld * sizeof(T), ctx->stream));
         jacobi_diag(ctx, A, B, -sigma, 1.0, dinv.as<double>());
-        if (poly_degree > 1) cheb.setup(ctx, A, B, sigma, poly_degree, poly_ratio, n, m, poly_half);
+        if (poly_degree > 1 && !cheb_ready) cheb->setup(ctx, A, B, sigma, poly_degree, poly_ratio, n, m, poly_half);
         bool learned_on = false;
         if (precond == LH_PRECOND_LEARNED) {
             learned.setup(ctx, ctx->mpnn, A, B, -sigma, 1.0);  // K = A - sigma B
@@ -183,11 +210,12 @@ struct Lobpcg {
         double hist[kStallWindow] = {};  // worst residual of the last iterations
 
         // start block
-        randn<double>(ctx, n, m, Sp(), ld, seed);
+        if (x0) copy_block<T, T>(ctx, n, m, x0, x0_ld, Sp(), ld);
+        else randn<T>(ctx, n, m, Sp(), ld, seed);
         spmm_B(Sp(), MSp(), m);
         if (!cholqr(0, m, true)) LH_THROW(LH_ERR_NUMERIC, "start block is rank deficient");
         if (!rayleigh_ritz(m)) LH_THROW(LH_ERR_NUMERIC, "initial Rayleigh-Ritz failed");
-        gemm_mfma_inplace(ctx, n, m, Sp(), ld, Cm.as<double>(), m, 1.0, false);
+        gemm_mfma_inplace<T>(ctx, n, m, Sp(), ld, Cm.as<double>(), m, 1.0, false);
         spmm_AB(Sp(), ASp(), MSp(), m);
         st.t_setup_us = now_us() - t_start;
 
@@ -195,11 +223,13 @@ struct Lobpcg {
         int it = 0;
         double t_iter0 = now_us();
         std::vector<double> r2(m), mx2(m);
+        double best = 0.0;
+        int since_best = 0;
         for (it = 0; it < max_iter; ++it) {
             // residual into W and the convergence test
-            residual<double>(ctx, n, m, ASp(), MSp(), ld, theta_d.as<double>(), Sp() + m, ld);
-            colsq<double>(ctx, n, m, Sp() + m, ld, cs1.as<double>(), wsc);
-            colsq<double>(ctx, n, m, MSp(), ld, cs2.as<double>(), wsc);
+            residual<T>(ctx, n, m, ASp(), MSp(), ld, theta_d.as<double>(), Sp() + m, ld);
+            colsq<T>(ctx, n, m, Sp() + m, ld, cs1.as<double>(), wsc);
+            colsq<T>(ctx, n, m, MSp(), ld, cs2.as<double>(), wsc);
             d2h(ctx, r2.data(), cs1.p, m * 8);
             d2h(ctx, mx2.data(), cs2.p, m * 8);
             const double floor = 1e-2 * std::fabs(theta[k - 1]);
@@ -231,14 +261,25 @@ struct Lobpcg {
                 fprintf(stderr, "[lobpcg] it %d conv %d/%d worst %.3e theta0 %.6e theta_k %.6e\n", it, nconv, k,
                         worst, theta[0], theta[k - 1]);
             if (nconv == k) break;
+            if (phase_one) {
+                // the FP32 residuals stop falling at the rounding floor
+                // (~u32 times the spread of the spectrum): hand over when the
+                // worst one has not improved by 10 % in 3 iterations
+                if (it == 0 || worst < 0.9 * best) {
+                    best = worst;
+                    since_best = 0;
+                } else if (++since_best >= 3) {
+                    break;
+                }
+            }
             // precondition W
             if (learned_on) {
-                learned.apply(Sp() + m, ld, m);
+                if constexpr (sizeof(T) == 8) learned.apply(Sp() + m, ld, m);
             } else if (poly_degree > 1) {
-                cheb.apply(Sp() + m, ld, m, cs1.as<double>());  // cs1: ||w_c||^2 from the test above
+                cheb->apply(Sp() + m, ld, m, cs1.as<double>());  // cs1: ||w_c||^2 from the test above
                 st.spmm_calls += poly_degree - 1;
             } else if (precond != LH_PRECOND_NONE) {
-                rowscale<double>(ctx, n, m, Sp() + m, ld, dinv.as<double>());
+                rowscale<T>(ctx, n, m, Sp() + m, ld, dinv.as<double>());
             }
             // [W P] orthogonal to X in the B inner product (classical Gram-Schmidt, twice)
             int nb = have_p ? 2 * m : m;
@@ -259,10 +300,10 @@ struct Lobpcg {
             const int N = m + nb;
             if (!rayleigh_ritz(N)) LH_THROW(LH_ERR_NUMERIC, "Rayleigh-Ritz failed at iteration %d", it);
             // X_new = S C ; P_new = S[:, m:N] C[m:N, :]
-            double* S2p = S2.as<double>();
-            gemm_nn<double>(ctx, n, N, m, Sp(), ld, Cm.as<double>(), m, S2p, ld, 1.0, 0.0);
-            gemm_nn<double>(ctx, n, N - m, m, Sp() + m, ld, Cm.as<double>() + (size_t)m * m, m, S2p + 2 * m, ld, 1.0,
-                            0.0);
+            T* S2p = S2.as<T>();
+            gemm_nn<T>(ctx, n, N, m, Sp(), ld, Cm.as<double>(), m, S2p, ld, 1.0, 0.0);
+            gemm_nn<T>(ctx, n, N - m, m, Sp() + m, ld, Cm.as<double>() + (size_t)m * m, m, S2p + 2 * m, ld, 1.0,
+                       0.0);
             std::swap(S, S2);
             spmm_AB(Sp(), ASp(), MSp(), m);
             have_p = true;
@@ -270,20 +311,29 @@ struct Lobpcg {
         HIPX(hipStreamSynchronize(ctx->stream));
         st.iterations = it;
         st.t_iter_us = now_us() - t_iter0;
+        if (!out_vecs) {  // first phase: X stays in S
+            st.t_total_us = now_us() - t_start;
+            return;
+        }
         // final: B-orthonormalise X and take the Ritz pairs of span(X) alone,
         // so that the returned vectors are B-orthonormal to rounding and the
         // eigenvalues are Rayleigh quotients of the returned vectors
         spmm_B(Sp(), MSp(), m);
         if (cholqr(0, m, true) && rayleigh_ritz(m)) {
-            gemm_mfma_inplace(ctx, n, m, Sp(), ld, Cm.as<double>(), m, 1.0, false);
+            gemm_mfma_inplace<T>(ctx, n, m, Sp(), ld, Cm.as<double>(), m, 1.0, false);
         }
         for (int j = 0; j < k; ++j) out_evals[j] = theta[j];
         {
             ArenaOff keep;  // the result outlives the solver's buffers
-            out_vecs.alloc(n * k * 8);
+            out_vecs->alloc(n * k * 8);
         }
-        if (reordered) ro.unpermute(ctx, Sp(), ld, out_vecs.as<double>(), k, n, k);
-        else copy_block<double, double>(ctx, n, k, Sp(), ld, out_vecs.as<double>(), k);
+        const Reorder* order = ext_ro ? ext_ro : (reordered ? &ro : nullptr);
+        if constexpr (sizeof(T) == 8) {
+            if (order) order->unpermute(ctx, Sp(), ld, out_vecs->as<double>(), k, n, k);
+            else copy_block<double, double>(ctx, n, k, Sp(), ld, out_vecs->as<double>(), k);
+        } else {
+            LH_THROW(LH_ERR_ARG, "FP32 blocks are not returned");
+        }
         HIPX(hipStreamSynchronize(ctx->stream));
         st.t_total_us = now_us() - t_start;
         if (st.converged < k) {
@@ -337,6 +387,10 @@ void dense_eigs(lh_ctx* ctx, const lh_csr* A, const lh_csr* B, int k, double* ev
     st.t_total_us = now_us() - t0;
 }
 
+// the mixed-precision solve's FP32 phase hands over at this relative residual
+// (or earlier, when its residuals stop falling)
+constexpr double kMixedSwitchTol = 1e-5;
+
 // one generalized eigenproblem on ctx's stream; throws lh_error on bad
 // arguments or failures, returns the result with code LH_OK or
 // LH_ERR_NOCONV (results of the last iteration)
@@ -345,7 +399,7 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
     const int64_t n = A->n();
     if (B->n() != n) LH_THROW(LH_ERR_ARG, "A and B differ in size (%lld vs %lld)", (long long)n, (long long)B->n());
     if (k <= 0 || k >= n) LH_THROW(LH_ERR_ARG, "k must satisfy 0 < k < n (k=%d, n=%lld)", k, (long long)n);
-    Lobpcg L{};
+    Lobpcg<double> L{};
     L.ctx = ctx;
     L.A = A;
     L.B = B;
@@ -391,7 +445,10 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
     if (store != LH_STORE_AUTO && store != LH_STORE_F32 && store != LH_STORE_F16)
         LH_THROW(LH_ERR_ARG, "poly_storage %d unknown", store);
     L.poly_ratio = (opts && opts->poly_ratio > 1.0) ? opts->poly_ratio : 4.0 * deg * deg;
-    if (opts && opts->precision != LH_PREC_FP64) LH_THROW(LH_ERR_ARG, "precision mode %d not available", opts->precision);
+    const int prec = opts ? opts->precision : LH_PREC_FP64;
+    if (prec != LH_PREC_FP64 && prec != LH_PREC_MIXED) LH_THROW(LH_ERR_ARG, "precision mode %d unknown", prec);
+    if (prec == LH_PREC_MIXED && pc == LH_PRECOND_LEARNED)
+        LH_THROW(LH_ERR_ARG, "the mixed-precision solver takes the Jacobi / Chebyshev-Jacobi preconditioner");
     std::unique_ptr<lh_eig> e(new lh_eig());
     e->ctx = ctx;
     e->n = n;
@@ -405,8 +462,52 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
         code = LH_OK;
         return e.release();
     }
-    L.run(e->evals.data(), e->evecs);
-    L.st.precision_mode = LH_PREC_FP64;
+    if (prec == LH_PREC_MIXED) {
+        // FP32 phase, then the FP64 solve from its X block (same order, same
+        // Chebyshev preconditioner)
+        Lobpcg<float> L1{};
+        L1.ctx = ctx;
+        L1.A = A;
+        L1.B = B;
+        L1.n = n;
+        L1.k = k;
+        L1.m = m;
+        L1.sigma = sigma;
+        L1.tol = std::max(L.tol, kMixedSwitchTol);
+        L1.max_iter = L.max_iter;
+        L1.seed = L.seed;
+        L1.verbose = L.verbose;
+        L1.precond = L.precond;
+        L1.poly_degree = L.poly_degree;
+        L1.poly_ratio = L.poly_ratio;
+        L1.poly_half = L.poly_half;
+        L1.phase_one = true;
+        L1.run(nullptr, nullptr);
+        DevBuf X0((size_t)n * m * 8);
+        copy_block<float, double>(ctx, n, m, L1.Sp(), L1.ld, X0.as<double>(), m);
+        L.A = L1.A;
+        L.B = L1.B;
+        L.ext_ro = L1.reordered ? &L1.ro : nullptr;
+        L.cheb = L1.cheb;
+        L.cheb_ready = L.poly_degree > 1;
+        L.x0 = X0.as<double>();
+        L.x0_ld = m;
+        L.max_iter = std::max(1, L.max_iter - L1.st.iterations);
+        L.run(e->evals.data(), &e->evecs);
+        L.st.iterations += L1.st.iterations;
+        L.st.iterations_fp32 = L1.st.iterations;
+        L.st.spmm_calls += L1.st.spmm_calls;
+        L.st.rr_calls += L1.st.rr_calls;
+        L.st.restarts += L1.st.restarts;
+        L.st.t_rr_us += L1.st.t_rr_us;
+        L.st.t_iter_us += L1.st.t_iter_us;
+        L.st.t_setup_us += L1.st.t_setup_us;
+        L.st.t_total_us += L1.st.t_total_us;
+        L.st.precision_mode = LH_PREC_MIXED;
+    } else {
+        L.run(e->evals.data(), &e->evecs);
+        L.st.precision_mode = LH_PREC_FP64;
+    }
     st = L.st;
     code = (L.st.converged < k) ? LH_ERR_NOCONV : LH_OK;
     return e.release();

@@ -223,6 +223,8 @@ void spmm_pair(lh_ctx* ctx, const lh_csr* a, const lh_csr* b, const XT* X, XT* Y
                int ldy);
 
 void refresh_dense_diag(lh_ctx* ctx, lh_csr* a);
+// FP32 copy of an FP64 matrix on the same pattern
+std::unique_ptr<lh_csr> csr_to_f32(lh_ctx* ctx, const lh_csr* s);
 
 // 16-byte vector of a block row: 4 floats or 2 doubles per lane
 template <typename T> struct Vec16;

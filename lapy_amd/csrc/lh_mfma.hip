@@ -2,9 +2,12 @@
 // eigensolver: the Gram matrix C = U^T V and the update Y = alpha U C + beta Y,
 // with U, V, Y row-major n x (p | q) blocks (n ~ 10^6, p, q <= 192).
 //
-// Both use v_mfma_f64_16x16x4f64 (one wave: D[16x16] += A[16x4] B[4x16]);
-// lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16], and holds
-// D[l / 16 + 4 r][l % 16] in accumulator r (r = 0..3).
+// FP64 blocks use v_mfma_f64_16x16x4f64, FP32 blocks (the mixed-precision
+// eigensolver) v_mfma_f32_16x16x4f32; both compute D[16x16] += A[16x4] B[4x16]
+// per wave, lane l supplying A[l % 16][l / 16] and B[l / 16][l % 16].  Lane l
+// holds D[l / 16 + 4 r][l % 16] (FP64) or D[4 (l / 16) + r][l % 16] (FP32) in
+// accumulator r (r = 0..3).  FP32 products accumulate in FP32 within a wave's
+// row range; the Gram partials are summed in FP64.
 //
 // Gram: the reduction runs over the rows.  A wave owns a 64 x 64 output tile
 // and a range of rows; per group of 4 rows each lane loads 4 consecutive
@@ -26,20 +29,47 @@ namespace {
 
 typedef double d2_t __attribute__((ext_vector_type(2)));
 typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ d4_t mfma16(double a, double b, d4_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+template <typename T>
+struct Mf;
+template <>
+struct Mf<double> {
+    typedef d4_t v4;
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // row of the 16 x 16 block held in accumulator r by the lanes of group kg
+    static __device__ __forceinline__ int drow(int kg, int r) { return kg + 4 * r; }
+};
+template <>
+struct Mf<float> {
+    typedef f4_t v4;
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int kg, int r) { return 4 * kg + r; }
+};
+
+template <typename T>
+__device__ __forceinline__ typename Mf<T>::v4 zero4() {
+    return (typename Mf<T>::v4){T(0), T(0), T(0), T(0)};
 }
 
 // four consecutive values row[c .. c+3] of a row-major block, zero outside
-// [0, ncols); VEC: the 32 bytes are 16-byte aligned and in bounds as a whole
-template <bool VEC>
-__device__ __forceinline__ d4_t load4(const double* __restrict__ row, int c, int ncols) {
-    d4_t v = {0.0, 0.0, 0.0, 0.0};
+// [0, ncols); VEC: the 4 values are 16-byte aligned (FP64: each pair) and in
+// bounds as a whole
+template <bool VEC, typename T>
+__device__ __forceinline__ typename Mf<T>::v4 load4(const T* __restrict__ row, int c, int ncols) {
+    typename Mf<T>::v4 v = zero4<T>();
     if (VEC && c + 3 < ncols) {
-        const d2_t lo = *reinterpret_cast<const d2_t*>(row + c);
-        const d2_t hi = *reinterpret_cast<const d2_t*>(row + c + 2);
-        v[0] = lo[0]; v[1] = lo[1]; v[2] = hi[0]; v[3] = hi[1];
+        if constexpr (sizeof(T) == 8) {
+            const d2_t lo = *reinterpret_cast<const d2_t*>(row + c);
+            const d2_t hi = *reinterpret_cast<const d2_t*>(row + c + 2);
+            v[0] = lo[0]; v[1] = lo[1]; v[2] = hi[0]; v[3] = hi[1];
+        } else {
+            v = *reinterpret_cast<const f4_t*>(row + c);
+        }
     } else {
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -57,10 +87,11 @@ __device__ __forceinline__ void tile_of(int t, int tv, int sym, int& ti, int& tj
     tj = ti + t;
 }
 
-template <bool VEC>
+template <bool VEC, typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-k_gram_mfma(int64_t n, int64_t rows_per_wave, int nwaves, const double* __restrict__ U, int ldu, int p,
-            const double* __restrict__ V, int ldv, int q, int tv, int sym, double* __restrict__ part) {
+k_gram_mfma(int64_t n, int64_t rows_per_wave, int nwaves, const T* __restrict__ U, int ldu, int p,
+            const T* __restrict__ V, int ldv, int q, int tv, int sym, double* __restrict__ part) {
+    typedef typename Mf<T>::v4 v4;
     const int lane = threadIdx.x & 63;
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= nwaves) return;  // whole wave exits together
@@ -71,23 +102,23 @@ k_gram_mfma(int64_t n, int64_t rows_per_wave, int nwaves, const double* __restri
     const int il = lane & 15, kg = lane >> 4;
     const int64_t rb = (int64_t)w * rows_per_wave;
     const int64_t re = (rb + rows_per_wave < n) ? rb + rows_per_wave : n;
-    d4_t acc[4][4];
+    v4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = (d4_t){0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < 4; ++b) acc[a][b] = zero4<T>();
     const int cu = u0 + 4 * il, cv = v0 + 4 * il;
     for (int64_t r0 = rb; r0 < re; r0 += 16) {
-        d4_t a[4], b[4];
+        v4 a[4], b[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int64_t r = r0 + 4 * s + kg;
             if (r < re) {
-                a[s] = load4<VEC>(U + r * ldu, cu, p);
-                b[s] = load4<VEC>(V + r * ldv, cv, q);
+                a[s] = load4<VEC, T>(U + r * ldu, cu, p);
+                b[s] = load4<VEC, T>(V + r * ldv, cv, q);
             } else {
-                a[s] = (d4_t){0.0, 0.0, 0.0, 0.0};
-                b[s] = (d4_t){0.0, 0.0, 0.0, 0.0};
+                a[s] = zero4<T>();
+                b[s] = zero4<T>();
             }
         }
 #pragma unroll
@@ -95,7 +126,7 @@ k_gram_mfma(int64_t n, int64_t rows_per_wave, int nwaves, const double* __restri
 #pragma unroll
             for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = mfma16(a[s][bi], b[s][bj], acc[bi][bj]);
+                for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = Mf<T>::mma(a[s][bi], b[s][bj], acc[bi][bj]);
     }
     // partial tile [w][tile][64 x 64] in natural order
     double* dst = part + ((int64_t)w * gridDim.y + tile) * 4096;
@@ -105,9 +136,9 @@ k_gram_mfma(int64_t n, int64_t rows_per_wave, int nwaves, const double* __restri
         for (int bj = 0; bj < 4; ++bj)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = 4 * (kg + 4 * r) + bi;  // tile-local column of U
-                const int j = 4 * il + bj;            // tile-local column of V
-                dst[i * 64 + j] = acc[bi][bj][r];
+                const int i = 4 * Mf<T>::drow(kg, r) + bi;  // tile-local column of U
+                const int j = 4 * il + bj;                  // tile-local column of V
+                dst[i * 64 + j] = (double)acc[bi][bj][r];
             }
 }
 
@@ -147,10 +178,11 @@ k_gram_mfma_reduce(int nwaves, int ntile, int p, int q, int tv, int sym, const d
 // j are zero), so a 64-column block only sums k < its last column + 1.
 // INPLACE (Y = U, p = q <= 256): a workgroup owns all column blocks of its
 // rows, and writes only after every wave of it has read its rows of U.
-template <bool VEC, bool INPLACE>
+template <bool VEC, bool INPLACE, typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-k_gemm_mfma(int64_t n, int p, int q, const double* U, int ldu, const double* __restrict__ Cm, int ldc, double* Y,
-            int ldy, double alpha, double beta, int upper, int nq) {
+k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restrict__ Cm, int ldc, T* Y, int ldy,
+            double alpha, double beta, int upper, int nq) {
+    typedef typename Mf<T>::v4 v4;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int il = lane & 15, kg = lane >> 4;
     int64_t rw;  // first row of this wave
@@ -165,58 +197,65 @@ k_gemm_mfma(int64_t n, int p, int q, const double* U, int ldu, const double* __r
         if (rw >= n) return;  // whole wave
     }
     const int kend = upper ? min(p, n0 + 64) : p;
-    d4_t acc[4][4];
+    v4 acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = (d4_t){0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < 4; ++b) acc[a][b] = zero4<T>();
     for (int k0 = 0; k0 < kend; k0 += 16) {
         const int kc = k0 + 4 * kg;  // this lane's 4 summed indices
-        d4_t a[4];
+        v4 a[4];
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi) {
             const int64_t r = rw + 16 * bi + il;
-            a[bi] = (r < n) ? load4<VEC>(U + r * ldu, kc, kend) : (d4_t){0.0, 0.0, 0.0, 0.0};
+            a[bi] = (r < n) ? load4<VEC, T>(U + r * ldu, kc, kend) : zero4<T>();
         }
-        double b[4][4];
+        T b[4][4];
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
             for (int bj = 0; bj < 4; ++bj) {
                 const int col = n0 + 16 * bj + il;
-                b[jj][bj] = (kc + jj < kend && col < q) ? Cm[(int64_t)(kc + jj) * ldc + col] : 0.0;
+                b[jj][bj] = (kc + jj < kend && col < q) ? (T)Cm[(int64_t)(kc + jj) * ldc + col] : T(0);
             }
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
             for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = mfma16(a[bi][jj], b[jj][bj], acc[bi][bj]);
+                for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = Mf<T>::mma(a[bi][jj], b[jj][bj], acc[bi][bj]);
     }
     if (INPLACE) __syncthreads();
 #pragma unroll
     for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int64_t row = rw + 16 * bi + kg + 4 * r;
+            const int64_t row = rw + 16 * bi + Mf<T>::drow(kg, r);
             if (row >= n) continue;
 #pragma unroll
             for (int bj = 0; bj < 4; ++bj) {
                 const int col = n0 + 16 * bj + il;
                 if (col >= q) continue;
-                double v = alpha * acc[bi][bj][r];
-                if (beta != 0.0) v += beta * Y[row * ldy + col];
-                Y[row * ldy + col] = v;
+                double v = alpha * (double)acc[bi][bj][r];
+                if (beta != 0.0) v += beta * (double)Y[row * ldy + col];
+                Y[row * ldy + col] = (T)v;
             }
         }
 }
 
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+// 16-byte vector loads (FP64: pairs, FP32: quadruples) at column offsets
+// divisible by 4
+template <typename T>
+bool vec_ok(const T* p, int ld) {
+    return aligned16(p) && ld % (int)(16 / sizeof(T)) == 0;
+}
 
 }  // namespace
 
-void gram_mfma(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q, double* C,
-               int ldc, DevBuf& ws, double alpha, bool sym) {
+template <typename T>
+void gram_mfma(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, int ldv, int q, double* C, int ldc,
+               DevBuf& ws, double alpha, bool sym) {
     if (sym && p != q) LH_THROW(LH_ERR_ARG, "symmetric Gram needs p == q");
     const int tu = (p + 63) / 64, tv = (q + 63) / 64;
     const int ntile = sym ? tu * (tu + 1) / 2 : tu * tv;
@@ -228,85 +267,150 @@ void gram_mfma(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const do
     rpw = ((rpw + 15) / 16) * 16;
     nw = (n + rpw - 1) / rpw;
     ws.ensure((size_t)nw * ntile * 4096 * sizeof(double));
-    const bool vec = aligned16(U) && aligned16(V) && ldu % 2 == 0 && ldv % 2 == 0;
+    const bool vec = vec_ok(U, ldu) && vec_ok(V, ldv);
     dim3 grid((unsigned)((nw + 3) / 4), (unsigned)ntile);
     if (vec)
-        k_gram_mfma<true><<<grid, 256, 0, ctx->stream>>>(n, rpw, (int)nw, U, ldu, p, V, ldv, q, tv, sym ? 1 : 0,
-                                                         ws.as<double>());
+        k_gram_mfma<true, T><<<grid, 256, 0, ctx->stream>>>(n, rpw, (int)nw, U, ldu, p, V, ldv, q, tv, sym ? 1 : 0,
+                                                            ws.as<double>());
     else
-        k_gram_mfma<false><<<grid, 256, 0, ctx->stream>>>(n, rpw, (int)nw, U, ldu, p, V, ldv, q, tv, sym ? 1 : 0,
-                                                          ws.as<double>());
+        k_gram_mfma<false, T><<<grid, 256, 0, ctx->stream>>>(n, rpw, (int)nw, U, ldu, p, V, ldv, q, tv, sym ? 1 : 0,
+                                                             ws.as<double>());
     LH_CHECK_LAUNCH();
     k_gram_mfma_reduce<<<nblocks((int64_t)p * q, 64), 256, 0, ctx->stream>>>((int)nw, ntile, p, q, tv, sym ? 1 : 0,
                                                                              ws.as<double>(), C, ldc, alpha);
     LH_CHECK_LAUNCH();
 }
 
-void gemm_mfma(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* C, int ldc, double* Y,
-               int ldy, double alpha, double beta, bool upper) {
+template <typename T>
+void gemm_mfma(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const double* C, int ldc, T* Y, int ldy,
+               double alpha, double beta, bool upper) {
     if (n == 0 || q == 0) return;
-    const bool vec = aligned16(U) && ldu % 2 == 0;
+    const bool vec = vec_ok(U, ldu);
     const int nq = (q + 63) / 64;
     dim3 grid(nblocks_xcd(n, 256), (unsigned)nq);
     const int up = upper ? 1 : 0;
     if (vec)
-        k_gemm_mfma<true, false><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta, up, nq);
+        k_gemm_mfma<true, false, T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta, up,
+                                                                   nq);
     else
-        k_gemm_mfma<false, false><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta, up, nq);
+        k_gemm_mfma<false, false, T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta, up,
+                                                                    nq);
     LH_CHECK_LAUNCH();
 }
 
-void gemm_mfma_inplace(lh_ctx* ctx, int64_t n, int p, double* X, int ldx, const double* C, int ldc, double alpha,
+template <typename T>
+void gemm_mfma_inplace(lh_ctx* ctx, int64_t n, int p, T* X, int ldx, const double* C, int ldc, double alpha,
                        bool upper) {
     if (n == 0 || p == 0) return;
     if (p > 256) LH_THROW(LH_ERR_ARG, "in-place block update needs width <= 256, got %d", p);
-    const bool vec = aligned16(X) && ldx % 2 == 0;
+    const bool vec = vec_ok(X, ldx);
     const int nq = (p + 63) / 64;
     const int rsub = nq == 1 ? 4 : (nq == 2 ? 2 : 1);
     const int bs = 64 * nq * rsub;
     const unsigned nb = nblocks_xcd(n, 64 * rsub);
     const int up = upper ? 1 : 0;
     if (vec)
-        k_gemm_mfma<true, true><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0, up, nq);
+        k_gemm_mfma<true, true, T><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0, up, nq);
     else
-        k_gemm_mfma<false, true><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0, up, nq);
+        k_gemm_mfma<false, true, T><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0, up, nq);
     LH_CHECK_LAUNCH();
 }
+
+template void gram_mfma<double>(lh_ctx*, int64_t, const double*, int, int, const double*, int, int, double*, int,
+                                DevBuf&, double, bool);
+template void gram_mfma<float>(lh_ctx*, int64_t, const float*, int, int, const float*, int, int, double*, int,
+                               DevBuf&, double, bool);
+template void gemm_mfma<double>(lh_ctx*, int64_t, int, int, const double*, int, const double*, int, double*, int,
+                                double, double, bool);
+template void gemm_mfma<float>(lh_ctx*, int64_t, int, int, const float*, int, const double*, int, float*, int, double,
+                               double, bool);
+template void gemm_mfma_inplace<double>(lh_ctx*, int64_t, int, double*, int, const double*, int, double, bool);
+template void gemm_mfma_inplace<float>(lh_ctx*, int64_t, int, float*, int, const double*, int, double, bool);
 
 }  // namespace lh
 
 using namespace lh;
 
+namespace {
+
+// average duration (us) of `iters` launches of f on ctx's stream
+template <typename F>
+double time_launches(lh_ctx* ctx, int iters, F&& f) {
+    hipEvent_t e0, e1;
+    HIPX(hipEventCreate(&e0));
+    HIPX(hipEventCreate(&e1));
+    HIPX(hipEventRecord(e0, ctx->stream));
+    for (int it = 0; it < iters; ++it) f();
+    HIPX(hipEventRecord(e1, ctx->stream));
+    HIPX(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPX(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 1000.0 * ms / iters;
+}
+
+// host FP64 block -> device block of T (converted on the device)
+template <typename T>
+void upload_block(lh_ctx* ctx, DevBuf& dst, const double* src, size_t count) {
+    dst.ensure(count * sizeof(T));
+    if constexpr (sizeof(T) == 8) {
+        h2d(ctx, dst.p, src, count * 8);
+    } else {
+        DevBuf tmp(count * 8);
+        h2d(ctx, tmp.p, src, count * 8);
+        copy_block<double, T>(ctx, (int64_t)count, 1, tmp.as<double>(), 1, dst.as<T>(), 1);
+    }
+}
+
+template <typename T>
+void block_gram(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q, bool sym,
+                double* C, int iters, double* avg_us) {
+    DevBuf du, dv, dc((size_t)p * q * 8), ws;
+    upload_block<T>(ctx, du, U, (size_t)n * ldu);
+    upload_block<T>(ctx, dv, V, (size_t)n * ldv);
+    auto run = [&] { gram_mfma<T>(ctx, n, du.as<T>(), ldu, p, dv.as<T>(), ldv, q, dc.as<double>(), q, ws, 1.0, sym); };
+    run();
+    if (avg_us && iters > 0) *avg_us = time_launches(ctx, iters, run);
+    d2h(ctx, C, dc.p, (size_t)p * q * 8);
+}
+
+template <typename T>
+void block_gemm(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* Cs, double* Y, int ldy,
+                double alpha, double beta, bool upper, bool inplace, int iters, double* avg_us) {
+    DevBuf du, dy, dc((size_t)p * q * 8);
+    if (!inplace) upload_block<T>(ctx, du, U, (size_t)n * ldu);
+    upload_block<T>(ctx, dy, Y, (size_t)n * ldy);
+    h2d(ctx, dc.p, Cs, (size_t)p * q * 8);
+    auto run = [&](double b) {
+        if (inplace) gemm_mfma_inplace<T>(ctx, n, p, dy.as<T>(), ldy, dc.as<double>(), q, alpha, upper);
+        else gemm_mfma<T>(ctx, n, p, q, du.as<T>(), ldu, dc.as<double>(), q, dy.as<T>(), ldy, alpha, b, upper);
+    };
+    run(beta);
+    if constexpr (sizeof(T) == 8) {
+        d2h(ctx, Y, dy.p, (size_t)n * ldy * 8);
+    } else {
+        DevBuf tmp((size_t)n * ldy * 8);
+        copy_block<T, double>(ctx, n * ldy, 1, dy.as<T>(), 1, tmp.as<double>(), 1);
+        d2h(ctx, Y, tmp.p, (size_t)n * ldy * 8);
+    }
+    if (avg_us && iters > 0) *avg_us = time_launches(ctx, iters, [&] { run(0.0); });
+}
+
+}  // namespace
+
 extern "C" {
 
-int lh_block_gram(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q, int sym,
+int lh_block_gram(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q, int flags,
                   double* C, int iters, double* avg_us) {
     if (!ctx || !U || !V || !C || n <= 0 || p <= 0 || q <= 0 || ldu < p || ldv < q) return LH_ERR_ARG;
     if (p > 1024 || q > 1024) return LH_ERR_ARG;
+    const bool sym = (flags & LH_GRAM_SYM) != 0, f32 = (flags & LH_BLOCK_F32) != 0;
     LH_API_BEGIN(ctx)
     HIPX(hipSetDevice(ctx->device));
     if (sym && p != q) LH_THROW(LH_ERR_ARG, "symmetric Gram needs p == q");
-    DevBuf du(n * ldu * 8), dv(n * ldv * 8), dc((size_t)p * q * 8), ws;
-    h2d(ctx, du.p, U, n * ldu * 8);
-    h2d(ctx, dv.p, V, n * ldv * 8);
-    gram_mfma(ctx, n, du.as<double>(), ldu, p, dv.as<double>(), ldv, q, dc.as<double>(), q, ws, 1.0, sym != 0);
-    if (avg_us && iters > 0) {
-        hipEvent_t e0, e1;
-        HIPX(hipEventCreate(&e0));
-        HIPX(hipEventCreate(&e1));
-        HIPX(hipEventRecord(e0, ctx->stream));
-        for (int it = 0; it < iters; ++it)
-            gram_mfma(ctx, n, du.as<double>(), ldu, p, dv.as<double>(), ldv, q, dc.as<double>(), q, ws, 1.0,
-                      sym != 0);
-        HIPX(hipEventRecord(e1, ctx->stream));
-        HIPX(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIPX(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        *avg_us = 1000.0 * ms / iters;
-    }
-    d2h(ctx, C, dc.p, (size_t)p * q * 8);
+    if (f32) block_gram<float>(ctx, n, U, ldu, p, V, ldv, q, sym, C, iters, avg_us);
+    else block_gram<double>(ctx, n, U, ldu, p, V, ldv, q, sym, C, iters, avg_us);
     LH_API_END(ctx)
 }
 
@@ -315,35 +419,13 @@ int lh_block_gemm(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu
     if (!ctx || !Cs || !Y || n <= 0 || p <= 0 || q <= 0 || ldy < q) return LH_ERR_ARG;
     if (p > 1024 || q > 1024) return LH_ERR_ARG;
     const bool upper = (flags & LH_GEMM_UPPER) != 0, inplace = (flags & LH_GEMM_INPLACE) != 0;
+    const bool f32 = (flags & LH_BLOCK_F32) != 0;
     if (inplace && (p != q || p > 256 || beta != 0.0)) return LH_ERR_ARG;
     if (!inplace && (!U || ldu < p)) return LH_ERR_ARG;
     LH_API_BEGIN(ctx)
     HIPX(hipSetDevice(ctx->device));
-    DevBuf du(inplace ? 8 : n * ldu * 8), dy(n * ldy * 8), dc((size_t)p * q * 8);
-    if (!inplace) h2d(ctx, du.p, U, n * ldu * 8);
-    h2d(ctx, dy.p, Y, n * ldy * 8);
-    h2d(ctx, dc.p, Cs, (size_t)p * q * 8);
-    if (inplace) gemm_mfma_inplace(ctx, n, p, dy.as<double>(), ldy, dc.as<double>(), q, alpha, upper);
-    else gemm_mfma(ctx, n, p, q, du.as<double>(), ldu, dc.as<double>(), q, dy.as<double>(), ldy, alpha, beta, upper);
-    d2h(ctx, Y, dy.p, n * ldy * 8);
-    if (avg_us && iters > 0) {
-        hipEvent_t e0, e1;
-        HIPX(hipEventCreate(&e0));
-        HIPX(hipEventCreate(&e1));
-        HIPX(hipEventRecord(e0, ctx->stream));
-        for (int it = 0; it < iters; ++it) {
-            if (inplace) gemm_mfma_inplace(ctx, n, p, dy.as<double>(), ldy, dc.as<double>(), q, alpha, upper);
-            else gemm_mfma(ctx, n, p, q, du.as<double>(), ldu, dc.as<double>(), q, dy.as<double>(), ldy, alpha, 0.0,
-                           upper);
-        }
-        HIPX(hipEventRecord(e1, ctx->stream));
-        HIPX(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIPX(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        *avg_us = 1000.0 * ms / iters;
-    }
+    if (f32) block_gemm<float>(ctx, n, p, q, U, ldu, Cs, Y, ldy, alpha, beta, upper, inplace, iters, avg_us);
+    else block_gemm<double>(ctx, n, p, q, U, ldu, Cs, Y, ldy, alpha, beta, upper, inplace, iters, avg_us);
     LH_API_END(ctx)
 }
 

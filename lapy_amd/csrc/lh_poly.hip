@@ -45,7 +45,8 @@ __global__ void k_dinv_from(int64_t n, const int32_t* __restrict__ diagpos, cons
 }
 
 // b = W (FP32 copy), x_1 = alpha D^-1 b, x_0 = 0
-__global__ void k_cheb_init(int64_t n, int m, const double* __restrict__ W, int ldw, const float* __restrict__ dinv,
+template <typename WT>
+__global__ void k_cheb_init(int64_t n, int m, const WT* __restrict__ W, int ldw, const float* __restrict__ dinv,
                             float alpha, float* __restrict__ bvec, float* __restrict__ x1, float* __restrict__ x0) {
     int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n * m) return;
@@ -133,7 +134,8 @@ __global__ void k_col_scales(int m, const double* __restrict__ colnorm2, double 
 }
 
 // b = s_c W, x_1 = alpha D^-1 b, x_0 = 0 (FP16)
-__global__ void k_cheb_init_h(int64_t n, int m, const double* __restrict__ W, int ldw, const float* __restrict__ dinv,
+template <typename WT>
+__global__ void k_cheb_init_h(int64_t n, int m, const WT* __restrict__ W, int ldw, const float* __restrict__ dinv,
                               const float* __restrict__ scale, float alpha, _Float16* __restrict__ bvec,
                               _Float16* __restrict__ x1, _Float16* __restrict__ x0) {
     int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,21 +180,23 @@ k_cheb_step_h(int64_t n, int m, const int32_t* __restrict__ rowptr, const int32_
     *reinterpret_cast<half8*>(xprev + o) = f2h_sat(r);
 }
 
+template <typename WT>
 __global__ void k_cheb_out_h(int64_t n, int m, const _Float16* __restrict__ x, const double* __restrict__ inv,
-                             double* __restrict__ W, int ldw) {
+                             WT* __restrict__ W, int ldw) {
     int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n * m) return;
     int64_t i = gid / m;
     int c = (int)(gid - i * m);
-    W[i * ldw + c] = (double)(float)x[gid] * inv[c];
+    W[i * ldw + c] = (WT)((double)(float)x[gid] * inv[c]);
 }
 
-__global__ void k_cheb_out(int64_t n, int m, const float* __restrict__ x, double* __restrict__ W, int ldw) {
+template <typename WT>
+__global__ void k_cheb_out(int64_t n, int m, const float* __restrict__ x, WT* __restrict__ W, int ldw) {
     int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n * m) return;
     int64_t i = gid / m;
     int c = (int)(gid - i * m);
-    W[i * ldw + c] = (double)x[gid];
+    W[i * ldw + c] = (WT)x[gid];
 }
 
 __global__ void k_to_f32(int64_t cnt, const double* __restrict__ s, float* __restrict__ d) {
@@ -377,7 +381,8 @@ void ChebPrecond::step(int m, const void* bvec, const void* xk, void* xp, double
 }
 
 // b, x_1, x_0 from W; returns the (x_k, x_prev) buffers
-void ChebPrecond::init(const double* W, int ldw, int m, const double* colnorm2, void** xk, void** xp) {
+template <typename WT>
+void ChebPrecond::init(const WT* W, int ldw, int m, const double* colnorm2, void** xk, void** xp) {
     const float fa = (float)(2.0 / (bmax + amin));
     if (half) {
         if (!colnorm2) LH_THROW(LH_ERR_ARG, "FP16 Chebyshev storage needs the column norms of the block");
@@ -397,7 +402,8 @@ void ChebPrecond::init(const double* W, int ldw, int m, const double* colnorm2, 
     *xp = D1.p;
 }
 
-void ChebPrecond::apply(double* W, int ldw, int m, const double* colnorm2) {
+template <typename WT>
+void ChebPrecond::apply(WT* W, int ldw, int m, const double* colnorm2) {
     // Chebyshev semi-iteration (Golub-Varga two-step form) for K x = w,
     // Jacobi-scaled, on [amin, bmax]
     if (m > mmax) LH_THROW(LH_ERR_ARG, "Chebyshev block width %d exceeds %d", m, mmax);
@@ -420,6 +426,8 @@ void ChebPrecond::apply(double* W, int ldw, int m, const double* colnorm2) {
     }
     LH_CHECK_LAUNCH();
 }
+template void ChebPrecond::apply<double>(double*, int, int, const double*);
+template void ChebPrecond::apply<float>(float*, int, int, const double*);
 
 double ChebPrecond::time_step(int m, int iters) {
     // random block, then `iters` back-to-back steps between HIP events

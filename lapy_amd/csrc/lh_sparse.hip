@@ -212,6 +212,19 @@ void copy_block_f64_f32(lh_ctx* ctx, const double* s, float* d, int64_t cnt) {
     LH_CHECK_LAUNCH();
 }
 
+std::unique_ptr<lh_csr> csr_to_f32(lh_ctx* ctx, const lh_csr* s) {
+    if (s->dtype != LH_F64) LH_THROW(LH_ERR_ARG, "expects an FP64 matrix");
+    auto d = std::make_unique<lh_csr>();
+    d->ctx = ctx;
+    d->pat = s->pat;
+    d->dtype = LH_F32;
+    d->is_diag = s->is_diag;
+    d->vals.alloc(s->nnz() * 4);
+    copy_block_f64_f32(ctx, s->vals.as<double>(), d->vals.as<float>(), s->nnz());
+    if (s->is_diag) refresh_dense_diag(ctx, d.get());
+    return d;
+}
+
 template <typename T>
 __global__ void k_fill_pattern(int64_t cnt, T* __restrict__ x) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -230,23 +243,13 @@ extern "C" int lh_time_spmm(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int pre
     const size_t es = precision == LH_PREC_FP64 ? 8 : 4;
     DevBuf X(n * ncol * es), YA(n * ncol * es), YB(n * ncol * es);
     // FP32 SpMM needs FP32 values: convert on the fly for the timing copy
-    lh_csr A32, B32;
+    std::unique_ptr<lh_csr> A32, B32;
     const lh_csr *pa = A, *pb = B;
-    auto to32 = [&](const lh_csr* s, lh_csr& d) {
-        d.ctx = ctx;
-        d.pat = s->pat;
-        d.dtype = LH_F32;
-        d.is_diag = s->is_diag;
-        d.vals.alloc(s->nnz() * 4);
-        copy_block_f64_f32(ctx, s->vals.as<double>(), d.vals.as<float>(), s->nnz());
-        if (s->is_diag) refresh_dense_diag(ctx, &d);
-    };
     if (precision != LH_PREC_FP64) {
-        if (A->dtype != LH_F64 || B->dtype != LH_F64) LH_THROW(LH_ERR_ARG, "expects FP64 matrices");
-        to32(A, A32);
-        to32(B, B32);
-        pa = &A32;
-        pb = &B32;
+        A32 = csr_to_f32(ctx, A);
+        B32 = csr_to_f32(ctx, B);
+        pa = A32.get();
+        pb = B32.get();
     }
     auto launch = [&]() {
         if (precision == LH_PREC_FP64) spmm_pair<double, double>(ctx, pa, pb, X.as<double>(), YA.as<double>(), YB.as<double>(), ncol, ncol, ncol);

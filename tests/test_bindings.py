@@ -40,7 +40,10 @@ def test_block_products_argument_types(fake):
     _lib.block_gemm(ctx, U, 4, np.zeros((4, 3)), np.zeros((10, 5)), 1.0, 0.5)
     _lib.block_gemm(ctx, U, 4, np.zeros((4, 3)), np.zeros((10, 5)), iters=2)
     _lib.block_gemm(ctx, None, 4, np.zeros((4, 4)), np.zeros((10, 6)), inplace=True, upper=True)
-    assert calls == ["lh_block_gram", "lh_block_gram", "lh_block_gemm", "lh_block_gemm", "lh_block_gemm"]
+    _lib.block_gram(ctx, U, U, 4, 4, f32=True)
+    _lib.block_gemm(ctx, U, 4, np.zeros((4, 3)), np.zeros((10, 5)), f32=True)
+    assert calls == ["lh_block_gram", "lh_block_gram", "lh_block_gemm", "lh_block_gemm", "lh_block_gemm",
+                     "lh_block_gram", "lh_block_gemm"]
 
 
 def test_eigs_and_timing_argument_types(fake):

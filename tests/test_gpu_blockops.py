@@ -104,3 +104,54 @@ def test_gemm_upper_skips_only_zero_blocks():
     y_full, _ = _lib.block_gemm(ctx, U, 128, Cm, Y0)
     y_up, _ = _lib.block_gemm(ctx, U, 128, Cm, Y0, upper=True)
     np.testing.assert_allclose(y_up, y_full, rtol=0, atol=1e-12)
+
+
+def _f32(a):
+    return a.astype(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("n,p,q,ld,sym", [
+    (10000, 64, 64, 192, False),
+    (10007, 192, 192, 192, True),
+    (3001, 56, 112, 168, False),
+    (2000, 100, 37, 101, False),
+    (17, 5, 3, 7, False),
+])
+def test_gram_fp32_matrix_cores(n, p, q, ld, sym):
+    """FP32 blocks (the mixed-precision eigensolver's): products of the
+    FP32-rounded inputs, FP32 accumulation within a wave's rows (relative
+    error ~1e-6), FP64 across waves."""
+    rng = np.random.default_rng(n + 3 * p + q)
+    U = rng.standard_normal((n, ld))
+    V = U * rng.uniform(0.5, 2.0, n)[:, None] if sym else rng.standard_normal((n, ld))
+    if sym:
+        q = p
+    C, _ = _lib.block_gram(_lib.default_context(), U, V, p, q, sym=sym, f32=True)
+    ref = _f32(U)[:, :p].T @ _f32(V)[:, :q]
+    np.testing.assert_allclose(C, ref, rtol=0, atol=2e-6 * np.sqrt(n) * 8)
+    # an FP64 Gram of the unrounded inputs differs by the rounding, far more
+    # than the FP32 Gram's own error: the rounding really happened
+    assert np.abs(C - U[:, :p].T @ V[:, :q]).max() > 1e-10
+
+
+@pytest.mark.parametrize("n,p,q,ld,alpha,beta,inplace,upper", [
+    (10000, 192, 64, 192, 1.0, 0.0, False, False),
+    (4099, 64, 128, 192, -1.0, 1.0, False, False),
+    (2000, 37, 100, 101, 1.0, 0.0, False, False),
+    (10007, 128, 128, 192, 1.0, 0.0, True, True),
+    (1001, 72, 72, 73, 1.0, 0.0, True, False),
+])
+def test_gemm_fp32_matrix_cores(n, p, q, ld, alpha, beta, inplace, upper):
+    rng = np.random.default_rng(n * 5 + p)
+    U = rng.standard_normal((n, ld))
+    Cm = rng.standard_normal((p, q))
+    if upper:
+        Cm = np.triu(Cm)
+    Y0 = rng.standard_normal((n, ld))
+    Y, _ = _lib.block_gemm(_lib.default_context(), None if inplace else U, p, Cm, Y0, alpha, beta, f32=True,
+                           inplace=inplace, upper=upper)
+    src = _f32(Y0) if inplace else _f32(U)
+    ref = _f32(Y0)
+    ref[:, :q] = alpha * (src[:, :p] @ _f32(Cm)) + beta * _f32(Y0)[:, :q]
+    np.testing.assert_allclose(Y, ref, rtol=0, atol=3e-6 * np.sqrt(p) * 8 * (1 + abs(beta)))
+    assert np.array_equal(Y[:, q:], _f32(Y0)[:, q:])

@@ -103,6 +103,33 @@ def test_ellipsoid_l6_preconditioner_storage(storage, degree):
     assert_eigs_match(lam, vec, lam_o, vec_o, b, k_complete=False)
 
 
+@pytest.mark.parametrize("level,lump", [(5, False), (6, False), (5, True)])
+def test_mixed_precision_matches_oracle(level, lump):
+    """precision="mixed": an FP32 phase (FP32 blocks, SpMM and matrix-core
+    Grams) hands its X block to FP64 iterations; the result meets the same
+    bar against the FP64 oracle, and the stats say how the work split."""
+    v, t = meshgen.large_ellipsoid(level=level)
+    fem_ = Solver(TriaMesh(v, t), lump=lump)
+    lam, vec = fem_.eigs(k=50, precision="mixed")
+    st = fem_.last_stats
+    assert st.precision_mode == 1
+    assert 0 < st.iterations_fp32 < st.iterations
+    a, b = fem.fem_tria(v, t, lump=lump)
+    lam_o, vec_o = fem.eigs(a, b, k=50)
+    assert_eigs_match(lam, vec, lam_o, vec_o, b, k_complete=False)
+
+
+def test_mixed_precision_tet_and_learned_rejected():
+    tet = TetMesh.read_vtk(data_path("cubeTetra.vtk"))
+    fem_ = Solver(tet)
+    lam, vec = fem_.eigs(k=10, precision="mixed")
+    a, b = fem.fem_tetra(tet.v, tet.t)
+    lam_o, vec_o = fem.eigs(a, b, k=10)
+    assert_eigs_match(lam, vec, lam_o, vec_o, b, k_complete=False)
+    with pytest.raises(ValueError):
+        fem_.eigs(k=10, precision="mixed", precond="learned")
+
+
 def test_dense_path_tiny_mesh():
     v, t = meshgen.icosphere(1)
     fem_ = Solver(TriaMesh(v, t))
