@@ -62,6 +62,10 @@ struct ChebPrecond {
     bool half = false;  // FP16 storage of the work blocks (FP32 arithmetic)
     double bmax = 2.0, amin = 0.03;
     DevBuf kv, dinv, kv32, dinv32, R, D0, D1, sc;
+    // ELL copy of the FP32 matrix (FP16 steps): ell_w slots per row, 0: none
+    static constexpr int kEllMaxW = 32;
+    int ell_w = 0;
+    DevBuf ecols, ekv;
     void setup(lh_ctx* c, const lh_csr* A, const lh_csr* B, double sigma, int deg, double ratio, int64_t n_,
                int mmax_, bool half_storage);
     // W (n x m, leading dimension ldw) <- p(D^-1 K) D^-1 W; colnorm2 (device,

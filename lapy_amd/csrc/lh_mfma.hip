@@ -176,9 +176,11 @@ k_gram_mfma_reduce(int nwaves, int ntile, int p, int q, int tv, int sym, const d
 
 // Y = alpha U C + beta Y.  upper: C is upper triangular (rows k > j of column
 // j are zero), so a 64-column block only sums k < its last column + 1.
-// INPLACE (Y = U, p = q <= 256): a workgroup owns all column blocks of its
-// rows, and writes only after every wave of it has read its rows of U.
-template <bool VEC, bool INPLACE, typename T>
+// GROUPED (q <= 256): a workgroup owns all column blocks of its rows, so the
+// rows of U are fetched from HBM once for all of them.  INPLACE (Y = U,
+// p = q <= 256, grouped): the workgroup writes only after every wave of it has
+// read its rows of U.
+template <bool VEC, bool INPLACE, bool GROUPED, typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restrict__ Cm, int ldc, T* Y, int ldy,
             double alpha, double beta, int upper, int nq) {
@@ -187,10 +189,11 @@ k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restri
     const int il = lane & 15, kg = lane >> 4;
     int64_t rw;  // first row of this wave
     int n0;      // first column of this wave
-    if (INPLACE) {
+    if (INPLACE || GROUPED) {
         const int rsub = (int)(blockDim.x >> 6) / nq;
         rw = xcd_block() * 64 * rsub + (int64_t)(wv / nq) * 64;
         n0 = (wv % nq) * 64;
+        if (!INPLACE && rw >= n) return;  // whole wave (no barrier follows)
     } else {
         rw = xcd_block() * 256 + (int64_t)wv * 64;
         n0 = blockIdx.y * 64;
@@ -287,14 +290,28 @@ void gemm_mfma(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const 
     if (n == 0 || q == 0) return;
     const bool vec = vec_ok(U, ldu);
     const int nq = (q + 63) / 64;
-    dim3 grid(nblocks_xcd(n, 256), (unsigned)nq);
     const int up = upper ? 1 : 0;
+    if (nq > 1 && nq <= 4) {
+        // grouped: nq column blocks x rsub row blocks of 64 per workgroup
+        const int rsub = nq == 2 ? 2 : 1;
+        const int bs = 64 * nq * rsub;
+        const unsigned nb = nblocks_xcd(n, 64 * rsub);
+        if (vec)
+            k_gemm_mfma<true, false, true, T><<<nb, bs, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta,
+                                                                         up, nq);
+        else
+            k_gemm_mfma<false, false, true, T><<<nb, bs, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha,
+                                                                          beta, up, nq);
+        LH_CHECK_LAUNCH();
+        return;
+    }
+    dim3 grid(nblocks_xcd(n, 256), (unsigned)nq);
     if (vec)
-        k_gemm_mfma<true, false, T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta, up,
-                                                                   nq);
+        k_gemm_mfma<true, false, false, T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta,
+                                                                          up, nq);
     else
-        k_gemm_mfma<false, false, T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta, up,
-                                                                    nq);
+        k_gemm_mfma<false, false, false, T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha,
+                                                                           beta, up, nq);
     LH_CHECK_LAUNCH();
 }
 
@@ -310,9 +327,11 @@ void gemm_mfma_inplace(lh_ctx* ctx, int64_t n, int p, T* X, int ldx, const doubl
     const unsigned nb = nblocks_xcd(n, 64 * rsub);
     const int up = upper ? 1 : 0;
     if (vec)
-        k_gemm_mfma<true, true, T><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0, up, nq);
+        k_gemm_mfma<true, true, true, T><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0, up,
+                                                                     nq);
     else
-        k_gemm_mfma<false, true, T><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0, up, nq);
+        k_gemm_mfma<false, true, true, T><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0, up,
+                                                                      nq);
     LH_CHECK_LAUNCH();
 }
 

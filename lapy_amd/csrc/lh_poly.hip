@@ -148,7 +148,10 @@ __global__ void k_cheb_init_h(int64_t n, int m, const WT* __restrict__ W, int ld
     x0[gid] = (_Float16)0.0f;
 }
 
-// one step, 8 columns (16 bytes) per lane; requires m % 8 == 0
+// one step, 8 columns (16 bytes) per lane; requires m % 8 == 0.  The row's
+// own x_k, b and x_{k-1} are loaded first, so that they are in flight with the
+// gathers; the gathers go four at a time (a partial last group repeats the
+// row's own x_k with weight 0)
 template <int LPR>
 __global__ void __launch_bounds__(256)
 k_cheb_step_h(int64_t n, int m, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
@@ -158,26 +161,106 @@ k_cheb_step_h(int64_t n, int m, const int32_t* __restrict__ rowptr, const int32_
     const int64_t row = gid / LPR;
     const int c0 = (int)(gid % LPR) * 8;
     if (row >= n || c0 >= m) return;
+    const int64_t o = row * m + c0;
+    const half8 hk = *reinterpret_cast<const half8*>(xk + o);
+    const half8 hb = *reinterpret_cast<const half8*>(bvec + o);
+    const half8 hp = *reinterpret_cast<const half8*>(xprev + o);
+    const float ad = alpha * dinv[row];
     const int32_t r0 = rowptr[row], r1 = rowptr[row + 1];
     float8 a0 = 0.0f, a1 = 0.0f;
-    int32_t s = r0;
-    for (; s + 1 < r1; s += 2) {
-        const int32_t j0 = cols[s], j1 = cols[s + 1];
-        const float w0 = kv[s], w1 = kv[s + 1];
-        const half8 x0 = *reinterpret_cast<const half8*>(xk + (int64_t)j0 * m + c0);
-        const half8 x1 = *reinterpret_cast<const half8*>(xk + (int64_t)j1 * m + c0);
-        a0 += w0 * h2f(x0);
-        a1 += w1 * h2f(x1);
+    for (int32_t s = r0; s < r1; s += 4) {
+        int32_t j[4];
+        float w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bool in = s + q < r1;
+            j[q] = in ? cols[s + q] : (int32_t)row;
+            w[q] = in ? kv[s + q] : 0.0f;
+        }
+        half8 x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[q] = *reinterpret_cast<const half8*>(xk + (int64_t)j[q] * m + c0);
+        a0 += w[0] * h2f(x[0]);
+        a1 += w[1] * h2f(x[1]);
+        a0 += w[2] * h2f(x[2]);
+        a1 += w[3] * h2f(x[3]);
     }
-    if (s < r1) a0 += kv[s] * h2f(*reinterpret_cast<const half8*>(xk + (int64_t)cols[s] * m + c0));
     a0 += a1;
-    const float ad = alpha * dinv[row];
-    const int64_t o = row * m + c0;
-    const float8 xkv = h2f(*reinterpret_cast<const half8*>(xk + o));
-    const float8 bv = h2f(*reinterpret_cast<const half8*>(bvec + o));
-    const float8 xp = h2f(*reinterpret_cast<const half8*>(xprev + o));
+    const float8 xkv = h2f(hk), bv = h2f(hb), xp = h2f(hp);
     const float8 r = omega * (xkv - xp + ad * (bv - a0)) + xp;
     *reinterpret_cast<half8*>(xprev + o) = f2h_sat(r);
+}
+
+// ELL form of the step: every row has W slots (W % 8 == 0, row-major; a
+// padding slot points at the row itself with weight 0), so the slots' columns
+// and weights do not wait for the row pointers, and eight gathers go out at
+// once
+template <int LPR>
+__global__ void __launch_bounds__(256)
+k_cheb_step_e(int64_t n, int m, int W, const int32_t* __restrict__ ecols, const float* __restrict__ ekv,
+              const float* __restrict__ dinv, const _Float16* __restrict__ bvec, const _Float16* __restrict__ xk,
+              _Float16* __restrict__ xprev, float alpha, float omega) {
+    const int64_t gid = xcd_block() * blockDim.x + threadIdx.x;
+    const int64_t row = gid / LPR;
+    const int c0 = (int)(gid % LPR) * 8;
+    if (row >= n || c0 >= m) return;
+    const int64_t o = row * m + c0;
+    const half8 hk = *reinterpret_cast<const half8*>(xk + o);
+    // b and x_{k-1} are streamed once per step: non-temporal
+    const half8 hb = __builtin_nontemporal_load(reinterpret_cast<const half8*>(bvec + o));
+    const half8 hp = __builtin_nontemporal_load(reinterpret_cast<const half8*>(xprev + o));
+    const float ad = alpha * dinv[row];
+    const int32_t* cr = ecols + row * W;
+    const float* wr = ekv + row * W;
+    float8 a0 = 0.0f, a1 = 0.0f;
+    for (int s = 0; s < W; s += 8) {
+        const int4 ja = *reinterpret_cast<const int4*>(cr + s);
+        const int4 jb = *reinterpret_cast<const int4*>(cr + s + 4);
+        const float4 wa = *reinterpret_cast<const float4*>(wr + s);
+        const float4 wb = *reinterpret_cast<const float4*>(wr + s + 4);
+        const int32_t j[8] = {ja.x, ja.y, ja.z, ja.w, jb.x, jb.y, jb.z, jb.w};
+        const float w[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
+        half8 x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = *reinterpret_cast<const half8*>(xk + (int64_t)j[q] * m + c0);
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+            a0 += w[q] * h2f(x[q]);
+            a1 += w[q + 1] * h2f(x[q + 1]);
+        }
+    }
+    a0 += a1;
+    const float8 xkv = h2f(hk), bv = h2f(hb), xp = h2f(hp);
+    const float8 r = omega * (xkv - xp + ad * (bv - a0)) + xp;
+    __builtin_nontemporal_store(f2h_sat(r), reinterpret_cast<half8*>(xprev + o));
+}
+
+// longest row of a pattern (into *w, which starts at 0)
+__global__ void k_max_rowlen(int64_t n, const int32_t* __restrict__ rowptr, int* __restrict__ w) {
+    __shared__ int red[256];
+    int best = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        best = max(best, rowptr[i + 1] - rowptr[i]);
+    red[threadIdx.x] = best;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(w, red[0]);
+}
+
+// CSR -> ELL with W slots per row (padding: the row itself, weight 0)
+__global__ void k_to_ell(int64_t n, int W, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+                         const float* __restrict__ kv, int32_t* __restrict__ ecols, float* __restrict__ ekv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t r0 = rowptr[i], len = rowptr[i + 1] - r0;
+    for (int q = 0; q < W; ++q) {
+        const bool in = q < len;
+        ecols[i * W + q] = in ? cols[r0 + q] : (int32_t)i;
+        ekv[i * W + q] = in ? kv[r0 + q] : 0.0f;
+    }
 }
 
 template <typename WT>
@@ -327,6 +410,26 @@ void ChebPrecond::setup(lh_ctx* c, const lh_csr* A, const lh_csr* B, double sigm
     k_to_f32<<<nblocks(nnz, 256), 256, 0, ctx->stream>>>(nnz, kv.as<double>(), kv32.as<float>());
     k_to_f32<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, dinv.as<double>(), dinv32.as<float>());
     LH_CHECK_LAUNCH();
+    // ELL copy for the FP16 step when the rows are short (surface and
+    // volume meshes: 7 and ~15 slots)
+    ell_w = 0;
+    if (half) {
+        DevBuf wmax(8);
+        memset_d(ctx, wmax.p, 0, 4);
+        k_max_rowlen<<<256, 256, 0, ctx->stream>>>(n, pat->rowptr.as<int32_t>(), wmax.as<int>());
+        LH_CHECK_LAUNCH();
+        int w = 0;
+        d2h(ctx, &w, wmax.p, 4);
+        w = (w + 7) / 8 * 8;
+        if (w > 0 && w <= kEllMaxW) {
+            ell_w = w;
+            ecols.alloc((size_t)n * w * 4);
+            ekv.alloc((size_t)n * w * 4);
+            k_to_ell<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, w, pat->rowptr.as<int32_t>(), pat->cols.as<int32_t>(),
+                                                              kv32.as<float>(), ecols.as<int32_t>(), ekv.as<float>());
+            LH_CHECK_LAUNCH();
+        }
+    }
     const size_t es = half ? 2 : 4;
     R.alloc(n * mmax * es);
     D0.alloc(n * mmax * es);
@@ -346,6 +449,20 @@ void ChebPrecond::step(int m, const void* bvec, const void* xk, void* xp, double
         auto* y = static_cast<_Float16*>(xp);
         const int vl = vec_lanes(m, 8);
         const unsigned nb = nblocks_xcd(n * vl, 256);
+        if (ell_w > 0) {
+            const int32_t* ec = ecols.as<int32_t>();
+            const float* ek = ekv.as<float>();
+            const int W = ell_w;
+            switch (vl) {
+                case 1: k_cheb_step_e<1><<<nb, 256, 0, ctx->stream>>>(n, m, W, ec, ek, dd, b, x, y, fa, fo); break;
+                case 2: k_cheb_step_e<2><<<nb, 256, 0, ctx->stream>>>(n, m, W, ec, ek, dd, b, x, y, fa, fo); break;
+                case 4: k_cheb_step_e<4><<<nb, 256, 0, ctx->stream>>>(n, m, W, ec, ek, dd, b, x, y, fa, fo); break;
+                case 8: k_cheb_step_e<8><<<nb, 256, 0, ctx->stream>>>(n, m, W, ec, ek, dd, b, x, y, fa, fo); break;
+                default: k_cheb_step_e<16><<<nb, 256, 0, ctx->stream>>>(n, m, W, ec, ek, dd, b, x, y, fa, fo); break;
+            }
+            LH_CHECK_LAUNCH();
+            return;
+        }
         switch (vl) {
             case 1: k_cheb_step_h<1><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
             case 2: k_cheb_step_h<2><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;

@@ -63,6 +63,9 @@ def test_gram_deterministic():
     (4099, 64, 128, 192, 192, -1.0, 1.0),
     (3001, 56, 112, 168, 168, 0.5, -2.0),
     (2000, 37, 100, 37, 101, 1.0, 0.0),
+    (5003, 64, 192, 64, 192, 1.0, 1.0),
+    (3001, 64, 256, 64, 256, -1.0, 0.5),
+    (2049, 48, 300, 48, 300, 1.0, 0.0),
     (17, 3, 5, 3, 5, 2.0, 1.0),
 ])
 def test_gemm_matches_numpy(n, p, q, ldu, ldy, alpha, beta):
