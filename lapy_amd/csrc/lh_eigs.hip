@@ -82,12 +82,18 @@ struct Lobpcg {
     T* ASp() { return AS.as<T>(); }
     T* MSp() { return MS.as<T>(); }
 
+    Ell ell;  // ELL copy of Am (and Bm when it shares Am's pattern)
+
     void spmm_AB(const T* X, T* AX, T* MX, int ncol) {
-        spmm_pair<T, T>(ctx, Am, Bm, X, AX, MX, ncol, ld, ld);
+        if (spmm_ell<T, T>(ctx, ell, X, AX, ell.two ? MX : nullptr, ncol, ld, ld)) {
+            if (!ell.two) spmm_ld<T, T>(ctx, Bm, X, MX, ncol, ld, ld);
+        } else {
+            spmm_pair<T, T>(ctx, Am, Bm, X, AX, MX, ncol, ld, ld);
+        }
         st.spmm_calls++;
     }
     void spmm_B(const T* X, T* MX, int ncol) {
-        spmm_ld<T, T>(ctx, Bm, X, MX, ncol, ld, ld);
+        if (!spmm_ell<T, T>(ctx, ell, X, nullptr, MX, ncol, ld, ld, 1)) spmm_ld<T, T>(ctx, Bm, X, MX, ncol, ld, ld);
         st.spmm_calls++;
     }
     int read_status() {
@@ -179,6 +185,7 @@ struct Lobpcg {
             Am = A32.get();
             Bm = B32.get();
         }
+        ell.build(ctx, Am, Bm);
         ld = 3 * m;
         S.alloc(n * ld * sizeof(T));
         S2.alloc(n * ld * sizeof(T));

@@ -226,6 +226,25 @@ void refresh_dense_diag(lh_ctx* ctx, lh_csr* a);
 // FP32 copy of an FP64 matrix on the same pattern
 std::unique_ptr<lh_csr> csr_to_f32(lh_ctx* ctx, const lh_csr* s);
 
+// ELL copy of one matrix or of two on one pattern, for the block products of
+// the eigensolver: W slots per row (W % 8 == 0, row-major), a padding slot
+// points at the row itself with value 0.  Without row pointers in the way,
+// the slots' columns and values and eight gathers go out together.
+struct Ell {
+    static constexpr int kMaxW = 32;
+    int64_t n = 0;
+    int W = 0;          // 0: not built (rows too long, or a diagonal matrix)
+    bool two = false;   // vb holds the second matrix
+    int dtype = LH_F64;
+    DevBuf cols, va, vb;
+    void build(lh_ctx* ctx, const lh_csr* a, const lh_csr* b);
+};
+// YA = a X (and YB = b X when the ELL holds two matrices and YB is given);
+// which = 1: YB = b X.  Returns false (nothing launched) when the copy is not
+// built or the block is not 16-byte vectorisable; the caller then uses CSR.
+template <typename VT, typename XT>
+bool spmm_ell(lh_ctx* ctx, const Ell& e, const XT* X, XT* YA, XT* YB, int ncol, int ldx, int ldy, int which = 0);
+
 // 16-byte vector of a block row: 4 floats or 2 doubles per lane
 template <typename T> struct Vec16;
 template <> struct Vec16<float> {

@@ -229,8 +229,23 @@ k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restri
                 for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = Mf<T>::mma(a[bi][jj], b[jj][bj], acc[bi][bj]);
     }
     if (INPLACE) __syncthreads();
+    // per row block: all of its Y values are read before any is written (a
+    // load cannot pass a store the compiler cannot prove distinct, so
+    // interleaving them would cost one memory round trip per value)
 #pragma unroll
-    for (int bi = 0; bi < 4; ++bi)
+    for (int bi = 0; bi < 4; ++bi) {
+        T yv[4][4];
+        if (beta != 0.0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = rw + 16 * bi + Mf<T>::drow(kg, r);
+#pragma unroll
+                for (int bj = 0; bj < 4; ++bj) {
+                    const int col = n0 + 16 * bj + il;
+                    yv[r][bj] = (row < n && col < q) ? Y[row * ldy + col] : T(0);
+                }
+            }
+        }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int64_t row = rw + 16 * bi + Mf<T>::drow(kg, r);
@@ -240,10 +255,11 @@ k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restri
                 const int col = n0 + 16 * bj + il;
                 if (col >= q) continue;
                 double v = alpha * (double)acc[bi][bj][r];
-                if (beta != 0.0) v += beta * (double)Y[row * ldy + col];
-                Y[row * ldy + col] = (T)v;
+                if (beta != 0.0) v += beta * (double)yv[r][bj];
+                __builtin_nontemporal_store((T)v, Y + row * ldy + col);  // streamed out
             }
         }
+    }
 }
 
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }

@@ -191,6 +191,160 @@ void spmm_pair(lh_ctx* ctx, const lh_csr* a, const lh_csr* b, const XT* X, XT* Y
     spmm_ld<VT, XT>(ctx, b, X, YB, ncol, ldx, ldy);
 }
 
+// ---- ELL forms ------------------------------------------------------------
+// non-temporal 16-byte store of a block-row vector
+__device__ __forceinline__ void nt_store(const double2& v, double* p) {
+    typedef double d2n __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store((d2n){v.x, v.y}, reinterpret_cast<d2n*>(p));
+}
+__device__ __forceinline__ void nt_store(const float4& v, float* p) {
+    typedef float f4n __attribute__((ext_vector_type(4)));
+    __builtin_nontemporal_store((f4n){v.x, v.y, v.z, v.w}, reinterpret_cast<f4n*>(p));
+}
+
+// TWO: YA = A X and YB = B X in one pass; else Y = M X with M's values in va
+template <typename VT, typename XT, int LPR, bool TWO>
+__global__ void __launch_bounds__(256)
+k_spmm_e(int64_t n, int W, const int32_t* __restrict__ ecols, const VT* __restrict__ va, const VT* __restrict__ vb,
+         const XT* __restrict__ X, XT* __restrict__ YA, XT* __restrict__ YB, int ncol, int ldx, int ldy) {
+    using V = typename Vec16<XT>::type;
+    const int64_t gid = xcd_block() * blockDim.x + threadIdx.x;
+    int64_t row = gid / LPR;
+    // one row per wavefront: the row, hence its slots, is wave-uniform and
+    // the slots come through the scalar cache (loads only)
+    if (LPR == 64) row = __builtin_amdgcn_readfirstlane((int)row);
+    const int c0 = (int)(gid % LPR) * Vec16<XT>::n;
+    if (row >= n || c0 >= ncol) return;
+    const int32_t* cr = ecols + row * W;
+    const VT* ar = va + row * W;
+    const VT* br = TWO ? vb + row * W : nullptr;
+    V a1 = vzero<XT>(), a2 = vzero<XT>(), b1 = vzero<XT>(), b2 = vzero<XT>();
+    for (int s = 0; s < W; s += 8) {
+        const int4 ja = *reinterpret_cast<const int4*>(cr + s);
+        const int4 jb = *reinterpret_cast<const int4*>(cr + s + 4);
+        const int32_t j[8] = {ja.x, ja.y, ja.z, ja.w, jb.x, jb.y, jb.z, jb.w};
+        V x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = *reinterpret_cast<const V*>(X + (int64_t)j[q] * ldx + c0);
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+            vfma<XT>(a1, (XT)ar[s + q], x[q]);
+            vfma<XT>(a2, (XT)ar[s + q + 1], x[q + 1]);
+            if (TWO) {
+                vfma<XT>(b1, (XT)br[s + q], x[q]);
+                vfma<XT>(b2, (XT)br[s + q + 1], x[q + 1]);
+            }
+        }
+    }
+    // the products are not read again by this pass: non-temporal stores
+    vadd<XT>(a1, a2);
+    nt_store(a1, YA + row * ldy + c0);
+    if (TWO) {
+        vadd<XT>(b1, b2);
+        nt_store(b1, YB + row * ldy + c0);
+    }
+}
+
+// longest row of a pattern (into *w, which starts at 0)
+__global__ void k_rowlen_max(int64_t n, const int32_t* __restrict__ rowptr, int* __restrict__ w) {
+    __shared__ int red[256];
+    int best = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        best = max(best, rowptr[i + 1] - rowptr[i]);
+    red[threadIdx.x] = best;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(w, red[0]);
+}
+
+template <typename VT>
+__global__ void k_csr_to_ell(int64_t n, int W, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+                             const VT* __restrict__ a, const VT* __restrict__ b, int32_t* __restrict__ ecols,
+                             VT* __restrict__ ea, VT* __restrict__ eb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t r0 = rowptr[i], len = rowptr[i + 1] - r0;
+    for (int q = 0; q < W; ++q) {
+        const bool in = q < len;
+        ecols[i * W + q] = in ? cols[r0 + q] : (int32_t)i;
+        ea[i * W + q] = in ? a[r0 + q] : VT(0);
+        if (b) eb[i * W + q] = in ? b[r0 + q] : VT(0);
+    }
+}
+
+void Ell::build(lh_ctx* ctx, const lh_csr* a, const lh_csr* b) {
+    W = 0;
+    two = false;
+    if (a->is_diag) return;
+    if (b && (b->is_diag || b->pat != a->pat || b->dtype != a->dtype)) b = nullptr;
+    n = a->n();
+    dtype = a->dtype;
+    const Pattern& p = *a->pat;
+    DevBuf wmax(8);
+    memset_d(ctx, wmax.p, 0, 4);
+    k_rowlen_max<<<256, 256, 0, ctx->stream>>>(n, p.rowptr.as<int32_t>(), wmax.as<int>());
+    LH_CHECK_LAUNCH();
+    int w = 0;
+    d2h(ctx, &w, wmax.p, 4);
+    w = (w + 7) / 8 * 8;
+    if (w <= 0 || w > kMaxW) return;
+    const size_t es = dtype == LH_F64 ? 8 : 4;
+    cols.alloc((size_t)n * w * 4);
+    va.alloc((size_t)n * w * es);
+    if (b) vb.alloc((size_t)n * w * es);
+    if (dtype == LH_F64)
+        k_csr_to_ell<double><<<nblocks(n, 256), 256, 0, ctx->stream>>>(
+            n, w, p.rowptr.as<int32_t>(), p.cols.as<int32_t>(), a->vals.as<double>(),
+            b ? b->vals.as<double>() : nullptr, cols.as<int32_t>(), va.as<double>(), b ? vb.as<double>() : nullptr);
+    else
+        k_csr_to_ell<float><<<nblocks(n, 256), 256, 0, ctx->stream>>>(
+            n, w, p.rowptr.as<int32_t>(), p.cols.as<int32_t>(), a->vals.as<float>(),
+            b ? b->vals.as<float>() : nullptr, cols.as<int32_t>(), va.as<float>(), b ? vb.as<float>() : nullptr);
+    LH_CHECK_LAUNCH();
+    W = w;
+    two = b != nullptr;
+}
+
+template <typename VT, typename XT>
+bool spmm_ell(lh_ctx* ctx, const Ell& e, const XT* X, XT* YA, XT* YB, int ncol, int ldx, int ldy, int which) {
+    if (e.W == 0 || (which == 1 && !e.two)) return false;
+    const bool pair = which == 0 && e.two && YB != nullptr;
+    XT* y = (which == 1) ? YB : YA;
+    auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) % 16) == 0; };
+    constexpr int v = Vec16<XT>::n;
+    const bool vec = ncol % v == 0 && ldx % v == 0 && ldy % v == 0 && vec_lanes(ncol, v) <= 64 && al(X) && al(y) &&
+                     (!pair || al(YB));
+    if (!vec) return false;
+    const int64_t n = e.n;
+    const int32_t* ec = e.cols.as<int32_t>();
+    const VT* a = (which == 1) ? e.vb.as<VT>() : e.va.as<VT>();
+    const VT* b = e.vb.as<VT>();
+    const int lanes = vec_lanes(ncol, v);
+    const unsigned nb = nblocks_xcd(n * lanes, 256);
+#define LH_ELL_LAUNCH(L)                                                                                     \
+    do {                                                                                                     \
+        if (pair) k_spmm_e<VT, XT, L, true><<<nb, 256, 0, ctx->stream>>>(n, e.W, ec, a, b, X, YA, YB, ncol, ldx, ldy); \
+        else k_spmm_e<VT, XT, L, false><<<nb, 256, 0, ctx->stream>>>(n, e.W, ec, a, nullptr, X, y, nullptr, ncol, ldx, ldy); \
+    } while (0)
+    switch (lanes) {
+        case 1: LH_ELL_LAUNCH(1); break;
+        case 2: LH_ELL_LAUNCH(2); break;
+        case 4: LH_ELL_LAUNCH(4); break;
+        case 8: LH_ELL_LAUNCH(8); break;
+        case 16: LH_ELL_LAUNCH(16); break;
+        case 32: LH_ELL_LAUNCH(32); break;
+        default: LH_ELL_LAUNCH(64); break;
+    }
+#undef LH_ELL_LAUNCH
+    LH_CHECK_LAUNCH();
+    return true;
+}
+template bool spmm_ell<double, double>(lh_ctx*, const Ell&, const double*, double*, double*, int, int, int, int);
+template bool spmm_ell<float, float>(lh_ctx*, const Ell&, const float*, float*, float*, int, int, int, int);
+
 template void spmm<double, double>(lh_ctx*, const lh_csr*, const double*, double*, int, int);
 template void spmm<float, double>(lh_ctx*, const lh_csr*, const double*, double*, int, int);
 template void spmm<float, float>(lh_ctx*, const lh_csr*, const float*, float*, int, int);
