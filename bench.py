@@ -344,16 +344,17 @@ def main():
     us_c = _lib.time_cheb_step(ctx, A, B, mcols, iters=50, reorder=True,
                                storage=_lib.LH_STORE_F16 if half else _lib.LH_STORE_F32)
     # x_k (gathered, counted once), b, x_{k-1} read + x_{k+1} write: 4 values per entry of the block
-    # (2 bytes each in FP16 storage, 4 in FP32); K values (FP32) + cols (int32): 8 nnz; rowptr + D^-1: 8 n
+    # (2 bytes each in FP16 storage, 4 in FP32); K values (FP32) + cols (int32): 8 nnz; row structure
+    # (row pointers or ELL padding) + D^-1: 8 n
     es = 2 if half else 4
     cheb_bytes = 4 * es * nv * mcols + 8 * nnz + 8 * nv
     ach_c = cheb_bytes / (us_c * 1e-6) / 1e9
-    kname = "k_cheb_step_h (FP16 storage)" if half else "k_cheb_step_v (FP32 storage)"
+    kname = "k_cheb_step_e (ELL, FP16 storage)" if half else "k_cheb_step_v (FP32 storage)"
     # traffic: measured by rocprofv3 on exactly this configuration (L8, m = 64, FP16 storage)
     profiled = args.level == 8 and mcols == 64 and half
     roofline = {"bound": "hbm", "achieved": round(ach_c, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach_c / HBM_PEAK_GBS, 4),
-                "traffic": profiled_traffic("cheb_step_h_l8_m64_traffic.txt") if profiled else None,
+                "traffic": profiled_traffic("cheb_step_e_l8_m64_traffic.txt") if profiled else None,
                 "kernel": f"{kname}, Chebyshev-Jacobi preconditioner step, m={mcols}",
                 "avg_us": round(us_c, 2), "algorithmic_bytes": int(cheb_bytes)}
     # the metric's second figure: numeric assembly

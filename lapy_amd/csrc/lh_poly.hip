@@ -194,15 +194,30 @@ k_cheb_step_h(int64_t n, int m, const int32_t* __restrict__ rowptr, const int32_
 // ELL form of the step: every row has W slots (W % 8 == 0, row-major; a
 // padding slot points at the row itself with weight 0), so the slots' columns
 // and weights do not wait for the row pointers, and eight gathers go out at
-// once
-template <int LPR>
+// once.  W == 8 (surface meshes): the workgroup's rows' slots are staged in
+// LDS by one coalesced load instead of being fetched by every lane of a row.
+template <int LPR, bool W8>
 __global__ void __launch_bounds__(256)
 k_cheb_step_e(int64_t n, int m, int W, const int32_t* __restrict__ ecols, const float* __restrict__ ekv,
               const float* __restrict__ dinv, const _Float16* __restrict__ bvec, const _Float16* __restrict__ xk,
               _Float16* __restrict__ xprev, float alpha, float omega) {
-    const int64_t gid = xcd_block() * blockDim.x + threadIdx.x;
+    constexpr int kRows = 256 / LPR;  // rows of a workgroup
+    __shared__ __align__(16) int32_t s_cols[W8 ? kRows * 8 : 4];
+    __shared__ __align__(16) float s_kv[W8 ? kRows * 8 : 4];
+    const int64_t blk = xcd_block();
+    const int64_t gid = blk * blockDim.x + threadIdx.x;
     const int64_t row = gid / LPR;
     const int c0 = (int)(gid % LPR) * 8;
+    if (W8) {
+        // rows blk * kRows ... (+kRows): kRows * 8 slots, contiguous
+        const int64_t base = blk * kRows * 8;
+        for (int t = threadIdx.x; t < kRows * 8; t += 256) {
+            const bool in = base + t < n * 8;
+            s_cols[t] = in ? ecols[base + t] : 0;
+            s_kv[t] = in ? ekv[base + t] : 0.0f;
+        }
+        __syncthreads();
+    }
     if (row >= n || c0 >= m) return;
     const int64_t o = row * m + c0;
     const half8 hk = *reinterpret_cast<const half8*>(xk + o);
@@ -210,10 +225,10 @@ k_cheb_step_e(int64_t n, int m, int W, const int32_t* __restrict__ ecols, const 
     const half8 hb = __builtin_nontemporal_load(reinterpret_cast<const half8*>(bvec + o));
     const half8 hp = __builtin_nontemporal_load(reinterpret_cast<const half8*>(xprev + o));
     const float ad = alpha * dinv[row];
-    const int32_t* cr = ecols + row * W;
-    const float* wr = ekv + row * W;
+    const int32_t* cr = W8 ? s_cols + (threadIdx.x / LPR) * 8 : ecols + row * W;
+    const float* wr = W8 ? s_kv + (threadIdx.x / LPR) * 8 : ekv + row * W;
     float8 a0 = 0.0f, a1 = 0.0f;
-    for (int s = 0; s < W; s += 8) {
+    for (int s = 0; s < (W8 ? 8 : W); s += 8) {
         const int4 ja = *reinterpret_cast<const int4*>(cr + s);
         const int4 jb = *reinterpret_cast<const int4*>(cr + s + 4);
         const float4 wa = *reinterpret_cast<const float4*>(wr + s);
@@ -453,13 +468,19 @@ void ChebPrecond::step(int m, const void* bvec, const void* xk, void* xp, double
             const int32_t* ec = ecols.as<int32_t>();
             const float* ek = ekv.as<float>();
             const int W = ell_w;
+#define LH_CHEB_E(L)                                                                                     \
+    do {                                                                                                 \
+        if (W == 8) k_cheb_step_e<L, true><<<nb, 256, 0, ctx->stream>>>(n, m, W, ec, ek, dd, b, x, y, fa, fo); \
+        else k_cheb_step_e<L, false><<<nb, 256, 0, ctx->stream>>>(n, m, W, ec, ek, dd, b, x, y, fa, fo);       \
+    } while (0)
             switch (vl) {
-                case 1: k_cheb_step_e<1><<<nb, 256, 0, ctx->stream>>>(n, m, W, ec, ek, dd, b, x, y, fa, fo); break;
-                case 2: k_cheb_step_e<2><<<nb, 256, 0, ctx->stream>>>(n, m, W, ec, ek, dd, b, x, y, fa, fo); break;
-                case 4: k_cheb_step_e<4><<<nb, 256, 0, ctx->stream>>>(n, m, W, ec, ek, dd, b, x, y, fa, fo); break;
-                case 8: k_cheb_step_e<8><<<nb, 256, 0, ctx->stream>>>(n, m, W, ec, ek, dd, b, x, y, fa, fo); break;
-                default: k_cheb_step_e<16><<<nb, 256, 0, ctx->stream>>>(n, m, W, ec, ek, dd, b, x, y, fa, fo); break;
+                case 1: LH_CHEB_E(1); break;
+                case 2: LH_CHEB_E(2); break;
+                case 4: LH_CHEB_E(4); break;
+                case 8: LH_CHEB_E(8); break;
+                default: LH_CHEB_E(16); break;
             }
+#undef LH_CHEB_E
             LH_CHECK_LAUNCH();
             return;
         }
