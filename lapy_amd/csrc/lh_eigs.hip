@@ -230,8 +230,8 @@ struct Lobpcg {
         int it = 0;
         double t_iter0 = now_us();
         std::vector<double> r2(m), mx2(m);
-        double best = 0.0;
-        int since_best = 0;
+        std::vector<double> gm_hist;  // first phase: residual history
+        constexpr int kMinFp32Iters = 6;
         for (it = 0; it < max_iter; ++it) {
             // residual into W and the convergence test
             residual<T>(ctx, n, m, ASp(), MSp(), ld, theta_d.as<double>(), Sp() + m, ld);
@@ -270,14 +270,17 @@ struct Lobpcg {
             if (nconv == k) break;
             if (phase_one) {
                 // the FP32 residuals stop falling at the rounding floor
-                // (~u32 times the spread of the spectrum): hand over when the
-                // worst one has not improved by 10 % in 3 iterations
-                if (it == 0 || worst < 0.9 * best) {
-                    best = worst;
-                    since_best = 0;
-                } else if (++since_best >= 3) {
-                    break;
-                }
+                // (~u32 ||A|| in absolute terms): hand over as soon as the
+                // geometric mean of the wanted pairs' absolute residuals
+                // ||A x - theta B x|| / ||B x|| is more than 0.55 times the
+                // previous iteration's (~0.4 times before the floor).
+                // Relative residuals would mislead early on, while the Ritz
+                // values fall by orders of magnitude towards the smallest
+                // eigenvalues.
+                double lg = 0.0;
+                for (int j = 0; j < k; ++j) lg += std::log(std::max(res[j], 1e-300));
+                gm_hist.push_back(std::exp(lg / k));
+                if (it >= kMinFp32Iters && gm_hist[it] > 0.55 * gm_hist[it - 1]) break;
             }
             // precondition W
             if (learned_on) {
