@@ -17,9 +17,13 @@ from .solver import Solver, import_optional_dependency
 logger = logging.getLogger(__name__)
 
 
-def diffusion(geometry, vids, m: float = 1.0, aniso=None, use_cholmod: bool = False, *, rtol: float = 1e-12):
+def diffusion(geometry, vids, m: float = 1.0, aniso=None, use_cholmod: bool = False, *, rtol: float = 1e-12,
+              precond: str = "jacobi"):
     """Heat diffusion from the vertices in ``vids`` (same arguments, errors and
-    return shapes as lapy/heat.py:114-232)."""
+    return shapes as lapy/heat.py:114-232).  Keyword-only additions: ``rtol``
+    of the PCG solve and ``precond`` ("jacobi", or "learned" for the network
+    installed on the context, which falls back to Jacobi when CG stalls)."""
+    pc = {"jacobi": _lib.LH_PRECOND_JACOBI, "learned": _lib.LH_PRECOND_LEARNED}[precond]
     if use_cholmod:
         import_optional_dependency("sksparse", raise_error=True)
     nv = len(geometry.v)
@@ -40,7 +44,8 @@ def diffusion(geometry, vids, m: float = 1.0, aniso=None, use_cholmod: bool = Fa
     for k, v in enumerate(vids_list):
         b0[v, k] = 1.0
     logger.info("Solver: PCG on the GPU")
-    vfunc, st = _lib.pcg(fem._ctx, a, b, 1.0, t, b0, rtol=rtol)
+    vfunc, st = _lib.pcg(fem._ctx, a, b, 1.0, t, b0, rtol=rtol, precond=pc)
+    fem.last_stats = st
     if n < nv:
         # trailing vertices outside every element: the matrix is smaller
         out = np.zeros((nv, len(vids_list)))

@@ -110,6 +110,11 @@ def test_ablation_eigensolver_and_heat(trained):
         print("heat PCG iterations: jacobi", sj.iterations, "learned", sl.iterations)
         np.testing.assert_allclose(xl, xj, rtol=0, atol=1e-8 * np.abs(xj).max())
         assert sl.iterations <= sj.iterations
+        # the same through the drop-in heat.diffusion
+        tm = TriaMesh(v, t)
+        u_j = heat.diffusion(tm, 0, rtol=1e-10)
+        u_l = heat.diffusion(tm, 0, rtol=1e-10, precond="learned")
+        np.testing.assert_allclose(u_l, u_j, rtol=0, atol=1e-8 * np.abs(u_j).max())
     finally:
         net.use(False)
 
