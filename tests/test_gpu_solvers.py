@@ -130,6 +130,39 @@ def test_mixed_precision_tet_and_learned_rejected():
         fem_.eigs(k=10, precision="mixed", precond="learned")
 
 
+def wheel(spokes=48, rings=100, seed=0):
+    """Planar disk: a hub vertex with `spokes` neighbours, then `rings` rings of
+    `spokes` vertices joined by triangulated quads (the hub's matrix row has
+    spokes + 1 entries)."""
+    rng = np.random.default_rng(seed)
+    ang = 2 * np.pi * np.arange(spokes) / spokes
+    v = [[0.0, 0.0, 0.0]]
+    for r in range(1, rings + 1):
+        rad = r + 0.2 * rng.uniform(-1, 1, spokes)
+        v += [[rad[i] * np.cos(ang[i]), rad[i] * np.sin(ang[i]), 0.0] for i in range(spokes)]
+    t = [[0, 1 + i, 1 + (i + 1) % spokes] for i in range(spokes)]
+    for r in range(rings - 1):
+        a0, b0 = 1 + r * spokes, 1 + (r + 1) * spokes
+        for i in range(spokes):
+            j = (i + 1) % spokes
+            t.append([a0 + i, b0 + i, b0 + j])
+            t.append([a0 + i, b0 + j, a0 + j])
+    return np.asarray(v), np.asarray(t)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_long_rows_take_the_csr_products(precision):
+    """A row longer than the ELL copies take (49 > 32 slots): the block
+    products and the Chebyshev steps use the CSR kernels; same bar."""
+    v, t = wheel()
+    fem_ = Solver(TriaMesh(v, t))
+    lam, vec = fem_.eigs(k=10, precision=precision)
+    a, b = fem.fem_tria(v, t)
+    assert a.indptr[1] - a.indptr[0] == 49
+    lam_o, vec_o = fem.eigs(a, b, k=10)
+    assert_eigs_match(lam, vec, lam_o, vec_o, b, k_complete=False)
+
+
 def test_dense_path_tiny_mesh():
     v, t = meshgen.icosphere(1)
     fem_ = Solver(TriaMesh(v, t))
