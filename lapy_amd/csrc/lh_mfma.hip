@@ -108,10 +108,11 @@ k_gram_mfma(int64_t n, int64_t rows_per_wave, int nwaves, const T* __restrict__ 
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = zero4<T>();
     const int cu = u0 + 4 * il, cv = v0 + 4 * il;
-    for (int64_t r0 = rb; r0 < re; r0 += 16) {
-        v4 a[4], b[4];
+    // 8 rows per step (2 MFMA k-groups of 4 rows); the next step's rows are
+    // in flight during this step's 32 MFMAs
+    auto load_rows = [&](int64_t r0, v4 (&a)[2], v4 (&b)[2]) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
+        for (int s = 0; s < 2; ++s) {
             const int64_t r = r0 + 4 * s + kg;
             if (r < re) {
                 a[s] = load4<VEC, T>(U + r * ldu, cu, p);
@@ -121,12 +122,26 @@ k_gram_mfma(int64_t n, int64_t rows_per_wave, int nwaves, const T* __restrict__ 
                 b[s] = zero4<T>();
             }
         }
+    };
+    v4 a[2], b[2];
+    if (rb < re) load_rows(rb, a, b);
+    for (int64_t r0 = rb; r0 < re; r0 += 8) {
+        v4 an[2], bn[2];
+        const bool more = r0 + 8 < re;
+        if (more) load_rows(r0 + 8, an, bn);
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+        for (int s = 0; s < 2; ++s)
 #pragma unroll
             for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
                 for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = Mf<T>::mma(a[s][bi], b[s][bj], acc[bi][bj]);
+        if (more) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                a[s] = an[s];
+                b[s] = bn[s];
+            }
+        }
     }
     // partial tile [w][tile][64 x 64] in natural order
     double* dst = part + ((int64_t)w * gridDim.y + tile) * 4096;
