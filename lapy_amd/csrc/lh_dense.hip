@@ -545,19 +545,21 @@ __global__ void k_colsq_partial(int64_t n, int m, const T* __restrict__ X, int l
     __syncthreads();
     if (rg == 0 && c < m) part[(int64_t)blockIdx.x * m + c] = ((s[0][threadIdx.x] + s[1][threadIdx.x]) + s[2][threadIdx.x]) + s[3][threadIdx.x];
 }
+// out[c] = sum over the nchunk partials of column c: one workgroup per
+// column, strided per-thread sums, then a fixed-order tree (deterministic)
 __global__ void __launch_bounds__(256) k_colsq_reduce(int nchunk, int m, const double* __restrict__ part,
                                                        double* __restrict__ out) {
-    __shared__ double sh[4][64];
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+    __shared__ double sh[256];
+    const int c = blockIdx.x;
     double s = 0.0;
-    if (c < m)
-        for (int k = g; k < nchunk; k += 4) s += part[(int64_t)k * m + c];
-    sh[g][threadIdx.x & 63] = s;
+    for (int k = threadIdx.x; k < nchunk; k += 256) s += part[(int64_t)k * m + c];
+    sh[threadIdx.x] = s;
     __syncthreads();
-    if (g == 0 && c < m) {
-        const int l = threadIdx.x;
-        out[c] = (sh[0][l] + sh[1][l]) + (sh[2][l] + sh[3][l]);
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[c] = sh[0];
 }
 template <typename T>
 void colsq(lh_ctx* ctx, int64_t n, int m, const T* X, int ld, double* out, DevBuf& ws) {
@@ -568,11 +570,110 @@ void colsq(lh_ctx* ctx, int64_t n, int m, const T* X, int ld, double* out, DevBu
     dim3 grid((unsigned)nchunk, (m + 63) / 64);
     k_colsq_partial<T><<<grid, 256, 0, ctx->stream>>>(n, m, X, ld, rpc, ws.as<double>());
     LH_CHECK_LAUNCH();
-    k_colsq_reduce<<<nblocks(m, 64), 256, 0, ctx->stream>>>((int)nchunk, m, ws.as<double>(), out);
+    k_colsq_reduce<<<m, 256, 0, ctx->stream>>>((int)nchunk, m, ws.as<double>(), out);
     LH_CHECK_LAUNCH();
 }
 template void colsq<double>(lh_ctx*, int64_t, int, const double*, int, double*, DevBuf&);
 template void colsq<float>(lh_ctx*, int64_t, int, const float*, int, double*, DevBuf&);
+
+// R = AX - MX diag(theta) fused with the squared column norms of R and of
+// MX (per-workgroup partials, summed in fixed order by k_colsq_reduce).
+// 16-byte lanes: LPR = m / V lanes per row (V = 2 doubles or 4 floats),
+// 256 / LPR rows per pass of the workgroup over its row range.
+template <typename T, int LPR>
+__global__ void __launch_bounds__(256)
+k_residual_norms(int64_t n, int m, const T* __restrict__ AX, const T* __restrict__ MX, int ldx,
+                 const double* __restrict__ theta, T* __restrict__ R, int ldr, int64_t rpc,
+                 double* __restrict__ pr, double* __restrict__ pm) {
+    using V = typename Vec16<T>::type;
+    constexpr int VN = Vec16<T>::n;
+    constexpr int RG = 256 / LPR;  // row groups
+    __shared__ double sr[RG][LPR * VN], sm[RG][LPR * VN];
+    const int lane = threadIdx.x % LPR, rg = threadIdx.x / LPR;
+    const int c0 = lane * VN;
+    double ar[VN], am[VN], th[VN];
+#pragma unroll
+    for (int q = 0; q < VN; ++q) {
+        ar[q] = 0.0;
+        am[q] = 0.0;
+        th[q] = c0 + q < m ? theta[c0 + q] : 0.0;
+    }
+    const int64_t rb = (int64_t)blockIdx.x * rpc, re = min(rb + rpc, n);
+    if (c0 < m) {
+        for (int64_t i = rb + rg; i < re; i += RG) {
+            const V a = *reinterpret_cast<const V*>(AX + i * ldx + c0);
+            const V b = *reinterpret_cast<const V*>(MX + i * ldx + c0);
+            const T* pa = reinterpret_cast<const T*>(&a);
+            const T* pb = reinterpret_cast<const T*>(&b);
+            V r;
+            T* prr = reinterpret_cast<T*>(&r);
+#pragma unroll
+            for (int q = 0; q < VN; ++q) {
+                const double rv = (double)pa[q] - th[q] * (double)pb[q];
+                prr[q] = (T)rv;
+                const double rq = (double)prr[q];
+                ar[q] += rq * rq;
+                am[q] += (double)pb[q] * (double)pb[q];
+            }
+            *reinterpret_cast<V*>(R + i * ldr + c0) = r;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < VN; ++q) {
+        sr[rg][c0 + q] = ar[q];
+        sm[rg][c0 + q] = am[q];
+    }
+    __syncthreads();
+    // column c summed over the row groups in fixed order
+    for (int c = threadIdx.x; c < m; c += 256) {
+        double tr = 0.0, tm = 0.0;
+        for (int g = 0; g < RG; ++g) {
+            tr += sr[g][c];
+            tm += sm[g][c];
+        }
+        pr[(int64_t)blockIdx.x * m + c] = tr;
+        pm[(int64_t)blockIdx.x * m + c] = tm;
+    }
+}
+
+template <typename T>
+void residual_norms(lh_ctx* ctx, int64_t n, int m, const T* AX, const T* MX, int ldx, const double* theta, T* R,
+                    int ldr, double* r2, double* mx2, DevBuf& ws) {
+    constexpr int VN = Vec16<T>::n;
+    auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) % 16) == 0; };
+    const int lanes = m / VN;
+    const bool vec = m % VN == 0 && ldx % VN == 0 && ldr % VN == 0 && al(AX) && al(MX) && al(R) &&
+                     (lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64);
+    if (!vec) {
+        residual<T>(ctx, n, m, AX, MX, ldx, theta, R, ldr);
+        colsq<T>(ctx, n, m, R, ldr, r2, ws);
+        colsq<T>(ctx, n, m, MX, ldx, mx2, ws);
+        return;
+    }
+    // ~2048 workgroups of whole row groups
+    const int rgs = 256 / lanes;
+    int64_t rpc = std::max<int64_t>(rgs, (n + 2047) / 2048);
+    rpc = (rpc + rgs - 1) / rgs * rgs;
+    const int64_t nchunk = (n + rpc - 1) / rpc;
+    ws.ensure((size_t)nchunk * m * 2 * sizeof(double));
+    double* pr = ws.as<double>();
+    double* pm = pr + nchunk * m;
+    switch (lanes) {
+        case 4: k_residual_norms<T, 4><<<(unsigned)nchunk, 256, 0, ctx->stream>>>(n, m, AX, MX, ldx, theta, R, ldr, rpc, pr, pm); break;
+        case 8: k_residual_norms<T, 8><<<(unsigned)nchunk, 256, 0, ctx->stream>>>(n, m, AX, MX, ldx, theta, R, ldr, rpc, pr, pm); break;
+        case 16: k_residual_norms<T, 16><<<(unsigned)nchunk, 256, 0, ctx->stream>>>(n, m, AX, MX, ldx, theta, R, ldr, rpc, pr, pm); break;
+        case 32: k_residual_norms<T, 32><<<(unsigned)nchunk, 256, 0, ctx->stream>>>(n, m, AX, MX, ldx, theta, R, ldr, rpc, pr, pm); break;
+        default: k_residual_norms<T, 64><<<(unsigned)nchunk, 256, 0, ctx->stream>>>(n, m, AX, MX, ldx, theta, R, ldr, rpc, pr, pm); break;
+    }
+    LH_CHECK_LAUNCH();
+    k_colsq_reduce<<<m, 256, 0, ctx->stream>>>((int)nchunk, m, pr, r2);
+    k_colsq_reduce<<<m, 256, 0, ctx->stream>>>((int)nchunk, m, pm, mx2);
+    LH_CHECK_LAUNCH();
+}
+template void residual_norms<double>(lh_ctx*, int64_t, int, const double*, const double*, int, const double*, double*,
+                                     int, double*, double*, DevBuf&);
+template void residual_norms<float>(lh_ctx*, int64_t, int, const float*, const float*, int, const double*, float*,
+                                    int, double*, double*, DevBuf&);
 
 // X[:, c] *= s[c]
 template <typename T>

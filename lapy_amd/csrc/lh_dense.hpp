@@ -38,6 +38,10 @@ template <typename T>
 void residual(lh_ctx* ctx, int64_t n, int m, const T* AX, const T* MX, int ldx, const double* theta, T* R, int ldr);
 template <typename T>
 void colsq(lh_ctx* ctx, int64_t n, int m, const T* X, int ld, double* out, DevBuf& ws);
+// residual() fused with colsq() of R (into r2) and of MX (into mx2)
+template <typename T>
+void residual_norms(lh_ctx* ctx, int64_t n, int m, const T* AX, const T* MX, int ldx, const double* theta, T* R,
+                    int ldr, double* r2, double* mx2, DevBuf& ws);
 template <typename T>
 void colscale(lh_ctx* ctx, int64_t n, int m, T* X, int ld, const double* s);
 template <typename T>

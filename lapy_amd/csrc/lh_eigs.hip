@@ -234,9 +234,8 @@ struct Lobpcg {
         constexpr int kMinFp32Iters = 6;
         for (it = 0; it < max_iter; ++it) {
             // residual into W and the convergence test
-            residual<T>(ctx, n, m, ASp(), MSp(), ld, theta_d.as<double>(), Sp() + m, ld);
-            colsq<T>(ctx, n, m, Sp() + m, ld, cs1.as<double>(), wsc);
-            colsq<T>(ctx, n, m, MSp(), ld, cs2.as<double>(), wsc);
+            residual_norms<T>(ctx, n, m, ASp(), MSp(), ld, theta_d.as<double>(), Sp() + m, ld, cs1.as<double>(),
+                              cs2.as<double>(), wsc);
             d2h(ctx, r2.data(), cs1.p, m * 8);
             d2h(ctx, mx2.data(), cs2.p, m * 8);
             const double floor = 1e-2 * std::fabs(theta[k - 1]);

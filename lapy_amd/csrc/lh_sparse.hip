@@ -152,9 +152,16 @@ void spmm_ld(lh_ctx* ctx, const lh_csr* a, const XT* X, XT* Y, int ncol, int ldx
         LH_CHECK_LAUNCH();
         return;
     }
-    if (lpr == 64) k_spmm<VT, XT, 64><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
-    else if (lpr == 32) k_spmm<VT, XT, 32><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
-    else k_spmm<VT, XT, 16><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
+    if (ncol <= 2) {
+        // one or two vectors (SpMV, power iteration): one thread per row
+        k_spmm<VT, XT, 1><<<nblocks_xcd(n, 256), 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
+    } else if (lpr == 64) {
+        k_spmm<VT, XT, 64><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
+    } else if (lpr == 32) {
+        k_spmm<VT, XT, 32><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
+    } else {
+        k_spmm<VT, XT, 16><<<nb, 256, 0, ctx->stream>>>(n, rp, cl, v, X, Y, ncol, ldx, ldy);
+    }
     LH_CHECK_LAUNCH();
 }
 
