@@ -160,9 +160,10 @@ def run_workload(args, ctx, _lib, meshgen):
         # learned network, trained first (untimed) on six other meshes of
         # the same family (levels 3 and 4, seeds outside the batch)
         popts = {"cheb": {}, "jacobi": {"poly_degree": 1},
-                 "learned": {"precond": _lib.LH_PRECOND_LEARNED}}[args.precond]
+                 "learned": {"precond": _lib.LH_PRECOND_LEARNED},
+                 "auto": {"precond": _lib.LH_PRECOND_AUTO}}[args.precond]
         net = None
-        if args.precond == "learned":
+        if args.precond in ("learned", "auto"):
             from lapy_amd import learned
 
             t0 = time.perf_counter()
@@ -186,16 +187,20 @@ def run_workload(args, ctx, _lib, meshgen):
                                         **popts)
         log(f"[bench] autotune: best workers {best}, sample seconds {secs} ({time.perf_counter() - t0:.1f} s)")
         del ms, As, Bs
-        for _ in range(args.warmup):
+        for w in range(args.warmup):
+            t1 = time.perf_counter()
             ms, As, Bs = assemble_all()
             res = _lib.eigs_batched(ctx, As, Bs, [k] * 256, keep_device=True, **popts)
             ctx.sync()
             del res, ms, As, Bs
+            log(f"[bench] batch warmup {w}: {time.perf_counter() - t1:.2f} s")
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for st_i in range(args.steps):
+            t1 = time.perf_counter()
             ms, As, Bs = assemble_all()
             res = _lib.eigs_batched(ctx, As, Bs, [k] * 256, keep_device=True, **popts)
             ctx.sync()
+            log(f"[bench] batch step {st_i}: {time.perf_counter() - t1:.2f} s")
         total = time.perf_counter() - t0
         err = None
         if gold:
@@ -252,8 +257,9 @@ def main():
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--workload", default="l8", choices=["l8", "small", "tet", "pcie", "batch"],
                     help="l8: the benchmark line (default); small / tet / pcie / batch: the other workloads")
-    ap.add_argument("--precond", default="cheb", choices=["cheb", "jacobi", "learned"],
-                    help="batch workload: Chebyshev-Jacobi (default), plain Jacobi or the learned network")
+    ap.add_argument("--precond", default="cheb", choices=["cheb", "jacobi", "learned", "auto"],
+                    help="batch workload: Chebyshev-Jacobi (default), plain Jacobi, the learned network, or auto "
+                         "(the network on the meshes where the polynomial would be plain Jacobi)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="skip the eigensolves; only the roofline timings of the two measured kernels")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -70,9 +70,12 @@ typedef struct lh_stats {
                               falling or reach 1e-5, then FP64 iterations from
                               its X block to the requested tolerance        */
 
-#define LH_PRECOND_JACOBI  0
-#define LH_PRECOND_LEARNED 1
+#define LH_PRECOND_JACOBI  0   /* Jacobi with the automatic Chebyshev polynomial */
+#define LH_PRECOND_LEARNED 1   /* the context's network (lh_ctx_set_mpnn)        */
 #define LH_PRECOND_NONE    2
+#define LH_PRECOND_AUTO    3   /* eigensolver: the network where the automatic
+                                  polynomial is plain Jacobi (small meshes) and
+                                  a network is set, else LH_PRECOND_JACOBI      */
 
 /* storage of the Chebyshev preconditioner's work blocks (arithmetic is FP32) */
 #define LH_STORE_AUTO 0
@@ -159,8 +162,9 @@ typedef struct lh_eig_opts {
                               (0: automatic, 4 * degree^2)                   */
     int32_t poly_storage;  /* LH_STORE_* for the Chebyshev work blocks        */
     int32_t no_fallback;   /* LH_PRECOND_LEARNED: 1 keeps the network even when
-                              the worst residual has not halved over the last
-                              8 iterations (0: switch to plain Jacobi then,
+                              the geometric mean of the wanted pairs' absolute
+                              residuals has not halved over the last 8
+                              iterations (0: switch to plain Jacobi then,
                               stats.fallback_used = 1)                        */
 } lh_eig_opts;
 

@@ -21,7 +21,7 @@ LH_OK, LH_ERR_ARG, LH_ERR_OOM, LH_ERR_HIP, LH_ERR_NOCONV, LH_ERR_NUMERIC = range
 LH_F64, LH_F32 = 0, 1
 LH_TRIA, LH_TRIA_ANISO, LH_TRIA_MASS, LH_TET = 0, 1, 2, 3
 LH_PREC_FP64, LH_PREC_MIXED = 0, 1
-LH_PRECOND_JACOBI, LH_PRECOND_LEARNED, LH_PRECOND_NONE = 0, 1, 2
+LH_PRECOND_JACOBI, LH_PRECOND_LEARNED, LH_PRECOND_NONE, LH_PRECOND_AUTO = 0, 1, 2, 3
 LH_STORE_AUTO, LH_STORE_F32, LH_STORE_F16 = 0, 1, 2
 
 _vp = C.c_void_p

@@ -214,7 +214,7 @@ struct Lobpcg {
             learned_on = true;
         }
         constexpr int kStallWindow = 8;
-        double hist[kStallWindow] = {};  // worst residual of the last iterations
+        double hist[kStallWindow] = {};  // residual measure of the last iterations
 
         // start block
         if (x0) copy_block<T, T>(ctx, n, m, x0, x0_ld, Sp(), ld);
@@ -249,15 +249,20 @@ struct Lobpcg {
             }
             st.max_resid = worst;
             st.converged = nconv;
-            // learned preconditioner: back to Jacobi when the worst residual
-            // has not halved over the last kStallWindow iterations.  (A test
-            // per iteration would reject every preconditioner of Jacobi's
-            // class: LOBPCG reduces the residual by 0.5 - 0.97 per iteration
-            // with them.)
+            // learned preconditioner: back to Jacobi when the geometric mean
+            // of the wanted pairs' absolute residuals ||A x - theta B x|| /
+            // ||B x|| has not halved over the last kStallWindow iterations.
+            // (A test per iteration would reject every preconditioner of
+            // Jacobi's class: LOBPCG reduces the residual by 0.5 - 0.97 per
+            // iteration with them; relative residuals can rise early on, while
+            // the Ritz values fall towards the smallest eigenvalues.)
             if (learned_on && !no_fallback) {
+                double lg = 0.0;
+                for (int j = 0; j < k; ++j) lg += std::log(std::max(res[j], 1e-300));
+                const double gm = std::exp(lg / k);
                 const double before = hist[it % kStallWindow];  // iteration it - kStallWindow
-                hist[it % kStallWindow] = worst;
-                if (it >= kStallWindow && worst > 0.5 * before) {
+                hist[it % kStallWindow] = gm;
+                if (it >= kStallWindow && gm > 0.5 * before) {
                     learned_on = false;
                     st.fallback_used = 1;
                     if (verbose) fprintf(stderr, "[lobpcg] it %d: learned preconditioner -> Jacobi\n", it);
@@ -433,9 +438,17 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
     // grows like n^(2/d)) until the iteration count stops falling (~100 at
     // 655k vertices: beyond it the block's own gap ratio limits LOBPCG);
     // poly_degree 1 is plain Jacobi
-    const int pc = opts ? opts->precond : LH_PRECOND_JACOBI;
-    if (pc != LH_PRECOND_JACOBI && pc != LH_PRECOND_LEARNED && pc != LH_PRECOND_NONE)
+    int pc = opts ? opts->precond : LH_PRECOND_JACOBI;
+    if (pc != LH_PRECOND_JACOBI && pc != LH_PRECOND_LEARNED && pc != LH_PRECOND_NONE && pc != LH_PRECOND_AUTO)
         LH_THROW(LH_ERR_ARG, "preconditioner %d unknown", pc);
+    const int auto_deg = std::min(128, (int)std::lround(std::sqrt((double)n) / 8.5));
+    if (pc == LH_PRECOND_AUTO) {
+        // the learned network where the automatic polynomial would be plain
+        // Jacobi (small meshes), Chebyshev-Jacobi elsewhere
+        const bool small = (opts && opts->poly_degree > 0) ? opts->poly_degree == 1 : auto_deg < 4;
+        const bool fp64 = !opts || opts->precision == LH_PREC_FP64;  // the mixed solve takes no network
+        pc = (ctx->mpnn && small && fp64) ? LH_PRECOND_LEARNED : LH_PRECOND_JACOBI;
+    }
     if (pc == LH_PRECOND_LEARNED && !ctx->mpnn)
         LH_THROW(LH_ERR_ARG, "LH_PRECOND_LEARNED needs a network on the context (lh_ctx_set_mpnn)");
     L.precond = pc;
@@ -443,11 +456,7 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
     const bool jac = pc == LH_PRECOND_JACOBI;
     int deg = opts ? opts->poly_degree : 0;
     if (!jac) deg = 1;
-    if (deg <= 0) {
-        deg = (int)std::lround(std::sqrt((double)n) / 8.5);
-        deg = std::min(deg, 128);
-        if (deg < 4) deg = 1;
-    }
+    if (deg <= 0) deg = auto_deg < 4 ? 1 : auto_deg;
     deg = std::max(1, std::min(deg, 1024));
     L.poly_degree = deg;
     const int store = opts ? opts->poly_storage : LH_STORE_AUTO;

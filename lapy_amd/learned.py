@@ -17,9 +17,10 @@ eigensolver's shift, sigma = -0.01).
 
 Use: ``net.use()`` makes it the context's LH_PRECOND_LEARNED for
 ``Solver.eigs(precond="learned")`` and ``heat.diffusion``'s PCG; both fall
-back to plain Jacobi when convergence stalls: LOBPCG when the worst residual
-has not halved over the last 8 iterations, PCG when it falls by less than half
-per iteration over three 4-iteration checks running.
+back to plain Jacobi when convergence stalls: LOBPCG when the geometric mean of
+the wanted pairs' residuals has not halved over the last 8 iterations, PCG when
+the residual falls by less than half per iteration over three 4-iteration checks
+running.
 """
 
 from __future__ import annotations

@@ -160,7 +160,7 @@ class Solver:
             raise ValueError(f"k must be between 1 and ndim(A)-1 (k={k}, n={n})")
         prec = {"fp64": _lib.LH_PREC_FP64, "mixed": _lib.LH_PREC_MIXED}[precision]
         pc = {"jacobi": _lib.LH_PRECOND_JACOBI, "learned": _lib.LH_PRECOND_LEARNED,
-              "none": _lib.LH_PRECOND_NONE}[precond]
+              "none": _lib.LH_PRECOND_NONE, "auto": _lib.LH_PRECOND_AUTO}[precond]
         store = {"auto": _lib.LH_STORE_AUTO, "f32": _lib.LH_STORE_F32, "f16": _lib.LH_STORE_F16}[poly_storage]
         kk = k
         while True:
