@@ -156,10 +156,10 @@ typedef struct lh_eig_opts {
     int32_t verbose;
     int32_t poly_degree;   /* with LH_PRECOND_JACOBI: degree of the Chebyshev
                               polynomial in D^-1 (A - sigma B) applied on top
-                              of Jacobi; 0: automatic (~sqrt(n)/8.5, at most
+                              of Jacobi; 0: automatic (~sqrt(n)/6.3, at most
                               128), 1: plain Jacobi                          */
     double  poly_ratio;    /* Chebyshev interval [lmax/ratio, lmax]
-                              (0: automatic, 4 * degree^2)                   */
+                              (0: automatic, 2 * degree^2)                   */
     int32_t poly_storage;  /* LH_STORE_* for the Chebyshev work blocks        */
     int32_t no_fallback;   /* LH_PRECOND_LEARNED: 1 keeps the network even when
                               the geometric mean of the wanted pairs' absolute

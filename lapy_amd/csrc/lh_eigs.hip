@@ -433,15 +433,15 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
     L.max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 20000;
     L.seed = opts ? opts->seed : 0;
     L.verbose = opts ? opts->verbose : 0;
-    // preconditioner: Jacobi, accelerated by a Chebyshev polynomial whose
-    // degree grows with sqrt(n) (the condition number of a mesh Laplacian
-    // grows like n^(2/d)) until the iteration count stops falling (~100 at
-    // 655k vertices: beyond it the block's own gap ratio limits LOBPCG);
-    // poly_degree 1 is plain Jacobi
+    // preconditioner: Jacobi, accelerated by a Chebyshev polynomial on
+    // [lmax / (2 d^2), lmax] whose degree d grows with sqrt(n) (the condition
+    // number of a mesh Laplacian grows like n^(2/dim)) until the iteration
+    // count stops falling (128 at 655k vertices: beyond it the block's own
+    // gap ratio limits LOBPCG); poly_degree 1 is plain Jacobi
     int pc = opts ? opts->precond : LH_PRECOND_JACOBI;
     if (pc != LH_PRECOND_JACOBI && pc != LH_PRECOND_LEARNED && pc != LH_PRECOND_NONE && pc != LH_PRECOND_AUTO)
         LH_THROW(LH_ERR_ARG, "preconditioner %d unknown", pc);
-    const int auto_deg = std::min(128, (int)std::lround(std::sqrt((double)n) / 8.5));
+    const int auto_deg = std::min(128, (int)std::lround(std::sqrt((double)n) / 6.3));
     if (pc == LH_PRECOND_AUTO) {
         // the learned network where the automatic polynomial would be plain
         // Jacobi (small meshes), Chebyshev-Jacobi elsewhere
@@ -462,7 +462,7 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
     const int store = opts ? opts->poly_storage : LH_STORE_AUTO;
     if (store != LH_STORE_AUTO && store != LH_STORE_F32 && store != LH_STORE_F16)
         LH_THROW(LH_ERR_ARG, "poly_storage %d unknown", store);
-    L.poly_ratio = (opts && opts->poly_ratio > 1.0) ? opts->poly_ratio : 4.0 * deg * deg;
+    L.poly_ratio = (opts && opts->poly_ratio > 1.0) ? opts->poly_ratio : 2.0 * deg * deg;
     const int prec = opts ? opts->precision : LH_PREC_FP64;
     if (prec != LH_PREC_FP64 && prec != LH_PREC_MIXED) LH_THROW(LH_ERR_ARG, "precision mode %d unknown", prec);
     if (prec == LH_PREC_MIXED && pc == LH_PRECOND_LEARNED)

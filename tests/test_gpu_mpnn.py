@@ -144,26 +144,26 @@ def test_learned_needs_a_network():
 
 
 def test_auto_uses_the_network_only_where_the_polynomial_is_jacobi(trained):
-    """LH_PRECOND_AUTO: the network on a small mesh (automatic polynomial of
-    degree < 4: plain Jacobi), Chebyshev-Jacobi on a larger one; results as
-    with the explicit choices."""
+    """LH_PRECOND_AUTO: the network where the polynomial is plain Jacobi
+    (here by poly_degree=1; automatically on meshes too small for a degree of
+    4), Chebyshev-Jacobi elsewhere; results as with the explicit choices."""
     net, _ = trained
     ctx = _lib.default_context()
-    small_v, small_t = mesh(3, 77)  # 642 vertices: polynomial off
-    large_v, large_t = mesh(5, 78)  # 10,242 vertices: polynomial on
+    v, t = mesh(3, 77)
+    dm, A, B, K = device_ops(ctx, v, t)
     net.use()
     try:
-        cases = (((small_v, small_t), _lib.LH_PRECOND_LEARNED), ((large_v, large_t), _lib.LH_PRECOND_JACOBI))
-        for (v, t), expect in cases:
-            dm, A, B, K = device_ops(ctx, v, t)
-            ev_a, _, st_a = _lib.eigs(ctx, A, B, 10, precond=_lib.LH_PRECOND_AUTO)
-            ev_e, _, st_e = _lib.eigs(ctx, A, B, 10, precond=expect)
-            assert st_a.iterations == st_e.iterations
-            np.testing.assert_array_equal(ev_a, ev_e)
+        ev_a, _, st_a = _lib.eigs(ctx, A, B, 10, precond=_lib.LH_PRECOND_AUTO, poly_degree=1)
+        ev_e, _, st_e = _lib.eigs(ctx, A, B, 10, precond=_lib.LH_PRECOND_LEARNED)
+        assert st_a.iterations == st_e.iterations
+        np.testing.assert_array_equal(ev_a, ev_e)
+        ev_a, _, st_a = _lib.eigs(ctx, A, B, 10, precond=_lib.LH_PRECOND_AUTO)
+        ev_e, _, st_e = _lib.eigs(ctx, A, B, 10)
+        assert st_a.iterations == st_e.iterations
+        np.testing.assert_array_equal(ev_a, ev_e)
     finally:
         net.use(False)
-    # without a network: Chebyshev-Jacobi everywhere
-    dm, A, B, K = device_ops(ctx, small_v, small_t)
-    ev_a, _, st_a = _lib.eigs(ctx, A, B, 10, precond=_lib.LH_PRECOND_AUTO)
-    ev_j, _, st_j = _lib.eigs(ctx, A, B, 10)
+    # without a network: Chebyshev-Jacobi (plain Jacobi with poly_degree=1)
+    ev_a, _, st_a = _lib.eigs(ctx, A, B, 10, precond=_lib.LH_PRECOND_AUTO, poly_degree=1)
+    ev_j, _, st_j = _lib.eigs(ctx, A, B, 10, poly_degree=1)
     np.testing.assert_array_equal(ev_a, ev_j)
