@@ -343,10 +343,12 @@ __device__ __forceinline__ bool lds_window_begin(OT* sA, OT* sB, VT* sx, VT* sy,
         if (HAS_B) sB[s] = OT(0);
         const int32_t c = cols[base + s];
         sc[s] = c;
-        const V3<VT> v = load_xyz(xyz, c);
-        sx[s] = v.x;
-        sy[s] = v.y;
-        sz[s] = v.z;
+        if (sx) {
+            const V3<VT> v = load_xyz(xyz, c);
+            sx[s] = v.x;
+            sy[s] = v.y;
+            sz[s] = v.z;
+        }
     }
     __syncthreads();
     return true;
@@ -466,7 +468,10 @@ __device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ 
     if constexpr (LUMP) Bval[lrowptr[i]] = bl;
 }
 
-template <typename VT, typename OT, int KIND, bool LUMP>
+// STAGE_XYZ: the slots' vertex coordinates are staged in LDS too (else each
+// element's three vertices are gathered from the AoS copy, which leaves room
+// for more workgroups per CU)
+template <typename VT, typename OT, int KIND, bool LUMP, bool STAGE_XYZ>
 __global__ void __launch_bounds__(256)
 k_tria_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
             const int32_t* __restrict__ diagpos, const int32_t* __restrict__ lrowptr,
@@ -479,10 +484,10 @@ k_tria_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __rest
     extern __shared__ __align__(16) unsigned char lds_raw[];
     OT* sA = reinterpret_cast<OT*>(lds_raw);
     OT* sB = sA + cap;
-    VT* sx = reinterpret_cast<VT*>(sB + cap);
-    VT* sy = sx + cap;
-    VT* sz = sy + cap;
-    int32_t* sc = reinterpret_cast<int32_t*>(sz + cap);
+    VT* sx = STAGE_XYZ ? reinterpret_cast<VT*>(sB + cap) : nullptr;
+    VT* sy = STAGE_XYZ ? sx + cap : nullptr;
+    VT* sz = STAGE_XYZ ? sy + cap : nullptr;
+    int32_t* sc = STAGE_XYZ ? reinterpret_cast<int32_t*>(sz + cap) : reinterpret_cast<int32_t*>(sB + cap);
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
     const int64_t i = i0 + threadIdx.x;
     const int64_t iend = (i0 + blockDim.x < n) ? i0 + blockDim.x : n;
@@ -801,6 +806,11 @@ void clamp_value(lh_ctx* ctx, lh_mesh* m, int kind, DevBuf& vm) {
     HIPX(hipStreamSynchronize(ctx->stream));
 }
 
+// triangles gather their vertices from the AoS copy instead of staging the
+// slots' coordinates in LDS: 20 instead of 44 bytes of LDS per slot, twice the
+// workgroups per CU (105 -> 81 us at 655k vertices)
+constexpr bool kStageTriaXyz = false;
+
 template <typename VT, typename OT>
 void launch_numeric(lh_ctx* ctx, lh_mesh* m, int kind, int lump, const double* u1, const double* u2,
                     const double* am, int clamp, const VT* vm, OT* Aval, OT* Bval, int32_t* degen) {
@@ -810,10 +820,12 @@ void launch_numeric(lh_ctx* ctx, lh_mesh* m, int kind, int lump, const double* u
     const int bs = kind == LH_TET ? 64 : 128;
     const unsigned nb = nblocks(n, bs);
     // LDS window: 1.25x the average row length times the rows of a
-    // workgroup, at most 64 KB (A and B values, three coordinates and the
-    // column index per slot)
+    // workgroup, at most 64 KB (A and B values and the column index per slot,
+    // plus three coordinates for tetrahedra); larger ranges take the global
+    // path
     const int64_t avg_row = (p.nnz + n - 1) / n;
-    const int64_t per_slot = 2 * sizeof(OT) + 3 * sizeof(VT) + sizeof(int32_t);
+    const bool stage_xyz = kind == LH_TET || kStageTriaXyz;
+    const int64_t per_slot = 2 * sizeof(OT) + (stage_xyz ? 3 * sizeof(VT) : 0) + sizeof(int32_t);
     const int64_t cap_max = (65536 / per_slot) / 64 * 64;
     const int cap = (int)std::min<int64_t>(cap_max, (avg_row * bs * 5 / 4 + 63) / 64 * 64);
     const size_t shm = (size_t)cap * per_slot;
@@ -822,8 +834,8 @@ void launch_numeric(lh_ctx* ctx, lh_mesh* m, int kind, int lump, const double* u
     const int32_t *rp = p.rowptr.as<int32_t>(), *cl = p.cols.as<int32_t>(), *dp = p.diagpos.as<int32_t>();
     const int32_t *ip = m->inc_ptr.as<int32_t>(), *ic = m->inc.as<int32_t>();
 #define TRIA_LAUNCH(K, L)                                                                          \
-    k_tria_rows<VT, OT, K, L><<<nb, bs, shm, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, iv, xyz, u1, u2, am, clamp, \
-                                                           vm, Aval, Bval, degen, cap)
+    k_tria_rows<VT, OT, K, L, kStageTriaXyz><<<nb, bs, shm, ctx->stream>>>(n, rp, cl, dp, lrp, ip, ic, iv, xyz, u1, \
+                                                                          u2, am, clamp, vm, Aval, Bval, degen, cap)
     if (kind == LH_TRIA) { if (lump) TRIA_LAUNCH(LH_TRIA, true); else TRIA_LAUNCH(LH_TRIA, false); }
     else if (kind == LH_TRIA_ANISO) { if (lump) TRIA_LAUNCH(LH_TRIA_ANISO, true); else TRIA_LAUNCH(LH_TRIA_ANISO, false); }
     else if (kind == LH_TRIA_MASS) { if (lump) TRIA_LAUNCH(LH_TRIA_MASS, true); else TRIA_LAUNCH(LH_TRIA_MASS, false); }
