@@ -384,10 +384,13 @@ __device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ 
     OT bl = OT(0);
     const int32_t dslot = diagpos[i];
     const VT vm = clamp ? vol_mean[0] : VT(0);
-    int4 ev_next = incv[pa];
-    for (int32_t p = pa; p < pb; ++p) {
-        const int4 ev = ev_next;
-        if (p + 1 < pb) ev_next = incv[p + 1];  // next record's load overlaps this element
+    // one element's contribution to this row: slots and values
+    struct Contrib {
+        int32_t sa, sb;
+        OT oa, ob, od, bij, bii, bl;
+    };
+    auto eval = [&](int32_t p, const int4 ev) -> Contrib {
+        Contrib c{};
         const int l = ev.w;
         const int32_t i1 = ev.x, i2 = ev.y, i3 = ev.z;
         // columns of this row's three entries: the other two vertices + diag
@@ -395,11 +398,11 @@ __device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ 
         if (l == 0) { ja = i2; jb = i3; }
         else if (l == 1) { ja = i1; jb = i3; }
         else { ja = i2; jb = i1; }
-        const int32_t sa = row_slot(rv, cols, r0, r1, off, ja);
-        const int32_t sb = row_slot(rv, cols, r0, r1, off, jb);
+        c.sa = row_slot(rv, cols, r0, r1, off, ja);
+        c.sb = row_slot(rv, cols, r0, r1, off, jb);
         V3<VT> v1, v2, v3;
         if (rv.x) {
-            const V3<VT> vi = slot_xyz(rv, dslot - off), va = slot_xyz(rv, sa - off), vb = slot_xyz(rv, sb - off);
+            const V3<VT> vi = slot_xyz(rv, dslot - off), va = slot_xyz(rv, c.sa - off), vb = slot_xyz(rv, c.sb - off);
             if (l == 0) { v1 = vi; v2 = va; v3 = vb; }
             else if (l == 1) { v1 = va; v2 = vi; v3 = vb; }
             else { v1 = vb; v2 = va; v3 = vi; }
@@ -420,14 +423,13 @@ __device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ 
         if constexpr (HAS_A) {
             // a12 (t1,t2), a23 (t2,t3), a31 (t3,t1); row t1: a12 -> t2, a31 -> t3
             // row t2: a12 -> t1, a23 -> t3; row t3: a23 -> t2, a31 -> t1
-            OT oa, ob, od;
             if constexpr (KIND == LH_TRIA) {
                 VT a12 = vdot(v3mv2, v1mv3) / vol;
                 VT a23 = vdot(v1mv3, v2mv1) / vol;
                 VT a31 = vdot(v2mv1, v3mv2) / vol;
-                if (l == 0) { oa = (OT)a12; ob = (OT)a31; od = (OT)(-a12 - a31); }
-                else if (l == 1) { oa = (OT)a12; ob = (OT)a23; od = (OT)(-a12 - a23); }
-                else { oa = (OT)a23; ob = (OT)a31; od = (OT)(-a31 - a23); }
+                if (l == 0) { c.oa = (OT)a12; c.ob = (OT)a31; c.od = (OT)(-a12 - a31); }
+                else if (l == 1) { c.oa = (OT)a12; c.ob = (OT)a23; c.od = (OT)(-a12 - a23); }
+                else { c.oa = (OT)a23; c.ob = (OT)a31; c.od = (OT)(-a31 - a23); }
             } else {
                 // lapy/solver.py:259-280: edges projected on (u1, u2), weighted
                 const int64_t e = inc[p] >> 2;
@@ -445,25 +447,55 @@ __device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ 
                 double a12 = ((p32_0 * w0) * p13_0 + (p32_1 * w1) * p13_1) / dv;
                 double a23 = ((p13_0 * w0) * p21_0 + (p13_1 * w1) * p21_1) / dv;
                 double a31 = ((p21_0 * w0) * p32_0 + (p21_1 * w1) * p32_1) / dv;
-                if (l == 0) { oa = (OT)a12; ob = (OT)a31; od = (OT)(-a12 - a31); }
-                else if (l == 1) { oa = (OT)a12; ob = (OT)a23; od = (OT)(-a12 - a23); }
-                else { oa = (OT)a23; ob = (OT)a31; od = (OT)(-a31 - a23); }
+                if (l == 0) { c.oa = (OT)a12; c.ob = (OT)a31; c.od = (OT)(-a12 - a31); }
+                else if (l == 1) { c.oa = (OT)a12; c.ob = (OT)a23; c.od = (OT)(-a12 - a23); }
+                else { c.oa = (OT)a23; c.ob = (OT)a31; c.od = (OT)(-a31 - a23); }
             }
-            Aacc[sa - off] += oa;
-            Aacc[sb - off] += ob;
-            Aacc[dslot - off] += od;
         }
         if constexpr (LUMP) {
-            if constexpr (KIND == LH_TRIA_MASS) bl += (OT)(vol / VT(3));
-            else bl += (OT)(vol / VT(12));
+            if constexpr (KIND == LH_TRIA_MASS) c.bl = (OT)(vol / VT(3));
+            else c.bl = (OT)(vol / VT(12));
         } else {
-            OT bij, bii;
-            if constexpr (KIND == LH_TRIA_MASS) { bij = (OT)(vol / VT(12)); bii = (OT)(vol / VT(6)); }
-            else { bij = (OT)(vol / VT(48)); bii = (OT)(vol / VT(24)); }
-            Bacc[sa - off] += bij;
-            Bacc[sb - off] += bij;
-            Bacc[dslot - off] += bii;
+            if constexpr (KIND == LH_TRIA_MASS) { c.bij = (OT)(vol / VT(12)); c.bii = (OT)(vol / VT(6)); }
+            else { c.bij = (OT)(vol / VT(48)); c.bii = (OT)(vol / VT(24)); }
         }
+        return c;
+    };
+    // sums in ascending element order, as the reference's duplicate sums
+    auto apply = [&](const Contrib& c) {
+        if constexpr (HAS_A) {
+            Aacc[c.sa - off] += c.oa;
+            Aacc[c.sb - off] += c.ob;
+            Aacc[dslot - off] += c.od;
+        }
+        if constexpr (LUMP) {
+            bl += c.bl;
+        } else {
+            Bacc[c.sa - off] += c.bij;
+            Bacc[c.sb - off] += c.bij;
+            Bacc[dslot - off] += c.bii;
+        }
+    };
+    // three elements evaluated together (independent work in flight), then
+    // accumulated one after the other
+    int32_t p = pa;
+    for (; p + 2 < pb; p += 3) {
+        const int4 e0 = incv[p], e1 = incv[p + 1], e2 = incv[p + 2];
+        const Contrib c0 = eval(p, e0);
+        const Contrib c1 = eval(p + 1, e1);
+        const Contrib c2 = eval(p + 2, e2);
+        apply(c0);
+        apply(c1);
+        apply(c2);
+    }
+    if (p + 1 < pb) {
+        const int4 e0 = incv[p], e1 = incv[p + 1];
+        const Contrib c0 = eval(p, e0);
+        const Contrib c1 = eval(p + 1, e1);
+        apply(c0);
+        apply(c1);
+    } else if (p < pb) {
+        apply(eval(p, incv[p]));
     }
     if constexpr (LUMP) Bval[lrowptr[i]] = bl;
 }
