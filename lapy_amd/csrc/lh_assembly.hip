@@ -153,6 +153,33 @@ __global__ void k_inc_verts(int64_t cnt, int nodes, const int32_t* __restrict__ 
     incv[p] = r;
 }
 
+// triangle incidence records: the slots of the row's two off-diagonal
+// entries, as offsets from the row start, packed beside the local corner:
+// w = l | off_a << 2 | off_b << 17 (15 bits each; kSlotSearch: rows too long,
+// the assembly searches the row's columns instead)
+constexpr uint32_t kSlotSearch = 0x7fff;
+__global__ void k_inc_slots(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+                            const int32_t* __restrict__ inc_ptr, int4* __restrict__ incv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t r0 = rowptr[i], r1 = rowptr[i + 1];
+    for (int32_t p = inc_ptr[i]; p < inc_ptr[i + 1]; ++p) {
+        int4 ev = incv[p];
+        const int l = ev.w & 3;
+        int32_t ja, jb;
+        if (l == 0) { ja = ev.y; jb = ev.z; }
+        else if (l == 1) { ja = ev.x; jb = ev.z; }
+        else { ja = ev.y; jb = ev.x; }
+        uint32_t oa = kSlotSearch, ob = kSlotSearch;
+        if (r1 - r0 < (int32_t)kSlotSearch) {
+            oa = (uint32_t)(find_slot(cols, r0, r1, ja) - r0);
+            ob = (uint32_t)(find_slot(cols, r0, r1, jb) - r0);
+        }
+        ev.w = (int32_t)((uint32_t)l | (oa << 2) | (ob << 17));
+        incv[p] = ev;
+    }
+}
+
 // AoS copy of the coordinates, one 4-wide record per vertex
 template <typename VT>
 __global__ void k_aos(int64_t nv, const VT* __restrict__ x, const VT* __restrict__ y, const VT* __restrict__ z,
@@ -341,8 +368,9 @@ __device__ __forceinline__ bool lds_window_begin(OT* sA, OT* sB, VT* sx, VT* sy,
     for (int32_t s = threadIdx.x; s < len; s += blockDim.x) {
         if (HAS_A) sA[s] = OT(0);
         if (HAS_B) sB[s] = OT(0);
+        if (!sc && !sx) continue;
         const int32_t c = cols[base + s];
-        sc[s] = c;
+        if (sc) sc[s] = c;
         if (sx) {
             const V3<VT> v = load_xyz(xyz, c);
             sx[s] = v.x;
@@ -391,15 +419,23 @@ __device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ 
     };
     auto eval = [&](int32_t p, const int4 ev) -> Contrib {
         Contrib c{};
-        const int l = ev.w;
+        const uint32_t w = (uint32_t)ev.w;
+        const int l = (int)(w & 3u);
         const int32_t i1 = ev.x, i2 = ev.y, i3 = ev.z;
-        // columns of this row's three entries: the other two vertices + diag
-        int32_t ja, jb;  // ja: "next" column, jb: "previous" column
-        if (l == 0) { ja = i2; jb = i3; }
-        else if (l == 1) { ja = i1; jb = i3; }
-        else { ja = i2; jb = i1; }
-        c.sa = row_slot(rv, cols, r0, r1, off, ja);
-        c.sb = row_slot(rv, cols, r0, r1, off, jb);
+        // slots of this row's two off-diagonal entries: precomputed in the
+        // record (ja: "next" column, jb: "previous" column), else searched
+        const uint32_t oa = (w >> 2) & kSlotSearch, ob = w >> 17;
+        if (oa != kSlotSearch) {
+            c.sa = r0 + (int32_t)oa;
+            c.sb = r0 + (int32_t)ob;
+        } else {
+            int32_t ja, jb;
+            if (l == 0) { ja = i2; jb = i3; }
+            else if (l == 1) { ja = i1; jb = i3; }
+            else { ja = i2; jb = i1; }
+            c.sa = row_slot(rv, cols, r0, r1, off, ja);
+            c.sb = row_slot(rv, cols, r0, r1, off, jb);
+        }
         V3<VT> v1, v2, v3;
         if (rv.x) {
             const V3<VT> vi = slot_xyz(rv, dslot - off), va = slot_xyz(rv, c.sa - off), vb = slot_xyz(rv, c.sb - off);
@@ -519,7 +555,9 @@ k_tria_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __rest
     VT* sx = STAGE_XYZ ? reinterpret_cast<VT*>(sB + cap) : nullptr;
     VT* sy = STAGE_XYZ ? sx + cap : nullptr;
     VT* sz = STAGE_XYZ ? sy + cap : nullptr;
-    int32_t* sc = STAGE_XYZ ? reinterpret_cast<int32_t*>(sz + cap) : reinterpret_cast<int32_t*>(sB + cap);
+    // the slots are precomputed in the incidence records: no column staging
+    // (the rare rows too long for the records search the global columns)
+    int32_t* sc = STAGE_XYZ ? reinterpret_cast<int32_t*>(sz + cap) : nullptr;
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
     const int64_t i = i0 + threadIdx.x;
     const int64_t iend = (i0 + blockDim.x < n) ? i0 + blockDim.x : n;
@@ -809,6 +847,11 @@ void build_topology(lh_ctx* ctx, lh_mesh* m) {
                                                           p->diagpos.as<int32_t>());
     LH_CHECK_LAUNCH();
     m->full = p;
+    if (nodes == 3) {
+        k_inc_slots<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, p->rowptr.as<int32_t>(), p->cols.as<int32_t>(),
+                                                              m->inc_ptr.as<int32_t>(), m->incv.as<int4>());
+        LH_CHECK_LAUNCH();
+    }
     // lumped pattern: diagonal of used vertices (scipy builds it from (i, i)
     // triplets of element vertices, lapy/solver.py:190-193)
     DevBuf used(n * 4);
@@ -857,7 +900,7 @@ void launch_numeric(lh_ctx* ctx, lh_mesh* m, int kind, int lump, const double* u
     // path
     const int64_t avg_row = (p.nnz + n - 1) / n;
     const bool stage_xyz = kind == LH_TET || kStageTriaXyz;
-    const int64_t per_slot = 2 * sizeof(OT) + (stage_xyz ? 3 * sizeof(VT) : 0) + sizeof(int32_t);
+    const int64_t per_slot = 2 * sizeof(OT) + (stage_xyz ? 3 * sizeof(VT) + sizeof(int32_t) : 0);
     const int64_t cap_max = (65536 / per_slot) / 64 * 64;
     const int cap = (int)std::min<int64_t>(cap_max, (avg_row * bs * 5 / 4 + 63) / 64 * 64);
     const size_t shm = (size_t)cap * per_slot;
