@@ -578,6 +578,11 @@ k_tria_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __rest
     }
 }
 
+// tetrahedra stage their slots' columns in LDS (for the slot searches) but
+// gather vertex coordinates from the AoS copy: 20 instead of 44 bytes of LDS
+// per slot, more workgroups per CU
+constexpr bool kStageTetXyz = false;
+
 // tetrahedra: lapy/solver.py:414-532
 template <typename VT, typename OT, bool LUMP>
 __device__ __forceinline__ void tet_row(int64_t i, const int32_t* __restrict__ rowptr,
@@ -684,10 +689,12 @@ k_tet_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restr
     extern __shared__ __align__(16) unsigned char lds_raw[];
     OT* sA = reinterpret_cast<OT*>(lds_raw);
     OT* sB = sA + cap;
-    VT* sx = reinterpret_cast<VT*>(sB + cap);
-    VT* sy = sx + cap;
-    VT* sz = sy + cap;
-    int32_t* sc = reinterpret_cast<int32_t*>(sz + cap);
+    // columns staged (the slot searches), coordinates gathered from the AoS
+    // copy unless kStageTetXyz
+    VT* sx = kStageTetXyz ? reinterpret_cast<VT*>(sB + cap) : nullptr;
+    VT* sy = kStageTetXyz ? sx + cap : nullptr;
+    VT* sz = kStageTetXyz ? sy + cap : nullptr;
+    int32_t* sc = kStageTetXyz ? reinterpret_cast<int32_t*>(sz + cap) : reinterpret_cast<int32_t*>(sB + cap);
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x;
     const int64_t i = i0 + threadIdx.x;
     const int64_t iend = (i0 + blockDim.x < n) ? i0 + blockDim.x : n;
@@ -899,8 +906,12 @@ void launch_numeric(lh_ctx* ctx, lh_mesh* m, int kind, int lump, const double* u
     // plus three coordinates for tetrahedra); larger ranges take the global
     // path
     const int64_t avg_row = (p.nnz + n - 1) / n;
-    const bool stage_xyz = kind == LH_TET || kStageTriaXyz;
-    const int64_t per_slot = 2 * sizeof(OT) + (stage_xyz ? 3 * sizeof(VT) + sizeof(int32_t) : 0);
+    // LDS per slot: A and B accumulators, plus the column (tetrahedra, or
+    // triangles with staged coordinates) and the three coordinates when staged
+    const bool stage_xyz = kind == LH_TET ? kStageTetXyz : kStageTriaXyz;
+    const bool stage_col = kind == LH_TET || kStageTriaXyz;
+    const int64_t per_slot =
+        2 * sizeof(OT) + (stage_col ? sizeof(int32_t) : 0) + (stage_xyz ? 3 * sizeof(VT) : 0);
     const int64_t cap_max = (65536 / per_slot) / 64 * 64;
     const int cap = (int)std::min<int64_t>(cap_max, (avg_row * bs * 5 / 4 + 63) / 64 * 64);
     const size_t shm = (size_t)cap * per_slot;
