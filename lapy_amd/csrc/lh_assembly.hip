@@ -601,7 +601,13 @@ __device__ __forceinline__ void tet_row(int64_t i, const int32_t* __restrict__ r
     OT bl = OT(0);
     const int32_t dslot = diagpos[i];
     const VT vm = clamp ? vol_mean[0] : VT(0);
-    for (int32_t p = pa; p < pb; ++p) {
+    // one element's contribution to this row: slots and values
+    struct Contrib {
+        int32_t s0, s1, s2;
+        OT a0, a1, a2, ad, bij, bii, bl;
+    };
+    auto eval = [&](int32_t p) -> Contrib {
+        Contrib c{};
         const int l = inc[p] & 3;
         const int4 ev = incv[p];
         const int32_t t1 = ev.x, t2 = ev.y, t3 = ev.z, t4 = ev.w;
@@ -611,6 +617,9 @@ __device__ __forceinline__ void tet_row(int64_t i, const int32_t* __restrict__ r
         const int32_t s0 = row_slot(rv, cols, r0, r1, off, c0);
         const int32_t s1 = row_slot(rv, cols, r0, r1, off, c1);
         const int32_t s2 = row_slot(rv, cols, r0, r1, off, c2);
+        c.s0 = s0;
+        c.s1 = s1;
+        c.s2 = s2;
         V3<VT> v1, v2, v3, v4;
         if (rv.x) {
             v1 = slot_xyz(rv, (l == 0 ? dslot : s0) - off);
@@ -656,25 +665,45 @@ __device__ __forceinline__ void tet_row(int64_t i, const int32_t* __restrict__ r
             o2 = ((-e14) * e13 + e11 * e34) / vol;  // a34
         }
         // diagonal from the row sum, in the reference's term order
-        VT dg;
-        if (l == 0) dg = (-o0 - o1) - o2;        // a11 = -a12 - a13 - a14
-        else if (l == 1) dg = (-o0 - o1) - o2;   // a22 = -a12 - a23 - a24
-        else if (l == 2) dg = (-o0 - o1) - o2;   // a33 = -a13 - a23 - a34
-        else dg = (-o0 - o1) - o2;               // a44 = -a14 - a24 - a34
-        Aacc[s0 - off] += (OT)(o0 / VT(6.0));
-        Aacc[s1 - off] += (OT)(o1 / VT(6.0));
-        Aacc[s2 - off] += (OT)(o2 / VT(6.0));
-        Aacc[dslot - off] += (OT)(dg / VT(6.0));
+        // (a11 = -a12 - a13 - a14, a22 = -a12 - a23 - a24, ...)
+        const VT dg = (-o0 - o1) - o2;
+        c.a0 = (OT)(o0 / VT(6.0));
+        c.a1 = (OT)(o1 / VT(6.0));
+        c.a2 = (OT)(o2 / VT(6.0));
+        c.ad = (OT)(dg / VT(6.0));
         if constexpr (LUMP) {
-            bl += (OT)(vol / VT(24.0));
+            c.bl = (OT)(vol / VT(24.0));
         } else {
-            OT bij = (OT)(vol / VT(120.0)), bii = (OT)(vol / VT(60.0));
-            Bacc[s0 - off] += bij;
-            Bacc[s1 - off] += bij;
-            Bacc[s2 - off] += bij;
-            Bacc[dslot - off] += bii;
+            c.bij = (OT)(vol / VT(120.0));
+            c.bii = (OT)(vol / VT(60.0));
         }
+        return c;
+    };
+    // sums in ascending element order, as the reference's duplicate sums
+    auto apply = [&](const Contrib& c) {
+        Aacc[c.s0 - off] += c.a0;
+        Aacc[c.s1 - off] += c.a1;
+        Aacc[c.s2 - off] += c.a2;
+        Aacc[dslot - off] += c.ad;
+        if constexpr (LUMP) {
+            bl += c.bl;
+        } else {
+            Bacc[c.s0 - off] += c.bij;
+            Bacc[c.s1 - off] += c.bij;
+            Bacc[c.s2 - off] += c.bij;
+            Bacc[dslot - off] += c.bii;
+        }
+    };
+    // two elements evaluated together (independent work in flight), then
+    // accumulated one after the other
+    int32_t p = pa;
+    for (; p + 1 < pb; p += 2) {
+        const Contrib c0 = eval(p);
+        const Contrib c1 = eval(p + 1);
+        apply(c0);
+        apply(c1);
     }
+    if (p < pb) apply(eval(p));
     if constexpr (LUMP) Bval[lrowptr[i]] = bl;
 }
 
