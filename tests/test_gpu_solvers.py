@@ -1,7 +1,7 @@
 """GPU eigensolver / PCG parity against the oracle (splu + eigsh restated) and
 the reference's own fixtures.
 
-Eigen bar (north star): eigenvalue relative error <= 1e-8 against the FP64
+Eigen bar: eigenvalue relative error <= 1e-8 against the FP64
 oracle (a zero eigenvalue is compared against 1e-8 * lambda_max), largest
 per-cluster principal angle <= 1e-6, B-orthonormal vectors."""
 
