@@ -266,7 +266,7 @@ def main():
                     help="only run one device-to-device copy of 3 x 22,369,056 bytes (for PMC scale factors)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "lapy_amd runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
     # the batched eigensolver runs one stream per worker; let HIP map up to 16

@@ -16,8 +16,8 @@ and, in ``metrics``, the comparison measures used to judge GPU results against
 it (eigenvalue relative error, per-cluster principal angles).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import this package, and only as the checker / the CPU
-baseline.  The product package ``lapy_amd`` never imports it.
+``bench.py`` may import this package, and only to check results or as the
+CPU baseline.  The product package ``lapy_amd`` never imports it.
 
 Pinning (how each function is checked independently of the GPU path):
 
