@@ -354,15 +354,18 @@ class DeviceMesh:
 
 def eigs(ctx: Context, A: DeviceCSR, B: DeviceCSR, k: int, sigma: float = -0.01, precision=LH_PREC_FP64,
          precond=LH_PRECOND_JACOBI, block=0, max_iter=0, tol=0.0, seed=0, verbose=0, keep_device=False,
-         poly_degree=0, poly_ratio=0.0, poly_storage=LH_STORE_AUTO, no_fallback=False):
-    """Generalized eigenpairs on the GPU; returns (evals, evecs, stats)."""
+         poly_degree=0, poly_ratio=0.0, poly_storage=LH_STORE_AUTO, no_fallback=False, allow_noconv=False):
+    """Generalized eigenpairs on the GPU; returns (evals, evecs, stats).
+    allow_noconv: return the Ritz pairs of the last iteration when max_iter
+    stops the solve (stats.converged < k) instead of raising NoConvergence."""
     opts = EigOpts(precision, precond, block, max_iter, tol, seed, verbose, poly_degree, poly_ratio, poly_storage,
                    int(bool(no_fallback)))
     st = Stats()
     h = _vp()
     code = lib().lh_eigs(ctx.handle, A.handle, B.handle, int(k), float(sigma), C.byref(opts), C.byref(h), C.byref(st))
     try:
-        check(code, ctx, "lh_eigs")
+        if not (allow_noconv and code == LH_ERR_NOCONV and h):
+            check(code, ctx, "lh_eigs")
         evals = np.empty(k, dtype=np.float64)
         if keep_device:
             check(lib().lh_eig_download(h, _ptr(evals), None), ctx)

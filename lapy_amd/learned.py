@@ -46,6 +46,24 @@ def training_residuals(n: int, W_offdiag_host, ncol: int, rng) -> np.ndarray:
     return R
 
 
+def solver_residuals(ctx, A, B, A_host, B_host, ncol: int, iterations, rng) -> list:
+    """Residual blocks the eigensolver itself produces: LOBPCG with the plain
+    Jacobi preconditioner (the one the network replaces), stopped after each
+    of the given iteration counts, and the residuals r = A x - theta B x of
+    its Ritz pairs (the preconditioner's inputs at that point); columns
+    normalised."""
+    out = []
+    for it in iterations:
+        seed = int(rng.integers(1 << 30))
+        ev, X, _ = _lib.eigs(ctx, A, B, ncol, poly_degree=1, max_iter=int(it), seed=seed, allow_noconv=True)
+        R = A_host @ X - (B_host @ X) * ev[None, :]
+        nr = np.linalg.norm(R, axis=0)
+        R = R[:, nr > 0] / nr[nr > 0]
+        if R.shape[1]:
+            out.append(np.ascontiguousarray(R))
+    return out
+
+
 class LearnedPreconditioner:
     """Weights of the learned preconditioner on one GPU context."""
 
@@ -72,30 +90,50 @@ class LearnedPreconditioner:
         with np.load(path, allow_pickle=False) as f:
             return cls(f["params"], ctx)
 
-    def train(self, meshes, steps: int = 200, ncol: int = 8, lr: float = 2e-3, seed: int = 0, log_every: int = 0):
+    def train(self, meshes, steps: int = 200, ncol: int = 8, lr: float = 2e-3, seed: int = 0, log_every: int = 0,
+              residuals: str = "noise"):
         """Adam on the mean loss over residual batches drawn from the given
-        meshes ((v, t) triangle meshes).  Returns the loss history."""
+        meshes ((v, t) triangle meshes).  residuals: "noise" (white and
+        graph-smoothed noise), "solver" (residual blocks of LOBPCG runs on
+        the meshes stopped after 1-12 iterations, sampled from a pool built
+        first) or "mixed" (each batch half of each).  Returns the loss
+        history."""
         import scipy.sparse as sp
 
+        if residuals not in ("noise", "solver", "mixed"):
+            raise ValueError(f"residuals must be 'noise', 'solver' or 'mixed', not {residuals!r}")
         rng = np.random.default_rng(seed)
         ops = []
         for v, t in meshes:
             dm = _lib.DeviceMesh(self.ctx, v, t)
             A, B = dm.assemble(_lib.LH_TRIA, False, np.float64)
-            K = (A.to_scipy_csc() + SHIFT * B.to_scipy_csc()).tocsr()
+            Ah, Bh = A.to_scipy_csc().tocsr(), B.to_scipy_csc().tocsr()
+            K = (Ah + SHIFT * Bh).tocsr()
             d = K.diagonal()
             s = 1.0 / np.sqrt(d)
             W = sp.diags(s) @ (K - sp.diags(d)) @ sp.diags(s)
             # averaging operator with row sums ~ 1: -W (cotan weights are negative off the diagonal)
-            ops.append((dm, A, B, sp.csr_matrix(-W)))
+            pool = []
+            if residuals != "noise":
+                pool = solver_residuals(self.ctx, A, B, Ah, Bh, ncol, (1, 2, 3, 5, 8, 12), rng)
+            ops.append((dm, A, B, sp.csr_matrix(-W), pool))
         theta = self.params.astype(np.float64)
         m1 = np.zeros_like(theta)
         m2 = np.zeros_like(theta)
         b1, b2, eps = 0.9, 0.999, 1e-8
         hist = []
         for it in range(1, steps + 1):
-            dm, A, B, Wm = ops[int(rng.integers(len(ops)))]
-            R = training_residuals(A.n, Wm, ncol, rng)
+            dm, A, B, Wm, pool = ops[int(rng.integers(len(ops)))]
+            if residuals == "noise" or not pool:
+                R = training_residuals(A.n, Wm, ncol, rng)
+            else:
+                blk = pool[int(rng.integers(len(pool)))]
+                cols = rng.choice(blk.shape[1], size=min(ncol, blk.shape[1]), replace=False)
+                R = blk[:, cols]
+                if residuals == "mixed":
+                    h = R.shape[1] // 2
+                    R = np.concatenate([R[:, :R.shape[1] - h], training_residuals(A.n, Wm, 2 * h, rng)[:, h:]], axis=1)
+                R = np.ascontiguousarray(R)
             loss, g = self.net.loss_grad(A, B, SHIFT, 1.0, R)
             hist.append(loss)
             m1 = b1 * m1 + (1 - b1) * g

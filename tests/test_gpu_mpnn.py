@@ -119,6 +119,59 @@ def test_ablation_eigensolver_and_heat(trained):
         net.use(False)
 
 
+def test_solver_residuals_are_ritz_residuals():
+    """Residual blocks of stopped LOBPCG runs: the solve stops at max_iter
+    without raising (allow_noconv), and r = A x - theta B x of Ritz pairs is
+    orthogonal to every Ritz vector (X^T A X = Theta, X^T B X = I)."""
+    ctx = _lib.default_context()
+    v, t = mesh(3, 5)
+    dm, A, B, K = device_ops(ctx, v, t)
+    Ah, Bh = A.to_scipy_csc().tocsr(), B.to_scipy_csc().tocsr()
+    with pytest.raises(_lib.NoConvergence):
+        _lib.eigs(ctx, A, B, 8, poly_degree=1, max_iter=2)
+    ev, X, st = _lib.eigs(ctx, A, B, 8, poly_degree=1, max_iter=2, allow_noconv=True)
+    assert st.converged < 8 and 1 <= st.iterations <= 2
+    np.testing.assert_allclose(X.T @ (Bh @ X), np.eye(8), atol=1e-10)
+    pool = learned.solver_residuals(ctx, A, B, Ah, Bh, 8, (1, 3), np.random.default_rng(0))
+    assert len(pool) == 2
+    for R in pool:
+        assert R.shape == (A.n, 8)
+        np.testing.assert_allclose(np.linalg.norm(R, axis=0), 1.0, rtol=1e-12)
+    ev1, X1, _ = _lib.eigs(ctx, A, B, 8, poly_degree=1, max_iter=3, seed=1, allow_noconv=True)
+    R1 = Ah @ X1 - (Bh @ X1) * ev1[None, :]
+    scale = np.outer(np.linalg.norm(X1, axis=0), np.linalg.norm(R1, axis=0))
+    assert np.all(np.abs(X1.T @ R1) <= 1e-8 * scale), np.abs(X1.T @ R1 / scale).max()
+
+
+def test_training_on_solver_residuals_beats_jacobi(trained):
+    """The network trained on the solver's own residual blocks (stopped
+    Jacobi-LOBPCG runs on the training meshes) also needs fewer LOBPCG
+    iterations than plain Jacobi on the held-out mesh."""
+    meshes = [mesh(lev, 100 + i) for i, lev in enumerate((3, 4, 3, 4, 3, 4))]
+    net = learned.LearnedPreconditioner()
+    hist = net.train(meshes, steps=300, ncol=8, lr=3e-3, seed=1, residuals="solver")
+    assert np.mean(hist[-30:]) < 0.8 * np.mean(hist[:10]), (hist[:10], hist[-30:])
+    noise_net, _ = trained
+    v, t = mesh(4, 999)
+    ctx = _lib.default_context()
+    dm, A, B, K = device_ops(ctx, v, t)
+    ev_j, _, st_j = _lib.eigs(ctx, A, B, 10, precond=_lib.LH_PRECOND_JACOBI, poly_degree=1)
+    its = {}
+    for name, n in (("solver", net), ("noise", noise_net)):
+        n.use()
+        try:
+            ev_l, _, st_l = _lib.eigs(ctx, A, B, 10, precond=_lib.LH_PRECOND_LEARNED, no_fallback=True)
+        finally:
+            n.use(False)
+        np.testing.assert_allclose(ev_l, ev_j, rtol=1e-8, atol=1e-10)
+        its[name] = st_l.iterations
+    print("LOBPCG iterations: jacobi", st_j.iterations, "learned (solver residuals)", its["solver"],
+          "learned (noise residuals)", its["noise"], "final losses", np.mean(hist[-30:]))
+    assert its["solver"] < st_j.iterations
+    with pytest.raises(ValueError):
+        net.train(meshes[:1], steps=1, residuals="recorded")
+
+
 def test_fallback_to_jacobi_with_a_bad_network():
     ctx = _lib.default_context()
     v, t = mesh(3, 42)
