@@ -394,7 +394,8 @@ def main():
         "config": {"workload": f"icosphere L{args.level} ellipsoid, FP64 assembly + k={args.k} eigenpairs",
                    "vertices": nv, "triangles": nf, "k": args.k, "precision": args.precision,
                    "lobpcg_iterations": st.iterations if st else None,
-                   "lobpcg_iterations_fp32": st.iterations_fp32 if st else None, "eig_rel_err_vs_oracle": eig_err,
+                   "lobpcg_iterations_fp32": st.iterations_fp32 if st else None,
+                   "chebyshev_block_columns": st.poly_columns if st else None, "eig_rel_err_vs_oracle": eig_err,
                    "max_rel_resid": st.max_resid if st else None},
         "roofline": roofline,
         "roofline_assembly": roofline_asm,

@@ -61,6 +61,9 @@ typedef struct lh_stats {
     int32_t restarts;
     int32_t iterations_fp32; /* LH_PREC_MIXED: iterations of the FP32 phase
                                 (included in `iterations`)                    */
+    int32_t poly_columns;    /* eigensolver: block columns the Chebyshev
+                                polynomial ran on, summed over iterations
+                                (converged wanted pairs get plain Jacobi)     */
 } lh_stats;
 
 #define LH_PREC_FP64   0   /* FP64 everywhere                                */

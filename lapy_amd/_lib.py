@@ -45,6 +45,7 @@ class Stats(C.Structure):
         ("rr_calls", C.c_int32),
         ("restarts", C.c_int32),
         ("iterations_fp32", C.c_int32),
+        ("poly_columns", C.c_int32),
     ]
 
     def as_dict(self):

@@ -709,6 +709,25 @@ void rowscale(lh_ctx* ctx, int64_t n, int m, T* X, int ld, const double* dinv) {
 template void rowscale<double>(lh_ctx*, int64_t, int, double*, int, const double*);
 template void rowscale<float>(lh_ctx*, int64_t, int, float*, int, const double*);
 
+// dst[:, c] = src[:, perm[c]] for c < m (perm on the device)
+template <typename T>
+__global__ void k_gather_cols(int64_t n, int m, const T* __restrict__ src, int lds, T* __restrict__ dst, int ldd,
+                              const int32_t* __restrict__ perm) {
+    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = gid / m;
+    int c = (int)(gid % m);
+    if (i >= n) return;
+    dst[i * ldd + c] = src[i * lds + perm[c]];
+}
+template <typename T>
+void gather_cols(lh_ctx* ctx, int64_t n, int m, const T* src, int lds, T* dst, int ldd, const int32_t* perm) {
+    if (n * m == 0) return;
+    k_gather_cols<T><<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, src, lds, dst, ldd, perm);
+    LH_CHECK_LAUNCH();
+}
+template void gather_cols<double>(lh_ctx*, int64_t, int, const double*, int, double*, int, const int32_t*);
+template void gather_cols<float>(lh_ctx*, int64_t, int, const float*, int, float*, int, const int32_t*);
+
 // copy a column block
 template <typename S, typename D>
 __global__ void k_copy_block(int64_t n, int m, const S* __restrict__ src, int lds, D* __restrict__ dst, int ldd) {

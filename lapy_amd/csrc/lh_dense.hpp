@@ -46,6 +46,9 @@ template <typename T>
 void colscale(lh_ctx* ctx, int64_t n, int m, T* X, int ld, const double* s);
 template <typename T>
 void rowscale(lh_ctx* ctx, int64_t n, int m, T* X, int ld, const double* dinv);
+// dst[:, c] = src[:, perm[c]], c < m; perm: m int32 on the device
+template <typename T>
+void gather_cols(lh_ctx* ctx, int64_t n, int m, const T* src, int lds, T* dst, int ldd, const int32_t* perm);
 template <typename S, typename D>
 void copy_block(lh_ctx* ctx, int64_t n, int m, const S* src, int lds, D* dst, int ldd);
 template <typename T>

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 
 using namespace lh;
 
@@ -36,6 +37,10 @@ namespace {
 double now_us() {
     return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
+
+// the polynomial skips the wanted pairs whose relative residual is at most
+// this many times the tolerance (they keep a plain Jacobi direction)
+constexpr double kPolyLockFactor = 1.0;
 
 // T: the block type.  double for the FP64 solve; float for the first phase
 // of the mixed-precision solve (FP32 blocks, FP32 SpMM, FP32 matrix-core
@@ -75,6 +80,7 @@ struct Lobpcg {
     uint64_t seed;
     DevBuf S, S2, AS, MS, dinv;
     DevBuf GA, GM, Cm, Gs, Ts, theta_d, rrwork, status, cs1, cs2, wsg, wsc;
+    DevBuf lockbuf;  // soft locking of the polynomial: column norms (m doubles), then the column order (m int32)
     std::vector<double> theta, res;
     lh_stats st{};
 
@@ -203,6 +209,7 @@ struct Lobpcg {
         status.alloc(64);
         cs1.alloc(N3 * 8);
         cs2.alloc(N3 * 8);
+        lockbuf.alloc(m * 12);
         theta.assign(m, 0.0);
         res.assign(m, 0.0);
         HIPX(hipMemsetAsync(S.p, 0, n * ld * sizeof(T), ctx->stream));
@@ -230,6 +237,9 @@ struct Lobpcg {
         int it = 0;
         double t_iter0 = now_us();
         std::vector<double> r2(m), mx2(m);
+        std::vector<char> conv(m, 0);
+        std::vector<int32_t> perm;
+        std::vector<char> lock_host((size_t)m * 12);
         std::vector<double> gm_hist;  // first phase: residual history
         constexpr int kMinFp32Iters = 6;
         for (it = 0; it < max_iter; ++it) {
@@ -245,6 +255,7 @@ struct Lobpcg {
                 res[j] = std::sqrt(r2[j] / std::max(mx2[j], 1e-300));
                 const double rel = res[j] / std::max(std::max(std::fabs(theta[j]), floor), 1e-300);
                 worst = std::max(worst, rel);
+                conv[j] = rel <= kPolyLockFactor * tol;
                 if (rel <= tol) ++nconv;
             }
             st.max_resid = worst;
@@ -290,7 +301,35 @@ struct Lobpcg {
             if (learned_on) {
                 if constexpr (sizeof(T) == 8) learned.apply(Sp() + m, ld, m);
             } else if (poly_degree > 1) {
-                cheb->apply(Sp() + m, ld, m, cs1.as<double>());  // cs1: ||w_c||^2 from the test above
+                // soft locking of the polynomial: the wanted pairs that pass
+                // the convergence test keep a plain Jacobi direction in W;
+                // the others and the guard columns k..m-1, gathered in front
+                // of W (ma of them, rounded up to 8), get the polynomial,
+                // when that saves 8 columns or more.  The order of W's
+                // columns does not matter to the Rayleigh-Ritz step.
+                perm.clear();
+                for (int j = 0; j < m; ++j)
+                    if (j >= k || !conv[j]) perm.push_back(j);
+                const int ma = std::min(m, ((int)perm.size() + 7) / 8 * 8);
+                if (m - ma >= 8) {
+                    for (int j = 0; j < k; ++j)
+                        if (conv[j]) perm.push_back(j);
+                    double* nrm = reinterpret_cast<double*>(lock_host.data());
+                    for (int c = 0; c < m; ++c) nrm[c] = r2[perm[c]];  // r2: ||w_c||^2
+                    std::memcpy(lock_host.data() + (size_t)m * 8, perm.data(), (size_t)m * 4);
+                    h2d(ctx, lockbuf.p, lock_host.data(), (size_t)m * 12);
+                    T* W = Sp() + m;
+                    T* tmp = S2.as<T>() + m;  // S2 is free until the update below
+                    gather_cols<T>(ctx, n, m, W, ld, tmp, ld,
+                                   reinterpret_cast<const int32_t*>(lockbuf.as<char>() + (size_t)m * 8));
+                    copy_block<T, T>(ctx, n, m, tmp, ld, W, ld);
+                    cheb->apply(W, ld, ma, lockbuf.as<double>());
+                    rowscale<T>(ctx, n, m - ma, W + ma, ld, dinv.as<double>());
+                    st.poly_columns += ma;
+                } else {
+                    cheb->apply(Sp() + m, ld, m, cs1.as<double>());  // cs1: ||w_c||^2 from the test above
+                    st.poly_columns += m;
+                }
                 st.spmm_calls += poly_degree - 1;
             } else if (precond != LH_PRECOND_NONE) {
                 rowscale<T>(ctx, n, m, Sp() + m, ld, dinv.as<double>());
@@ -515,6 +554,7 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
         L.st.iterations += L1.st.iterations;
         L.st.iterations_fp32 = L1.st.iterations;
         L.st.spmm_calls += L1.st.spmm_calls;
+        L.st.poly_columns += L1.st.poly_columns;
         L.st.rr_calls += L1.st.rr_calls;
         L.st.restarts += L1.st.restarts;
         L.st.t_rr_us += L1.st.t_rr_us;
