@@ -12,7 +12,7 @@ library replaces:
 * ``fem.diffusion``       -- lapy/heat.py:114-232     (``heat.diffusion``)
 * ``fem.avg_edge_length`` -- lapy/tria_mesh.py:735-748, lapy/tet_mesh.py:182-195
 
-and, in ``metrics``, the comparison measures used to judge GPU results against
+and, in ``metrics``, the comparison measures used to compare GPU results with
 it (eigenvalue relative error, per-cluster principal angles).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of

@@ -1,5 +1,5 @@
 """Pin the CPU oracle (oracle/fem.py) against what the mathematics and the
-reference's own fixtures fix, before it is used to judge the GPU path."""
+reference's own fixtures fix, before it is used to check the GPU path."""
 
 import numpy as np
 import pytest
