@@ -427,7 +427,6 @@ void rr_solve(lh_ctx* ctx, int N, int nev, double* GA, double* GM, double* C, do
 // T with T^T G T = I for a small SPD Gram G (m x m, m <= 128): G is
 // equilibrated to unit diagonal (D G D), optionally shifted, factorised
 // G_s = L L^T in LDS, and T = D L^-T.  status 1 on a non-positive pivot.
-constexpr int kCholMax = 128;
 __global__ void __launch_bounds__(RB) k_cholinv(int m, const double* __restrict__ G, double* __restrict__ T,
                                                 double shift, int* __restrict__ status) {
     __shared__ double L[kCholMax * kCholMax];

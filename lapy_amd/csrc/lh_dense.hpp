@@ -33,6 +33,8 @@ void rr_std(lh_ctx* ctx, int N, int nev, const double* G, int ldg, double* C, do
             int* status, unsigned long long* prof = nullptr);
 void rr_solve(lh_ctx* ctx, int N, int nev, double* GA, double* GM, double* C, double* theta, double* work,
               int* status);
+// widest Gram cholinv factorises (one workgroup, the factor in LDS)
+constexpr int kCholMax = 128;
 void cholinv(lh_ctx* ctx, int m, double* G, double* T, double shift, int* status);
 template <typename T>
 void residual(lh_ctx* ctx, int64_t n, int m, const T* AX, const T* MX, int ldx, const double* theta, T* R, int ldr);

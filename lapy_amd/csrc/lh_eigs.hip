@@ -113,6 +113,19 @@ struct Lobpcg {
     // pivot (Cholesky-QR2).  Returns false when the block is numerically rank
     // deficient.  A X and B X of the block are recomputed afterwards.
     bool cholqr(int off, int c, bool recompute_A) {
+        if (c > kCholMax) {
+            // wider than one factorisation: the first half (W of [W P]),
+            // then the second half projected against it (twice) and
+            // factorised on its own.  A rank-deficient second half fails
+            // as the whole block would.
+            if (c > 2 * kCholMax) LH_THROW(LH_ERR_ARG, "Cholesky-QR of %d columns (at most %d)", c, 2 * kCholMax);
+            const int c1 = (c + 1) / 2, c2 = c - c1;
+            if (!cholqr(off, c1, recompute_A)) return false;
+            project_out(off, c1, off + c1, c2);
+            project_out(off, c1, off + c1, c2);
+            spmm_B(Sp() + off + c1, MSp() + off + c1, c2);
+            return cholqr(off + c1, c2, recompute_A);
+        }
         T* X = Sp() + off;
         T* MX = MSp() + off;
         for (int pass = 0; pass < 2; ++pass) {

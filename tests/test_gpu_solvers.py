@@ -90,6 +90,21 @@ def test_ellipsoid_k50():
     assert_eigs_match(lam, vec, lam_o, vec_o, b, k_complete=False)
 
 
+@pytest.mark.parametrize("k", [60, 120])
+def test_large_k_wide_blocks(k):
+    """k = 60 and 120 give 72- and 128-column blocks: [W P] is wider than one
+    Cholesky factorisation (128 columns) and is orthonormalised in two halves,
+    and the trial basis is wider than the LDS Rayleigh-Ritz solver (192);
+    the eigenpairs match the oracle."""
+    v, t = meshgen.large_ellipsoid(level=5)
+    fem_ = Solver(TriaMesh(v, t))
+    lam, vec = fem_.eigs(k=k)
+    a, b = fem.fem_tria(v, t)
+    lam_o, vec_o = fem.eigs(a, b, k=k)
+    assert_eigs_match(lam, vec, lam_o, vec_o, b, k_complete=False)
+    print("k", k, "iterations", fem_.last_stats.iterations, "restarts", fem_.last_stats.restarts)
+
+
 @pytest.mark.parametrize("storage,degree", [("f32", 0), ("f16", 0), ("f16", 40), ("f32", 1)])
 def test_ellipsoid_l6_preconditioner_storage(storage, degree):
     """Icosphere level 6 ellipsoid (40,962 vertices), k=50: the eigenpairs do
