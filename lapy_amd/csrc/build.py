@@ -41,7 +41,10 @@ SOURCES = {
     "lh_eigs.hip": [],
     "lh_pcg.hip": [],
     "lh_mpnn.hip": [],
+    "lh_rrlib.hip": [],
 }
+# rocSOLVER (and its rocBLAS) only for the Rayleigh-Ritz of wide trial bases
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def _compile(name: str, extra: list[str]) -> str:
@@ -60,7 +63,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(lambda kv: _compile(*kv), SOURCES.items()))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, *LIBS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -211,6 +211,7 @@ void lh_ctx_destroy(lh_ctx* ctx) {
     ctx->scratch.release();
     ctx->small.release();
     if (ctx->host_small) (void)hipHostFree(ctx->host_small);
+    release_blas(ctx);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -33,6 +33,11 @@ void rr_std(lh_ctx* ctx, int N, int nev, const double* G, int ldg, double* C, do
             int* status, unsigned long long* prof = nullptr);
 void rr_solve(lh_ctx* ctx, int N, int nev, double* GA, double* GM, double* C, double* theta, double* work,
               int* status);
+// the same generalized problem through rocSOLVER (sygvd) for trial bases
+// wider than the LDS solvers take; work: rr_wide_work_doubles(N) doubles
+size_t rr_wide_work_doubles(int N);
+void rr_solve_wide(lh_ctx* ctx, int N, int nev, const double* GA, const double* GM, double* C, double* theta,
+                   double* work, int* status);
 // widest Gram cholinv factorises (one workgroup, the factor in LDS)
 constexpr int kCholMax = 128;
 void cholinv(lh_ctx* ctx, int m, double* G, double* T, double shift, int* status);

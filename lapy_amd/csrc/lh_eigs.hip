@@ -116,9 +116,9 @@ struct Lobpcg {
         if (c > kCholMax) {
             // wider than one factorisation: the first half (W of [W P]),
             // then the second half projected against it (twice) and
-            // factorised on its own.  A rank-deficient second half fails
-            // as the whole block would.
-            if (c > 2 * kCholMax) LH_THROW(LH_ERR_ARG, "Cholesky-QR of %d columns (at most %d)", c, 2 * kCholMax);
+            // factorised on its own, each half split again while wider
+            // than kCholMax.  A rank-deficient second half fails as the
+            // whole block would.
             const int c1 = (c + 1) / 2, c2 = c - c1;
             if (!cholqr(off, c1, recompute_A)) return false;
             project_out(off, c1, off + c1, c2);
@@ -162,7 +162,8 @@ struct Lobpcg {
     // The blocks of S are B-orthonormalised (X by the previous Rayleigh-Ritz,
     // W and P by projection + Cholesky-QR), so S^T B S = I to rounding and
     // the small problem is a standard one; the generalized path (Cholesky of
-    // S^T B S) is kept for trial bases wider than the LDS solver takes.
+    // S^T B S, through rocSOLVER) is kept for trial bases wider than the LDS
+    // solver takes.
     bool rayleigh_ritz(int N) {
         double t0 = now_us();
         gram<T>(ctx, n, Sp(), ld, N, ASp(), ld, N, GA.as<double>(), N, wsg, 1.0, true);
@@ -171,8 +172,8 @@ struct Lobpcg {
                    status.as<int>());
         } else {
             gram<T>(ctx, n, Sp(), ld, N, MSp(), ld, N, GM.as<double>(), N, wsg, 1.0, true);
-            rr_solve(ctx, N, m, GA.as<double>(), GM.as<double>(), Cm.as<double>(), theta_d.as<double>(),
-                     rrwork.as<double>(), status.as<int>());
+            rr_solve_wide(ctx, N, m, GA.as<double>(), GM.as<double>(), Cm.as<double>(), theta_d.as<double>(),
+                          rrwork.as<double>(), status.as<int>());
         }
         int s = read_status();
         st.rr_calls++;
@@ -218,7 +219,7 @@ struct Lobpcg {
         Gs.alloc(4 * m * m * 8);
         Ts.alloc(4 * m * m * 8);
         theta_d.alloc(N3 * 8);
-        rrwork.alloc(std::max(rr_work_doubles(N3, m), rr_std_work_doubles(N3, m)) * 8);
+        rrwork.alloc(std::max({rr_work_doubles(N3, m), rr_std_work_doubles(N3, m), rr_wide_work_doubles(N3)}) * 8);
         status.alloc(64);
         cs1.alloc(N3 * 8);
         cs2.alloc(N3 * 8);
@@ -475,11 +476,11 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
     // block = k wanted + guard vectors, rounded to a multiple of 8
     int m = opts && opts->block > 0 ? opts->block : ((std::max(k + 4, (k * 6 + 4) / 5) + 7) / 8) * 8;
     m = std::max(m, k);
-    if (m > 128) m = std::max(k, 128);
+    if (m > 512) m = std::max(k, 512);
     if (3 * m > n) m = std::max(k, (int)(n / 3));
     L.m = m;
     L.poly_half = (opts && opts->poly_storage == LH_STORE_F16) ||
-                  ((!opts || opts->poly_storage == LH_STORE_AUTO) && m % 8 == 0);
+                  ((!opts || opts->poly_storage == LH_STORE_AUTO) && m % 8 == 0 && m <= 512);
     if (L.poly_half && m % 8 != 0) LH_THROW(LH_ERR_ARG, "FP16 Chebyshev storage needs a block size divisible by 8");
     L.tol = (opts && opts->tol > 0) ? opts->tol : 1e-9;
     L.max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : 20000;

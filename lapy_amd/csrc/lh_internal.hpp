@@ -137,7 +137,11 @@ struct lh_ctx {
     lh_mpnn* mpnn = nullptr;  // network used by LH_PRECOND_LEARNED (not owned)
     void* host_small = nullptr;   // pinned host staging (hipHostMalloc)
     size_t host_small_bytes = 0;
+    void* blas = nullptr;  // rocBLAS handle of the wide Rayleigh-Ritz solver, made on first use
 };
+
+// frees ctx->blas (lh_rrlib.hip)
+void release_blas(lh_ctx* ctx);
 
 // CSR pattern (rows sorted, int32), shared by A and consistent B.
 struct Pattern {

@@ -374,6 +374,7 @@ void ChebPrecond::setup(lh_ctx* c, const lh_csr* A, const lh_csr* B, double sigm
     ctx = c;
     mmax = mmax_;
     half = half_storage;
+    if (half && mmax > 512) LH_THROW(LH_ERR_ARG, "FP16 Chebyshev storage takes at most 512 block columns (%d)", mmax);
     pat = A->pat;
     n = n_;
     degree = deg;
@@ -478,7 +479,9 @@ void ChebPrecond::step(int m, const void* bvec, const void* xk, void* xp, double
                 case 2: LH_CHEB_E(2); break;
                 case 4: LH_CHEB_E(4); break;
                 case 8: LH_CHEB_E(8); break;
-                default: LH_CHEB_E(16); break;
+                case 16: LH_CHEB_E(16); break;
+                case 32: LH_CHEB_E(32); break;
+                default: LH_CHEB_E(64); break;  // vl <= 64: checked at setup
             }
 #undef LH_CHEB_E
             LH_CHECK_LAUNCH();
@@ -489,7 +492,9 @@ void ChebPrecond::step(int m, const void* bvec, const void* xk, void* xp, double
             case 2: k_cheb_step_h<2><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
             case 4: k_cheb_step_h<4><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
             case 8: k_cheb_step_h<8><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
-            default: k_cheb_step_h<16><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+            case 16: k_cheb_step_h<16><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+            case 32: k_cheb_step_h<32><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
+            default: k_cheb_step_h<64><<<nb, 256, 0, ctx->stream>>>(n, m, rp, cl, kk, dd, b, x, y, fa, fo); break;
         }
         LH_CHECK_LAUNCH();
         return;
@@ -527,7 +532,7 @@ void ChebPrecond::init(const WT* W, int ldw, int m, const double* colnorm2, void
         sc.ensure(m * 16);
         float* scale = sc.as<float>();
         double* inv = reinterpret_cast<double*>(sc.as<char>() + 8 * m);
-        k_col_scales<<<1, 256, 0, ctx->stream>>>(m, colnorm2, std::sqrt((double)n) / 16.0, scale, inv);
+        k_col_scales<<<nblocks(m, 256), 256, 0, ctx->stream>>>(m, colnorm2, std::sqrt((double)n) / 16.0, scale, inv);
         k_cheb_init_h<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, W, ldw, dinv32.as<float>(), scale, fa,
                                                                     R.as<_Float16>(), D0.as<_Float16>(),
                                                                     D1.as<_Float16>());

@@ -90,13 +90,14 @@ def test_ellipsoid_k50():
     assert_eigs_match(lam, vec, lam_o, vec_o, b, k_complete=False)
 
 
-@pytest.mark.parametrize("k", [60, 120])
-def test_large_k_wide_blocks(k):
-    """k = 60 and 120 give 72- and 128-column blocks: [W P] is wider than one
-    Cholesky factorisation (128 columns) and is orthonormalised in two halves,
-    and the trial basis is wider than the LDS Rayleigh-Ritz solver (192);
-    the eigenpairs match the oracle."""
-    v, t = meshgen.large_ellipsoid(level=5)
+@pytest.mark.parametrize("level,k", [(5, 60), (5, 120), (4, 200)])
+def test_large_k_wide_blocks(level, k):
+    """k = 60, 120 and 200 give 72-, 144- and 240-column blocks: [W P] is
+    wider than one Cholesky factorisation (128 columns) and is orthonormalised
+    in halves (quarters from 2m > 256 on), and the trial basis is wider than
+    the LDS Rayleigh-Ritz solver (192 columns) and goes to rocSOLVER; the
+    eigenpairs match the oracle."""
+    v, t = meshgen.large_ellipsoid(level=level)
     fem_ = Solver(TriaMesh(v, t))
     lam, vec = fem_.eigs(k=k)
     a, b = fem.fem_tria(v, t)
