@@ -458,6 +458,15 @@ void dense_eigs(lh_ctx* ctx, const lh_csr* A, const lh_csr* B, int k, double* ev
 // (or earlier, when its residuals stop falling)
 constexpr double kMixedSwitchTol = 1e-5;
 
+// block = k wanted + guard vectors, rounded to a multiple of 8, at most 512
+// columns unless k itself is larger (before the 3m <= n limit of the mesh)
+int block_width(int k, const lh_eig_opts* opts) {
+    int m = opts && opts->block > 0 ? opts->block : ((std::max(k + 4, (k * 6 + 4) / 5) + 7) / 8) * 8;
+    m = std::max(m, k);
+    if (m > 512) m = std::max(k, 512);
+    return m;
+}
+
 // one generalized eigenproblem on ctx's stream; throws lh_error on bad
 // arguments or failures, returns the result with code LH_OK or
 // LH_ERR_NOCONV (results of the last iteration)
@@ -473,10 +482,7 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
     L.n = n;
     L.k = k;
     L.sigma = sigma;
-    // block = k wanted + guard vectors, rounded to a multiple of 8
-    int m = opts && opts->block > 0 ? opts->block : ((std::max(k + 4, (k * 6 + 4) / 5) + 7) / 8) * 8;
-    m = std::max(m, k);
-    if (m > 512) m = std::max(k, 512);
+    int m = block_width(k, opts);
     if (3 * m > n) m = std::max(k, (int)(n / 3));
     L.m = m;
     L.poly_half = (opts && opts->poly_storage == LH_STORE_F16) ||
@@ -619,7 +625,7 @@ void solve_batch(lh_ctx* ctx, int nmesh, lh_csr* const* A, lh_csr* const* B, con
         nmax = std::max<int64_t>(nmax, A[i]->n());
         kmax = std::max(kmax, k[i]);
     }
-    const int mmax = std::min(128, ((std::max(kmax + 4, (kmax * 6 + 4) / 5) + 7) / 8) * 8);
+    const int mmax = block_width(kmax, opts);
     nw = std::max(1, std::min(nw, nmesh));
     ensure_workers(ctx, nw);
     const size_t abytes = arena_bytes(nmax, mmax);
