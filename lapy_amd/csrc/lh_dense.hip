@@ -429,10 +429,13 @@ void rr_solve(lh_ctx* ctx, int N, int nev, double* GA, double* GM, double* C, do
 // G_s = L L^T in LDS, and T = D L^-T.  status 1 on a non-positive pivot.
 __global__ void __launch_bounds__(RB) k_cholinv(int m, const double* __restrict__ G, double* __restrict__ T,
                                                 double shift, int* __restrict__ status) {
-    __shared__ double L[kCholMax * kCholMax];
+    // row stride m + 1 (odd for even m): a column's entries fall on
+    // different LDS banks
+    __shared__ double L[kCholMax * (kCholMax + 1)];
     __shared__ double dsc[kCholMax];
     __shared__ int flag[1];
     const int tid = threadIdx.x;
+    const int lw = (m % 2 == 0) ? m + 1 : m;
     if (tid == 0) flag[0] = 0;
     for (int i = tid; i < m; i += RB) {
         const double d = G[i * m + i];
@@ -448,52 +451,57 @@ __global__ void __launch_bounds__(RB) k_cholinv(int m, const double* __restrict_
         const int i = idx / m, j = idx - (idx / m) * m;
         double v = 0.5 * (G[i * m + j] + G[j * m + i]) * dsc[i] * dsc[j];
         if (i == j) v = 1.0 + shift;
-        L[idx] = v;
+        L[i * lw + j] = v;
     }
     __syncthreads();
     // right-looking Cholesky; lanes own rows of the trailing matrix.  The
     // smallest pivot (relative to the unit diagonal) is reported in
     // ((double*)status)[1]; below 1e-12 the block counts as rank deficient.
+    // Column j is read unscaled during its step and divided by its pivot
+    // root after the loop (step j writes only columns > j): one barrier per
+    // step.
+    __shared__ double ldg[kCholMax];  // the pivot roots, L's diagonal
     double minpiv = 1.0;
     for (int j = 0; j < m; ++j) {
-        const double djj = L[j * m + j];
+        const double djj = L[j * lw + j];
         minpiv = fmin(minpiv, djj);
         if (!(djj > 1e-12)) {
             if (tid == 0) { status[0] = 1; reinterpret_cast<double*>(status)[1] = djj; }
             return;
         }
         const double ljj = sqrt(djj);
+        if (tid == 0) ldg[j] = ljj;
         for (int i = j + 1 + (tid >> 4); i < m; i += RB / 16) {
-            const double lij = L[i * m + j] / ljj;
-            for (int kk = j + 1 + (tid & 15); kk <= i; kk += 16) L[i * m + kk] -= lij * (L[kk * m + j] / ljj);
+            const double lij = L[i * lw + j] / ljj;
+            for (int kk = j + 1 + (tid & 15); kk <= i; kk += 16) L[i * lw + kk] -= lij * (L[kk * lw + j] / ljj);
         }
         __syncthreads();
-        for (int i = j + 1 + tid; i < m; i += RB) L[i * m + j] /= ljj;
-        if (tid == 0) L[j * m + j] = ljj;
-        __syncthreads();
     }
+    for (int idx = tid; idx < m * m; idx += RB) {
+        const int i = idx / m, j = idx - (idx / m) * m;
+        if (i > j) L[i * lw + j] /= ldg[j];
+        else if (i == j) L[i * lw + i] = ldg[i];
+    }
+    __syncthreads();
     // Tu = L^-T (upper triangular) by back substitution over rows i, all
     // columns c >= i in parallel, the dot split over 8 lanes per column.  Tu
     // is kept in the (unused) upper triangle of L, L's diagonal in ldg.
-    __shared__ double ldg[kCholMax];
-    for (int i = tid; i < m; i += RB) ldg[i] = L[i * m + i];
-    __syncthreads();
     for (int i = m - 1; i >= 0; --i) {
         const int c = i + (tid >> 3), l8 = tid & 7;
         double s = 0.0;
         if (c < m)
-            for (int r = i + 1 + l8; r <= c; r += 8) s += L[r * m + i] * L[r * m + c];
+            for (int r = i + 1 + l8; r <= c; r += 8) s += L[r * lw + i] * L[r * lw + c];
         s += __shfl_xor(s, 1, 64);
         s += __shfl_xor(s, 2, 64);
         s += __shfl_xor(s, 4, 64);
         __syncthreads();  // all reads of row i's inputs done before row i is written
-        if (c < m && l8 == 0) L[i * m + c] = ((c == i ? 1.0 : 0.0) - s) / ldg[i];
+        if (c < m && l8 == 0) L[i * lw + c] = ((c == i ? 1.0 : 0.0) - s) / ldg[i];
         __syncthreads();
     }
     // T = D Tu
     for (int idx = tid; idx < m * m; idx += RB) {
         const int i = idx / m, c = idx - (idx / m) * m;
-        T[idx] = (c >= i) ? dsc[i] * L[idx] : 0.0;
+        T[idx] = (c >= i) ? dsc[i] * L[i * lw + c] : 0.0;
     }
     if (tid == 0) { status[0] = 0; reinterpret_cast<double*>(status)[1] = minpiv; }
 }
