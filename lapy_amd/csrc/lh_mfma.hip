@@ -157,35 +157,50 @@ k_gram_mfma(int64_t n, int64_t rows_per_wave, int nwaves, const T* __restrict__ 
             }
 }
 
-// C[i][j] = alpha * sum over waves (fixed order: 4 interleaved lane groups,
-// combined pairwise); sym: the lower triangle mirrors the upper tiles
-__global__ void __launch_bounds__(256)
+// C[i][j] = alpha * sum over waves (fixed order: 8 interleaved lane groups,
+// combined pairwise).  One thread per element of a stored tile, so that
+// consecutive lanes read consecutive partials; sym: the stored tiles are the
+// upper ones (ti <= tj) and an off-diagonal tile also writes its mirror
+// C[j][i].
+__global__ void __launch_bounds__(512)
 k_gram_mfma_reduce(int nwaves, int ntile, int p, int q, int tv, int sym, const double* __restrict__ part,
                    double* __restrict__ C, int ldc, double alpha) {
-    __shared__ double sh[4][64];
-    const int o = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+    __shared__ double sh[8][64];
+    const int64_t o = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    const int g = threadIdx.x >> 6;
+    int i = p, j = q;
+    int ti = 0, tj = 0;
     double s = 0.0;
-    int i = 0, j = 0;
-    if (o < p * q) {
-        i = o / q;
-        j = o - i * q;
-        int ti = i >> 6, tj = j >> 6, li = i & 63, lj = j & 63;
-        if (sym && ti > tj) {
-            const int t = ti; ti = tj; tj = t;
-            const int l = li; li = lj; lj = l;
+    if (o < (int64_t)ntile * 4096) {
+        const int tile = (int)(o >> 12), e = (int)(o & 4095);
+        if (sym) {
+            // tile = ti * tv - ti (ti - 1) / 2 + (tj - ti), ti <= tj
+            int first = 0;
+            while (first + (tv - ti) <= tile) {
+                first += tv - ti;
+                ++ti;
+            }
+            tj = ti + (tile - first);
+        } else {
+            ti = tile / tv;
+            tj = tile - ti * tv;
         }
-        int tile;
-        if (sym) tile = ti * tv - (ti * (ti - 1)) / 2 + (tj - ti);
-        else tile = ti * tv + tj;
-        const double* src = part + (int64_t)tile * 4096 + li * 64 + lj;
-        const int64_t stride = (int64_t)ntile * 4096;
-        for (int c = g; c < nwaves; c += 4) s += src[c * stride];
+        i = ti * 64 + (e >> 6);
+        j = tj * 64 + (e & 63);
+        if (i < p && j < q) {
+            const double* src = part + o;
+            const int64_t stride = (int64_t)ntile * 4096;
+            for (int c = g; c < nwaves; c += 8) s += src[c * stride];
+        }
     }
     sh[g][threadIdx.x & 63] = s;
     __syncthreads();
-    if (g == 0 && o < p * q) {
+    if (g == 0 && i < p && j < q) {
         const int l = threadIdx.x;
-        C[(int64_t)i * ldc + j] = alpha * ((sh[0][l] + sh[1][l]) + (sh[2][l] + sh[3][l]));
+        const double v =
+            alpha * (((sh[0][l] + sh[1][l]) + (sh[2][l] + sh[3][l])) + ((sh[4][l] + sh[5][l]) + (sh[6][l] + sh[7][l])));
+        C[(int64_t)i * ldc + j] = v;
+        if (sym && ti != tj) C[(int64_t)j * ldc + i] = v;
     }
 }
 
@@ -310,8 +325,8 @@ void gram_mfma(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, i
         k_gram_mfma<false, T><<<grid, 256, 0, ctx->stream>>>(n, rpw, (int)nw, U, ldu, p, V, ldv, q, tv, sym ? 1 : 0,
                                                              ws.as<double>());
     LH_CHECK_LAUNCH();
-    k_gram_mfma_reduce<<<nblocks((int64_t)p * q, 64), 256, 0, ctx->stream>>>((int)nw, ntile, p, q, tv, sym ? 1 : 0,
-                                                                             ws.as<double>(), C, ldc, alpha);
+    k_gram_mfma_reduce<<<nblocks((int64_t)ntile * 4096, 64), 512, 0, ctx->stream>>>(
+        (int)nw, ntile, p, q, tv, sym ? 1 : 0, ws.as<double>(), C, ldc, alpha);
     LH_CHECK_LAUNCH();
 }
 
