@@ -88,6 +88,12 @@ def rel_err(evals, gold_evals):
     return float(np.max(np.abs(np.asarray(evals) - gold) / den))
 
 
+def fem_oracle_diffusion(v, t):
+    from oracle import fem
+
+    return fem.diffusion(v, t, 0, m=1.0)
+
+
 def run_workload(args, ctx, _lib, meshgen):
     """The other workloads of the benchmark plan (one JSON line each; the
     default L8 line is the contract line):
@@ -100,7 +106,10 @@ def run_workload(args, ctx, _lib, meshgen):
              download);
       batch: 256 seeded ellipsoids of icosphere levels 3-6, k = 50 each,
              assembled per mesh and solved by the batched eigensolver with
-             the autotuned worker count."""
+             the autotuned worker count;
+      geodesic: heat-method distance on the L8 ellipsoid from vertex 0: heat
+             diffusion (m = 1), then compute_geodesic_f (unit gradient,
+             divergence, Poisson solve by PCG), host arrays in and out."""
     from lapy_amd import TetMesh, heat
 
     out = {"unit": "eigenpairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -151,6 +160,35 @@ def run_workload(args, ctx, _lib, meshgen):
                 ref = np.asarray(gold["heat_probe_values"])
                 herr = float(np.max(np.abs(u[idx] - ref)) / np.max(np.abs(ref)))
             out["config"].update({"heat_seconds": round(th, 4), "heat_rtol": 1e-10, "heat_probe_rel_err": herr})
+    elif args.workload == "geodesic":
+        from lapy_amd import TriaMesh, diffgeo
+
+        v, t = meshgen.large_ellipsoid(level=args.level)
+        mesh = TriaMesh(v, t)
+        u = heat.diffusion(mesh, 0, m=1.0)
+        g = diffgeo.compute_geodesic_f(mesh, u)  # warm: device mesh, topology
+        for _ in range(args.warmup):
+            g = diffgeo.compute_geodesic_f(mesh, u)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g = diffgeo.compute_geodesic_f(mesh, u)
+        total = time.perf_counter() - t0
+        out.update({"metric": "geodesic distance fields/sec (heat method)", "unit": "fields/s",
+                    "value": round(args.steps / total, 4), "ms_per_step": round(1000 * total / args.steps, 2),
+                    "config": {"workload": f"icosphere L{args.level} ellipsoid, geodesic distance from vertex 0 "
+                                           "(host arrays in and out: PCIe inclusive)",
+                               "vertices": len(v), "elements": len(t), "source_distance": float(g[0]),
+                               "max_distance": float(g.max())}})
+        if not args.no_cpu_baseline:
+            from oracle import diffgeo as odg
+
+            vs, ts = meshgen.large_ellipsoid(level=6)
+            us = fem_oracle_diffusion(vs, ts)
+            c0 = time.perf_counter()
+            odg.geodesic_f(vs, ts, us)
+            ct = time.perf_counter() - c0
+            out["cpu_baseline"] = {"value": round(1.0 / ct, 4), "unit": "fields/s", "cores": 1, "kind": "port",
+                                   "sample": f"oracle geodesic_f (splu) on the L6 ellipsoid, {len(vs)} vertices"}
     elif args.workload == "batch":
         meshes = meshgen.shape_batch(count=256)
         k = 50
@@ -255,7 +293,7 @@ def main():
     ap.add_argument("--cpu-level", type=int, default=7)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", type=int, default=0)
-    ap.add_argument("--workload", default="l8", choices=["l8", "small", "tet", "pcie", "batch"],
+    ap.add_argument("--workload", default="l8", choices=["l8", "small", "tet", "pcie", "batch", "geodesic"],
                     help="l8: the benchmark line (default); small / tet / pcie / batch: the other workloads")
     ap.add_argument("--precond", default="cheb", choices=["cheb", "jacobi", "learned", "auto"],
                     help="batch workload: Chebyshev-Jacobi (default), plain Jacobi, the learned network, or auto "

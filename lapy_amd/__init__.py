@@ -4,7 +4,7 @@ AMD Instinct MI355X (gfx950), behind LaPy's own Python API.
     from lapy_amd import TriaMesh, TetMesh, Solver, heat, shapedna
 """
 
-from . import heat, shapedna  # noqa: F401
+from . import diffgeo, heat, shapedna  # noqa: F401
 from .mesh import TetMesh, TriaMesh  # noqa: F401
 from .solver import Solver  # noqa: F401
 

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "liblapyhip.so")
 LH_OK, LH_ERR_ARG, LH_ERR_OOM, LH_ERR_HIP, LH_ERR_NOCONV, LH_ERR_NUMERIC = range(6)
 LH_F64, LH_F32 = 0, 1
 LH_TRIA, LH_TRIA_ANISO, LH_TRIA_MASS, LH_TET = 0, 1, 2, 3
+LH_DG_GRADIENT, LH_DG_DIVERGENCE, LH_DG_GEODESIC_RHS = 0, 1, 2
 LH_PREC_FP64, LH_PREC_MIXED = 0, 1
 LH_PRECOND_JACOBI, LH_PRECOND_LEARNED, LH_PRECOND_NONE, LH_PRECOND_AUTO = 0, 1, 2, 3
 LH_STORE_AUTO, LH_STORE_F32, LH_STORE_F16 = 0, 1, 2
@@ -91,6 +92,7 @@ SIGNATURES = {
     "lh_mesh_free": (None, [_vp]),
     "lh_mesh_pattern_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
     "lh_mesh_avg_edge_length": (C.c_int, [_vp, _dp]),
+    "lh_tria_diffgeo": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp]),
     "lh_assemble": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.POINTER(_vp), C.POINTER(_vp)]),
     "lh_assemble_into": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "lh_csr_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -299,6 +301,7 @@ class DeviceMesh:
         self.handle = h
         self.nodes = td.shape[1]
         self.vdtype = vd.dtype
+        self.nv, self.ne = vd.shape[0], td.shape[0]
 
     def clone(self):
         """Device-to-device copy without the cached topology."""
@@ -306,6 +309,7 @@ class DeviceMesh:
         check(lib().lh_mesh_clone(self.handle, C.byref(h)), self.ctx, "lh_mesh_clone")
         other = DeviceMesh.__new__(DeviceMesh)
         other.ctx, other.handle, other.nodes, other.vdtype = self.ctx, h, self.nodes, self.vdtype
+        other.nv, other.ne = self.nv, self.ne
         return other
 
     def time_assembly(self, kind, lump, A, B, iters=20):
@@ -323,6 +327,16 @@ class DeviceMesh:
         out = C.c_double()
         check(lib().lh_mesh_avg_edge_length(self.handle, C.byref(out)), self.ctx)
         return out.value
+
+    def tria_diffgeo(self, mode, data, nf):
+        """Gradient (mode 0), divergence (1) or geodesic right-hand side (2) on
+        a triangle mesh; data is (nv, nf) for modes 0/2, (ne, nf, 3) for 1."""
+        d = np.ascontiguousarray(data, dtype=np.float64)
+        ne, nv = self.ne, self.nv
+        out = np.empty((ne, nf, 3) if mode == LH_DG_GRADIENT else (nv, nf), dtype=np.float64)
+        check(lib().lh_tria_diffgeo(self.ctx.handle, self.handle, int(mode), _ptr(d), int(nf), _ptr(out)),
+              self.ctx, "lh_tria_diffgeo")
+        return out
 
     def assemble(self, kind, lump, dtype=np.float64, u1=None, u2=None, aniso_mat=None):
         a, b = _vp(), _vp()

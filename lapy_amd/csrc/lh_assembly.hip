@@ -1007,6 +1007,139 @@ void upload_aniso(lh_ctx* ctx, lh_mesh* m, const void* u1, const void* u2, const
     h2d(ctx, a.am.p, am, m->ne * 2 * 8);
 }
 
+
+// ---------------------------------------------------------------------------
+// gradient, divergence and the heat-method right-hand side on triangles
+// (lapy/diffgeo.py:222-387, 117-164).  Geometry is evaluated in the vertex
+// dtype with numpy's operation order, the per-function arithmetic in float64
+// (numpy promotes the float32 geometry against the float64 functions).
+// Functions are row-major (n, nf); fields row-major (ne, nf, 3); the divergence
+// terms of element e, corner c, function k sit at ((e*3 + c)*nf + k).
+// ---------------------------------------------------------------------------
+template <typename VT>
+struct TriGeo {
+    V3<VT> e0, e1, e2;  // e2 = v1 - v0, e0 = v2 - v1, e1 = v0 - v2
+    VT ln;              // |cross(e2, -e1)|, 1 below machine epsilon
+    V3<VT> n;           // cross(e2, -e1) (unscaled)
+};
+
+template <typename VT>
+__device__ __forceinline__ TriGeo<VT> tri_geo(const VT* __restrict__ xyz, const int32_t* __restrict__ te) {
+    auto ld = [&](int32_t i) {
+        const VT* p = xyz + 4 * (int64_t)i;
+        return V3<VT>{p[0], p[1], p[2]};
+    };
+    const V3<VT> v0 = ld(te[0]), v1 = ld(te[1]), v2 = ld(te[2]);
+    TriGeo<VT> g;
+    g.e2 = vsub(v1, v0);
+    g.e0 = vsub(v2, v1);
+    g.e1 = vsub(v0, v2);
+    const V3<VT> me1{-g.e1.x, -g.e1.y, -g.e1.z};
+    g.n = vcross(g.e2, me1);
+    g.ln = dsqrt(vdot(g.n, g.n));
+    if (g.ln < (VT)kEps) g.ln = VT(1);
+    return g;
+}
+
+__device__ __forceinline__ V3<double> to_d(const V3<float>& a) { return {(double)a.x, (double)a.y, (double)a.z}; }
+__device__ __forceinline__ V3<double> to_d(const V3<double>& a) { return a; }
+
+// lni * cross(n * lni, f0 e0 + f1 e1 + f2 e2)  (diffgeo.py:270-283)
+template <typename VT>
+__device__ __forceinline__ V3<double> tri_grad(const TriGeo<VT>& g, VT lni, const V3<VT>& nn, double f0, double f1,
+                                               double f2) {
+    const V3<double> e0 = to_d(g.e0), e1 = to_d(g.e1), e2 = to_d(g.e2);
+    V3<double> c;
+    c.x = (f0 * e0.x + f1 * e1.x) + f2 * e2.x;
+    c.y = (f0 * e0.y + f1 * e1.y) + f2 * e2.y;
+    c.z = (f0 * e0.z + f1 * e1.z) + f2 * e2.z;
+    const V3<double> r = vcross(to_d(nn), c);
+    const double l = (double)lni;
+    return {l * r.x, l * r.y, l * r.z};
+}
+
+// gradf / sqrt((gradf**2).sum()) with np.nan_to_num (diffgeo.py:148-151)
+__device__ __forceinline__ double nan_to_num(double a) {
+    if (isnan(a)) return 0.0;
+    if (isinf(a)) return a > 0 ? DBL_MAX : -DBL_MAX;
+    return a;
+}
+__device__ __forceinline__ V3<double> unit(const V3<double>& g) {
+    const double s = sqrt(vdot(g, g));
+    return {nan_to_num(g.x / s), nan_to_num(g.y / s), nan_to_num(g.z / s)};
+}
+
+// the differences of cot-weighted edges that the divergence dots with the field
+// (diffgeo.py:350-362)
+template <typename VT>
+__device__ __forceinline__ void tri_div_rows(const TriGeo<VT>& g, V3<double>& d0, V3<double>& d1, V3<double>& d2) {
+    const V3<VT> me0{-g.e0.x, -g.e0.y, -g.e0.z}, me1{-g.e1.x, -g.e1.y, -g.e1.z}, me2{-g.e2.x, -g.e2.y, -g.e2.z};
+    const VT cot0 = vdot(g.e2, me1) / g.ln, cot1 = vdot(g.e0, me2) / g.ln, cot2 = vdot(g.e1, me0) / g.ln;
+    const V3<VT> c0{cot0 * g.e0.x, cot0 * g.e0.y, cot0 * g.e0.z};
+    const V3<VT> c1{cot1 * g.e1.x, cot1 * g.e1.y, cot1 * g.e1.z};
+    const V3<VT> c2{cot2 * g.e2.x, cot2 * g.e2.y, cot2 * g.e2.z};
+    d0 = to_d(vsub(c2, c1));
+    d1 = to_d(vsub(c0, c2));
+    d2 = to_d(vsub(c1, c0));
+}
+
+// MODE 0: gradient field out; 1: divergence terms of a field in; 2: fused
+// gradient -> unit field -> divergence terms (the geodesic right-hand side)
+template <typename VT, int MODE>
+__global__ void __launch_bounds__(256) k_tria_diffgeo(int64_t ne, int nf, const int32_t* __restrict__ t,
+                                                      const VT* __restrict__ xyz, const double* __restrict__ in,
+                                                      double* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const int32_t te[3] = {t[3 * e], t[3 * e + 1], t[3 * e + 2]};
+    const TriGeo<VT> g = tri_geo<VT>(xyz, te);
+    VT lni = VT(1) / g.ln;
+    const V3<VT> nn{g.n.x * lni, g.n.y * lni, g.n.z * lni};
+    V3<double> d0, d1, d2;
+    if constexpr (MODE != 0) tri_div_rows<VT>(g, d0, d1, d2);
+    for (int k = 0; k < nf; ++k) {
+        V3<double> fld;
+        if constexpr (MODE == 1) {
+            const double* p = in + (e * nf + k) * 3;
+            fld = {p[0], p[1], p[2]};
+        } else {
+            fld = tri_grad<VT>(g, lni, nn, in[(int64_t)te[0] * nf + k], in[(int64_t)te[1] * nf + k],
+                               in[(int64_t)te[2] * nf + k]);
+            if constexpr (MODE == 2) fld = unit(fld);
+        }
+        if constexpr (MODE == 0) {
+            double* p = out + (e * nf + k) * 3;
+            p[0] = fld.x;
+            p[1] = fld.y;
+            p[2] = fld.z;
+        } else {
+            out[(e * 3 + 0) * nf + k] = vdot(d0, fld);
+            out[(e * 3 + 1) * nf + k] = vdot(d1, fld);
+            out[(e * 3 + 2) * nf + k] = vdot(d2, fld);
+        }
+    }
+}
+
+// vertex sums of the divergence terms: 0.5 * sum over incident corners, corner
+// 0 of every incident element in ascending element order, then corner 1, then
+// corner 2 -- the order of the (n, 3F) scatter product of diffgeo.py:377-386
+__global__ void __launch_bounds__(256) k_tria_div_gather(int64_t n, int nf, const int32_t* __restrict__ inc_ptr,
+                                                         const int32_t* __restrict__ inc,
+                                                         const double* __restrict__ terms, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t b = inc_ptr[i], en = inc_ptr[i + 1];
+    for (int k = 0; k < nf; ++k) {
+        double acc = 0.0;
+        for (int c = 0; c < 3; ++c)
+            for (int32_t q = b; q < en; ++q) {
+                const int32_t r = inc[q];
+                if ((r & 3) == c) acc += terms[((int64_t)(r >> 2) * 3 + c) * nf + k];
+            }
+        out[i * nf + k] = 0.5 * acc;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1238,6 +1371,48 @@ int lh_assemble_into(lh_ctx* ctx, lh_mesh* m, int kind, int lump, const void* u1
     if (kind == LH_TRIA_ANISO) upload_aniso(ctx, m, u1, u2, aniso_mat, an);
     run_numeric(ctx, m, kind, lump, an.u1.as<double>(), an.u2.as<double>(), an.am.as<double>(),
                 kind == LH_TRIA_MASS ? nullptr : A, B);
+    LH_API_END(ctx)
+}
+
+
+// mode 0: gradient of nf vertex functions; 1: divergence of nf element
+// fields; 2: divergence of the unit gradient of nf vertex functions
+int lh_tria_diffgeo(lh_ctx* ctx, lh_mesh* m, int mode, const double* in, int nf, double* out) {
+    if (!ctx || !m || !in || !out || nf <= 0) return LH_ERR_ARG;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    if (m->nodes != 3) LH_THROW(LH_ERR_ARG, "gradient/divergence: triangle meshes only");
+    if (mode < 0 || mode > 2) LH_THROW(LH_ERR_ARG, "unknown mode %d", mode);
+    build_topology(ctx, m);
+    const int64_t ne = m->ne, nv = m->nv, n = m->n;
+    const size_t in_bytes = (size_t)(mode == 1 ? ne * 3 : nv) * nf * 8;
+    DevBuf din(in_bytes), dterm((size_t)ne * 3 * nf * 8);
+    h2d(ctx, din.p, in, in_bytes);
+    const unsigned nb = nblocks(ne, 256) ? nblocks(ne, 256) : 1;
+    const bool f64 = m->vdtype == LH_F64;
+#define LH_DG(MODE)                                                                                              \
+    if (f64)                                                                                                     \
+        k_tria_diffgeo<double, MODE><<<nb, 256, 0, ctx->stream>>>(ne, nf, m->t.as<int32_t>(), m->xyz.as<double>(), \
+                                                                  din.as<double>(), dterm.as<double>());         \
+    else                                                                                                         \
+        k_tria_diffgeo<float, MODE><<<nb, 256, 0, ctx->stream>>>(ne, nf, m->t.as<int32_t>(), m->xyz.as<float>(),  \
+                                                                 din.as<double>(), dterm.as<double>());
+    if (mode == 0) { LH_DG(0) } else if (mode == 1) { LH_DG(1) } else { LH_DG(2) }
+#undef LH_DG
+    LH_CHECK_LAUNCH();
+    if (mode == 0) {
+        d2h(ctx, out, dterm.p, (size_t)ne * 3 * nf * 8);
+    } else {
+        // vertices past max(t) have no incident elements: their divergence is 0
+        DevBuf dout((size_t)nv * nf * 8);
+        if (nv > n) memset_d(ctx, dout.as<double>() + n * nf, 0, (size_t)(nv - n) * nf * 8);
+        if (n > 0)
+            k_tria_div_gather<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, nf, m->inc_ptr.as<int32_t>(),
+                                                                       m->inc.as<int32_t>(), dterm.as<double>(),
+                                                                       dout.as<double>());
+        LH_CHECK_LAUNCH();
+        d2h(ctx, out, dout.p, (size_t)nv * nf * 8);
+    }
     LH_API_END(ctx)
 }
 

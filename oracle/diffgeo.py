@@ -1,0 +1,102 @@
+"""numpy/scipy restatement of LaPy's triangle-mesh gradient, divergence and
+heat-method geodesic distance (test infrastructure only).
+
+Same numpy operations in the same order as the reference, geometry in the
+vertex dtype, per-function arithmetic in float64, so the GPU library can be
+checked against these numbers.  Only tests, the smoke check and the CPU
+baseline of the benchmark import this module.
+"""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+from scipy import sparse
+
+from . import fem
+
+
+def _edges(v, t):
+    v0 = v[t[:, 0], :]
+    v1 = v[t[:, 1], :]
+    v2 = v[t[:, 2], :]
+    return v1 - v0, v2 - v1, v0 - v2  # e2, e0, e1
+
+
+def tria_gradient(v, t, vfunc):
+    """lapy/diffgeo.py:222-300: per-triangle gradient of a vertex function,
+    ``(F, 3)`` for ``(n,)`` input, ``(F, nf, 3)`` for ``(n, nf)``."""
+    vfunc = np.asarray(vfunc)
+    e2, e0, e1 = _edges(v, t)
+    n = np.cross(e2, -e1)
+    ln = np.sqrt(np.sum(n * n, axis=1))
+    ln[ln < sys.float_info.epsilon] = 1
+    lni = np.divide(1.0, ln)[:, np.newaxis]
+    n *= lni
+    if vfunc.ndim == 1:
+        c0 = vfunc[t[:, 0], np.newaxis] * e0
+        c1 = vfunc[t[:, 1], np.newaxis] * e1
+        c2 = vfunc[t[:, 2], np.newaxis] * e2
+        return lni * np.cross(n, (c0 + c1 + c2))
+    f0, f1, f2 = vfunc[t[:, 0], :], vfunc[t[:, 1], :], vfunc[t[:, 2], :]
+    c = (f0[:, :, None] * e0[:, None, :] + f1[:, :, None] * e1[:, None, :]
+         + f2[:, :, None] * e2[:, None, :])
+    return lni[:, np.newaxis, :] * np.cross(n[:, np.newaxis, :], c)
+
+
+def tria_divergence_terms(v, t, tfunc):
+    """lapy/diffgeo.py:346-375: the three per-triangle terms x0, x1, x2
+    (each ``(F,)`` or ``(F, nf)``) before the vertex sum."""
+    tfunc = np.asarray(tfunc)
+    e2, e0, e1 = _edges(v, t)
+    n = np.cross(e2, -e1)
+    ln = np.sqrt(np.sum(n * n, axis=1))
+    ln[ln < sys.float_info.epsilon] = 1
+    cot0 = (e2 * (-e1)).sum(1) / ln
+    cot1 = (e0 * (-e2)).sum(1) / ln
+    cot2 = (e1 * (-e0)).sum(1) / ln
+    c0 = cot0[:, np.newaxis] * e0
+    c1 = cot1[:, np.newaxis] * e1
+    c2 = cot2[:, np.newaxis] * e2
+    if tfunc.ndim == 2:
+        return (((c2 - c1) * tfunc).sum(1), ((c0 - c2) * tfunc).sum(1), ((c1 - c0) * tfunc).sum(1))
+    return (((c2 - c1)[:, None, :] * tfunc).sum(-1), ((c0 - c2)[:, None, :] * tfunc).sum(-1),
+            ((c1 - c0)[:, None, :] * tfunc).sum(-1))
+
+
+def tria_divergence(v, t, tfunc):
+    """lapy/diffgeo.py:303-387: integrated divergence at the vertices."""
+    tfunc = np.asarray(tfunc)
+    x0, x1, x2 = tria_divergence_terms(v, t, tfunc)
+    if tfunc.ndim == 2:
+        i = np.column_stack((t[:, 0], t[:, 1], t[:, 2])).reshape(-1)
+        j = np.zeros((3 * len(t), 1), dtype=int).reshape(-1)
+        dat = np.column_stack((x0, x1, x2)).reshape(-1)
+        return np.squeeze(np.asarray(0.5 * sparse.csc_matrix((dat, (i, j))).todense(), dtype=tfunc.dtype))
+    n_t = len(t)
+    rows = np.concatenate([t[:, 0], t[:, 1], t[:, 2]])
+    s = sparse.csr_matrix((np.ones(3 * n_t), (rows, np.arange(3 * n_t))), shape=(v.shape[0], 3 * n_t))
+    return 0.5 * s.dot(np.vstack([x0, x1, x2]))
+
+
+def normalized_gradient(gradf):
+    """lapy/diffgeo.py:148-162: unit gradient, 0 where the gradient is 0."""
+    if gradf.ndim == 2:
+        g = gradf / np.sqrt((gradf ** 2).sum(1))[:, np.newaxis]
+    else:
+        g = gradf / np.sqrt((gradf ** 2).sum(-1))[:, :, np.newaxis]
+    return np.nan_to_num(g)
+
+
+def geodesic_f(v, t, vfunc):
+    """lapy/diffgeo.py:117-164 for a triangle mesh: gradient, normalise,
+    divergence, then ``Solver(lump=True).poisson`` with the mass replaced by
+    the identity (an splu solve of the stiffness), shifted to minimum 0."""
+    vfunc = np.asarray(vfunc)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        gradnorm = normalized_gradient(tria_gradient(v, t, vfunc))
+    divf = tria_divergence(v, t, gradnorm)
+    a, _ = fem.fem_tria(v, t, lump=True)
+    vf = fem.poisson(a, sparse.eye(a.shape[0], dtype=a.dtype), divf)
+    return vf - vf.min(axis=0)

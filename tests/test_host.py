@@ -74,3 +74,20 @@ def test_meshgen_seeded():
     assert np.array_equal(a[0], b[0])
     batch = meshgen.shape_batch(count=4)
     assert len(batch) == 4 and all(x[1].shape[1] == 3 for x in batch)
+
+
+def test_diffgeo_dispatch_errors():
+    # lapy/diffgeo.py:51-54, 80-85: unknown types raise ValueError; tetrahedra are not built here
+    from lapy_amd import diffgeo
+
+    class Polygon:
+        v = np.zeros((3, 3))
+        t = np.zeros((1, 3), int)
+
+    for fn, arg in ((diffgeo.compute_gradient, np.zeros(3)), (diffgeo.compute_divergence, np.zeros((1, 3))),
+                    (diffgeo.compute_geodesic_f, np.zeros(3))):
+        with pytest.raises(ValueError, match='Geometry type "Polygon" unknown'):
+            fn(Polygon(), arg)
+    tet = TetMesh(*meshgen.tet_cube(2))
+    with pytest.raises(NotImplementedError, match="TetMesh"):
+        diffgeo.compute_gradient(tet, np.zeros(len(tet.v)))

@@ -1,0 +1,108 @@
+"""Pin the oracle's gradient, divergence and heat-method geodesics
+(oracle/diffgeo.py) against closed forms, the discrete identity
+div(grad f) = -A f, and the reference's geodesic known answer."""
+
+import numpy as np
+import pytest
+
+from conftest import data_path
+from lapy_amd import meshgen
+from lapy_amd.mesh import _read_off
+from oracle import diffgeo, fem
+
+
+def boundary_vertices(t):
+    """Vertices on edges that belong to one triangle only."""
+    e = np.sort(np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]]), axis=1)
+    uniq, cnt = np.unique(e, axis=0, return_counts=True)
+    return np.unique(uniq[cnt == 1])
+
+
+def test_gradient_of_linear_function_is_exact():
+    # on a planar mesh the P1 gradient of a*x + b*y + c is (a, b, 0) on every triangle
+    v, t = _read_off(data_path("square-mesh.off"))
+    v = v.astype(np.float64)
+    f = 2.0 * v[:, 0] - 3.0 * v[:, 1] + 0.5
+    g = diffgeo.tria_gradient(v, t, f)
+    assert g.shape == (len(t), 3)
+    np.testing.assert_allclose(g, np.tile([2.0, -3.0, 0.0], (len(t), 1)), atol=1e-11)
+    # a rotated copy of the mesh rotates the gradient
+    c, s = np.cos(0.3), np.sin(0.3)
+    rot = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1.0]]) @ np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+    gr = diffgeo.tria_gradient(v @ rot.T, t, f)
+    np.testing.assert_allclose(gr, g @ rot.T, atol=1e-11)
+
+
+def test_divergence_of_gradient_is_minus_stiffness():
+    # the integrated divergence of the P1 gradient is the cotan Laplacian: div(grad f) = -A f
+    v, t = meshgen.perturbed_sphere(3, np.random.default_rng(3), axes=(1.3, 0.8, 1.0), sigma=0.03)
+    a, _ = fem.fem_tria(v, t)
+    f = np.random.default_rng(4).standard_normal((len(v), 3))
+    d = diffgeo.tria_divergence(v, t, diffgeo.tria_gradient(v, t, f))
+    assert d.shape == (len(v), 3)
+    ref = -(a @ f)
+    np.testing.assert_allclose(d, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    for k in range(3):  # one column at a time gives the same numbers
+        dk = diffgeo.tria_divergence(v, t, diffgeo.tria_gradient(v, t, f[:, k]))
+        np.testing.assert_allclose(dk, d[:, k], rtol=1e-13, atol=1e-14)
+
+
+def test_planted_mistakes_fail_the_identity():
+    # copies of the oracle's gradient with one plausible mistake each: a dropped
+    # edge term, a transposed corner (f at the wrong vertex), the rotation the wrong way
+    v, t = meshgen.perturbed_sphere(2, np.random.default_rng(5), axes=(1.2, 1.0, 0.9), sigma=0.02)
+    a, _ = fem.fem_tria(v, t)
+    f = np.random.default_rng(6).standard_normal(len(v))
+    e2, e0, e1 = diffgeo._edges(v, t)
+    n = np.cross(e2, -e1)
+    lni = (1.0 / np.sqrt(np.sum(n * n, axis=1)))[:, None]
+    n = n * lni
+    c0, c1, c2 = f[t[:, 0], None] * e0, f[t[:, 1], None] * e1, f[t[:, 2], None] * e2
+    good = lni * np.cross(n, c0 + c1 + c2)
+    np.testing.assert_allclose(good, diffgeo.tria_gradient(v, t, f), rtol=1e-12, atol=1e-12)
+    bads = [lni * np.cross(n, c0 + c1),
+            lni * np.cross(n, f[t[:, 1], None] * e0 + f[t[:, 0], None] * e1 + c2),
+            lni * np.cross(c0 + c1 + c2, n)]
+    ref = -(a @ f)
+    for bad in bads:
+        d_bad = diffgeo.tria_divergence(v, t, bad)
+        assert np.abs(d_bad - ref).max() > 1e-2 * np.abs(ref).max()
+    np.testing.assert_allclose(diffgeo.tria_divergence(v, t, good), ref, atol=1e-12 * np.abs(ref).max())
+
+
+def test_geodesics_known_answer_square():
+    # reference test_TriaMesh_Geodesics.py:128-193: heat from the boundary, m=1,
+    # max distance 0.60497826 (expected_outcomes.json), the H = -A route equal to compute_geodesic_f
+    v, t = _read_off(data_path("square-mesh.off"))
+    bvert = boundary_vertices(t)
+    u = fem.diffusion(v, t, bvert, m=1.0)
+    gf = diffgeo.geodesic_f(v, t, u)
+    assert np.isclose(gf.max(), 0.60497826)
+    from scipy.sparse.linalg import splu
+
+    tf = diffgeo.tria_gradient(v, t, u)
+    with np.errstate(invalid="ignore"):
+        x_field = np.nan_to_num(-tf / np.sqrt((tf ** 2).sum(1))[:, np.newaxis])
+    divx = diffgeo.tria_divergence(v, t, x_field)
+    a, _ = fem.fem_tria(v, t, lump=True)
+    x = splu((-a).tocsc()).solve(divx)
+    x = x - x.min()
+    assert np.isclose(x.max(), 0.60497826)
+    assert np.allclose(np.abs(gf - x).max(), 0.0)
+    # distance-like: zero on the boundary's farthest-from-centre point, largest near the centre
+    centre = np.argmin(((v[:, :2] - 0.5) ** 2).sum(1))
+    assert gf[centre] > 0.9 * gf.max()
+
+
+def test_geodesics_multi_column_matches_single():
+    v, t = meshgen.perturbed_sphere(3, np.random.default_rng(7), axes=(1.1, 0.9, 1.0), sigma=0.02)
+    us = np.stack([fem.diffusion(v, t, [s], m=1.0) for s in (0, 100)], axis=1)
+    nf = us.shape[1]
+    g = diffgeo.geodesic_f(v, t, us)
+    assert g.shape == (len(v), nf)
+    # one column as (n, 1) comes back squeezed to (n,), as Solver.poisson squeezes a single right-hand side
+    assert diffgeo.geodesic_f(v, t, us[:, :1]).shape == (len(v),)
+    for k in range(nf):
+        np.testing.assert_allclose(g[:, k], diffgeo.geodesic_f(v, t, us[:, k]), rtol=1e-9, atol=1e-10)
+    # source vertex is the closest point: distance 0 there, positive elsewhere
+    assert g[0, 0] == pytest.approx(0.0, abs=1e-3 * g[:, 0].max())
