@@ -100,3 +100,42 @@ def geodesic_f(v, t, vfunc):
     a, _ = fem.fem_tria(v, t, lump=True)
     vf = fem.poisson(a, sparse.eye(a.shape[0], dtype=a.dtype), divf)
     return vf - vf.min(axis=0)
+
+
+def normalize(v, t):
+    """lapy/tria_mesh.py:990-1017 (area-weighted centroid, total area) and
+    :1390-1405 (normalize_): translate the centroid to the origin, scale to
+    unit surface area."""
+    v0, v1, v2 = v[t[:, 0], :], v[t[:, 1], :], v[t[:, 2], :]
+    cr = np.cross(v2 - v1, v0 - v2)
+    areas = 0.5 * np.sqrt(np.sum(cr * cr, axis=1))
+    totalarea = areas.sum()
+    areas = areas / totalarea
+    centers = (1.0 / 3.0) * (v0 + v1 + v2)
+    centroid = np.sum(centers * areas[:, np.newaxis], axis=0)
+    if totalarea <= 0:
+        raise ValueError("Mesh surface area must be positive to normalize.")
+    return (1.0 / np.sqrt(totalarea)) * (v - centroid)
+
+
+def mean_curvature_flow(v, t, max_iter=30, stop_eps=1e-13, step=1.0):
+    """lapy/diffgeo.py:523-607: fixed lumped stiffness of the normalised
+    mesh; each step solves (M + step*A) v = M v with the current lumped mass
+    (spsolve), renormalises, and stops when trace((dv' M dv)**2) < stop_eps.
+    Returns (vertices, steps taken)."""
+    from scipy.sparse.linalg import spsolve
+
+    v = normalize(np.asarray(v), t)
+    a_mat, _ = fem.fem_tria(v, t, lump=True)
+    x = 0
+    for x in range(max_iter):
+        vlast = v
+        mass = fem.fem_tria_mass(v, t, lump=True)
+        mass_v = mass.dot(v)
+        v = spsolve(mass + step * a_mat, mass_v)
+        v = normalize(v, t)
+        dv = v - vlast
+        diff = np.trace(np.square(np.matmul(np.transpose(dv), mass.dot(dv))))
+        if diff < stop_eps:
+            break
+    return v, x + 1

@@ -96,3 +96,25 @@ def test_geodesics_sphere_multi_source():
     rel_close(gf, ref, 1e-8)
     assert gf[0, 0] < 0.05 * gf[:, 0].max()
     assert gf[5000, 1] < 0.05 * gf[:, 1].max()
+
+
+def test_mean_curvature_flow_fixed_steps():
+    # five steps (no early stop): PCG to 1e-12 against the oracle's spsolve
+    print("case: mean curvature flow L4 sphere, 5 steps", flush=True)
+    v, t = sphere(4, 31)
+    w = diffgeo.tria_mean_curvature_flow(TriaMesh(v, t), max_iter=5, step=0.5)
+    w_ref, it = odg.mean_curvature_flow(v, t, max_iter=5, step=0.5)
+    assert it == 5
+    np.testing.assert_allclose(w.v, w_ref, rtol=0, atol=1e-9 * np.abs(w_ref).max())
+    np.testing.assert_array_equal(w.t, t)
+
+
+def test_mean_curvature_flow_to_convergence():
+    print("case: mean curvature flow L3 ellipsoid to convergence", flush=True)
+    v, t = sphere(3, 32)
+    w = diffgeo.tria_mean_curvature_flow(TriaMesh(v, t))
+    w_ref, it = odg.mean_curvature_flow(v, t)
+    assert it < 30
+    np.testing.assert_allclose(w.v, w_ref, rtol=0, atol=1e-6 * np.abs(w_ref).max())
+    r = np.linalg.norm(w.v, axis=1)
+    assert r.max() / r.min() < 1.05

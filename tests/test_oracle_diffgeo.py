@@ -106,3 +106,36 @@ def test_geodesics_multi_column_matches_single():
         np.testing.assert_allclose(g[:, k], diffgeo.geodesic_f(v, t, us[:, k]), rtol=1e-9, atol=1e-10)
     # source vertex is the closest point: distance 0 there, positive elsewhere
     assert g[0, 0] == pytest.approx(0.0, abs=1e-3 * g[:, 0].max())
+
+
+def test_normalize_closed_form():
+    # the square [0, 2]^2 (area 4, centroid (1, 1, 0)) maps to (v - (1, 1, 0)) / 2
+    v, t = _read_off(data_path("square-mesh.off"))
+    v2 = 2.0 * v.astype(np.float64)
+    w = diffgeo.normalize(v2, t)
+    np.testing.assert_allclose(w, (v2 - [1.0, 1.0, 0.0]) / 2.0, atol=1e-12)
+
+
+def test_mean_curvature_flow_one_step_is_the_dense_solve():
+    # pins the system (M + step*A) v = M v and the renormalisation on a dense solve
+    v, t = meshgen.perturbed_sphere(2, np.random.default_rng(8), axes=(1.4, 0.8, 1.0), sigma=0.03)
+    w1, it = diffgeo.mean_curvature_flow(v, t, max_iter=1, step=0.5)
+    assert it == 1
+    v0 = diffgeo.normalize(v, t)
+    a, _ = fem.fem_tria(v0, t, lump=True)
+    m = fem.fem_tria_mass(v0, t, lump=True).toarray()
+    x = np.linalg.solve(m + 0.5 * a.toarray(), m @ v0)
+    np.testing.assert_allclose(w1, diffgeo.normalize(x, t), atol=1e-12)
+
+
+def test_mean_curvature_flow_rounds_a_closed_mesh():
+    # closed meshes flow towards the sphere of unit area: radius 1/sqrt(4 pi)
+    v, t = meshgen.perturbed_sphere(3, np.random.default_rng(1), axes=(1.5, 0.7, 1.0), sigma=0.03)
+    w, it = diffgeo.mean_curvature_flow(v, t)
+    assert it < 30
+    r = np.linalg.norm(w, axis=1)
+    r0 = np.linalg.norm(diffgeo.normalize(v, t), axis=1)
+    assert r.max() / r.min() < 1.05 < r0.max() / r0.min()
+    assert abs(np.median(r) - 1.0 / np.sqrt(4 * np.pi)) < 0.01 / np.sqrt(4 * np.pi)
+    cr = np.cross(w[t[:, 1]] - w[t[:, 0]], w[t[:, 2]] - w[t[:, 0]])
+    assert 0.5 * np.linalg.norm(cr, axis=1).sum() == pytest.approx(1.0, rel=1e-12)
