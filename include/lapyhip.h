@@ -135,18 +135,20 @@ int lh_assemble_into(lh_ctx* ctx, lh_mesh* m, int kind, int lump,
                      const void* u1, const void* u2, const void* aniso_mat,
                      lh_csr* A, lh_csr* B);
 
-/* ---- gradient / divergence on triangles ------------------------------------
+/* ---- gradient / divergence on triangles and tetrahedra ----------------------
  * mode LH_DG_GRADIENT: in = nf vertex functions (nv x nf, float64, row-major),
- *   out = per-triangle gradients (ne x nf x 3) -- lapy/diffgeo.py:222-300
- *   (tria_compute_gradient; nf = 1 is the (n,) -> (ne, 3) form).
- * mode LH_DG_DIVERGENCE: in = per-triangle fields (ne x nf x 3), out = their
- *   integrated divergence at the vertices (nv x nf) -- lapy/diffgeo.py:303-387.
+ *   out = per-element gradients (ne x nf x 3) -- lapy/diffgeo.py:222-300
+ *   (tria_compute_gradient) and :846-918 (tet_compute_gradient); nf = 1 is
+ *   the (n,) -> (ne, 3) form.
+ * mode LH_DG_DIVERGENCE: in = per-element fields (ne x nf x 3), out = their
+ *   integrated divergence at the vertices (nv x nf) -- lapy/diffgeo.py:303-387
+ *   and :921-1006.
  * mode LH_DG_GEODESIC_RHS: in = nf vertex functions, out = the integrated
  *   divergence of their normalised gradients (nan_to_num'd), the right-hand
  *   side that compute_geodesic_f hands to Solver.poisson, lapy/diffgeo.py:117-164.
  * Geometry arithmetic runs in v's dtype, the function arithmetic in float64. */
 enum { LH_DG_GRADIENT = 0, LH_DG_DIVERGENCE = 1, LH_DG_GEODESIC_RHS = 2 };
-int lh_tria_diffgeo(lh_ctx* ctx, lh_mesh* m, int mode, const double* in, int nf, double* out);
+int lh_diffgeo(lh_ctx* ctx, lh_mesh* m, int mode, const double* in, int nf, double* out);
 
 /* ---- CSR matrices ----------------------------------------------------------- */
 /* A CSR of a symmetric matrix equals its CSC, which is how the façade hands

@@ -92,7 +92,7 @@ SIGNATURES = {
     "lh_mesh_free": (None, [_vp]),
     "lh_mesh_pattern_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
     "lh_mesh_avg_edge_length": (C.c_int, [_vp, _dp]),
-    "lh_tria_diffgeo": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp]),
+    "lh_diffgeo": (C.c_int, [_vp, _vp, C.c_int, _vp, C.c_int, _vp]),
     "lh_assemble": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, C.POINTER(_vp), C.POINTER(_vp)]),
     "lh_assemble_into": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "lh_csr_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -328,14 +328,14 @@ class DeviceMesh:
         check(lib().lh_mesh_avg_edge_length(self.handle, C.byref(out)), self.ctx)
         return out.value
 
-    def tria_diffgeo(self, mode, data, nf):
+    def diffgeo(self, mode, data, nf):
         """Gradient (mode 0), divergence (1) or geodesic right-hand side (2) on
         a triangle mesh; data is (nv, nf) for modes 0/2, (ne, nf, 3) for 1."""
         d = np.ascontiguousarray(data, dtype=np.float64)
         ne, nv = self.ne, self.nv
         out = np.empty((ne, nf, 3) if mode == LH_DG_GRADIENT else (nv, nf), dtype=np.float64)
-        check(lib().lh_tria_diffgeo(self.ctx.handle, self.handle, int(mode), _ptr(d), int(nf), _ptr(out)),
-              self.ctx, "lh_tria_diffgeo")
+        check(lib().lh_diffgeo(self.ctx.handle, self.handle, int(mode), _ptr(d), int(nf), _ptr(out)),
+              self.ctx, "lh_diffgeo")
         return out
 
     def assemble(self, kind, lump, dtype=np.float64, u1=None, u2=None, aniso_mat=None):

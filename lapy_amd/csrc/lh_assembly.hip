@@ -1120,23 +1120,88 @@ __global__ void __launch_bounds__(256) k_tria_diffgeo(int64_t ne, int nf, const 
     }
 }
 
-// vertex sums of the divergence terms: 0.5 * sum over incident corners, corner
-// 0 of every incident element in ascending element order, then corner 1, then
-// corner 2 -- the order of the (n, 3F) scatter product of diffgeo.py:377-386
-__global__ void __launch_bounds__(256) k_tria_div_gather(int64_t n, int nf, const int32_t* __restrict__ inc_ptr,
-                                                         const int32_t* __restrict__ inc,
-                                                         const double* __restrict__ terms, double* __restrict__ out) {
+// the same three modes on tetrahedra (lapy/diffgeo.py:846-1006): gradient
+// from the three edge cross products over |6V| (so a negatively oriented
+// tetrahedron yields -grad, as in the reference), divergence terms as the
+// field's dot products with the four face normals
+template <typename VT, int MODE>
+__global__ void __launch_bounds__(256) k_tet_diffgeo(int64_t ne, int nf, const int32_t* __restrict__ t,
+                                                     const VT* __restrict__ xyz, const double* __restrict__ in,
+                                                     double* __restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const int32_t te[4] = {t[4 * e], t[4 * e + 1], t[4 * e + 2], t[4 * e + 3]};
+    auto ld = [&](int32_t i) {
+        const VT* p = xyz + 4 * (int64_t)i;
+        return V3<VT>{p[0], p[1], p[2]};
+    };
+    const V3<VT> v0 = ld(te[0]), v1 = ld(te[1]), v2 = ld(te[2]), v3 = ld(te[3]);
+    V3<double> cr25{}, cr34{}, cr2e0{}, nrm[4];
+    VT voli = VT(1);
+    if constexpr (MODE != 1) {
+        const V3<VT> e0 = vsub(v1, v0), e2 = vsub(v0, v2), e3 = vsub(v3, v0), e4 = vsub(v3, v1), e5 = vsub(v3, v2);
+        VT vol = fabs(vdot(e3, vcross(e0, e2)));
+        if (vol < (VT)kEps) vol = VT(1);
+        voli = VT(1) / vol;
+        const V3<VT> me2{-e2.x, -e2.y, -e2.z};
+        cr25 = to_d(vcross(e2, e5));
+        cr34 = to_d(vcross(e3, e4));
+        cr2e0 = to_d(vcross(me2, e0));
+    }
+    if constexpr (MODE != 0) {
+        const V3<VT> e0 = vsub(v1, v0), e1 = vsub(v2, v1), e2 = vsub(v2, v0), e3 = vsub(v3, v0), e4 = vsub(v3, v1);
+        nrm[0] = to_d(vcross(e1, e4));
+        nrm[1] = to_d(vcross(e3, e2));
+        nrm[2] = to_d(vcross(e0, e3));
+        nrm[3] = to_d(vcross(e2, e0));
+    }
+    for (int k = 0; k < nf; ++k) {
+        V3<double> fld;
+        if constexpr (MODE == 1) {
+            const double* p = in + (e * nf + k) * 3;
+            fld = {p[0], p[1], p[2]};
+        } else {
+            const double f0 = in[(int64_t)te[0] * nf + k];
+            const double d1 = in[(int64_t)te[1] * nf + k] - f0, d2 = in[(int64_t)te[2] * nf + k] - f0,
+                         d3 = in[(int64_t)te[3] * nf + k] - f0;
+            const double l = (double)voli;
+            fld.x = l * ((d1 * cr25.x + d2 * cr34.x) + d3 * cr2e0.x);
+            fld.y = l * ((d1 * cr25.y + d2 * cr34.y) + d3 * cr2e0.y);
+            fld.z = l * ((d1 * cr25.z + d2 * cr34.z) + d3 * cr2e0.z);
+            if constexpr (MODE == 2) fld = unit(fld);
+        }
+        if constexpr (MODE == 0) {
+            double* p = out + (e * nf + k) * 3;
+            p[0] = fld.x;
+            p[1] = fld.y;
+            p[2] = fld.z;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) out[(e * 4 + c) * nf + k] = vdot(nrm[c], fld);
+        }
+    }
+}
+
+// vertex sums of the divergence terms: scale * the sum over incident corners,
+// corner 0 of every incident element in ascending element order, then corner
+// 1, and so on -- the order of the (n, NODES*F) scatter products of
+// diffgeo.py:377-386 and :995-1004.  scale is 0.5 for triangles, -1/6 for
+// tetrahedra.
+template <int NODES>
+__global__ void __launch_bounds__(256) k_div_gather(int64_t n, int nf, const int32_t* __restrict__ inc_ptr,
+                                                    const int32_t* __restrict__ inc, const double* __restrict__ terms,
+                                                    double scale, double* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t b = inc_ptr[i], en = inc_ptr[i + 1];
     for (int k = 0; k < nf; ++k) {
         double acc = 0.0;
-        for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < NODES; ++c)
             for (int32_t q = b; q < en; ++q) {
                 const int32_t r = inc[q];
-                if ((r & 3) == c) acc += terms[((int64_t)(r >> 2) * 3 + c) * nf + k];
+                if ((r & 3) == c) acc += terms[((int64_t)(r >> 2) * NODES + c) * nf + k];
             }
-        out[i * nf + k] = 0.5 * acc;
+        out[i * nf + k] = scale * acc;
     }
 }
 
@@ -1377,39 +1442,49 @@ int lh_assemble_into(lh_ctx* ctx, lh_mesh* m, int kind, int lump, const void* u1
 
 // mode 0: gradient of nf vertex functions; 1: divergence of nf element
 // fields; 2: divergence of the unit gradient of nf vertex functions
-int lh_tria_diffgeo(lh_ctx* ctx, lh_mesh* m, int mode, const double* in, int nf, double* out) {
+int lh_diffgeo(lh_ctx* ctx, lh_mesh* m, int mode, const double* in, int nf, double* out) {
     if (!ctx || !m || !in || !out || nf <= 0) return LH_ERR_ARG;
     LH_API_BEGIN(ctx)
     HIPX(hipSetDevice(ctx->device));
-    if (m->nodes != 3) LH_THROW(LH_ERR_ARG, "gradient/divergence: triangle meshes only");
     if (mode < 0 || mode > 2) LH_THROW(LH_ERR_ARG, "unknown mode %d", mode);
     build_topology(ctx, m);
+    const int nodes = m->nodes;
     const int64_t ne = m->ne, nv = m->nv, n = m->n;
     const size_t in_bytes = (size_t)(mode == 1 ? ne * 3 : nv) * nf * 8;
-    DevBuf din(in_bytes), dterm((size_t)ne * 3 * nf * 8);
+    const size_t term_bytes = (size_t)ne * (mode == 0 ? 3 : nodes) * nf * 8;
+    DevBuf din(in_bytes), dterm(term_bytes);
     h2d(ctx, din.p, in, in_bytes);
     const unsigned nb = nblocks(ne, 256) ? nblocks(ne, 256) : 1;
     const bool f64 = m->vdtype == LH_F64;
-#define LH_DG(MODE)                                                                                              \
+#define LH_DG(KERNEL, MODE)                                                                                      \
     if (f64)                                                                                                     \
-        k_tria_diffgeo<double, MODE><<<nb, 256, 0, ctx->stream>>>(ne, nf, m->t.as<int32_t>(), m->xyz.as<double>(), \
-                                                                  din.as<double>(), dterm.as<double>());         \
+        KERNEL<double, MODE><<<nb, 256, 0, ctx->stream>>>(ne, nf, m->t.as<int32_t>(), m->xyz.as<double>(),      \
+                                                          din.as<double>(), dterm.as<double>());                 \
     else                                                                                                         \
-        k_tria_diffgeo<float, MODE><<<nb, 256, 0, ctx->stream>>>(ne, nf, m->t.as<int32_t>(), m->xyz.as<float>(),  \
-                                                                 din.as<double>(), dterm.as<double>());
-    if (mode == 0) { LH_DG(0) } else if (mode == 1) { LH_DG(1) } else { LH_DG(2) }
+        KERNEL<float, MODE><<<nb, 256, 0, ctx->stream>>>(ne, nf, m->t.as<int32_t>(), m->xyz.as<float>(),        \
+                                                         din.as<double>(), dterm.as<double>());
+    if (nodes == 3) {
+        if (mode == 0) { LH_DG(k_tria_diffgeo, 0) } else if (mode == 1) { LH_DG(k_tria_diffgeo, 1) } else { LH_DG(k_tria_diffgeo, 2) }
+    } else {
+        if (mode == 0) { LH_DG(k_tet_diffgeo, 0) } else if (mode == 1) { LH_DG(k_tet_diffgeo, 1) } else { LH_DG(k_tet_diffgeo, 2) }
+    }
 #undef LH_DG
     LH_CHECK_LAUNCH();
     if (mode == 0) {
-        d2h(ctx, out, dterm.p, (size_t)ne * 3 * nf * 8);
+        d2h(ctx, out, dterm.p, term_bytes);
     } else {
         // vertices past max(t) have no incident elements: their divergence is 0
         DevBuf dout((size_t)nv * nf * 8);
         if (nv > n) memset_d(ctx, dout.as<double>() + n * nf, 0, (size_t)(nv - n) * nf * 8);
-        if (n > 0)
-            k_tria_div_gather<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, nf, m->inc_ptr.as<int32_t>(),
-                                                                       m->inc.as<int32_t>(), dterm.as<double>(),
-                                                                       dout.as<double>());
+        if (n > 0) {
+            if (nodes == 3)
+                k_div_gather<3><<<nblocks(n, 256), 256, 0, ctx->stream>>>(
+                    n, nf, m->inc_ptr.as<int32_t>(), m->inc.as<int32_t>(), dterm.as<double>(), 0.5, dout.as<double>());
+            else
+                k_div_gather<4><<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, nf, m->inc_ptr.as<int32_t>(),
+                                                                         m->inc.as<int32_t>(), dterm.as<double>(),
+                                                                         -(1.0 / 6.0), dout.as<double>());
+        }
         LH_CHECK_LAUNCH();
         d2h(ctx, out, dout.p, (size_t)nv * nf * 8);
     }

@@ -1,12 +1,12 @@
-"""Gradient, divergence and heat-method geodesic distance on triangle meshes
-(lapy/diffgeo.py:26-164, 222-387), on the GPU.
+"""Gradient, divergence and heat-method geodesic distance on triangle and
+tetrahedral meshes (lapy/diffgeo.py:26-164, 222-387, 846-1006), and mean
+curvature flow of triangle meshes (lapy/diffgeo.py:523-607), on the GPU.
 
 Same names, arguments, dispatch and return shapes as the reference.  The
 element work (gradient, normalisation, divergence terms) and the vertex sums
-run in ``lh_tria_diffgeo``; the Poisson solve of ``compute_geodesic_f`` is the
+run in ``lh_diffgeo``; the Poisson solve of ``compute_geodesic_f`` is the
 library's PCG on the lumped Solver's stiffness with the mass replaced by the
-identity, as the reference's ``fem.mass = sparse.eye(n)`` does.  Tetrahedral
-meshes are not built here and raise ``NotImplementedError``.
+identity, as the reference's ``fem.mass = sparse.eye(n)`` does.
 """
 
 from __future__ import annotations
@@ -20,10 +20,8 @@ from .solver import Solver
 
 def _kind(geom):
     name = type(geom).__name__
-    if name == "TriaMesh":
+    if name in ("TriaMesh", "TetMesh"):
         return name
-    if name == "TetMesh":
-        raise NotImplementedError("gradient/divergence/geodesics on TetMesh are not built; triangle meshes only")
     raise ValueError('Geometry type "' + name + '" unknown')
 
 
@@ -33,9 +31,9 @@ def _dim(geom):
 
 
 def compute_gradient(geom, vfunc: np.ndarray) -> np.ndarray:
-    """Per-triangle gradient of a vertex function (lapy/diffgeo.py:26-54,
-    222-300): ``(n_triangles, 3)`` for ``(n_vertices,)`` input,
-    ``(n_triangles, n_functions, 3)`` for ``(n_vertices, n_functions)``."""
+    """Per-element gradient of a vertex function (lapy/diffgeo.py:26-54,
+    222-300, 846-918): ``(n_elements, 3)`` for ``(n_vertices,)`` input,
+    ``(n_elements, n_functions, 3)`` for ``(n_vertices, n_functions)``."""
     _kind(geom)
     vfunc = np.asarray(vfunc)
     scalar = vfunc.ndim == 1
@@ -43,23 +41,24 @@ def compute_gradient(geom, vfunc: np.ndarray) -> np.ndarray:
     dm = device_mesh(geom)
     if f2.shape[0] != dm.nv:
         raise ValueError("vfunc must have one row per vertex")
-    g = dm.tria_diffgeo(_lib.LH_DG_GRADIENT, f2, f2.shape[1])
+    g = dm.diffgeo(_lib.LH_DG_GRADIENT, f2, f2.shape[1])
     return g[:, 0, :] if scalar else g
 
 
 def compute_divergence(geom, tfunc: np.ndarray) -> np.ndarray:
-    """Integrated divergence at the vertices of a per-triangle vector field
-    (lapy/diffgeo.py:57-85, 303-387): ``(n,)`` for ``(n_triangles, 3)`` input
-    (n = max(t)+1, the inferred shape of the reference's ``csc_matrix``),
-    ``(n_vertices, n_functions)`` for ``(n_triangles, n_functions, 3)``."""
+    """Integrated divergence at the vertices of a per-element vector field
+    (lapy/diffgeo.py:57-85, 303-387, 921-1006): ``(n,)`` for
+    ``(n_elements, 3)`` input (n = max(t)+1, the inferred shape of the
+    reference's ``csc_matrix``), ``(n_vertices, n_functions)`` for
+    ``(n_elements, n_functions, 3)``."""
     _kind(geom)
     tfunc = np.asarray(tfunc)
     scalar = tfunc.ndim == 2
     f3 = tfunc[:, None, :] if scalar else tfunc
     dm = device_mesh(geom)
     if f3.shape[0] != dm.ne or f3.shape[-1] != 3:
-        raise ValueError("tfunc must have shape (n_triangles, 3) or (n_triangles, n_functions, 3)")
-    d = dm.tria_diffgeo(_lib.LH_DG_DIVERGENCE, f3, f3.shape[1])
+        raise ValueError("tfunc must have shape (n_elements, 3) or (n_elements, n_functions, 3)")
+    d = dm.diffgeo(_lib.LH_DG_DIVERGENCE, f3, f3.shape[1])
     if scalar:
         return d[:_dim(geom), 0].astype(tfunc.dtype, copy=False)
     return d
@@ -77,7 +76,7 @@ def compute_geodesic_f(geom, vfunc: np.ndarray, use_cholmod: bool = False) -> np
     dm = device_mesh(geom, fem._ctx)
     if f2.shape[0] != dm.nv:
         raise ValueError("vfunc must have one row per vertex")
-    divf = dm.tria_diffgeo(_lib.LH_DG_GEODESIC_RHS, f2, f2.shape[1])
+    divf = dm.diffgeo(_lib.LH_DG_GEODESIC_RHS, f2, f2.shape[1])
     a_dev = fem.device_matrices()[0]
     if divf.shape[0] != a_dev.n:
         raise ValueError("h should be either scalar or array with first dim matching A")

@@ -1,5 +1,5 @@
-"""numpy/scipy restatement of LaPy's triangle-mesh gradient, divergence and
-heat-method geodesic distance (test infrastructure only).
+"""numpy/scipy restatement of LaPy's gradient, divergence, heat-method
+geodesic distance and mean curvature flow (test infrastructure only).
 
 Same numpy operations in the same order as the reference, geometry in the
 vertex dtype, per-function arithmetic in float64, so the GPU library can be
@@ -80,6 +80,56 @@ def tria_divergence(v, t, tfunc):
     return 0.5 * s.dot(np.vstack([x0, x1, x2]))
 
 
+def tet_gradient(v, t, vfunc):
+    """lapy/diffgeo.py:846-918: per-tetrahedron gradient, ``(T, 3)`` or ``(T, nf, 3)``."""
+    vfunc = np.asarray(vfunc)
+    v0, v1, v2, v3 = (v[t[:, c], :] for c in range(4))
+    e0, e2, e3, e4, e5 = v1 - v0, v0 - v2, v3 - v0, v3 - v1, v3 - v2
+    cr = np.cross(e0, e2)
+    vol = np.abs(np.sum(e3 * cr, axis=1))
+    vol[vol < sys.float_info.epsilon] = 1
+    voli = np.divide(1.0, vol)[:, np.newaxis]
+    cr25, cr34, cr2e0 = np.cross(e2, e5), np.cross(e3, e4), np.cross(-e2, e0)
+    if vfunc.ndim == 1:
+        c1 = (vfunc[t[:, 1], np.newaxis] - vfunc[t[:, 0], np.newaxis]) * cr25
+        c2 = (vfunc[t[:, 2], np.newaxis] - vfunc[t[:, 0], np.newaxis]) * cr34
+        c3 = (vfunc[t[:, 3], np.newaxis] - vfunc[t[:, 0], np.newaxis]) * cr2e0
+        return voli * (c1 + c2 + c3)
+    d1, d2, d3 = (vfunc[t[:, c], :] - vfunc[t[:, 0], :] for c in (1, 2, 3))
+    c = (d1[:, :, None] * cr25[:, None, :] + d2[:, :, None] * cr34[:, None, :] + d3[:, :, None] * cr2e0[:, None, :])
+    return voli[:, np.newaxis, :] * c
+
+
+def tet_divergence(v, t, tfunc):
+    """lapy/diffgeo.py:921-1006: integrated divergence at the vertices,
+    -1/6 of the field's flux through the faces opposite each vertex."""
+    tfunc = np.asarray(tfunc)
+    v0, v1, v2, v3 = (v[t[:, c], :] for c in range(4))
+    e0, e1, e2, e3, e4 = v1 - v0, v2 - v1, v2 - v0, v3 - v0, v3 - v1
+    ns = (np.cross(e1, e4), np.cross(e3, e2), np.cross(e0, e3), np.cross(e2, e0))
+    n_t = len(t)
+    if tfunc.ndim == 2:
+        xs = [(n * tfunc).sum(1) for n in ns]
+        i = np.column_stack((t[:, 0], t[:, 1], t[:, 2], t[:, 3])).reshape(-1)
+        j = np.zeros((4 * n_t, 1), dtype=int).reshape(-1)
+        dat = np.column_stack(xs).reshape(-1)
+        return -np.squeeze(np.asarray((1.0 / 6.0) * sparse.csc_matrix((dat, (i, j))).todense(), dtype=tfunc.dtype))
+    xs = [(n[:, np.newaxis, :] * tfunc).sum(-1) for n in ns]
+    rows = np.concatenate([t[:, 0], t[:, 1], t[:, 2], t[:, 3]])
+    s = sparse.csr_matrix((np.ones(4 * n_t), (rows, np.arange(4 * n_t))), shape=(v.shape[0], 4 * n_t))
+    return -(1.0 / 6.0) * s.dot(np.vstack(xs))
+
+
+def gradient(v, t, vfunc):
+    """lapy/diffgeo.py:26-54: dispatch on nodes per element."""
+    return tet_gradient(v, t, vfunc) if t.shape[1] == 4 else tria_gradient(v, t, vfunc)
+
+
+def divergence(v, t, tfunc):
+    """lapy/diffgeo.py:57-85."""
+    return tet_divergence(v, t, tfunc) if t.shape[1] == 4 else tria_divergence(v, t, tfunc)
+
+
 def normalized_gradient(gradf):
     """lapy/diffgeo.py:148-162: unit gradient, 0 where the gradient is 0."""
     if gradf.ndim == 2:
@@ -90,14 +140,14 @@ def normalized_gradient(gradf):
 
 
 def geodesic_f(v, t, vfunc):
-    """lapy/diffgeo.py:117-164 for a triangle mesh: gradient, normalise,
+    """lapy/diffgeo.py:117-164 (triangles or tetrahedra): gradient, normalise,
     divergence, then ``Solver(lump=True).poisson`` with the mass replaced by
     the identity (an splu solve of the stiffness), shifted to minimum 0."""
     vfunc = np.asarray(vfunc)
     with np.errstate(invalid="ignore", divide="ignore"):
-        gradnorm = normalized_gradient(tria_gradient(v, t, vfunc))
-    divf = tria_divergence(v, t, gradnorm)
-    a, _ = fem.fem_tria(v, t, lump=True)
+        gradnorm = normalized_gradient(gradient(v, t, vfunc))
+    divf = divergence(v, t, gradnorm)
+    a, _ = (fem.fem_tetra if t.shape[1] == 4 else fem.fem_tria)(v, t, lump=True)
     vf = fem.poisson(a, sparse.eye(a.shape[0], dtype=a.dtype), divf)
     return vf - vf.min(axis=0)
 

@@ -118,3 +118,42 @@ def test_mean_curvature_flow_to_convergence():
     np.testing.assert_allclose(w.v, w_ref, rtol=0, atol=1e-6 * np.abs(w_ref).max())
     r = np.linalg.norm(w.v, axis=1)
     assert r.max() / r.min() < 1.05
+
+
+def boundary_faces_vertices(t):
+    f = np.sort(np.concatenate([t[:, [1, 2, 3]], t[:, [0, 2, 3]], t[:, [0, 1, 3]], t[:, [0, 1, 2]]]), axis=1)
+    uniq, cnt = np.unique(f, axis=0, return_counts=True)
+    return np.unique(uniq[cnt == 1])
+
+
+@pytest.mark.parametrize("nf", [1, 2])
+def test_tet_gradient_divergence(nf):
+    print(f"case: tet gradient/divergence 6^3 jittered cube nf={nf}", flush=True)
+    from lapy_amd import TetMesh
+
+    v, t = meshgen.tet_cube(6, jitter=0.15, seed=5)
+    mesh = TetMesh(v, t)
+    f = np.random.default_rng(13).standard_normal((len(v), nf))
+    f_in = f[:, 0] if nf == 1 else f
+    g = diffgeo.compute_gradient(mesh, f_in)
+    g_ref = odg.tet_gradient(v, t, f_in)
+    assert g.shape == g_ref.shape
+    rel_close(g, g_ref, 1e-12)
+    d = diffgeo.compute_divergence(mesh, g_ref)
+    d_ref = odg.tet_divergence(v, t, g_ref)
+    assert d.shape == d_ref.shape
+    rel_close(d, d_ref, 1e-12)
+    a, _ = fem.fem_tetra(v, t)
+    rel_close(d, -(a @ f_in), 1e-11)
+
+
+def test_tet_geodesics_known_answer_cube():
+    print("case: tet geodesics cubeTetra.vtk from the boundary", flush=True)
+    from lapy_amd import TetMesh
+
+    mesh = TetMesh.read_vtk(data_path("cubeTetra.vtk"))
+    v, t = np.asarray(mesh.v), np.asarray(mesh.t)
+    u = heat.diffusion(mesh, boundary_faces_vertices(t), m=1.0)
+    gf = diffgeo.compute_geodesic_f(mesh, u)
+    assert np.isclose(gf.max(), 0.69931495), gf.max()
+    rel_close(gf, odg.geodesic_f(v, t, u), 1e-8)

@@ -77,7 +77,7 @@ def test_meshgen_seeded():
 
 
 def test_diffgeo_dispatch_errors():
-    # lapy/diffgeo.py:51-54, 80-85: unknown types raise ValueError; tetrahedra are not built here
+    # lapy/diffgeo.py:51-54, 80-85: unknown geometry types raise ValueError
     from lapy_amd import diffgeo
 
     class Polygon:
@@ -88,6 +88,3 @@ def test_diffgeo_dispatch_errors():
                     (diffgeo.compute_geodesic_f, np.zeros(3))):
         with pytest.raises(ValueError, match='Geometry type "Polygon" unknown'):
             fn(Polygon(), arg)
-    tet = TetMesh(*meshgen.tet_cube(2))
-    with pytest.raises(NotImplementedError, match="TetMesh"):
-        diffgeo.compute_gradient(tet, np.zeros(len(tet.v)))

@@ -139,3 +139,49 @@ def test_mean_curvature_flow_rounds_a_closed_mesh():
     assert abs(np.median(r) - 1.0 / np.sqrt(4 * np.pi)) < 0.01 / np.sqrt(4 * np.pi)
     cr = np.cross(w[t[:, 1]] - w[t[:, 0]], w[t[:, 2]] - w[t[:, 0]])
     assert 0.5 * np.linalg.norm(cr, axis=1).sum() == pytest.approx(1.0, rel=1e-12)
+
+
+def boundary_faces_vertices(t):
+    """Vertices on tetrahedron faces that belong to one tetrahedron only."""
+    f = np.sort(np.concatenate([t[:, [1, 2, 3]], t[:, [0, 2, 3]], t[:, [0, 1, 3]], t[:, [0, 1, 2]]]), axis=1)
+    uniq, cnt = np.unique(f, axis=0, return_counts=True)
+    return np.unique(uniq[cnt == 1])
+
+
+def test_tet_gradient_linear_and_identity():
+    v, t = meshgen.tet_cube(4, jitter=0.15, seed=3)
+    g = diffgeo.tet_gradient(v, t, 2.0 * v[:, 0] - 3.0 * v[:, 1] + 0.5 * v[:, 2])
+    # the reference divides by |6V|, so a negatively oriented tetrahedron (half of this split) gets -grad
+    e0, e2, e3 = v[t[:, 1]] - v[t[:, 0]], v[t[:, 0]] - v[t[:, 2]], v[t[:, 3]] - v[t[:, 0]]
+    sign = np.sign(np.sum(e3 * np.cross(e0, e2), axis=1))
+    assert (sign > 0).any() and (sign < 0).any()
+    np.testing.assert_allclose(g, sign[:, None] * np.array([2.0, -3.0, 0.5]), atol=1e-11)
+    a, _ = fem.fem_tetra(v, t)
+    f = np.random.default_rng(9).standard_normal((len(v), 2))
+    d = diffgeo.tet_divergence(v, t, diffgeo.tet_gradient(v, t, f))
+    ref = -(a @ f)
+    np.testing.assert_allclose(d, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    d1 = diffgeo.tet_divergence(v, t, diffgeo.tet_gradient(v, t, f[:, 0]))
+    np.testing.assert_allclose(d1, d[:, 0], rtol=1e-13, atol=1e-14)
+
+
+def test_tet_geodesics_known_answer_cube():
+    # reference test_TetMesh_Geodesics.py:115-172: heat from the boundary of cubeTetra.vtk, m=1,
+    # max distance 0.69931495 (expected_outcomes.json)
+    from lapy_amd.mesh import _read_vtk
+    from scipy.sparse.linalg import splu
+
+    v, t = _read_vtk(data_path("cubeTetra.vtk"), 4)
+    bvert = boundary_faces_vertices(t)
+    u = fem.diffusion(v, t, bvert, m=1.0)
+    tf = diffgeo.tet_gradient(v, t, u)
+    with np.errstate(invalid="ignore"):
+        x_field = np.nan_to_num(-tf / np.sqrt((tf ** 2).sum(1))[:, np.newaxis])
+    divx = diffgeo.tet_divergence(v, t, x_field)
+    a, _ = fem.fem_tetra(v, t, lump=True)
+    x = splu(a.tocsc()).solve(-divx)
+    x = x - x.min()
+    assert np.isclose(x.max(), 0.69931495)
+    gf = diffgeo.geodesic_f(v, t, u)
+    assert np.isclose(gf.max(), 0.69931495)
+    assert np.abs(gf - x).max() < 1e-8
