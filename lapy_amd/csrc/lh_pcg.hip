@@ -6,11 +6,14 @@
 // sides are solved together as a row-major block (one CG recurrence per
 // column).  The operator K is formed once on the device on A's pattern
 // (B diagonal or on the same pattern); Dirichlet rows/columns are removed by
-// masking, not by slicing.  With alpha == 0 and no Dirichlet rows, A is
-// singular with the constants as null space (closed mesh / pure Neumann): the
-// right-hand side is projected onto the complement of the constants and the
-// mean-free solution is returned (SuperLU returns an arbitrary member of the
-// solution set).
+// masking, not by slicing.  With alpha == 0 and no Dirichlet rows (pure
+// Neumann), A is singular with the constants as null space when its row sums
+// vanish to rounding: the right-hand side is then projected onto the
+// complement of the constants and the mean-free solution is returned (SuperLU
+// returns an arbitrary member of the solution set).  Element matrices computed
+// from float32 coordinates have row sums of ~1e-8 of the diagonal; such an A is
+// regular (its smallest eigenvalue ~1e-9) and is solved as it stands, as
+// SuperLU does, with no projection.
 #include "lh_dense.hpp"
 
 #include <chrono>
@@ -226,6 +229,27 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
         jacobi_diag(ctx, &K, nullptr, 0.0, 1.0, dinv.as<double>());
         K.vals.p = nullptr;  // not owned
     }
+    bool any_dirichlet = false;
+    if (dmask) {
+        for (int64_t i = 0; i < n; ++i) if (dmask[i]) { any_dirichlet = true; break; }
+    }
+    // pure Neumann: singular (constants as null space) only if max_i |(K 1)_i| / K_ii is at rounding level
+    bool singular = false;
+    if (alpha == 0.0 && !any_dirichlet) {
+        DevBuf one(n * 8), rowsum(n * 8);
+        k_fill<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, 1.0, one.as<double>());
+        k_spmv_masked<<<nblocks(n * 16, 256), 256, 0, ctx->stream>>>(n, A->pat->rowptr.as<int32_t>(),
+                                                                    A->pat->cols.as<int32_t>(), kv.as<double>(),
+                                                                    nullptr, one.as<double>(),
+                                                                    rowsum.as<double>(), 1);
+        LH_CHECK_LAUNCH();
+        std::vector<double> hs(n), hd(n);
+        d2h(ctx, hs.data(), rowsum.p, n * 8);
+        d2h(ctx, hd.data(), dinv.p, n * 8);
+        double worst = 0.0;
+        for (int64_t i = 0; i < n; ++i) worst = std::max(worst, std::fabs(hs[i] * hd[i]));
+        singular = worst <= 1e-12;
+    }
     const int precond = opts ? opts->precond : LH_PRECOND_JACOBI;
     if (precond != LH_PRECOND_JACOBI && precond != LH_PRECOND_LEARNED && precond != LH_PRECOND_NONE)
         LH_THROW(LH_ERR_ARG, "preconditioner %d unknown", precond);
@@ -247,16 +271,11 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
         }
     DevBuf dB(n * m * 8), dX(n * m * 8), dR(n * m * 8), dZ(n * m * 8), dP(n * m * 8), dQ(n * m * 8), dm;
     const uint8_t* dmp = nullptr;
-    bool any_dirichlet = false;
-    if (dmask) {
-        for (int64_t i = 0; i < n; ++i) if (dmask[i]) { any_dirichlet = true; break; }
-    }
     if (any_dirichlet) {
         dm.alloc(n);
         h2d(ctx, dm.p, dmask, n);
         dmp = dm.as<uint8_t>();
     }
-    const bool singular = (alpha == 0.0) && !any_dirichlet;
     h2d(ctx, dB.p, hb.data(), n * m * 8);
     // r = b on free rows (x starts at 0 there)
     HIPX(hipMemsetAsync(dX.p, 0, n * m * 8, ctx->stream));
