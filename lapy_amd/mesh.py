@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import _lib
+from . import _lib, curvature
 
 
 class TriaMesh:
@@ -64,6 +64,25 @@ class TriaMesh:
     def area(self) -> float:
         v1, v2, v3 = self.v[self.t[:, 0]], self.v[self.t[:, 1]], self.v[self.t[:, 2]]
         return float(0.5 * np.linalg.norm(np.cross(v2 - v1, v3 - v1), axis=1).sum())
+
+    def tria_normals(self) -> np.ndarray:
+        """Unit triangle normals (lapy/tria_mesh.py:750-773)."""
+        return curvature.tria_normals(self.v, self.t)
+
+    def vertex_normals(self) -> np.ndarray:
+        """Area-weighted unit vertex normals (lapy/tria_mesh.py:775-817)."""
+        return curvature.vertex_normals(self.v, self.t)
+
+    def curvature(self, smoothit: int = 3):
+        """Per-vertex principal curvatures and directions
+        (lapy/tria_mesh.py:1078-1219): ``u_min, u_max, c_min, c_max, c_mean,
+        c_gauss, normals``."""
+        return curvature.curvature(self.v, self.t, smoothit)
+
+    def curvature_tria(self, smoothit: int = 3):
+        """Per-triangle principal directions and curvatures, the input of
+        ``Solver(aniso=...)`` (lapy/tria_mesh.py:1221-1279)."""
+        return curvature.curvature_tria(self.v, self.t, smoothit)
 
 
 class TetMesh:

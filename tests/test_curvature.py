@@ -1,0 +1,169 @@
+"""Principal curvatures (lapy_amd.curvature, the input of Solver(aniso=...)),
+pinned by what the geometry fixes: a plane has zero curvature, a sphere is
+umbilic (equal principal curvatures, normals radial), a cylinder's flat
+principal direction is its axis, and the estimator is scale-invariant (edge
+lengths are normalised by their mean) and rotation-equivariant. A wrong
+tensor entry (e.g. xz and yy exchanged) breaks the equivariance check."""
+
+import numpy as np
+import pytest
+
+from lapy_amd import TriaMesh, curvature, meshgen
+
+
+def _rotation(seed):
+    q, r = np.linalg.qr(np.random.default_rng(seed).standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+def _cylinder(nu=48, nz=40, radius=1.0, height=3.0):
+    """Open tube around the z axis, triangles oriented outwards."""
+    th = 2 * np.pi * np.arange(nu) / nu
+    z = np.linspace(0.0, height, nz)
+    v = np.array([[radius * np.cos(a), radius * np.sin(a), h] for h in z for a in th])
+    t = []
+    for k in range(nz - 1):
+        for i in range(nu):
+            a, b = k * nu + i, k * nu + (i + 1) % nu
+            c, d = a + nu, b + nu
+            t += [[a, b, d], [a, d, c]]
+    return v, np.array(t)
+
+
+def _grid(n=12):
+    x, y = np.meshgrid(np.linspace(0, 1, n), np.linspace(0, 1, n))
+    v = np.column_stack([x.ravel(), y.ravel(), np.zeros(n * n)])
+    t = []
+    for j in range(n - 1):
+        for i in range(n - 1):
+            a = j * n + i
+            t += [[a, a + 1, a + n + 1], [a, a + n + 1, a + n]]
+    return v, np.array(t)
+
+
+def test_inner_edges_brute_force():
+    v, t = meshgen.icosphere(2)
+    vids, tids = curvature.inner_edges(v, t)
+    assert len(vids) == 3 * len(t) // 2  # closed mesh: every edge is inner
+    assert np.all(vids[:, 0] < vids[:, 1])
+    assert np.all(np.diff(vids[:, 0] * len(v) + vids[:, 1]) > 0)
+    for (i, j), (ta, tb) in zip(vids, tids):
+        assert any((t[ta][k], t[ta][(k + 1) % 3]) == (i, j) for k in range(3))
+        assert any((t[tb][k], t[tb][(k + 1) % 3]) == (j, i) for k in range(3))
+    # an open grid has boundary edges, which are left out
+    gv, gt = _grid(5)
+    gvids, _ = curvature.inner_edges(gv, gt)
+    assert len(gvids) == 56 - 16  # 20 + 20 + 16 diagonals, 16 on the boundary
+
+
+def test_plane_has_zero_curvature():
+    v, t = _grid()
+    u_min, u_max, c_min, c_max, c_mean, c_gauss, normals = curvature.curvature(v, t)
+    for c in (c_min, c_max, c_mean, c_gauss):
+        assert np.abs(c).max() == 0.0
+    np.testing.assert_allclose(np.abs(normals[:, 2]), 1.0, atol=1e-12)
+
+
+def test_sphere_is_umbilic():
+    v, t = meshgen.icosphere(4)
+    u_min, u_max, c_min, c_max, c_mean, c_gauss, normals = curvature.curvature(v, t)
+    assert np.all(c_min <= c_max)
+    assert np.abs(c_max - c_min).max() < 0.05 * np.abs(c_mean).max()
+    # outward normals; the signed dihedral angles make a convex surface negative
+    np.testing.assert_allclose(np.sum(normals * v, axis=1), 1.0, atol=1e-3)
+    assert c_mean.max() < 0
+    # orthonormal frame with u_min x u_max along the normal
+    np.testing.assert_allclose(np.sum(u_min * u_max, axis=1), 0.0, atol=1e-10)
+    np.testing.assert_allclose(np.sum(np.cross(u_min, u_max) * normals, axis=1), 1.0, atol=1e-10)
+
+
+def test_cylinder_axis_is_the_flat_direction():
+    v, t = _cylinder()
+    u_min, u_max, c_min, c_max, c_mean, c_gauss, normals = curvature.curvature(v, t)
+    z = v[:, 2]
+    inner = (z > 0.8) & (z < 2.2)  # away from the open ends
+    # convex (negative) around, flat along the axis
+    assert c_min[inner].max() < 0
+    assert np.abs(c_max[inner]).max() < 0.05 * np.abs(c_min[inner]).min()
+    np.testing.assert_allclose(np.abs(u_max[inner, 2]), 1.0, atol=1e-3)
+    np.testing.assert_allclose(np.abs(u_min[inner, 2]), 0.0, atol=1e-3)
+    # the circumferential and normal directions share the eigenvalue 0 there, so
+    # the vertex frame is arbitrary in that plane; the per-triangle projection
+    # (the input of the anisotropic assembly) makes u_min circumferential
+    u1, u2, c1, c2 = curvature.curvature_tria(v, t)
+    tinner = np.all(inner[t], axis=1)
+    np.testing.assert_allclose(np.abs(u1[tinner, 2]), 0.0, atol=1e-3)
+    np.testing.assert_allclose(np.abs(u2[tinner, 2]), 1.0, atol=1e-3)
+    circ = np.cross(curvature.tria_normals(v, t)[tinner], [0.0, 0.0, 1.0])
+    np.testing.assert_allclose(np.abs(np.sum(u1[tinner] * circ, axis=1)), 1.0, atol=1e-3)
+    assert np.all(c1[tinner] < 0) and np.abs(c2[tinner]).max() < 0.05 * np.abs(c1[tinner]).min()
+
+
+def test_scale_invariant_and_rotation_equivariant():
+    v, t = meshgen.perturbed_sphere(3, np.random.default_rng(4), axes=(1.4, 0.7, 1.0), sigma=0.02)
+    base = curvature.curvature(v, t)
+    scaled = curvature.curvature(2.5 * v, t)
+    for a, b in zip(base[2:6], scaled[2:6]):
+        np.testing.assert_allclose(b, a, rtol=1e-9, atol=1e-12)
+    r = _rotation(9)
+    rot = curvature.curvature(v @ r.T, t)
+    for a, b in zip(base[2:6], rot[2:6]):
+        np.testing.assert_allclose(b, a, rtol=1e-9, atol=1e-12)
+    # directions rotate with the mesh (up to the sign of an eigenvector)
+    sep = np.abs(base[3] - base[2]) > 1e-3 * np.abs(base[3]).max()
+    for k in (0, 1):
+        np.testing.assert_allclose(np.abs(np.sum((base[k] @ r.T) * rot[k], axis=1))[sep], 1.0, atol=1e-6)
+    np.testing.assert_allclose(base[6] @ r.T, rot[6], atol=1e-9)
+
+
+def test_curvature_tria_frame():
+    v, t = meshgen.perturbed_sphere(3, np.random.default_rng(5), axes=(1.2, 0.9, 1.0), sigma=0.02)
+    mesh = TriaMesh(v, t)
+    u1, u2, c1, c2 = mesh.curvature_tria(smoothit=3)
+    tn = mesh.tria_normals()
+    assert u1.shape == u2.shape == (len(t), 3) and c1.shape == c2.shape == (len(t),)
+    np.testing.assert_allclose(np.linalg.norm(u1, axis=1), 1.0, atol=1e-12)
+    np.testing.assert_allclose(np.linalg.norm(u2, axis=1), 1.0, atol=1e-12)
+    np.testing.assert_allclose(np.sum(u1 * tn, axis=1), 0.0, atol=1e-12)
+    np.testing.assert_allclose(np.sum(u1 * u2, axis=1), 0.0, atol=1e-12)
+    np.testing.assert_allclose(np.sum(np.cross(u1, u2) * tn, axis=1), 1.0, atol=1e-12)
+    _, _, cmin, cmax, _, _, _ = mesh.curvature(3)
+    np.testing.assert_allclose(c1, cmin[t].mean(axis=1), rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(c2, cmax[t].mean(axis=1), rtol=1e-12, atol=1e-15)
+
+
+def test_unoriented_mesh_raises():
+    v, t = meshgen.icosphere(1)
+    t = t.copy()
+    t[0] = t[0, ::-1]
+    with pytest.raises(ValueError, match="oriented"):
+        curvature.curvature(v, t)
+
+
+@pytest.mark.gpu
+def test_gpu_solver_aniso_with_own_curvature():
+    """Solver(TriaMesh, aniso=...) runs end to end on a lapy_amd mesh: its
+    curvature_tria feeds the GPU anisotropic assembly, which equals the
+    oracle's fem_tria_aniso on the same inputs (lapy/solver.py:74-93)."""
+    from oracle import fem
+    from lapy_amd import Solver
+
+    print("case: Solver(aniso=(3, 3)) on a perturbed L4 ellipsoid", flush=True)
+    v, t = meshgen.perturbed_sphere(4, np.random.default_rng(6), axes=(1.5, 0.8, 1.0), sigma=0.01)
+    mesh = TriaMesh(v, t)
+    s = Solver(mesh, lump=False, aniso=(3.0, 3.0))
+    u1, u2, c1, c2 = mesh.curvature_tria(smoothit=10)
+    am = np.empty((len(t), 2))
+    am[:, 1] = np.exp(-3.0 * np.abs(c1))
+    am[:, 0] = np.exp(-3.0 * np.abs(c2))
+    a_ref, b_ref = fem.fem_tria_aniso(v, t, u1, u2, am, lump=False)
+    a, b = s.stiffness, s.mass
+    np.testing.assert_array_equal(a.indptr, a_ref.indptr)
+    np.testing.assert_array_equal(a.indices, a_ref.indices)
+    np.testing.assert_allclose(a.data, a_ref.data, rtol=1e-12, atol=1e-14 * np.abs(a_ref.data).max())
+    np.testing.assert_allclose(b.data, b_ref.data, rtol=1e-12, atol=1e-14 * np.abs(b_ref.data).max())
+    # anisotropy changes the operator but keeps A 1 = 0
+    a_iso, _ = fem.fem_tria(v, t)
+    assert np.abs(a.data - a_iso.data).max() > 1e-3 * np.abs(a_iso.data).max()
+    assert np.abs(a @ np.ones(len(v))).max() < 1e-12 * np.abs(a.data).max()
