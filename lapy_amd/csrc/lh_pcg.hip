@@ -81,12 +81,42 @@ __global__ void k_coldot_partial(int64_t n, int m, const double* __restrict__ X,
     if (rg == 0 && c < m)
         part[(int64_t)blockIdx.x * m + c] = ((s[0][threadIdx.x] + s[1][threadIdx.x]) + s[2][threadIdx.x]) + s[3][threadIdx.x];
 }
+// m <= 64 columns: all 256 threads read, thread t at column t % m and row
+// offset t / m, so consecutive threads read consecutive addresses; the block's
+// partial of each column is summed in a fixed order
+__global__ void k_coldot_partial_narrow(int64_t n, int m, const double* __restrict__ X,
+                                        const double* __restrict__ Y, int64_t rpc, double* __restrict__ part) {
+    __shared__ double s[256];
+    const int t = threadIdx.x;
+    const int per = 256 / m;
+    double acc = 0.0;
+    if (t < per * m) {
+        const int c = t % m;
+        const int64_t rb = (int64_t)blockIdx.x * rpc, re = min(rb + rpc, n);
+        for (int64_t i = rb + t / m; i < re; i += per) acc += X[i * m + c] * Y[i * m + c];
+    }
+    s[t] = acc;
+    __syncthreads();
+    if (t < m) {
+        double sum = 0.0;
+        for (int k = 0; k < per; ++k) sum += s[t + k * m];
+        part[(int64_t)blockIdx.x * m + t] = sum;
+    }
+}
+// one 256-thread block per column: strided partial sums, then a fixed-order tree
 __global__ void k_reduce_cols(int nchunk, int m, const double* __restrict__ part, double* __restrict__ out) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m) return;
-    double s = 0.0;
-    for (int k = 0; k < nchunk; ++k) s += part[(int64_t)k * m + c];
-    out[c] = s;
+    __shared__ double s[256];
+    const int c = blockIdx.x;
+    const int t = threadIdx.x;
+    double acc = 0.0;
+    for (int k = t; k < nchunk; k += 256) acc += part[(int64_t)k * m + c];
+    s[t] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s[t] += s[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[c] = s[0];
 }
 
 struct ColDot {
@@ -97,16 +127,20 @@ struct ColDot {
     void init(int64_t n_, int m_) {
         n = n_;
         m = m_;
-        nchunk = std::min<int64_t>(256, std::max<int64_t>(1, n / 512));
+        nchunk = std::min<int64_t>(1024, std::max<int64_t>(1, n / 512));
         rpc = (n + nchunk - 1) / nchunk;
         nchunk = (n + rpc - 1) / rpc;
         part.alloc(nchunk * m * 8);
     }
     void run(lh_ctx* ctx, const double* X, const double* Y, double* out) {
-        dim3 grid((unsigned)nchunk, (m + 63) / 64);
-        k_coldot_partial<<<grid, 256, 0, ctx->stream>>>(n, m, X, Y, rpc, part.as<double>());
+        if (m <= 64) {
+            k_coldot_partial_narrow<<<(unsigned)nchunk, 256, 0, ctx->stream>>>(n, m, X, Y, rpc, part.as<double>());
+        } else {
+            dim3 grid((unsigned)nchunk, (m + 63) / 64);
+            k_coldot_partial<<<grid, 256, 0, ctx->stream>>>(n, m, X, Y, rpc, part.as<double>());
+        }
         LH_CHECK_LAUNCH();
-        k_reduce_cols<<<nblocks(m, 64), 64, 0, ctx->stream>>>((int)nchunk, m, part.as<double>(), out);
+        k_reduce_cols<<<(unsigned)m, 256, 0, ctx->stream>>>((int)nchunk, m, part.as<double>(), out);
         LH_CHECK_LAUNCH();
     }
 };
