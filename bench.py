@@ -34,29 +34,63 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(level: int, k: int):
-    """The oracle (numpy assembly + splu + eigsh, as lapy/solver.py) timed on
-    this host on a bounded sample of the same mesh family."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_eigs_seconds(v, t, k, threads):
+    """One run of the oracle pipeline (numpy assembly, scipy splu + eigsh
+    shift-invert, as lapy/solver.py:106-194 and :667-716) with the BLAS/OpenMP
+    pools limited to `threads` (None: all).  Returns (assembly s, factor+eigsh s)."""
     import threadpoolctl
 
-    from lapy_amd import meshgen
     from oracle import fem
 
-    v, t = meshgen.large_ellipsoid(level=level)
-    with threadpoolctl.threadpool_limits(1):
+    with threadpoolctl.threadpool_limits(threads):
         t0 = time.perf_counter()
         a, b = fem.fem_tria(v, t)
         t1 = time.perf_counter()
-        lam, _ = fem.eigs(a, b, k=k)
+        fem.eigs(a, b, k=k)
         t2 = time.perf_counter()
+    return t1 - t0, t2 - t1
+
+
+def cpu_baseline(level: int, k: int, thread_level: int):
+    """The oracle timed on this host's cores, on the metric's own mesh
+    (icosphere level `level`, one run with BLAS threads = 1: SuperLU, ARPACK
+    and numpy's element arithmetic are single-threaded, which is how LaPy runs).
+    The all-cores leg (BLAS and OpenMP pools unrestricted) is timed beside a
+    second one-thread run on the smaller level `thread_level`, to show what
+    extra threads give the reference's path."""
+    from lapy_amd import meshgen
+
+    v, t = meshgen.large_ellipsoid(level=level)
+    a1, e1 = _oracle_eigs_seconds(v, t, k, 1)
+    vs, ts = meshgen.large_ellipsoid(level=thread_level)
+    sa1, se1 = _oracle_eigs_seconds(vs, ts, k, 1)
+    san, sen = _oracle_eigs_seconds(vs, ts, k, None)
+    ncpu = os.cpu_count()
+    aff = len(os.sched_getaffinity(0))
     return {
-        "value": k / (t2 - t0),
+        "value": round(k / (a1 + e1), 4),
         "unit": "eigenpairs/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"oracle pipeline (numpy assembly + scipy splu + eigsh shift-invert, as lapy Solver.eigs) "
-                   f"on icosphere level {level} ellipsoid ({len(v)} vertices), k={k}, one run, BLAS threads=1; "
-                   f"assembly {t1 - t0:.2f} s, factor+eigsh {t2 - t1:.2f} s; host cpu_count={os.cpu_count()}"),
+        "sample": (f"oracle pipeline (numpy assembly + scipy splu + eigsh shift-invert, as lapy Solver.eigs) on the "
+                   f"metric's mesh, icosphere level {level} ellipsoid ({len(v)} vertices), k={k}, one run, BLAS "
+                   f"threads=1: assembly {a1:.2f} s, factor+eigsh {e1:.2f} s. All-cores check on level "
+                   f"{thread_level} ({len(vs)} vertices): {k / (sa1 + se1):.3f} eigenpairs/s with 1 thread, "
+                   f"{k / (san + sen):.3f} with all {aff} threads. CPU: {_cpu_model()}, os.cpu_count()={ncpu}, "
+                   f"sched_getaffinity={aff}"),
+        "all_cores": {"value": round(k / (san + sen), 4), "threads": aff, "level": thread_level,
+                      "one_thread_value": round(k / (sa1 + se1), 4)},
     }
 
 
@@ -290,8 +324,13 @@ def main():
     ap.add_argument("--poly-degree", type=int, default=0)
     ap.add_argument("--poly-ratio", type=float, default=0.0)
     ap.add_argument("--poly-storage", default="auto", choices=["auto", "f32", "f16"])
-    ap.add_argument("--cpu-level", type=int, default=7)
+    ap.add_argument("--cpu-level", type=int, default=8,
+                    help="icosphere level of the CPU baseline's main run (the metric's mesh: 8)")
+    ap.add_argument("--cpu-thread-level", type=int, default=7,
+                    help="level of the CPU baseline's one-thread vs all-cores comparison")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the PCIe-inclusive and mixed-precision figures of the L8 line")
     ap.add_argument("--verbose", type=int, default=0)
     ap.add_argument("--workload", default="l8", choices=["l8", "small", "tet", "pcie", "batch", "geodesic"],
                     help="l8: the benchmark line (default); small / tet / pcie / batch: the other workloads")
@@ -378,6 +417,56 @@ def main():
         den = np.where(np.abs(gold) < 1e-6 * scale, scale, np.abs(gold))
         eig_err = float(np.max(np.abs(evals - gold) / den))
 
+    # extra figures beside the contract value (not part of it): the same
+    # workload PCIe-inclusive (host mesh arrays in, host eigenpairs out), and
+    # the mixed-precision solve on the same mesh
+    extras = {}
+    if args.steps and not args.no_extras:
+        def pcie_step():
+            mh = _lib.DeviceMesh(ctx, v, t)  # host -> device
+            Ah, Bh = mh.assemble(_lib.LH_TRIA, False, np.float64)
+            return _lib.eigs(ctx, Ah, Bh, args.k, -0.01, precision=prec, tol=args.tol, block=args.block,
+                             poly_degree=args.poly_degree, poly_ratio=args.poly_ratio, poly_storage=storage)
+
+        pcie_step()
+        n_extra = 3
+        t0 = time.perf_counter()
+        for _ in range(n_extra):
+            ev_p, vec_p, st_p = pcie_step()
+        dt = time.perf_counter() - t0
+        extras["pcie_inclusive"] = {
+            "value": round(args.k * n_extra / dt, 4), "unit": "eigenpairs/s", "steps": n_extra,
+            "ms_per_step": round(1000 * dt / n_extra, 2),
+            "what": "host mesh arrays -> device mesh, symbolic + numeric assembly, eigensolve, eigenvalues and "
+                    "eigenvectors downloaded to host memory",
+            "eigenvector_bytes_downloaded": int(vec_p.nbytes),
+            "eig_rel_err_vs_oracle": rel_err(ev_p, gold) if os.path.exists(gold_path) else None}
+        log(f"[bench] pcie-inclusive: {1000 * dt / n_extra:.1f} ms per step")
+        if args.precision == "fp64":
+            def mixed_step():
+                mm = base.clone()
+                Am, Bm = mm.assemble(_lib.LH_TRIA, False, np.float64)
+                ev_m, eh_m, st_m = _lib.eigs(ctx, Am, Bm, args.k, -0.01, precision=_lib.LH_PREC_MIXED,
+                                             keep_device=True)
+                ctx.sync()
+                return ev_m, st_m
+
+            mixed_step()
+            t0 = time.perf_counter()
+            for _ in range(n_extra):
+                ev_m, st_m = mixed_step()
+            dt = time.perf_counter() - t0
+            mixed_err = None
+            if os.path.exists(gold_path):
+                mixed_err = rel_err(ev_m, gold)
+            extras["mixed_precision"] = {
+                "value": round(args.k * n_extra / dt, 4), "unit": "eigenpairs/s", "steps": n_extra,
+                "ms_per_step": round(1000 * dt / n_extra, 2), "lobpcg_iterations": st_m.iterations,
+                "lobpcg_iterations_fp32": st_m.iterations_fp32, "eig_rel_err_vs_oracle": mixed_err,
+                "what": "same workload, precision='mixed' (FP32 phase, then FP64 to the same tolerance)"}
+            log(f"[bench] mixed precision: {1000 * dt / n_extra:.1f} ms per step, "
+                f"{st_m.iterations_fp32}/{st_m.iterations} iterations in FP32")
+
     # rooflines, kernel durations from HIP events on the library's stream
     m = base.clone()
     A, B = m.assemble(_lib.LH_TRIA, False, np.float64)
@@ -414,7 +503,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log(f"[bench] cpu baseline: oracle on level {args.cpu_level}")
-        cpu = cpu_baseline(args.cpu_level, args.k)
+        cpu = cpu_baseline(args.cpu_level, args.k, args.cpu_thread_level)
 
     out = {
         "metric": "eigenpairs/sec (k=50) at lambda rel err <=1e-8 vs FP64 oracle",
@@ -439,6 +528,7 @@ def main():
         "roofline_assembly": roofline_asm,
         "cpu_baseline": cpu,
     }
+    out.update(extras)
     print(json.dumps(out), flush=True)
 
 

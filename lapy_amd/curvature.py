@@ -75,14 +75,18 @@ def inner_edges(v, t):
 
 
 def _smoothing_matrix(v, t):
-    """Row-stochastic neighbour average (``tria_mesh.py:1700-1718``): row i
-    holds ``adj_ij / sum_j adj_ij`` with ``adj_ij`` the number of triangles on
-    edge (i, j). The reference weights both by the vertex area, which cancels;
-    a vertex of zero area keeps a zero row, as there."""
+    """Row-stochastic neighbour average (``tria_mesh.py:1696-1715``): row i
+    holds ``1 / deg_i`` at each of its ``deg_i`` distinct edge neighbours. The
+    edge adjacency is binarised first (``tria_mesh.py:1707``), so an inner
+    edge (two triangles) and a boundary edge (one) weigh the same. The
+    reference weights both by the vertex area, which cancels; a vertex of zero
+    area keeps a zero row, as there."""
     n = v.shape[0]
     i = t[:, [0, 1, 1, 2, 2, 0]].reshape(-1)
     j = t[:, [1, 0, 2, 1, 0, 2]].reshape(-1)
     adj = sparse.csr_matrix((np.ones(i.size), (i, j)), shape=(n, n))
+    adj.sum_duplicates()
+    adj.data[:] = 1.0
     v0, v1, v2 = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
     cr = np.cross(v1 - v0, v2 - v0)
     area = 0.5 * np.sqrt(np.sum(cr * cr, axis=1))

@@ -61,6 +61,29 @@ def subspace_angles(x, y, mass, lam_ref, cluster_rtol=1e-3, complete=None):
     return out
 
 
+def projector_probes(x, mass, lam_ref, idx, nr=4, seed=11, cluster_rtol=1e-3):
+    """Basis-free samples of each cluster's eigenspace.
+
+    For every cluster ``g`` of ``lam_ref`` (see ``clusters``) and ``nr`` seeded
+    standard-normal vectors ``r``, returns ``(P_g r)[idx]`` with the
+    M-orthogonal projector ``P_g = X_g X_g^T M`` onto the span of the cluster's
+    M-orthonormal columns.  ``P_g`` does not depend on the signs or the basis
+    chosen inside the cluster, so two solvers' results can be compared at a
+    few hundred vertices: ``|P_g - P'_g| r`` is bounded by ``2 sin(theta)``
+    times ``|r|_M`` for the largest principal angle ``theta`` between the
+    spaces.  Output: a list of ``(columns, array (nr, len(idx)))``.
+    """
+    n = x.shape[0]
+    r = np.random.default_rng(seed).standard_normal((n, nr))
+    mr = mass @ r
+    out = []
+    for g in clusters(lam_ref, cluster_rtol):
+        xg = x[:, g]
+        p = xg @ (xg.T @ mr)
+        out.append((list(g), np.asarray(p[idx]).T.copy()))
+    return out
+
+
 def sign_normalise(x):
     """Flip each column so that its largest-magnitude entry is positive."""
     x = np.array(x, copy=True)

@@ -7,7 +7,9 @@ runs the oracle pipeline (oracle/fem.py: numpy assembly as lapy/solver.py:
 lapy_amd.meshgen.large_ellipsoid(level=LEVEL) and stores the K smallest
 eigenvalues, eigenvector probes and timings in
 tests/golden/ellipsoid_l{LEVEL}_k{K}.json.  Probes: the sign-normalised
-eigenvectors at 256 seeded vertex indices.
+eigenvectors at 256 seeded vertex indices, and per eigenvalue cluster the
+basis-free projections P_g r = X_g X_g^T B r of 4 seeded normal vectors r
+(seed 11) at the same vertices (oracle.metrics.projector_probes).
 """
 
 import json
@@ -39,6 +41,9 @@ def main(level: int, k: int):
         "evals": lam.tolist(),
         "probe_index": idx.tolist(),
         "probe_vectors": vs[idx].tolist(),
+        "cluster_probe_seed": 11,
+        "cluster_probes": [{"cols": g, "values": p.tolist()}
+                           for g, p in metrics.projector_probes(vec, b, lam, idx, seed=11)],
         "oracle_seconds": {"assembly": t1 - t0, "splu_eigsh": t2 - t1},
     }
     path = os.path.join(HERE, f"ellipsoid_l{level}_k{k}.json")

@@ -7,7 +7,9 @@ cube, 6 tetrahedra per cell in the split of the reference's cubeTetra.vtk,
 interior vertices jittered U(-0.15 h, 0.15 h)):
   * eigenpairs: numpy tetrahedral assembly (lapy/solver.py:380-533), splu +
     eigsh shift-invert (lapy/solver.py:667-716), the K smallest eigenvalues
-    and sign-normalised eigenvector probes at 256 seeded vertices;
+    and sign-normalised eigenvector probes at 256 seeded vertices, plus per
+    eigenvalue cluster the basis-free projections P_g r = X_g X_g^T B r of 4
+    seeded normal vectors r (seed 11) at the same vertices;
   * heat diffusion from vertex 0 with t = 1e-3 (lapy/heat.py:114-232, i.e.
     m = 1e-3 / avg_edge_length^2), values at the same probe vertices.
 Output: tests/golden/volume_c{CELLS}_k{K}.json.
@@ -47,6 +49,9 @@ def main(cells: int, k: int):
         "evals": lam.tolist(),
         "probe_index": idx.tolist(),
         "probe_vectors": vs[idx].tolist(),
+        "cluster_probe_seed": 11,
+        "cluster_probes": [{"cols": g, "values": p.tolist()}
+                           for g, p in metrics.projector_probes(vec, b, lam, idx, seed=11)],
         "avg_edge_length": ael,
         "heat_m": m,
         "heat_probe_values": u[idx].tolist(),

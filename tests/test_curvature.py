@@ -167,3 +167,51 @@ def test_gpu_solver_aniso_with_own_curvature():
     a_iso, _ = fem.fem_tria(v, t)
     assert np.abs(a.data - a_iso.data).max() > 1e-3 * np.abs(a_iso.data).max()
     assert np.abs(a @ np.ones(len(v))).max() < 1e-12 * np.abs(a.data).max()
+
+
+def _reference_smoothing(v, t):
+    """The smoothing operator of lapy/tria_mesh.py:1696-1715 written out step by
+    step: the symmetric edge adjacency of :486-509 (edge counts), binarised,
+    each row scaled by the vertex area of :715-733, rows normalised to sum 1
+    (a zero row sum kept as a zero row)."""
+    n = v.shape[0]
+    i = np.column_stack((t[:, 0], t[:, 1], t[:, 1], t[:, 2], t[:, 2], t[:, 0])).reshape(-1)
+    j = np.column_stack((t[:, 1], t[:, 0], t[:, 2], t[:, 1], t[:, 0], t[:, 2])).reshape(-1)
+    adj = np.zeros((n, n))
+    np.add.at(adj, (i, j), 1.0)
+    counts = adj.copy()
+    adj[adj != 0] = 1.0
+    cr = np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]])
+    area = 0.5 * np.sqrt(np.sum(cr * cr, axis=1))
+    varea = np.bincount(t.reshape(-1), np.repeat(area, 3), minlength=n) / 3.0
+    adj2 = adj * varea[:, None]
+    rowsum = adj2.sum(axis=1, keepdims=True)
+    rowsum[rowsum == 0] = 1.0
+    return adj2 / rowsum, counts
+
+
+def test_smoothing_matrix_open_mesh_matches_reference_steps():
+    """On an open mesh (square-mesh.off lifted by a height field) the boundary
+    edges lie on one triangle and the inner edges on two; the reference
+    binarises the adjacency, so every neighbour weighs the same.  The
+    smoothing matrix equals the reference's steps at every vertex, the
+    boundary included, and so do smoothed curvatures derived from it."""
+    from conftest import data_path
+    from lapy_amd.mesh import _read_off
+
+    v, t = _read_off(data_path("square-mesh.off"))
+    v = v.astype(np.float64)
+    v[:, 2] = 0.15 * np.sin(3 * v[:, 0]) * np.cos(2 * v[:, 1])
+    s = curvature._smoothing_matrix(v, t).toarray()
+    ref, counts = _reference_smoothing(v, t)
+    assert set(np.unique(counts)) == {0.0, 1.0, 2.0}  # boundary and inner edges both present
+    np.testing.assert_allclose(s, ref, rtol=0, atol=1e-15)
+    # planted mistake: weighting neighbours by their edge counts differs at the boundary
+    bad = counts / counts.sum(axis=1, keepdims=True)
+    assert np.abs(bad - ref).max() > 1e-2
+    # a vertex function smoothed ten times agrees too
+    f = np.random.default_rng(5).standard_normal(len(v))
+    fs, fr = f.copy(), f.copy()
+    for _ in range(10):
+        fs, fr = s @ fs, ref @ fr
+    np.testing.assert_allclose(fs, fr, rtol=1e-13, atol=1e-15)

@@ -66,6 +66,75 @@ def test_aniso_isotropic_reduces_to_cotan():
     np.testing.assert_allclose(b1.toarray(), b0.toarray(), atol=1e-15)
 
 
+def _planar_aniso_energy(v, t, u1, u2, am, g):
+    """sum_T area_T g^T K_T g for the linear function f = g . (x, y), with the
+    per-triangle tensor K_T = am[T,1] u1 u1^T + am[T,0] u2 u2^T.
+
+    On a linear element the reference's a_ij = (R^T e_i)^T D (R^T e_j) / vol
+    (lapy/solver.py:259-280, D = diag(aniso_mat[T]) in the basis (u1, u2),
+    e_i the edge opposite vertex i and R the in-plane quarter turn with
+    grad(phi_i) = R e_i / (2 area)) is the integral of grad(phi_i)^T K
+    grad(phi_j) with K = R D R^T: the quarter turn swaps the axes, so the
+    weight aniso_mat[:,1] acts along u1 and aniso_mat[:,0] along u2.  That is
+    the pairing lapy/solver.py:88-89 relies on: column 1 = exp(-a1 |c1|)
+    weighs diffusion along u1, the direction of c1."""
+    e1 = v[t[:, 1]] - v[t[:, 0]]
+    e2 = v[t[:, 2]] - v[t[:, 0]]
+    area = 0.5 * np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0])
+    g1 = u1[:, :2] @ g
+    g2 = u2[:, :2] @ g
+    return float(np.sum(area * (am[:, 1] * g1 * g1 + am[:, 0] * g2 * g2)))
+
+
+@pytest.mark.parametrize("varying", [False, True])
+def test_aniso_planar_closed_form(varying):
+    """Pins the anisotropic weighting of lapy/solver.py:259-280, which the
+    isotropic reduction leaves open: on the planar square-mesh.off with an
+    in-plane orthonormal basis (u1, u2) per triangle and weights aniso_mat,
+    f^T A f of a linear function f = g . (x, y) is the exact energy
+    sum_T area_T g^T K_T g, K_T = aniso_mat[T,1] u1 u1^T + aniso_mat[T,0] u2 u2^T
+    (derivation in _planar_aniso_energy).  With u1 = x, u2 = y and constant
+    (alpha, beta): x^T A x = beta * area, y^T A y = alpha * area, x^T A y = 0.
+    Per-triangle random angles and weights pin the row indexing too; a
+    swapped weight column and a swapped basis pairing must fail."""
+    v, t = _read_off(data_path("square-mesh.off"))
+    v = v.astype(np.float64)
+    nt = len(t)
+    rng = np.random.default_rng(17)
+    if varying:
+        phi = rng.uniform(0, np.pi, nt)
+        am = rng.uniform(0.2, 3.0, (nt, 2))
+    else:
+        phi = np.zeros(nt)
+        am = np.tile([0.3, 1.7], (nt, 1))
+    z = np.zeros(nt)
+    u1 = np.column_stack((np.cos(phi), np.sin(phi), z))
+    u2 = np.column_stack((-np.sin(phi), np.cos(phi), z))
+    a, _ = fem.fem_tria_aniso(v, t, u1, u2, am)
+    gs = [np.array([1.0, 0.0]), np.array([0.0, 1.0]), np.array([0.6, -1.3])]
+
+    def energies(aa):
+        out = []
+        for g in gs:
+            f = v[:, :2] @ g
+            out.append(f @ (aa @ f))
+        return np.array(out)
+
+    exp = np.array([_planar_aniso_energy(v, t, u1, u2, am, g) for g in gs])
+    np.testing.assert_allclose(energies(a), exp, rtol=1e-12)
+    if not varying:
+        area = 1.0  # the unit square
+        x, y = v[:, 0], v[:, 1]
+        assert x @ (a @ x) == pytest.approx(1.7 * area, rel=1e-12)
+        assert y @ (a @ y) == pytest.approx(0.3 * area, rel=1e-12)
+        assert abs(x @ (a @ y)) < 1e-12
+    # planted mistakes: weight columns swapped, basis vectors swapped
+    a_sw, _ = fem.fem_tria_aniso(v, t, u1, u2, am[:, ::-1].copy())
+    a_bs, _ = fem.fem_tria_aniso(v, t, u2, u1, am)
+    for bad in (a_sw, a_bs):
+        assert np.max(np.abs(energies(bad) - exp) / np.abs(exp)) > 1e-3
+
+
 # ---------------------------------------------------------------- invariants
 @pytest.mark.parametrize("lump", [False, True])
 def test_tria_invariants(lump):
