@@ -303,6 +303,23 @@ int lh_block_gram(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const
                   int flags, double* C, int iters, double* avg_us);
 int lh_block_gemm(lh_ctx* ctx, int64_t n, int p, int q, const double* U, int ldu, const double* Cs, double* Y,
                   int ldy, double alpha, double beta, int flags, int iters, double* avg_us);
+/* lh_block_gram without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  U (n x p,
+ * leading dimension ldu), V (n x q, ldv) and C (p x q, row-major, contiguous)
+ * are DEVICE pointers to FP64 blocks; flags: 0 or LH_GRAM_SYM.  The product
+ * is launched on the context's stream and the call returns without waiting
+ * (lh_sync, or any later synchronising call, completes it).  The per-wave
+ * partial tiles live in one fixed 16 MiB device array per GPU
+ * (512 tiles of 64 x 64 doubles), so p and q are limited to
+ * LH_GRAM_NOALLOC_MAX_COLS (larger: LH_ERR_ARG); n is not limited (more rows
+ * per wave).  All calls on one GPU share that array: calls that overlap on
+ * one GPU (several contexts' streams or threads at once) race.  The rows are
+ * split over fewer waves than in lh_block_gram, so the two agree to rounding,
+ * not bit for bit. */
+#define LH_GRAM_NOALLOC_MAX_COLS 1024
+int lh_block_gram_noalloc(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q,
+                          int flags, double* C);
 /* The Rayleigh-Ritz eigensolver on its own (no counterpart in the reference;
  * exposed for its tests and timing): smallest nev eigenpairs of the symmetric
  * N x N host matrix H (N <= 192).  evals (nev), Z (N x nev row-major); when

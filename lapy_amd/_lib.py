@@ -121,6 +121,7 @@ SIGNATURES = {
     "lh_mpnn_loss_grad": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp, C.c_int, _dp, _vp]),
     "lh_block_gram": (C.c_int, [_vp, C.c_int64, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
                                 _dp]),
+    "lh_block_gram_noalloc": (C.c_int, [_vp, C.c_int64, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp]),
     "lh_block_gemm": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, C.c_double,
                                 C.c_double, C.c_int, C.c_int, _dp]),
 }
@@ -531,6 +532,7 @@ def time_cheb_step(ctx: Context, A: DeviceCSR, B: DeviceCSR, ncol: int, iters: i
 
 
 LH_GRAM_SYM, LH_BLOCK_F32 = 1, 4
+LH_GRAM_NOALLOC_MAX_COLS = 1024  # lh_block_gram_noalloc's column limit (its fixed scratch)
 
 
 def block_gram(ctx: Context, U: np.ndarray, V: np.ndarray, p: int, q: int, sym: bool = False, iters: int = 0,

@@ -189,6 +189,7 @@ int lh_ctx_create(int device, lh_ctx** out) {
         hipDeviceProp_t prop;
         HIPX(hipGetDeviceProperties(&prop, device));
         ctx->n_cu = prop.multiProcessorCount;
+        noalloc_scratch_init(device);
         ctx->host_small_bytes = 1 << 20;
         HIPX(hipHostMalloc(&ctx->host_small, ctx->host_small_bytes, hipHostMallocDefault));
     } catch (const lh_error& e) {

@@ -143,6 +143,11 @@ struct lh_ctx {
 // frees ctx->blas (lh_rrlib.hip)
 void release_blas(lh_ctx* ctx);
 
+// device address of lh_block_gram_noalloc's fixed scratch on `device`, looked
+// up once per device with that device current (lh_mfma.hip; called by
+// lh_ctx_create)
+void noalloc_scratch_init(int device);
+
 // CSR pattern (rows sorted, int32), shared by A and consistent B.
 struct Pattern {
     int64_t n = 0, nnz = 0;

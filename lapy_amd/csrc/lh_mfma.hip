@@ -23,6 +23,8 @@
 // matching 4 rows of C (L2-resident) as B.
 #include "lh_dense.hpp"
 
+#include <mutex>
+
 namespace lh {
 
 namespace {
@@ -302,32 +304,59 @@ bool vec_ok(const T* p, int ld) {
 
 }  // namespace
 
+namespace {
+
+// row split of a Gram: nw waves per output tile, rpw rows each (a multiple of
+// 16); part holds nw * ntile partial tiles of 64 x 64 doubles
+struct GramPlan {
+    int tv = 0, ntile = 0;
+    int64_t nw = 0, rpw = 0;
+};
+
+// about two waves per SIMD over all tiles, at least 64 rows per wave;
+// max_parts > 0 caps nw * ntile (the fixed scratch of lh_block_gram_noalloc)
+GramPlan gram_plan(const lh_ctx* ctx, int64_t n, int p, int q, bool sym, int64_t max_parts) {
+    GramPlan g;
+    const int tu = (p + 63) / 64;
+    g.tv = (q + 63) / 64;
+    g.ntile = sym ? tu * (tu + 1) / 2 : tu * g.tv;
+    const int64_t target = (int64_t)ctx->n_cu * 8;
+    int64_t nw = std::max<int64_t>(1, target / g.ntile);
+    nw = std::min<int64_t>(nw, (n + 63) / 64);
+    if (max_parts > 0) nw = std::min<int64_t>(nw, std::max<int64_t>(1, max_parts / g.ntile));
+    int64_t rpw = (n + nw - 1) / nw;
+    rpw = ((rpw + 15) / 16) * 16;
+    g.rpw = rpw;
+    g.nw = (n + rpw - 1) / rpw;  // <= the nw above
+    return g;
+}
+
+template <typename T>
+void gram_launch(lh_ctx* ctx, const GramPlan& g, int64_t n, const T* U, int ldu, int p, const T* V, int ldv, int q,
+                 bool sym, double* part, double* C, int ldc, double alpha) {
+    const bool vec = vec_ok(U, ldu) && vec_ok(V, ldv);
+    dim3 grid((unsigned)((g.nw + 3) / 4), (unsigned)g.ntile);
+    if (vec)
+        k_gram_mfma<true, T><<<grid, 256, 0, ctx->stream>>>(n, g.rpw, (int)g.nw, U, ldu, p, V, ldv, q, g.tv,
+                                                            sym ? 1 : 0, part);
+    else
+        k_gram_mfma<false, T><<<grid, 256, 0, ctx->stream>>>(n, g.rpw, (int)g.nw, U, ldu, p, V, ldv, q, g.tv,
+                                                             sym ? 1 : 0, part);
+    LH_CHECK_LAUNCH();
+    k_gram_mfma_reduce<<<nblocks((int64_t)g.ntile * 4096, 64), 512, 0, ctx->stream>>>(
+        (int)g.nw, g.ntile, p, q, g.tv, sym ? 1 : 0, part, C, ldc, alpha);
+    LH_CHECK_LAUNCH();
+}
+
+}  // namespace
+
 template <typename T>
 void gram_mfma(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, int ldv, int q, double* C, int ldc,
                DevBuf& ws, double alpha, bool sym) {
     if (sym && p != q) LH_THROW(LH_ERR_ARG, "symmetric Gram needs p == q");
-    const int tu = (p + 63) / 64, tv = (q + 63) / 64;
-    const int ntile = sym ? tu * (tu + 1) / 2 : tu * tv;
-    // about two waves per SIMD over all tiles, at least 64 rows per wave
-    const int64_t target = (int64_t)ctx->n_cu * 8;
-    int64_t nw = std::max<int64_t>(1, target / ntile);
-    nw = std::min<int64_t>(nw, (n + 63) / 64);
-    int64_t rpw = (n + nw - 1) / nw;
-    rpw = ((rpw + 15) / 16) * 16;
-    nw = (n + rpw - 1) / rpw;
-    ws.ensure((size_t)nw * ntile * 4096 * sizeof(double));
-    const bool vec = vec_ok(U, ldu) && vec_ok(V, ldv);
-    dim3 grid((unsigned)((nw + 3) / 4), (unsigned)ntile);
-    if (vec)
-        k_gram_mfma<true, T><<<grid, 256, 0, ctx->stream>>>(n, rpw, (int)nw, U, ldu, p, V, ldv, q, tv, sym ? 1 : 0,
-                                                            ws.as<double>());
-    else
-        k_gram_mfma<false, T><<<grid, 256, 0, ctx->stream>>>(n, rpw, (int)nw, U, ldu, p, V, ldv, q, tv, sym ? 1 : 0,
-                                                             ws.as<double>());
-    LH_CHECK_LAUNCH();
-    k_gram_mfma_reduce<<<nblocks((int64_t)ntile * 4096, 64), 512, 0, ctx->stream>>>(
-        (int)nw, ntile, p, q, tv, sym ? 1 : 0, ws.as<double>(), C, ldc, alpha);
-    LH_CHECK_LAUNCH();
+    const GramPlan g = gram_plan(ctx, n, p, q, sym, 0);
+    ws.ensure((size_t)g.nw * g.ntile * 4096 * sizeof(double));
+    gram_launch<T>(ctx, g, n, U, ldu, p, V, ldv, q, sym, ws.as<double>(), C, ldc, alpha);
 }
 
 template <typename T>
@@ -392,9 +421,39 @@ template void gemm_mfma<float>(lh_ctx*, int64_t, int, int, const float*, int, co
 template void gemm_mfma_inplace<double>(lh_ctx*, int64_t, int, double*, int, const double*, int, double, bool);
 template void gemm_mfma_inplace<float>(lh_ctx*, int64_t, int, float*, int, const double*, int, double, bool);
 
+// Scratch of lh_block_gram_noalloc: the per-wave partial tiles of its Gram, a
+// fixed device array (each device holds its own copy while the process lives).
+// Host code never passes the array's name to a kernel (on the host it is only a
+// stand-in for the symbol): every use goes through the device address that
+// hipGetSymbolAddress returned for the current device.
+constexpr int64_t NOALLOC_PART_TILES = 512;  // 512 x 64 x 64 doubles = 16 MiB
+static __device__ __align__(256) double scratch[NOALLOC_PART_TILES * 4096];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+static double* noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return (device >= 0 && device < NOALLOC_MAX_DEVICES) ? g_scratch_ptr[device] : nullptr;
+}
+
 }  // namespace lh
 
 using namespace lh;
+
+void noalloc_scratch_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        LH_THROW(LH_ERR_ARG, "device %d beyond the %d devices of the no-allocation scratch", device,
+                 NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    HIPX(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));  // the current device's copy
+    g_scratch_ptr[device] = static_cast<double*>(p);
+}
 
 namespace {
 
@@ -476,6 +535,27 @@ int lh_block_gram(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const
     if (sym && p != q) LH_THROW(LH_ERR_ARG, "symmetric Gram needs p == q");
     if (f32) block_gram<float>(ctx, n, U, ldu, p, V, ldv, q, sym, C, iters, avg_us);
     else block_gram<double>(ctx, n, U, ldu, p, V, ldv, q, sym, C, iters, avg_us);
+    LH_API_END(ctx)
+}
+
+int lh_block_gram_noalloc(lh_ctx* ctx, int64_t n, const double* U, int ldu, int p, const double* V, int ldv, int q,
+                          int flags, double* C) {
+    if (!ctx || !U || !V || !C || n <= 0 || p <= 0 || q <= 0 || ldu < p || ldv < q) return LH_ERR_ARG;
+    LH_API_BEGIN(ctx)
+    if (flags & ~LH_GRAM_SYM) LH_THROW(LH_ERR_ARG, "lh_block_gram_noalloc: FP64 blocks only (flags %d)", flags);
+    if (p > LH_GRAM_NOALLOC_MAX_COLS || q > LH_GRAM_NOALLOC_MAX_COLS)
+        LH_THROW(LH_ERR_ARG, "lh_block_gram_noalloc: %d x %d Gram exceeds its fixed scratch (at most %d columns)", p,
+                 q, LH_GRAM_NOALLOC_MAX_COLS);
+    const bool sym = (flags & LH_GRAM_SYM) != 0;
+    if (sym && p != q) LH_THROW(LH_ERR_ARG, "symmetric Gram needs p == q");
+    HIPX(hipSetDevice(ctx->device));
+    double* part = noalloc_scratch(ctx->device);
+    if (!part) LH_THROW(LH_ERR_HIP, "no-allocation scratch of device %d not set up", ctx->device);
+    const GramPlan g = gram_plan(ctx, n, p, q, sym, NOALLOC_PART_TILES);
+    if (g.nw * g.ntile > NOALLOC_PART_TILES)
+        LH_THROW(LH_ERR_ARG, "lh_block_gram_noalloc: %lld partial tiles exceed the scratch's %lld",
+                 (long long)(g.nw * g.ntile), (long long)NOALLOC_PART_TILES);
+    gram_launch<double>(ctx, g, n, U, ldu, p, V, ldv, q, sym, part, C, q, 1.0);
     LH_API_END(ctx)
 }
 
