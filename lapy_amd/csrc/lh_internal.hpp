@@ -148,12 +148,18 @@ void release_blas(lh_ctx* ctx);
 // lh_ctx_create)
 void noalloc_scratch_init(int device);
 
+namespace lh {
+struct Reorder;
+}
 // CSR pattern (rows sorted, int32), shared by A and consistent B.
 struct Pattern {
     int64_t n = 0, nnz = 0;
     DevBuf rowptr;   // n+1 int32
     DevBuf cols;     // nnz int32
     DevBuf diagpos;  // n int32 (-1: row without diagonal)
+    // locality reordering of this pattern, built by the first solve that
+    // wants it and kept for later solves on the same pattern
+    std::shared_ptr<lh::Reorder> ro;
 };
 
 struct lh_csr {

@@ -5,15 +5,35 @@
 // A_ff x_f = b_f after removing Dirichlet rows/columns).  The right-hand
 // sides are solved together as a row-major block (one CG recurrence per
 // column).  The operator K is formed once on the device on A's pattern
-// (B diagonal or on the same pattern); Dirichlet rows/columns are removed by
-// masking, not by slicing.  With alpha == 0 and no Dirichlet rows (pure
-// Neumann), A is singular with the constants as null space when its row sums
-// vanish to rounding: the right-hand side is then projected onto the
-// complement of the constants and the mean-free solution is returned (SuperLU
-// returns an arbitrary member of the solution set).  Element matrices computed
-// from float32 coordinates have row sums of ~1e-8 of the diagonal; such an A is
-// regular (its smallest eigenvalue ~1e-9) and is solved as it stands, as
-// SuperLU does, with no projection.
+// (B diagonal or on the same pattern), with the Dirichlet rows and columns
+// zeroed in it, so the iteration never reads the mask: a zero row and a zero
+// Jacobi weight keep r, z, p and q at zero there.  With alpha == 0 and no
+// Dirichlet rows (pure Neumann), A is singular with the constants as null
+// space when its row sums vanish to rounding: the right-hand side is then
+// projected onto the complement of the constants, each iteration removes the
+// residual's mean before updating it (so rounding cannot build up a
+// component CG cannot reduce), and the mean-free solution is returned
+// (SuperLU returns an arbitrary member of the solution set).  Element matrices
+// computed from float32 coordinates have row sums of ~1e-8 of the diagonal;
+// such an A is regular (its smallest eigenvalue ~1e-9) and is solved as it
+// stands, as SuperLU does, with no projection.
+//
+// One iteration is two launches (Jacobi or no preconditioner):
+//   k_cg_dir:    q = K z + beta q, p = z + beta p (so q = K p without a
+//                second product), block partials of p.q; the last block to
+//                finish sums the partials in a fixed order and forms alpha;
+//   k_cg_update: x += alpha p, r -= alpha q, z = D^-1 r, block partials of
+//                r.z and r.r; the last block forms beta, retires converged
+//                columns and sets a done flag when none is left.
+// K z is a CSR product with LPR lanes per row, the row's entries split over
+// the lanes and summed by lane shuffles.  Systems of 4,096 rows or more are
+// solved in the breadth-first locality order of the pattern (built once per
+// pattern and kept with it), and each workgroup walks one contiguous range of
+// rows, with the ranges of one XCD adjacent: the rows gathered by a range
+// then sit in that XCD's L2.  Sums over blocks are always taken in block
+// order, so results do not depend on scheduling.  The host checks the done
+// flag once per chunk of iterations; iterations enqueued after convergence
+// exit at once.
 #include "lh_dense.hpp"
 
 #include <chrono>
@@ -22,6 +42,10 @@
 using namespace lh;
 
 namespace {
+
+constexpr int kBS = 256;    // threads per block of the set-up kernels
+constexpr int kIBS = 1024;  // threads per block of the iteration kernels
+constexpr int kMC = 8;      // right-hand sides per launch (wider blocks: several launches)
 
 // K = beta * A + alpha * B on A's pattern
 template <typename VT>
@@ -40,172 +64,616 @@ __global__ void k_add_diag(int64_t n, const int32_t* __restrict__ diagpos, const
     if (diagpos[i] < 0) { atomicOr(missing, 1); return; }
     kv[diagpos[i]] += alpha * (double)d[i];
 }
-
-// Dirichlet-masked SpMM on K: y = mask (K (mask x))
-__global__ void k_spmv_masked(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
-                              const double* __restrict__ kv, const uint8_t* __restrict__ dm, const double* __restrict__ X,
-                              double* __restrict__ Y, int nrhs) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t row = gid / 16;
-    const int lane = (int)(gid % 16);
-    if (row >= n) return;
-    if (dm && dm[row]) {
-        for (int c = lane; c < nrhs; c += 16) Y[row * nrhs + c] = 0.0;
-        return;
-    }
-    const int32_t r0 = rowptr[row], r1 = rowptr[row + 1];
-    for (int c = lane; c < nrhs; c += 16) {
-        double acc = 0.0;
-        for (int32_t s = r0; s < r1; ++s) {
-            int32_t j = cols[s];
-            if (dm && dm[j]) continue;
-            acc += kv[s] * X[(int64_t)j * nrhs + c];
-        }
-        Y[row * nrhs + c] = acc;
-    }
+// Dirichlet rows and columns of K set to zero (row i: all of it; else the
+// entries in Dirichlet columns)
+__global__ void k_mask_k(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+                         const uint8_t* __restrict__ dm, double* __restrict__ kv) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool drow = dm[i] != 0;
+    for (int32_t s = rowptr[i]; s < rowptr[i + 1]; ++s)
+        if (drow || dm[cols[s]]) kv[s] = 0.0;
+}
+// Y = mask ? 0 : X (rows of a row-major n x m block)
+__global__ void k_mask_rows(int64_t n, int m, const uint8_t* __restrict__ dm, const double* __restrict__ X,
+                            double* __restrict__ Y) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * m) return;
+    Y[g] = (dm && dm[g / m]) ? 0.0 : X[g];
+}
+// no preconditioner: weight 1, except 0 where the Jacobi weight is 0
+// (Dirichlet rows, whose row of K is zero)
+__global__ void k_dinv_unit(int64_t n, double* __restrict__ dinv) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dinv[i] = dinv[i] != 0.0 ? 1.0 : 0.0;
 }
 
-// per-column partial dot products of two blocks, deterministic chunks
-__global__ void k_coldot_partial(int64_t n, int m, const double* __restrict__ X, const double* __restrict__ Y,
-                                 int64_t rpc, double* __restrict__ part) {
-    const int c = threadIdx.x % 64 + 64 * blockIdx.y;
-    const int rg = threadIdx.x / 64;
-    __shared__ double s[4][64];
-    double acc = 0.0;
-    if (c < m) {
-        int64_t rb = (int64_t)blockIdx.x * rpc, re = min(rb + rpc, n);
-        for (int64_t i = rb + rg; i < re; i += 4) acc += X[i * m + c] * Y[i * m + c];
-    }
-    s[rg][threadIdx.x % 64] = acc;
+// ---- block reductions -------------------------------------------------------
+// Blocks exchange their partial sums through global memory between kernels
+// (each kernel's partials are read by the next kernel), never inside one
+// kernel: a device-scope fence per block (the usual last-block pattern) makes
+// every block write back its L2 and costs more than the work on this GPU.
+
+// sum of v over the block's threads in a fixed order (wave shuffles, then the
+// waves in order); every thread gets the result
+__device__ __forceinline__ double block_allsum(double v, double* sh /* blockDim / 64 */) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    __syncthreads();  // sh free (an earlier call's readers are done)
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (rg == 0 && c < m)
-        part[(int64_t)blockIdx.x * m + c] = ((s[0][threadIdx.x] + s[1][threadIdx.x]) + s[2][threadIdx.x]) + s[3][threadIdx.x];
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
+    return t;
 }
-// m <= 64 columns: all 256 threads read, thread t at column t % m and row
-// offset t / m, so consecutive threads read consecutive addresses; the block's
-// partial of each column is summed in a fixed order
-__global__ void k_coldot_partial_narrow(int64_t n, int m, const double* __restrict__ X,
-                                        const double* __restrict__ Y, int64_t rpc, double* __restrict__ part) {
-    __shared__ double s[256];
-    const int t = threadIdx.x;
-    const int per = 256 / m;
-    double acc = 0.0;
-    if (t < per * m) {
-        const int c = t % m;
-        const int64_t rb = (int64_t)blockIdx.x * rpc, re = min(rb + rpc, n);
-        for (int64_t i = rb + t / m; i < re; i += per) acc += X[i * m + c] * Y[i * m + c];
-    }
-    s[t] = acc;
-    __syncthreads();
-    if (t < m) {
-        double sum = 0.0;
-        for (int k = 0; k < per; ++k) sum += s[t + k * m];
-        part[(int64_t)blockIdx.x * m + t] = sum;
-    }
-}
-// one 256-thread block per column: strided partial sums, then a fixed-order tree
-__global__ void k_reduce_cols(int nchunk, int m, const double* __restrict__ part, double* __restrict__ out) {
-    __shared__ double s[256];
-    const int c = blockIdx.x;
-    const int t = threadIdx.x;
-    double acc = 0.0;
-    for (int k = t; k < nchunk; k += 256) acc += part[(int64_t)k * m + c];
-    s[t] = acc;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) s[t] += s[t + w];
-        __syncthreads();
-    }
-    if (t == 0) out[c] = s[0];
+// sum over nparts partials part[b * m + c] (b = 0 .. nparts-1), the same
+// fixed order in every block; every thread gets the result
+__device__ __forceinline__ double allsum_partials(const double* __restrict__ part, int nparts, int m, int c,
+                                                  double* sh) {
+    double t = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) t += part[(int64_t)b * m + c];
+    return block_allsum(t, sh);
 }
 
-struct ColDot {
-    int64_t n;
-    int m;
-    int64_t nchunk, rpc;
-    DevBuf part;
-    void init(int64_t n_, int m_) {
-        n = n_;
-        m = m_;
-        nchunk = std::min<int64_t>(1024, std::max<int64_t>(1, n / 512));
-        rpc = (n + nchunk - 1) / nchunk;
-        nchunk = (n + rpc - 1) / rpc;
-        part.alloc(nchunk * m * 8);
-    }
-    void run(lh_ctx* ctx, const double* X, const double* Y, double* out) {
-        if (m <= 64) {
-            k_coldot_partial_narrow<<<(unsigned)nchunk, 256, 0, ctx->stream>>>(n, m, X, Y, rpc, part.as<double>());
-        } else {
-            dim3 grid((unsigned)nchunk, (m + 63) / 64);
-            k_coldot_partial<<<grid, 256, 0, ctx->stream>>>(n, m, X, Y, rpc, part.as<double>());
-        }
-        LH_CHECK_LAUNCH();
-        k_reduce_cols<<<(unsigned)m, 256, 0, ctx->stream>>>((int)nchunk, m, part.as<double>(), out);
-        LH_CHECK_LAUNCH();
-    }
+// CG state on the device (m columns).  Iteration k reads r_k.z_k and r_k.r_k
+// from the partial buffers of parity k % 2 and the update writes those of
+// r_{k+1} into parity (k + 1) % 2, so a kernel never reads partials that
+// blocks of the same kernel are writing.
+struct CgState {
+    const double* tol2;  // rtol^2 |b|^2 per column
+    double* rz[2];       // partials of r.z (nb_upd x m)
+    double* rr[2];       // partials of r.r (nb_upd x m)
+    double* pq;          // partials of p.q (nb_dir x m)
+    double* sr;          // partials of sum(r) (nb_dir x m, singular systems)
+    int nb_upd, nb_dir;
+    int* status;         // [0]: done, [1]: iterations done (written by block 0)
+    double* rr_out;      // r.r of the latest iterate (written by block 0)
 };
 
-// a = rz / pq per column (0 where converged/inactive)
-__global__ void k_alpha(int m, const double* __restrict__ rz, const double* __restrict__ pq,
-                        const int* __restrict__ active, double* __restrict__ a) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m) return;
-    a[c] = (active[c] && pq[c] != 0.0) ? rz[c] / pq[c] : 0.0;
-}
-// x += a p ; r -= a q
-__global__ void k_update_xr(int64_t n, int m, const double* __restrict__ a, const double* __restrict__ p,
-                            const double* __restrict__ q, double* __restrict__ x, double* __restrict__ r) {
-    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n * m) return;
-    int c = (int)(gid % m);
-    x[gid] += a[c] * p[gid];
-    r[gid] -= a[c] * q[gid];
-}
-// z = dinv * r (masked; optional projection handled separately)
-__global__ void k_precond(int64_t n, int m, const double* __restrict__ dinv, const uint8_t* __restrict__ dm,
-                          const double* __restrict__ r, double* __restrict__ z) {
-    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n * m) return;
-    int64_t i = gid / m;
-    z[gid] = (dm && dm[i]) ? 0.0 : dinv[i] * r[gid];
-}
-// beta = rz_new / rz_old ; p = z + beta p ; rz_old = rz_new
-// z = 0 on Dirichlet rows
-__global__ void k_mask_rows(int64_t n, int m, const uint8_t* __restrict__ dm, double* __restrict__ z) {
-    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n * m) return;
-    if (dm[gid / m]) z[gid] = 0.0;
+// ---- direction: q = K z + beta q, p = z + beta p, partials of p.q ----------
+// gridDim.x is a multiple of 8; logical block xcd_block() walks rows
+// [lb * rpb, (lb + 1) * rpb)
+template <int LPR, int MC, bool SING>
+__global__ void __launch_bounds__(kIBS, (MC == 1 ? 2 : 1))
+k_cg_dir(int64_t n, int m, int c0, int mc, int par, const int32_t* __restrict__ rowptr,
+         const int32_t* __restrict__ cols, const double* __restrict__ kv, const double* __restrict__ Z,
+         double* __restrict__ P, double* __restrict__ Q, const double* __restrict__ R, CgState s) {
+    if (s.status[0]) return;  // converged: the enqueued iterations end here
+    __shared__ double sh[kIBS / 64];
+    constexpr int GPB = kIBS / LPR;
+    const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
+    const int64_t rpb = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t rb = xcd_block() * rpb, re = rb + rpb < n ? rb + rpb : n;
+    double bet[MC], pq[MC], sr[MC];
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        bet[c] = 0.0;
+        pq[c] = sr[c] = 0.0;
+        if (c < mc) {
+            // beta_k = rz_k / rz_{k-1}; 0 for a converged column (and at k = 0,
+            // where P and Q hold zeros anyway)
+            const double rzk = allsum_partials(s.rz[par], s.nb_upd, m, c0 + c, sh);
+            const double rzo = allsum_partials(s.rz[par ^ 1], s.nb_upd, m, c0 + c, sh);
+            const double rrk = allsum_partials(s.rr[par], s.nb_upd, m, c0 + c, sh);
+            bet[c] = (rrk > s.tol2[c0 + c] && rzo != 0.0) ? rzk / rzo : 0.0;
+        }
+    }
+    for (int64_t i = rb + grp; i < re; i += GPB) {
+        const int32_t r0 = rowptr[i], r1 = rowptr[i + 1];
+        double acc[MC];
+#pragma unroll
+        for (int c = 0; c < MC; ++c) acc[c] = 0.0;
+        for (int32_t q = r0 + lane; q < r1; q += LPR) {
+            const int64_t j = cols[q];
+            const double w = kv[q];
+            const double* zr = Z + j * m + c0;
+#pragma unroll
+            for (int c = 0; c < MC; ++c)
+                if (c < mc) acc[c] += w * zr[c];
+        }
+#pragma unroll
+        for (int c = 0; c < MC; ++c)
+#pragma unroll
+            for (int o = LPR / 2; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, LPR);
+        if (lane == 0) {
+            const int64_t o = i * m + c0;
+#pragma unroll
+            for (int c = 0; c < MC; ++c) {
+                if (c >= mc) continue;
+                const double pn = Z[o + c] + bet[c] * P[o + c];
+                const double qn = acc[c] + bet[c] * Q[o + c];
+                P[o + c] = pn;
+                Q[o + c] = qn;
+                pq[c] += pn * qn;
+                if (SING) sr[c] += R[o + c];
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        if (c >= mc) continue;
+        const double bp = block_allsum(pq[c], sh);
+        if (threadIdx.x == 0) s.pq[(int64_t)blockIdx.x * m + c0 + c] = bp;
+        if (SING) {
+            const double bs = block_allsum(sr[c], sh);
+            if (threadIdx.x == 0) s.sr[(int64_t)blockIdx.x * m + c0 + c] = bs;
+        }
+    }
 }
 
-__global__ void k_beta(int m, const double* __restrict__ rzn, double* __restrict__ rzo, double* __restrict__ b,
-                       const int* __restrict__ active) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m) return;
-    b[c] = (active[c] && rzo[c] != 0.0) ? rzn[c] / rzo[c] : 0.0;
-    rzo[c] = rzn[c];
+// the same on the operator's ELL copy (W slots per row, padding slots point
+// at the row itself with weight 0): one thread per row, the row's own values
+// and its slots loaded before the gathers, which then go out together;
+// 256-thread blocks, 8 waves per SIMD for W = 8.  A column's arithmetic (the
+// slots summed in order, the same blocks and partials) does not depend on how
+// many columns share the launch, so a block of right-hand sides gives what
+// one solve per column gives.
+constexpr int kEBS = 256;
+template <int W, int MC, bool SING>
+__global__ void __launch_bounds__(kEBS) __attribute__((amdgpu_waves_per_eu((W <= 8 && MC == 1) ? 8 : 4)))
+k_cg_dir_ell(int64_t n, int m, int c0, int mc, int par, const int32_t* __restrict__ ecols,
+             const double* __restrict__ ev, const double* __restrict__ Z, double* __restrict__ P,
+             double* __restrict__ Q, const double* __restrict__ R, CgState s) {
+    if (s.status[0]) return;
+    __shared__ double sh[kEBS / 64];
+    const int64_t rpb = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t rb = xcd_block() * rpb, re = rb + rpb < n ? rb + rpb : n;
+    double bet[MC], pq[MC], sr[MC];
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        bet[c] = 0.0;
+        pq[c] = sr[c] = 0.0;
+        if (c < mc) {
+            const double rzk = allsum_partials(s.rz[par], s.nb_upd, m, c0 + c, sh);
+            const double rzo = allsum_partials(s.rz[par ^ 1], s.nb_upd, m, c0 + c, sh);
+            const double rrk = allsum_partials(s.rr[par], s.nb_upd, m, c0 + c, sh);
+            bet[c] = (rrk > s.tol2[c0 + c] && rzo != 0.0) ? rzk / rzo : 0.0;
+        }
+    }
+    for (int64_t i = rb + threadIdx.x; i < re; i += kEBS) {
+        const int64_t o = i * m + c0;
+        double zo[MC], po[MC], qo[MC], ro[MC], acc[MC];
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            zo[c] = po[c] = qo[c] = ro[c] = acc[c] = 0.0;
+            if (c < mc) {
+                zo[c] = Z[o + c];
+                po[c] = P[o + c];
+                qo[c] = Q[o + c];
+                if (SING) ro[c] = R[o + c];
+            }
+        }
+        // slots in groups of G (all W at once for one column), in order
+        constexpr int G = MC == 1 ? W : 8;
+#pragma unroll
+        for (int k0 = 0; k0 < W; k0 += G) {
+            int32_t col[G];
+            double val[G];
+            const int4* cp = reinterpret_cast<const int4*>(ecols + i * W + k0);
+            const double2* vp = reinterpret_cast<const double2*>(ev + i * W + k0);
+#pragma unroll
+            for (int k = 0; k < G / 4; ++k) {
+                const int4 q4 = cp[k];
+                col[4 * k] = q4.x;
+                col[4 * k + 1] = q4.y;
+                col[4 * k + 2] = q4.z;
+                col[4 * k + 3] = q4.w;
+            }
+#pragma unroll
+            for (int k = 0; k < G / 2; ++k) {
+                const double2 v2 = vp[k];
+                val[2 * k] = v2.x;
+                val[2 * k + 1] = v2.y;
+            }
+            if (MC == 1) {
+                double g[G];
+#pragma unroll
+                for (int k = 0; k < G; ++k) g[k] = Z[(int64_t)col[k] * m + c0];
+#pragma unroll
+                for (int k = 0; k < G; ++k) acc[0] += val[k] * g[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    const double* zr = Z + (int64_t)col[k] * m + c0;
+#pragma unroll
+                    for (int c = 0; c < MC; ++c)
+                        if (c < mc) acc[c] += val[k] * zr[c];
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < MC; ++c) {
+            if (c >= mc) continue;
+            const double pn = zo[c] + bet[c] * po[c];
+            const double qn = acc[c] + bet[c] * qo[c];
+            P[o + c] = pn;
+            Q[o + c] = qn;
+            pq[c] += pn * qn;
+            if (SING) sr[c] += ro[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        if (c >= mc) continue;
+        const double bp = block_allsum(pq[c], sh);
+        if (threadIdx.x == 0) s.pq[(int64_t)blockIdx.x * m + c0 + c] = bp;
+        if (SING) {
+            const double bs = block_allsum(sr[c], sh);
+            if (threadIdx.x == 0) s.sr[(int64_t)blockIdx.x * m + c0 + c] = bs;
+        }
+    }
 }
-__global__ void k_update_p(int64_t n, int m, const double* __restrict__ b, const double* __restrict__ z,
-                           double* __restrict__ p) {
-    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n * m) return;
-    int c = (int)(gid % m);
-    p[gid] = z[gid] + b[c] * p[gid];
+
+// ---- update: x += alpha p, r = r - mean - alpha q, (z = D^-1 r) -------------
+// alpha_k = rz_k / p.q for an active column (r_k.r_k > tol2), else 0
+template <int MC, bool SING, bool JAC>
+__global__ void __launch_bounds__(kIBS)
+k_cg_update(int64_t n, int m, int c0, int mc, int par, const double* __restrict__ dinv, const double* __restrict__ P,
+            const double* __restrict__ Q, double* __restrict__ X, double* __restrict__ R, double* __restrict__ Z,
+            CgState s, int last_chunk) {
+    if (s.status[0]) return;
+    __shared__ double sh[kIBS / 64];
+    // contiguous rows per block, an even count; a thread takes two adjacent
+    // rows per step (one 16-byte load per operand for one column)
+    int64_t rpb = (n + gridDim.x - 1) / gridDim.x;
+    rpb += rpb & 1;
+    const int64_t rb = blockIdx.x * rpb, re = rb + rpb < n ? rb + rpb : n;
+    double al[MC], mu[MC], rz[MC], rr[MC];
+    int nact = 0;
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        al[c] = mu[c] = rz[c] = rr[c] = 0.0;
+        if (c < mc) {
+            const int cc = c0 + c;
+            const double rzk = allsum_partials(s.rz[par], s.nb_upd, m, cc, sh);
+            const double rrk = allsum_partials(s.rr[par], s.nb_upd, m, cc, sh);
+            const double pqk = allsum_partials(s.pq, s.nb_dir, m, cc, sh);
+            const bool act = rrk > s.tol2[cc];
+            nact += act ? 1 : 0;
+            al[c] = (act && pqk != 0.0) ? rzk / pqk : 0.0;
+            if (SING && act) mu[c] = allsum_partials(s.sr, s.nb_dir, m, cc, sh) / (double)n;
+            if (blockIdx.x == 0 && threadIdx.x == 0) s.rr_out[cc] = rrk;
+        }
+    }
+    for (int64_t i = rb + 2 * (int64_t)threadIdx.x; i < re; i += 2 * kIBS) {
+        const bool two = i + 1 < re;
+        double vr[2][MC], vq[2][MC], vp[2][MC], vx[2][MC], vd[2];
+        if (MC == 1 && m == 1 && two) {
+            const double2 r2 = *reinterpret_cast<const double2*>(R + i);
+            const double2 q2 = *reinterpret_cast<const double2*>(Q + i);
+            const double2 p2 = *reinterpret_cast<const double2*>(P + i);
+            const double2 x2 = *reinterpret_cast<const double2*>(X + i);
+            vr[0][0] = r2.x; vr[1][0] = r2.y;
+            vq[0][0] = q2.x; vq[1][0] = q2.y;
+            vp[0][0] = p2.x; vp[1][0] = p2.y;
+            vx[0][0] = x2.x; vx[1][0] = x2.y;
+            if (JAC) {
+                const double2 d2 = *reinterpret_cast<const double2*>(dinv + i);
+                vd[0] = d2.x;
+                vd[1] = d2.y;
+            } else {
+                vd[0] = vd[1] = 0.0;
+            }
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int64_t row = (h == 0 || two) ? i + h : i;  // an odd tail repeats row i, unused
+                const int64_t o = row * m + c0;
+                vd[h] = JAC ? dinv[row] : 0.0;
+#pragma unroll
+                for (int c = 0; c < MC; ++c) {
+                    vr[h][c] = vq[h][c] = vp[h][c] = vx[h][c] = 0.0;
+                    if (c < mc) {
+                        vr[h][c] = R[o + c];
+                        vq[h][c] = Q[o + c];
+                        vp[h][c] = P[o + c];
+                        vx[h][c] = X[o + c];
+                    }
+                }
+            }
+        }
+        double nr[2][MC], nx[2][MC], nz[2][MC];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int c = 0; c < MC; ++c) {
+                double r = vr[h][c];
+                if (SING) r -= mu[c];
+                r -= al[c] * vq[h][c];
+                nr[h][c] = r;
+                nx[h][c] = vx[h][c] + al[c] * vp[h][c];
+                nz[h][c] = vd[h] * r;
+                if (h == 0 || two) {
+                    if (JAC) rz[c] += r * nz[h][c];
+                    rr[c] += r * r;
+                }
+            }
+        if (MC == 1 && m == 1 && two) {
+            *reinterpret_cast<double2*>(R + i) = make_double2(nr[0][0], nr[1][0]);
+            *reinterpret_cast<double2*>(X + i) = make_double2(nx[0][0], nx[1][0]);
+            if (JAC) *reinterpret_cast<double2*>(Z + i) = make_double2(nz[0][0], nz[1][0]);
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (h == 1 && !two) continue;
+                const int64_t o = (i + h) * m + c0;
+#pragma unroll
+                for (int c = 0; c < MC; ++c) {
+                    if (c >= mc) continue;
+                    R[o + c] = nr[h][c];
+                    X[o + c] = nx[h][c];
+                    if (JAC) Z[o + c] = nz[h][c];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        if (c >= mc) continue;
+        const double b1 = block_allsum(rr[c], sh);
+        if (threadIdx.x == 0) s.rr[par ^ 1][(int64_t)blockIdx.x * m + c0 + c] = b1;
+        if (JAC) {
+            const double b2 = block_allsum(rz[c], sh);
+            if (threadIdx.x == 0) s.rz[par ^ 1][(int64_t)blockIdx.x * m + c0 + c] = b2;
+        }
+    }
+    // status for the host (and the early exit of later launches): every block
+    // reached the same decision from the same partials; block 0 records it.
+    // The columns of this launch are all converged only if nact == 0; the
+    // iteration counts once, on its last column chunk.
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (c0 == 0) s.status[2] = 0;
+        s.status[2] += nact;
+        if (last_chunk) {
+            if (s.status[2] == 0) s.status[0] = 1;
+            else s.status[1] += 1;
+        }
+    }
 }
-// column means over free rows -> subtract (projection off the constants)
+
+// ---- partials of r.z (after an external preconditioner) into parity par ----
+template <int MC>
+__global__ void __launch_bounds__(kBS)
+k_cg_rz(int64_t n, int m, int c0, int mc, const double* __restrict__ R, const double* __restrict__ Z,
+        double* __restrict__ part, const int* __restrict__ status) {
+    if (status && status[0]) return;
+    __shared__ double sh[kBS / 64];
+    double rz[MC];
+#pragma unroll
+    for (int c = 0; c < MC; ++c) rz[c] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+        const int64_t o = i * m + c0;
+#pragma unroll
+        for (int c = 0; c < MC; ++c)
+            if (c < mc) rz[c] += R[o + c] * Z[o + c];
+    }
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        if (c >= mc) continue;
+        const double b = block_allsum(rz[c], sh);
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * m + c0 + c] = b;
+    }
+}
+
+// ---- set-up reductions -------------------------------------------------------
+// partials of sum_i A[i, c] * (B ? B[i, c] : 1) (one per block), and the
+// total from them by a one-block kernel
+template <int MC>
+__global__ void __launch_bounds__(kBS)
+k_coldot_part(int64_t n, int m, int c0, int mc, const double* __restrict__ A, const double* __restrict__ B,
+              double* __restrict__ part) {
+    __shared__ double sh[kBS / 64];
+    double acc[MC];
+#pragma unroll
+    for (int c = 0; c < MC; ++c) acc[c] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kBS + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBS) {
+        const int64_t o = i * m + c0;
+#pragma unroll
+        for (int c = 0; c < MC; ++c)
+            if (c < mc) acc[c] += A[o + c] * (B ? B[o + c] : 1.0);
+    }
+#pragma unroll
+    for (int c = 0; c < MC; ++c) {
+        if (c >= mc) continue;
+        const double b = block_allsum(acc[c], sh);
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * m + c0 + c] = b;
+    }
+}
+__global__ void __launch_bounds__(kBS)
+k_sum_parts(int nparts, int m, const double* __restrict__ part, double* __restrict__ out) {
+    __shared__ double sh[kBS / 64];
+    for (int c = 0; c < m; ++c) {
+        const double t = allsum_partials(part, nparts, m, c, sh);
+        if (threadIdx.x == 0) out[c] = t;
+    }
+}
+// out[i, :] = in[perm[i], :] (row-major n x m; perm: new row -> old row)
+__global__ void k_perm_rows(int64_t n, int m, const int32_t* __restrict__ perm, const double* __restrict__ in,
+                            double* __restrict__ out) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * m) return;
+    const int64_t i = g / m;
+    out[g] = in[(int64_t)perm[i] * m + (g - i * m)];
+}
+// X[:, c] -= sums[c] / n
 __global__ void k_sub_colmean(int64_t n, int m, double* __restrict__ x, const double* __restrict__ sums) {
-    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n * m) return;
-    int c = (int)(gid % m);
-    x[gid] -= sums[c] / (double)n;
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * m) return;
+    x[g] -= sums[g % m] / (double)n;
 }
-__global__ void k_fill(int64_t cnt, double v, double* __restrict__ x) {
-    int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid < cnt) x[gid] = v;
+// Y = B - K X (row-major n x m, LPR lanes per row)
+template <int LPR>
+__global__ void __launch_bounds__(kBS)
+k_resid(int64_t n, int m, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
+        const double* __restrict__ kv, const double* __restrict__ X, const double* __restrict__ B,
+        double* __restrict__ Y) {
+    constexpr int GPB = kBS / LPR;
+    const int lane = threadIdx.x % LPR;
+    const int64_t i = (int64_t)blockIdx.x * GPB + threadIdx.x / LPR;
+    if (i >= n) return;  // whole LPR group
+    for (int c = 0; c < m; ++c) {
+        double acc = 0.0;
+        for (int32_t q = rowptr[i] + lane; q < rowptr[i + 1]; q += LPR) acc += kv[q] * X[(int64_t)cols[q] * m + c];
+#pragma unroll
+        for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+        if (lane == 0) Y[i * m + c] = B[i * m + c] - acc;
+    }
+}
+// per-block maxima of |(K 1)_i| * dinv_i (row sums relative to the diagonal)
+template <int LPR>
+__global__ void __launch_bounds__(kBS)
+k_rowsum_rel(int64_t n, const int32_t* __restrict__ rowptr, const double* __restrict__ kv,
+             const double* __restrict__ dinv, double* __restrict__ part) {
+    __shared__ double sh[kBS / 64];
+    constexpr int GPB = kBS / LPR;
+    const int lane = threadIdx.x % LPR;
+    double mx = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * GPB + threadIdx.x / LPR; i < n; i += (int64_t)gridDim.x * GPB) {
+        double acc = 0.0;
+        for (int32_t q = rowptr[i] + lane; q < rowptr[i + 1]; q += LPR) acc += kv[q];
+#pragma unroll
+        for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+        if (lane == 0) mx = fmax(mx, fabs(acc * dinv[i]));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = 0.0;
+        for (int k = 0; k < kBS / 64; ++k) b = fmax(b, sh[k]);
+        part[blockIdx.x] = b;
+    }
 }
 
 double now_us() {
     return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
+
+// lanes per row for rows of about `avg` entries
+int lanes_per_row(double avg) { return avg <= 4.0 ? 4 : avg <= 8.0 ? 8 : avg <= 16.0 ? 16 : 32; }
+
+struct Launcher {
+    lh_ctx* ctx;
+    int64_t n;
+    int m;
+    int lpr;
+    unsigned nb;      // blocks of the row-wise kernels (partials per column)
+    unsigned nb_dir;  // blocks of k_cg_dir
+    const int32_t* rp;
+    const int32_t* cl;
+    const double* kv;
+
+    int ew = 0;  // ELL width of the operator's copy (0: CSR kernels)
+    unsigned nb_ell = 0;  // blocks of k_cg_dir_ell
+    const int32_t* ecols = nullptr;
+    const double* ev = nullptr;
+
+    template <int MC>
+    void dir_mc(int c0, int mc, int par, bool sing, const double* Z, double* P, double* Q, const double* R,
+                const CgState& s) {
+        if constexpr (MC <= 4) {
+          if (ew) {
+#define LH_DIRE(W_, S) \
+    k_cg_dir_ell<W_, MC, S><<<nb_ell, kEBS, 0, ctx->stream>>>(n, m, c0, mc, par, ecols, ev, Z, P, Q, R, s)
+            switch (ew) {
+                case 8: if (sing) LH_DIRE(8, true); else LH_DIRE(8, false); break;
+                case 16: if (sing) LH_DIRE(16, true); else LH_DIRE(16, false); break;
+                case 24: if (sing) LH_DIRE(24, true); else LH_DIRE(24, false); break;
+                default: if (sing) LH_DIRE(32, true); else LH_DIRE(32, false); break;
+            }
+#undef LH_DIRE
+            LH_CHECK_LAUNCH();
+            return;
+          }
+        }
+#define LH_DIR(L, S) \
+    k_cg_dir<L, MC, S><<<nb_dir, kIBS, 0, ctx->stream>>>(n, m, c0, mc, par, rp, cl, kv, Z, P, Q, R, s)
+        switch (lpr) {
+            case 4: if (sing) LH_DIR(4, true); else LH_DIR(4, false); break;
+            case 8: if (sing) LH_DIR(8, true); else LH_DIR(8, false); break;
+            case 16: if (sing) LH_DIR(16, true); else LH_DIR(16, false); break;
+            default: if (sing) LH_DIR(32, true); else LH_DIR(32, false); break;
+        }
+#undef LH_DIR
+        LH_CHECK_LAUNCH();
+    }
+    void dir(int par, bool sing, const double* Z, double* P, double* Q, const double* R, const CgState& s) {
+        const int w = ew ? 4 : kMC;  // the ELL kernel takes four columns at most (eight would spill)
+        for (int c0 = 0; c0 < m; c0 += w) {
+            const int mc = std::min(w, m - c0);
+            if (mc == 1) dir_mc<1>(c0, mc, par, sing, Z, P, Q, R, s);
+            else if (mc <= 4) dir_mc<4>(c0, mc, par, sing, Z, P, Q, R, s);
+            else dir_mc<8>(c0, mc, par, sing, Z, P, Q, R, s);
+        }
+    }
+    template <int MC>
+    void update_mc(int c0, int mc, int par, bool sing, bool jac, const double* dinv, const double* P,
+                   const double* Q, double* X, double* R, double* Z, const CgState& s, int last) {
+#define LH_UPD(S, J) \
+    k_cg_update<MC, S, J><<<nb, kIBS, 0, ctx->stream>>>(n, m, c0, mc, par, dinv, P, Q, X, R, Z, s, last)
+        if (sing) { if (jac) LH_UPD(true, true); else LH_UPD(true, false); }
+        else { if (jac) LH_UPD(false, true); else LH_UPD(false, false); }
+#undef LH_UPD
+        LH_CHECK_LAUNCH();
+    }
+    // two columns per launch at most (two rows of four columns would spill)
+    void update(int par, bool sing, bool jac, const double* dinv, const double* P, const double* Q, double* X,
+                double* R, double* Z, const CgState& s) {
+        for (int c0 = 0; c0 < m; c0 += 2) {
+            const int mc = std::min(2, m - c0);
+            const int last = c0 + 2 >= m;
+            if (mc == 1) update_mc<1>(c0, mc, par, sing, jac, dinv, P, Q, X, R, Z, s, last);
+            else update_mc<2>(c0, mc, par, sing, jac, dinv, P, Q, X, R, Z, s, last);
+        }
+    }
+    // partials (nb per column) of r.z into part
+    void rz(const double* R, const double* Z, double* part, const int* status) {
+        for (int c0 = 0; c0 < m; c0 += kMC) {
+            const int mc = std::min(kMC, m - c0);
+            if (mc == 1) k_cg_rz<1><<<nb, kBS, 0, ctx->stream>>>(n, m, c0, mc, R, Z, part, status);
+            else if (mc <= 4) k_cg_rz<4><<<nb, kBS, 0, ctx->stream>>>(n, m, c0, mc, R, Z, part, status);
+            else k_cg_rz<8><<<nb, kBS, 0, ctx->stream>>>(n, m, c0, mc, R, Z, part, status);
+            LH_CHECK_LAUNCH();
+        }
+    }
+    // partials (nb per column) of sum_i A[i, c] B[i, c] (B null: sum of A)
+    void coldot_part(const double* A, const double* B, double* part) {
+        for (int c0 = 0; c0 < m; c0 += kMC) {
+            const int mc = std::min(kMC, m - c0);
+            if (mc == 1) k_coldot_part<1><<<nb, kBS, 0, ctx->stream>>>(n, m, c0, mc, A, B, part);
+            else if (mc <= 4) k_coldot_part<4><<<nb, kBS, 0, ctx->stream>>>(n, m, c0, mc, A, B, part);
+            else k_coldot_part<8><<<nb, kBS, 0, ctx->stream>>>(n, m, c0, mc, A, B, part);
+            LH_CHECK_LAUNCH();
+        }
+    }
+    // out[c] = sum_i A[i, c] B[i, c] (device, m doubles)
+    void coldot(const double* A, const double* B, double* part, double* out) {
+        coldot_part(A, B, part);
+        k_sum_parts<<<1, kBS, 0, ctx->stream>>>((int)nb, m, part, out);
+        LH_CHECK_LAUNCH();
+    }
+    void resid(const double* X, const double* B, double* Y) {
+        const unsigned g = nblocks(n * lpr, kBS);
+        switch (lpr) {
+            case 4: k_resid<4><<<g, kBS, 0, ctx->stream>>>(n, m, rp, cl, kv, X, B, Y); break;
+            case 8: k_resid<8><<<g, kBS, 0, ctx->stream>>>(n, m, rp, cl, kv, X, B, Y); break;
+            case 16: k_resid<16><<<g, kBS, 0, ctx->stream>>>(n, m, rp, cl, kv, X, B, Y); break;
+            default: k_resid<32><<<g, kBS, 0, ctx->stream>>>(n, m, rp, cl, kv, X, B, Y); break;
+        }
+        LH_CHECK_LAUNCH();
+    }
+    // nb_dir block maxima of |(K 1)_i| dinv_i
+    void rowsum_rel(const double* dinv, double* part) {
+        switch (lpr) {
+            case 4: k_rowsum_rel<4><<<nb_dir, kBS, 0, ctx->stream>>>(n, rp, kv, dinv, part); break;
+            case 8: k_rowsum_rel<8><<<nb_dir, kBS, 0, ctx->stream>>>(n, rp, kv, dinv, part); break;
+            case 16: k_rowsum_rel<16><<<nb_dir, kBS, 0, ctx->stream>>>(n, rp, kv, dinv, part); break;
+            default: k_rowsum_rel<32><<<nb_dir, kBS, 0, ctx->stream>>>(n, rp, kv, dinv, part); break;
+        }
+        LH_CHECK_LAUNCH();
+    }
+};
 
 }  // namespace
 
@@ -222,8 +690,14 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
     if (alpha != 0.0 && !B) LH_THROW(LH_ERR_ARG, "alpha != 0 needs B");
     const int m = nrhs;
     const double rtol = (opts && opts->rtol > 0) ? opts->rtol : 1e-10;
-    const int max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : (int)std::max<int64_t>(1000, 10 * n);
+    const int64_t max_iter = (opts && opts->max_iter > 0) ? opts->max_iter : std::max<int64_t>(1000, 10 * n);
     const int verbose = opts ? opts->verbose : 0;
+    const int precond = opts ? opts->precond : LH_PRECOND_JACOBI;
+    if (precond != LH_PRECOND_JACOBI && precond != LH_PRECOND_LEARNED && precond != LH_PRECOND_NONE)
+        LH_THROW(LH_ERR_ARG, "preconditioner %d unknown", precond);
+    if (precond == LH_PRECOND_LEARNED && !ctx->mpnn)
+        LH_THROW(LH_ERR_ARG, "LH_PRECOND_LEARNED needs a network on the context (lh_ctx_set_mpnn)");
+    if ((size_t)m * 16 + 64 > ctx->host_small_bytes) LH_THROW(LH_ERR_ARG, "too many right-hand sides (%d)", m);
     lh_stats st{};
     // operator values on A's pattern
     DevBuf kv(nnz * 8), missing(8);
@@ -252,177 +726,257 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
     int miss = 0;
     d2h(ctx, &miss, missing.p, 4);
     if (miss) LH_THROW(LH_ERR_ARG, "B has diagonal entries outside A's pattern");
-    // Jacobi from K's diagonal
-    DevBuf dinv(n * 8);
-    {
-        lh_csr K;  // view with K values for jacobi_diag
+    bool any_dirichlet = false;
+    if (dmask)
+        for (int64_t i = 0; i < n; ++i)
+            if (dmask[i]) { any_dirichlet = true; break; }
+    DevBuf dm;
+    const uint8_t* dmp = nullptr;
+    if (any_dirichlet) {
+        dm.alloc(n);
+        h2d(ctx, dm.p, dmask, n);
+        dmp = dm.as<uint8_t>();
+        k_mask_k<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, A->pat->rowptr.as<int32_t>(),
+                                                           A->pat->cols.as<int32_t>(), dmp, kv.as<double>());
+        LH_CHECK_LAUNCH();
+    }
+    // the locality order of the pattern (not with the learned network, which
+    // runs on A's own order): K permuted once per solve, the order itself
+    // built once per pattern
+    std::shared_ptr<Reorder> ro;
+    if (n >= 4096 && precond != LH_PRECOND_LEARNED) {
+        if (!A->pat->ro) {
+            ArenaOff off;  // kept with the pattern, beyond this call
+            auto r = std::make_shared<Reorder>();
+            r->build(ctx, *A->pat);
+            A->pat->ro = r;
+        }
+        ro = A->pat->ro;
+    }
+    std::unique_ptr<lh_csr> Kp;
+    const Pattern* kpat = A->pat.get();
+    const double* kvp = kv.as<double>();
+    if (ro) {
+        lh_csr K;  // view of the operator's values on A's pattern
         K.ctx = ctx;
         K.pat = A->pat;
         K.dtype = LH_F64;
         K.vals.p = kv.p;
+        Kp = ro->apply(ctx, &K);
+        K.vals.p = nullptr;  // not owned
+        kpat = Kp->pat.get();
+        kvp = Kp->vals.as<double>();
+    }
+    // Jacobi weights from the (masked) operator's diagonal: 0 on Dirichlet rows
+    DevBuf dinv(n * 8);
+    {
+        lh_csr K;  // view for jacobi_diag
+        K.ctx = ctx;
+        K.pat = ro ? Kp->pat : A->pat;
+        K.dtype = LH_F64;
+        K.vals.p = const_cast<double*>(kvp);
         jacobi_diag(ctx, &K, nullptr, 0.0, 1.0, dinv.as<double>());
         K.vals.p = nullptr;  // not owned
     }
-    bool any_dirichlet = false;
-    if (dmask) {
-        for (int64_t i = 0; i < n; ++i) if (dmask[i]) { any_dirichlet = true; break; }
+    Launcher L;
+    L.ctx = ctx;
+    L.n = n;
+    L.m = m;
+    L.lpr = lanes_per_row(n > 0 ? (double)nnz / (double)n : 1.0);
+    L.nb = std::max(1u, std::min<unsigned>(nblocks(n, 2 * kIBS), (unsigned)ctx->n_cu));
+    L.nb_dir = std::max<unsigned>(kXCDs, std::min<unsigned>(nblocks_xcd(n * L.lpr, kIBS), (unsigned)ctx->n_cu * 2));
+    L.nb_dir = (L.nb_dir + kXCDs - 1) / kXCDs * kXCDs;
+    L.rp = kpat->rowptr.as<int32_t>();
+    L.cl = kpat->cols.as<int32_t>();
+    L.kv = kvp;
+    // ELL copy of K for the direction kernel (rows of at most 32 entries)
+    Ell kell;
+    {
+        lh_csr K;  // view
+        K.ctx = ctx;
+        K.pat = ro ? Kp->pat : A->pat;
+        K.dtype = LH_F64;
+        K.vals.p = const_cast<double*>(kvp);
+        kell.build(ctx, &K, nullptr);
+        K.vals.p = nullptr;  // not owned
     }
+    if (kell.W) {
+        L.ew = kell.W;
+        L.ecols = kell.cols.as<int32_t>();
+        L.ev = kell.va.as<double>();
+        L.nb_ell = std::max<unsigned>(kXCDs, std::min<unsigned>(nblocks_xcd(n, kEBS), (unsigned)ctx->n_cu * 8));
+        L.nb_ell = (L.nb_ell + kXCDs - 1) / kXCDs * kXCDs;
+        L.nb_dir = L.nb_ell;  // the p.q partials come from the ELL launch
+    }
+    // device state
+    const size_t pu = (size_t)L.nb * m, pd = (size_t)L.nb_dir * m;
+    DevBuf parts((5 * pu + 2 * pd) * 8), scal((size_t)3 * m * 8), ints(16);
+    double* pp = parts.as<double>();
+    CgState s;
+    s.rz[0] = pp;
+    s.rz[1] = pp + pu;
+    s.rr[0] = pp + 2 * pu;
+    s.rr[1] = pp + 3 * pu;
+    double* ptmp = pp + 4 * pu;  // set-up reductions
+    s.pq = pp + 5 * pu;
+    s.sr = s.pq + pd;
+    s.nb_upd = (int)L.nb;
+    s.nb_dir = (int)L.nb_dir;
+    double* tol2 = scal.as<double>();
+    double* sums = tol2 + m;
+    double* rrout = sums + m;
+    s.tol2 = tol2;
+    s.rr_out = rrout;
+    s.status = ints.as<int>();
+    memset_d(ctx, parts.p, 0, parts.bytes);
+    memset_d(ctx, ints.p, 0, 16);
     // pure Neumann: singular (constants as null space) only if max_i |(K 1)_i| / K_ii is at rounding level
     bool singular = false;
     if (alpha == 0.0 && !any_dirichlet) {
-        DevBuf one(n * 8), rowsum(n * 8);
-        k_fill<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, 1.0, one.as<double>());
-        k_spmv_masked<<<nblocks(n * 16, 256), 256, 0, ctx->stream>>>(n, A->pat->rowptr.as<int32_t>(),
-                                                                    A->pat->cols.as<int32_t>(), kv.as<double>(),
-                                                                    nullptr, one.as<double>(),
-                                                                    rowsum.as<double>(), 1);
-        LH_CHECK_LAUNCH();
-        std::vector<double> hs(n), hd(n);
-        d2h(ctx, hs.data(), rowsum.p, n * 8);
-        d2h(ctx, hd.data(), dinv.p, n * 8);
+        L.rowsum_rel(dinv.as<double>(), s.pq);
+        std::vector<double> hmax(L.nb_dir);
+        d2h(ctx, hmax.data(), s.pq, (size_t)L.nb_dir * 8);
         double worst = 0.0;
-        for (int64_t i = 0; i < n; ++i) worst = std::max(worst, std::fabs(hs[i] * hd[i]));
+        for (double v : hmax) worst = std::max(worst, v);
         singular = worst <= 1e-12;
     }
-    const int precond = opts ? opts->precond : LH_PRECOND_JACOBI;
-    if (precond != LH_PRECOND_JACOBI && precond != LH_PRECOND_LEARNED && precond != LH_PRECOND_NONE)
-        LH_THROW(LH_ERR_ARG, "preconditioner %d unknown", precond);
-    if (precond == LH_PRECOND_NONE) k_fill<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, 1.0, dinv.as<double>());
+    if (precond == LH_PRECOND_NONE)
+        k_dinv_unit<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, dinv.as<double>());
     bool learned_on = false;
     LearnedPrecond learned;
     if (precond == LH_PRECOND_LEARNED) {
-        if (!ctx->mpnn) LH_THROW(LH_ERR_ARG, "LH_PRECOND_LEARNED needs a network on the context (lh_ctx_set_mpnn)");
         learned.setup(ctx, ctx->mpnn, A, B, alpha, beta);
         learned_on = true;
     }
     const bool no_fallback = opts && opts->no_fallback;
-    // host -> device (column-major host, row-major device)
+    // host -> device (column-major host, row-major device): b and the start x
     std::vector<double> hb((size_t)n * m), hx((size_t)n * m);
     for (int64_t i = 0; i < n; ++i)
         for (int c = 0; c < m; ++c) {
             hb[i * m + c] = b[(int64_t)c * n + i];
             hx[i * m + c] = x[(int64_t)c * n + i];
         }
-    DevBuf dB(n * m * 8), dX(n * m * 8), dR(n * m * 8), dZ(n * m * 8), dP(n * m * 8), dQ(n * m * 8), dm;
-    const uint8_t* dmp = nullptr;
-    if (any_dirichlet) {
-        dm.alloc(n);
-        h2d(ctx, dm.p, dmask, n);
-        dmp = dm.as<uint8_t>();
+    const size_t blk = (size_t)n * m * 8;
+    DevBuf dB(blk), dX(blk), dR(blk), dZ(blk), dP(blk), dQ(blk);
+    h2d(ctx, dZ.p, hb.data(), blk);
+    h2d(ctx, dP.p, hx.data(), blk);
+    // b and x0 with the Dirichlet rows cleared (those rows are not solved
+    // for), in the solve's row order
+    if (ro) {
+        k_mask_rows<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dmp, dZ.as<double>(), dR.as<double>());
+        k_mask_rows<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dmp, dP.as<double>(), dQ.as<double>());
+        k_perm_rows<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, ro->perm.as<int32_t>(), dR.as<double>(),
+                                                                 dB.as<double>());
+        k_perm_rows<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, ro->perm.as<int32_t>(), dQ.as<double>(),
+                                                                 dX.as<double>());
+    } else {
+        k_mask_rows<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dmp, dZ.as<double>(), dB.as<double>());
+        k_mask_rows<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dmp, dP.as<double>(), dX.as<double>());
     }
-    h2d(ctx, dB.p, hb.data(), n * m * 8);
-    // r = b on free rows (x starts at 0 there)
-    HIPX(hipMemsetAsync(dX.p, 0, n * m * 8, ctx->stream));
-    {
-        // r = mask ? 0 : b   (reuse k_precond with unit scaling)
-        DevBuf ones(n * 8);
-        k_fill<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, 1.0, ones.as<double>());
-        k_precond<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, ones.as<double>(), dmp, dB.as<double>(),
-                                                              dR.as<double>());
+    LH_CHECK_LAUNCH();
+    if (singular) {  // b onto the complement of the constants
+        L.coldot(dB.as<double>(), nullptr, ptmp, sums);
+        k_sub_colmean<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dB.as<double>(), sums);
         LH_CHECK_LAUNCH();
-        HIPX(hipStreamSynchronize(ctx->stream));
     }
-    ColDot cd;
-    cd.init(n, m);
-    DevBuf ones_blk, sums(m * 8);
+    // r = b - K x0 (the caller's x warm-starts the free rows)
+    L.resid(dX.as<double>(), dB.as<double>(), dR.as<double>());
     if (singular) {
-        ones_blk.alloc(n * m * 8);
-        k_fill<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n * m, 1.0, ones_blk.as<double>());
-        cd.run(ctx, ones_blk.as<double>(), dR.as<double>(), sums.as<double>());
-        k_sub_colmean<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dR.as<double>(), sums.as<double>());
+        L.coldot(dR.as<double>(), nullptr, ptmp, sums);
+        k_sub_colmean<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dR.as<double>(), sums);
+        LH_CHECK_LAUNCH();
     }
-    DevBuf rz(m * 8), rzn(m * 8), pq(m * 8), al(m * 8), be(m * 8), rr(m * 8), act(m * 4);
-    std::vector<double> bnorm2(m), rnorm2(m);
-    cd.run(ctx, dR.as<double>(), dR.as<double>(), rr.as<double>());
-    d2h(ctx, bnorm2.data(), rr.p, m * 8);
-    std::vector<int> active(m, 1);
-    int nconv = 0;
-    for (int c = 0; c < m; ++c)
-        if (bnorm2[c] == 0.0) { active[c] = 0; ++nconv; }
-    h2d(ctx, act.p, active.data(), m * 4);
-    auto precondition = [&]() {
-        if (learned_on) {
-            HIPX(hipMemcpyAsync(dZ.p, dR.p, n * m * 8, hipMemcpyDeviceToDevice, ctx->stream));
-            learned.apply(dZ.as<double>(), m, m);
-            if (dmp) k_mask_rows<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dmp, dZ.as<double>());
-        } else {
-            k_precond<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dinv.as<double>(), dmp, dR.as<double>(),
-                                                                  dZ.as<double>());
-        }
+    std::vector<double> hbb(m), htol(m);
+    L.coldot(dB.as<double>(), dB.as<double>(), ptmp, sums);
+    d2h(ctx, hbb.data(), sums, (size_t)m * 8);
+    // a column with b = 0 is never active (its solution is x = 0)
+    for (int c = 0; c < m; ++c) htol[c] = hbb[c] > 0.0 ? rtol * rtol * hbb[c] : HUGE_VAL;
+    h2d(ctx, tol2, htol.data(), (size_t)m * 8);
+    // partials of r_0.r_0 into parity 0
+    L.coldot_part(dR.as<double>(), dR.as<double>(), s.rr[0]);
+    // z = M^-1 r with the learned network (Dirichlet rows kept at zero)
+    auto apply_learned = [&]() {
+        HIPX(hipMemcpyAsync(dZ.p, dR.p, blk, hipMemcpyDeviceToDevice, ctx->stream));
+        learned.apply(dZ.as<double>(), m, m);
+        if (dmp) k_mask_rows<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dmp, dZ.as<double>(), dZ.as<double>());
         LH_CHECK_LAUNCH();
     };
-    precondition();
-    HIPX(hipMemcpyAsync(dP.p, dZ.p, n * m * 8, hipMemcpyDeviceToDevice, ctx->stream));
-    cd.run(ctx, dR.as<double>(), dZ.as<double>(), rz.as<double>());
-    const int32_t* rp = A->pat->rowptr.as<int32_t>();
-    const int32_t* cl = A->pat->cols.as<int32_t>();
-    int it = 0;
+    // a first direction at iteration k: z = M^-1 r and the partials of r.z
+    // into parity k % 2, P = Q = 0 (so p = z, q = K z whatever beta is)
+    auto start_direction = [&](bool jac, int64_t k) {
+        if (jac) {
+            HIPX(hipMemcpyAsync(dZ.p, dR.p, blk, hipMemcpyDeviceToDevice, ctx->stream));
+            rowscale<double>(ctx, n, m, dZ.as<double>(), m, dinv.as<double>());
+        } else {
+            apply_learned();
+        }
+        L.rz(dR.as<double>(), dZ.as<double>(), s.rz[k & 1], nullptr);
+        HIPX(hipMemsetAsync(dP.p, 0, blk, ctx->stream));
+        HIPX(hipMemsetAsync(dQ.p, 0, blk, ctx->stream));
+    };
+    start_direction(!learned_on, 0);
+    int64_t enq = 0;
+    int done = 0, iters = 0;
     double worst = 0.0, prev_worst = 1.0;
     int slow = 0;
-    const int check_every = 4;
-    for (it = 0; it < max_iter && nconv < m; ++it) {
-        k_spmv_masked<<<nblocks(n * 16, 256), 256, 0, ctx->stream>>>(n, rp, cl, kv.as<double>(), dmp, dP.as<double>(),
-                                                                    dQ.as<double>(), m);
-        LH_CHECK_LAUNCH();
-        st.spmm_calls++;
-        cd.run(ctx, dP.as<double>(), dQ.as<double>(), pq.as<double>());
-        k_alpha<<<nblocks(m, 64), 64, 0, ctx->stream>>>(m, rz.as<double>(), pq.as<double>(), act.as<int>(),
-                                                        al.as<double>());
-        k_update_xr<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, al.as<double>(), dP.as<double>(),
-                                                                 dQ.as<double>(), dX.as<double>(), dR.as<double>());
-        if (singular) {
-            cd.run(ctx, ones_blk.as<double>(), dR.as<double>(), sums.as<double>());
-            k_sub_colmean<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dR.as<double>(), sums.as<double>());
-        }
-        precondition();
-        cd.run(ctx, dR.as<double>(), dZ.as<double>(), rzn.as<double>());
-        k_beta<<<nblocks(m, 64), 64, 0, ctx->stream>>>(m, rzn.as<double>(), rz.as<double>(), be.as<double>(),
-                                                       act.as<int>());
-        k_update_p<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, be.as<double>(), dZ.as<double>(),
-                                                                dP.as<double>());
-        LH_CHECK_LAUNCH();
-        if ((it + 1) % check_every == 0) {
-            cd.run(ctx, dR.as<double>(), dR.as<double>(), rr.as<double>());
-            d2h(ctx, rnorm2.data(), rr.p, m * 8);
-            worst = 0.0;
-            bool changed = false;
-            for (int c = 0; c < m; ++c) {
-                if (!active[c]) continue;
-                double rel = std::sqrt(rnorm2[c] / bnorm2[c]);
-                worst = std::max(worst, rel);
-                if (rel <= rtol) { active[c] = 0; ++nconv; changed = true; }
+    int* hstat = static_cast<int*>(ctx->host_small);
+    double* hrr_pin = reinterpret_cast<double*>(static_cast<char*>(ctx->host_small) + 64);
+    while (!done && enq < max_iter) {
+        const int chunk = (int)std::min<int64_t>(learned_on ? 4 : 16, max_iter - enq);
+        for (int kk = 0; kk < chunk; ++kk) {
+            const int par = (int)((enq + kk) & 1);
+            L.dir(par, singular, dZ.as<double>(), dP.as<double>(), dQ.as<double>(), dR.as<double>(), s);
+            L.update(par, singular, !learned_on, dinv.as<double>(), dP.as<double>(), dQ.as<double>(),
+                     dX.as<double>(), dR.as<double>(), dZ.as<double>(), s);
+            if (learned_on) {
+                apply_learned();
+                L.rz(dR.as<double>(), dZ.as<double>(), s.rz[par ^ 1], s.status);
             }
-            if (changed) h2d(ctx, act.p, active.data(), m * 4);
-            // learned preconditioner: back to Jacobi (and a CG restart) when
-            // the residual falls by less than half per iteration over three
-            // checks running
-            if (learned_on && !no_fallback) {
-                const double per_it = prev_worst > 0.0 ? std::pow(worst / prev_worst, 1.0 / check_every) : 0.0;
-                slow = (per_it > 0.5) ? slow + 1 : 0;
-                if (slow >= 3) {
-                    learned_on = false;
-                    st.fallback_used = 1;
-                    precondition();
-                    cd.run(ctx, dR.as<double>(), dZ.as<double>(), rz.as<double>());
-                    HIPX(hipMemcpyAsync(dP.p, dZ.p, n * m * 8, hipMemcpyDeviceToDevice, ctx->stream));
-                }
-            }
-            prev_worst = worst;
-            if (verbose && ((it + 1) % 100 == 0))
-                fprintf(stderr, "[pcg] it %d conv %d/%d worst %.3e\n", it + 1, nconv, m, worst);
+            st.spmm_calls++;
         }
+        enq += chunk;
+        HIPX(hipMemcpyAsync(hstat, s.status, 12, hipMemcpyDeviceToHost, ctx->stream));
+        HIPX(hipMemcpyAsync(hrr_pin, rrout, (size_t)m * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPX(hipStreamSynchronize(ctx->stream));
+        done = hstat[0];
+        iters = hstat[1];
+        worst = 0.0;
+        for (int c = 0; c < m; ++c)
+            if (hbb[c] > 0.0) worst = std::max(worst, std::sqrt(hrr_pin[c] / hbb[c]));
+        // learned preconditioner: back to Jacobi (and a CG restart) when the
+        // residual falls by less than half per iteration over three checks
+        // running
+        if (learned_on && !no_fallback && !done) {
+            const double per_it = prev_worst > 0.0 ? std::pow(worst / prev_worst, 1.0 / chunk) : 0.0;
+            slow = (per_it > 0.5) ? slow + 1 : 0;
+            if (slow >= 3) {
+                learned_on = false;
+                st.fallback_used = 1;
+                start_direction(true, enq);
+            }
+        }
+        prev_worst = worst;
+        if (verbose && (enq % 256 == 0)) fprintf(stderr, "[pcg] it %d worst %.3e\n", iters, worst);
     }
     if (singular) {
-        cd.run(ctx, ones_blk.as<double>(), dX.as<double>(), sums.as<double>());
-        k_sub_colmean<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dX.as<double>(), sums.as<double>());
+        L.coldot(dX.as<double>(), nullptr, ptmp, sums);
+        k_sub_colmean<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, dX.as<double>(), sums);
+        LH_CHECK_LAUNCH();
     }
-    d2h(ctx, hb.data(), dX.p, n * m * 8);
+    if (ro) {
+        ro->unpermute(ctx, dX.as<double>(), m, dZ.as<double>(), m, n, m);
+        d2h(ctx, hb.data(), dZ.p, blk);
+    } else {
+        d2h(ctx, hb.data(), dX.p, blk);
+    }
+    int nconv = 0;
+    for (int c = 0; c < m; ++c) nconv += (hbb[c] == 0.0 || hrr_pin[c] <= htol[c] || done) ? 1 : 0;
     for (int64_t i = 0; i < n; ++i) {
         const bool dir = any_dirichlet && dmask[i];
         for (int c = 0; c < m; ++c)
-            if (!dir) x[(int64_t)c * n + i] = hb[i * m + c];
+            if (!dir) x[(int64_t)c * n + i] = hbb[c] > 0.0 ? hb[i * m + c] : 0.0;  // b = 0: x = 0
     }
-    st.iterations = it;
+    st.iterations = iters;
     st.converged = nconv;
     st.max_resid = worst;
     st.t_total_us = now_us() - t0;

@@ -16,8 +16,10 @@ seeded vectors agree with the oracle's at 256 probe vertices to
 PROJ_TOL * max |P_g r| (a principal angle theta moves P_g r by at most
 2 sin(theta) |r|_B; the eigenvector bar theta <= 1e-6 is the tolerance's
 scale).  The last cluster is skipped: k columns may cut it short.
-Heat: PCG to relative residual 1e-10; probe values within 1e-8 of the
-largest probe value, and the sum over all vertices within 1e-9 relative.
+Heat: PCG to relative residual 1e-10 (checked on the downloaded matrices);
+probe values within 1e-8 of the largest probe value and the sum over all
+vertices within 1e-8 relative (the condition number of M + t A, ~50 here,
+times the residual bound).
 """
 
 import json
@@ -97,13 +99,23 @@ def test_volume_c64_k30_and_heat():
     assert ael == pytest.approx(gold["avg_edge_length"], rel=1e-14)
     m = 1e-3 / (ael * ael)
     u = heat.diffusion(mesh, 0, m=m, rtol=1e-10)
+    # the solve itself: (M_lumped + t A) u = e_0 to relative residual 1e-10
+    lumped = Solver(mesh, lump=True)
+    t_step = m * ael * ael
+    b0 = np.zeros(len(u))
+    b0[0] = 1.0
+    res = np.linalg.norm(lumped.mass @ u + t_step * (lumped.stiffness @ u) - b0)
+    print(f"heat: true relative residual {res:.3e}")
+    assert res <= 2e-10
+    # against splu: the error of a residual-1e-10 solve is bounded by
+    # cond(M + t A) * 1e-10, and cond ~ 1 + t lambda_max(M^-1 A) ~ 50 here
     idx = np.asarray(gold["probe_index"])
     ref = np.asarray(gold["heat_probe_values"])
     d = np.abs(u[idx] - ref).max() / np.abs(ref).max()
     s = abs(u.sum() - gold["heat_sum"]) / abs(gold["heat_sum"])
     print(f"heat: probe difference {d:.3e} of the largest probe, sum rel err {s:.3e}")
     assert d <= 1e-8
-    assert s <= 1e-9
+    assert s <= 1e-8
 
 
 def test_batch_256_k50():

@@ -296,3 +296,54 @@ def test_mass_reassignment_is_used(square):
     x = fem_.poisson(h, dtup=dtup)
     x_o = fem.poisson(fem_.stiffness, sparse.eye(n, format="csc"), h, dtup=dtup)
     np.testing.assert_allclose(x, x_o, rtol=1e-8, atol=1e-10 * np.abs(x_o).max())
+
+
+def test_pcg_many_rhs_warm_start_and_singular(square):
+    """lh_pcg directly: 11 right-hand sides (two launches of 8 and 3 columns
+    per kernel) against the oracle's splu with Dirichlet rows; a warm start
+    from the solution converges at once and one from a nearby point in fewer
+    iterations than from zero; a zero right-hand side gives x = 0 whatever the
+    start; the pure-Neumann (singular) system gives the mean-free solution."""
+    from scipy import sparse
+    from scipy.sparse.linalg import splu
+
+    from lapy_amd import _lib
+
+    ctx = _lib.default_context()
+    fem_ = Solver(square, lump=True)
+    a, b = fem_.stiffness, fem_.mass
+    A = _lib.DeviceCSR.from_scipy(ctx, a)
+    n = a.shape[0]
+    rng = np.random.default_rng(8)
+    rhs = rng.standard_normal((n, 11))
+    dmask = np.zeros(n, np.uint8)
+    dmask[[0, 5, 17]] = 1
+    free = np.flatnonzero(dmask == 0)
+    x, st = _lib.pcg(ctx, A, None, 0.0, 1.0, rhs, dmask=dmask, rtol=1e-12)
+    lu = splu(sparse.csc_matrix(a[free][:, free]))
+    x_o = lu.solve(rhs[free])
+    np.testing.assert_allclose(x[free], x_o, rtol=1e-8, atol=1e-10 * np.abs(x_o).max())
+    assert np.all(x[dmask == 1] == 0.0)  # left as given (zero start)
+    it_cold = st.iterations
+    x_w, st_w = _lib.pcg(ctx, A, None, 0.0, 1.0, rhs[:, 0], x0=x[:, 0], dmask=dmask, rtol=1e-10)
+    assert st_w.iterations == 0
+    near = x[:, 0] + 1e-6 * rng.standard_normal(n)
+    x_n, st_n = _lib.pcg(ctx, A, None, 0.0, 1.0, rhs[:, 0], x0=near, dmask=dmask, rtol=1e-12)
+    _, st_c = _lib.pcg(ctx, A, None, 0.0, 1.0, rhs[:, 0], dmask=dmask, rtol=1e-12)
+    print("iterations: 11 columns", it_cold, "warm", st_n.iterations, "cold", st_c.iterations)
+    assert st_n.iterations < st_c.iterations
+    np.testing.assert_allclose(x_n[free], x[free, 0], rtol=1e-8, atol=1e-10 * np.abs(x_o).max())
+    z, _ = _lib.pcg(ctx, A, None, 0.0, 1.0, np.zeros(n), x0=rng.standard_normal(n), dmask=dmask)
+    assert np.abs(z[free]).max() == 0.0
+    # singular (a closed FP64 surface, no Dirichlet rows, no mass term): the
+    # mean-free solution of A x = b - mean(b)
+    vs, ts = meshgen.perturbed_sphere(3, np.random.default_rng(1), axes=(1.3, 0.8, 1.0), sigma=0.02)
+    fs = Solver(TriaMesh(vs, ts), lump=True)
+    as_ = fs.stiffness
+    As = _lib.DeviceCSR.from_scipy(ctx, as_)
+    bs = rng.standard_normal((len(vs), 3))
+    xs, _ = _lib.pcg(ctx, As, None, 0.0, 1.0, bs, rtol=1e-12)
+    pb = bs - bs.mean(axis=0)
+    for c in range(3):
+        assert abs(xs[:, c].mean()) < 1e-12 * np.abs(xs[:, c]).max()
+        assert np.linalg.norm(as_ @ xs[:, c] - pb[:, c]) <= 1e-10 * np.linalg.norm(pb[:, c])
