@@ -257,18 +257,27 @@ int lh_time_cheb_step(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int iters, in
                       double* avg_us);
 /* ---- learned preconditioner (no counterpart in the reference) -------------
  * A 3-layer message-passing network of width 32 on the operator's graph,
- * linear in the residual: for K = beta A + alpha B, s = diag(K)^-1/2 and the
- * normalised off-diagonal weights W^ = S (K - diag K) S,
- *   H0 = (s r) u^T,  H_{l+1} = H_l Ws_l^T + (W^ H_l) Wn_l^T,  z = s (s r + H3 v)
- * (Jacobi plus a learned correction; v = 0 is exactly Jacobi).  Weights are
- * bfloat16 on the device, arithmetic FP32.  Parameters (lh_mpnn_param_count()
- * floats): u (32), then Ws_l, Wn_l (32 x 32 row-major each) for l = 0, 1, 2,
- * then v (32).  params NULL: seeded small random weights with v = 0.
+ * linear in the residual: for K = beta A + alpha B, s = diag(K)^-1/2, the
+ * normalised off-diagonal weights W^ = S (K - diag K) S and per-vertex
+ * features phi_i = (1, log(d_i / mean d), log(mu_i / mean mu),
+ * sum_j W^_ij) (d = diag K, mu = row sums of B; 0 without B), gates
+ * gam_l = 1 + tanh(phi Gam_l):
+ *   H0 = (s r) u^T,  H_{l+1} = gam_l * (H_l Ws_l^T + (W^ H_l) Wn_l^T),
+ *   Q x = H3 v,  z = s (s r + Q^T Q s r)
+ * (Jacobi plus a learned correction, symmetric positive definite; v = 0 is
+ * exactly Jacobi).  Weights and hidden states are bfloat16 on the device,
+ * the layer products run on the matrix cores (FP32 accumulation).
+ * Parameters (lh_mpnn_param_count() floats): u (32), then Ws_l, Wn_l
+ * (32 x 32 row-major each) for l = 0, 1, 2, then v (32), then Gam_l (4 x 32
+ * row-major) for l = 0, 1, 2.  params NULL: seeded small random weights with
+ * Gam = 0 (gates 1).
  * lh_ctx_set_mpnn selects the network LH_PRECOND_LEARNED uses in lh_eigs
  * (K = A - sigma B) and lh_pcg (K = alpha B + beta A); NULL clears it.
  * lh_mpnn_apply: Z = P R for host R, Z (n x nc row-major).
  * lh_mpnn_loss_grad: training loss L = (1/nc) sum_c ||S(K z_c - r_c)||^2 /
- * ||S r_c||^2 and dL/dparams (host, FP64). */
+ * ||S r_c||^2 and dL/dparams (host, FP64).
+ * lh_mpnn_time_layer: average duration (us) of one forward layer on an
+ * n x nc block (HIP events on the context's stream, `iters` launches). */
 int  lh_mpnn_param_count(void);
 int  lh_mpnn_create(lh_ctx* ctx, const float* params, lh_mpnn** out);
 void lh_mpnn_free(lh_mpnn* net);
@@ -279,6 +288,8 @@ int  lh_mpnn_apply(lh_mpnn* net, lh_csr* A, lh_csr* B, double alpha, double beta
                    double* Z);
 int  lh_mpnn_loss_grad(lh_mpnn* net, lh_csr* A, lh_csr* B, double alpha, double beta, const double* R, int nc,
                        double* loss, double* grad);
+int  lh_mpnn_time_layer(lh_mpnn* net, lh_csr* A, lh_csr* B, double alpha, double beta, int nc, int iters,
+                        double* avg_us);
 
 /* The eigensolver's tall-skinny block products on their own (no counterpart
  * in the reference, whose eigsh does its dense work inside ARPACK; exposed

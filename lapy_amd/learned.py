@@ -1,19 +1,23 @@
 """Learned message-passing preconditioner: training and use.
 
-The network (lh_mpnn, hand-written HIP forward and backward kernels) is a
-linear map Q: three message-passing layers of width 32 over the operator's
-normalised off-diagonal weights W^ = S (K - diag K) S, S = diag(K)^-1/2,
-lifted from and read out to one scalar per vertex.  The preconditioner is
+The network (lh_mpnn, hand-written HIP forward and backward kernels, the
+layer products on bfloat16 matrix cores) is a linear map Q: three
+message-passing layers of width 32 over the operator's normalised
+off-diagonal weights W^ = S (K - diag K) S, S = diag(K)^-1/2, each gated per
+vertex by features of the mesh (diagonal of K, lumped mass, row sums of
+W^), lifted from and read out to one scalar per vertex.  The preconditioner
+is
 
     z = S (I + Q^T Q) S r
 
 symmetric positive definite for any weights (as CG and LOBPCG require),
 exactly Jacobi when the readout is zero, with Q^T evaluated by the backward
 kernels.  It is trained to make K z close to r.  Training here: Adam on the
-host over the 6,208 parameters, loss and gradient from the GPU
+host over the 6,592 parameters, loss and gradient from the GPU
 (lh_mpnn_loss_grad), residuals drawn as white noise mixed with
 graph-smoothed noise on the training meshes' operators K = A + 0.01 B (the
-eigensolver's shift, sigma = -0.01).
+eigensolver's shift, sigma = -0.01), or taken from the eigensolver's own
+stopped runs.
 
 Use: ``net.use()`` makes it the context's LH_PRECOND_LEARNED for
 ``Solver.eigs(precond="learned")`` and ``heat.diffusion``'s PCG; both fall

@@ -30,21 +30,40 @@ def random_params(seed, scale=0.15):
     return (scale * np.random.default_rng(seed).standard_normal(mpnn.n_params())).astype(np.float32)
 
 
+V_SLICE = slice(mpnn.F + 2 * mpnn.LAYERS * mpnn.F * mpnn.F, 2 * mpnn.F + 2 * mpnn.LAYERS * mpnn.F * mpnn.F)
+
+
+def lumped_mass(B):
+    return np.asarray(B.to_scipy_csc().sum(axis=1)).ravel()
+
+
 def test_forward_matches_oracle_and_is_jacobi_at_zero_readout():
+    """The GPU map (bfloat16 states, matrix-core products, FP32 sums) against
+    the oracle with the same bfloat16 storage points: the two differ by FP32
+    against FP64 sums and the rare state that rounds to a neighbouring
+    bfloat16 value, so 2e-3 of max |z| bounds every entry and the typical
+    difference is far smaller."""
     ctx = _lib.default_context()
     v, t = mesh(2, 1)
     dm, A, B, K = device_ops(ctx, v, t)
+    mu = lumped_mass(B)
     p = random_params(3)
     net = _lib.Mpnn(ctx, p)
     R = np.random.default_rng(4).standard_normal((A.n, 5))
     Z = net.apply(A, B, 0.01, 1.0, R)
-    Zo = mpnn.forward(K, R, p.astype(np.float64))
-    assert np.abs(Z - Zo).max() <= 2e-5 * np.abs(Zo).max()
-    # linear in R (FP32 arithmetic)
+    Zo = mpnn.forward(K, R, p.astype(np.float64), mass=mu)
+    d = np.abs(Z - Zo)
+    print("forward: max", d.max() / np.abs(Zo).max(), "mean", d.mean() / np.abs(Zo).mean())
+    assert d.max() <= 2e-3 * np.abs(Zo).max()
+    assert d.mean() <= 2e-4 * np.abs(Zo).mean()
+    # the exact map (no bfloat16 states) is within bfloat16 precision
+    Ze = mpnn.forward(K, R, p.astype(np.float64), mass=mu, bf16_acts=False)
+    assert np.abs(Z - Ze).max() <= 3e-2 * np.abs(Ze).max()
+    # linear in R up to the states' rounding
     Z2 = net.apply(A, B, 0.01, 1.0, 2.0 * R[:, :1] - R[:, 1:2])
-    np.testing.assert_allclose(Z2[:, 0], 2.0 * Z[:, 0] - Z[:, 1], rtol=0, atol=1e-5 * np.abs(Z).max())
+    np.testing.assert_allclose(Z2[:, 0], 2.0 * Z[:, 0] - Z[:, 1], rtol=0, atol=2e-2 * np.abs(Z).max())
     p0 = p.copy()
-    p0[-mpnn.F:] = 0.0
+    p0[V_SLICE] = 0.0
     net.params = p0
     Z0 = net.apply(A, B, 0.01, 1.0, R)
     np.testing.assert_allclose(Z0, R / K.diagonal()[:, None], rtol=1e-6)
@@ -54,16 +73,22 @@ def test_loss_and_gradient_match_oracle():
     ctx = _lib.default_context()
     v, t = mesh(2, 5)
     dm, A, B, K = device_ops(ctx, v, t)
+    mu = lumped_mass(B)
     p = random_params(6, scale=0.2)
     net = _lib.Mpnn(ctx, p)
     R = np.random.default_rng(7).standard_normal((A.n, 4))
     loss, g = net.loss_grad(A, B, 0.01, 1.0, R)
-    loss_o, g_o = mpnn.loss_and_grad(K, R, p.astype(np.float64))
-    assert abs(loss - loss_o) <= 1e-5 * abs(loss_o)
-    assert np.linalg.norm(g - g_o) <= 1e-4 * np.linalg.norm(g_o), (np.linalg.norm(g - g_o), np.linalg.norm(g_o))
-    # each parameter block
-    for sl in (slice(0, mpnn.F), slice(mpnn.F, mpnn.F + mpnn.F * mpnn.F), slice(-mpnn.F, None)):
-        assert np.linalg.norm(g[sl] - g_o[sl]) <= 1e-4 * max(np.linalg.norm(g_o[sl]), 1e-12)
+    loss_o, g_o = mpnn.loss_and_grad(K, R, p.astype(np.float64), mass=mu)
+    print("loss", loss, loss_o, "grad rel", np.linalg.norm(g - g_o) / np.linalg.norm(g_o))
+    assert abs(loss - loss_o) <= 2e-3 * abs(loss_o)
+    assert np.linalg.norm(g - g_o) <= 1e-2 * np.linalg.norm(g_o), (np.linalg.norm(g - g_o), np.linalg.norm(g_o))
+    # each parameter block (u, Ws_0, Wn_0, ..., v, Gam_0..2)
+    F, L, NFI = mpnn.F, mpnn.LAYERS, mpnn.NFI
+    edges = [0, F] + [F + (i + 1) * F * F for i in range(2 * L)]
+    edges += [edges[-1] + F] + [edges[-1] + F + (i + 1) * NFI * F for i in range(L)]
+    for a, b in zip(edges[:-1], edges[1:]):
+        rel = np.linalg.norm(g[a:b] - g_o[a:b]) / max(np.linalg.norm(g_o[a:b]), 1e-12)
+        assert rel <= 3e-2, (a, b, rel)
     # deterministic
     loss2, g2 = net.loss_grad(A, B, 0.01, 1.0, R)
     assert loss2 == loss and np.array_equal(g2, g)
