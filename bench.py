@@ -286,6 +286,9 @@ def run_workload(args, ctx, _lib, meshgen):
                                "fallbacks": int(sum(r[2].fallback_used for r in res)),
                                "autotune_seconds": secs, "lobpcg_iterations_min_max": [min(its), max(its)],
                                "eig_rel_err_vs_oracle": err}})
+    elif args.workload == "learned":
+        run_learned(args, ctx, _lib, meshgen, out)
+        return
     else:  # pcie
         v, t = meshgen.large_ellipsoid(level=8)
         k = 50
@@ -311,6 +314,138 @@ def run_workload(args, ctx, _lib, meshgen):
     print(json.dumps(out), flush=True)
 
 
+def run_learned(args, ctx, _lib, meshgen, out):
+    """The learned preconditioner at its specified scale: trained on the
+    solver's own residuals of 512 seeded meshes of icosphere levels 4-7, then
+    applied inside the batched LOBPCG (k = 50) and the heat PCG on 64
+    held-out level-7 meshes (163,842 vertices each), against plain Jacobi
+    (the fallback and the plain form) and Chebyshev-Jacobi (the default).
+    Every solve ends at the same FP64 residual test, so the eigenvalues of
+    all forms agree; the line reports throughput, iterations and fallbacks of
+    each form and the roofline of the network's layer kernel."""
+    from lapy_amd import TriaMesh, learned
+
+    k = 50
+    t0 = time.perf_counter()
+    net = learned.LearnedPreconditioner(ctx=ctx)
+    if args.kernels_only:  # only the layer kernel's timing (for the profiler)
+        v, t = meshgen.held_out_mesh(0)
+        A0, B0 = _lib.DeviceMesh(ctx, v, t).assemble(_lib.LH_TRIA, False, np.float64)
+        us = net.net.time_layer(A0, B0, 0.01, 1.0, 64, iters=30)
+        us_u = net.net.time_layer(A0, B0, 0.01, 1.0, 64, iters=30, reorder=False)
+        print(json.dumps({"kernel": "k_mp_fwd", "avg_us_locality_order": us, "avg_us_mesh_order": us_u,
+                          "vertices": A0.n, "columns": 64}), flush=True)
+        return
+    hist = net.train_collection(meshgen.training_mesh, args.train_count, epochs=1, steps_per_mesh=2, ncol=8,
+                                lr=3e-3, seed=1, log=log)
+    train_s = time.perf_counter() - t0
+    log(f"[bench] trained on {args.train_count} meshes in {train_s:.1f} s: loss {np.mean(hist[:32]):.3f} -> "
+        f"{np.mean(hist[-64:]):.3f}")
+    held = [meshgen.held_out_mesh(i) for i in range(args.eval_count)]
+    dms = [_lib.DeviceMesh(ctx, v, t) for v, t in held]
+    mats = [m.assemble(_lib.LH_TRIA, False, np.float64) for m in dms]
+    # roofline of the layer kernel: one forward layer on an n x 64 block
+    A0, B0 = mats[0]
+    n0 = A0.n
+    nnz0 = A0.nnz
+    us_layer = net.net.time_layer(A0, B0, 0.01, 1.0, 64, iters=30)  # the solver's locality order
+    us_layer_mesh_order = net.net.time_layer(A0, B0, 0.01, 1.0, 64, iters=30, reorder=False)
+    pairs = n0 * 64
+    # algorithmic bytes: H_l read once and H_{l+1} written (64 B each per
+    # pair, bfloat16), the graph (rowptr, columns, FP32 weights) and the
+    # vertex features; flops: the two 32 x 32 products per pair
+    layer_bytes = 128 * pairs + 4 * (n0 + 1) + 8 * nnz0 + 16 * n0
+    layer_flops = 2 * 2 * 32 * 32 * pairs
+    ach = layer_bytes / (us_layer * 1e-6) / 1e9
+    log(f"[bench] layer kernel: {us_layer:.1f} us, {ach:.0f} GB/s, {layer_flops / (us_layer * 1e-6) / 1e12:.1f} TFLOP/s")
+    net.use()
+    # (name, options, meshes): the form without the fallback runs on the
+    # first chunk only, to show how the network itself fares
+    chunk = 16
+    nall = len(mats)
+    forms = [("chebyshev_fp64", {}, nall), ("jacobi_fp64", {"poly_degree": 1}, nall),
+             ("learned_fp64", {"precond": _lib.LH_PRECOND_LEARNED}, nall),
+             ("learned_mixed", {"precond": _lib.LH_PRECOND_LEARNED, "precision": _lib.LH_PREC_MIXED}, nall),
+             ("jacobi_mixed", {"poly_degree": 1, "precision": _lib.LH_PREC_MIXED}, nall),
+             ("learned_fp64_no_fallback", {"precond": _lib.LH_PRECOND_LEARNED, "no_fallback": True,
+                                           "max_iter": 2500}, min(chunk, nall))]
+    results = {}
+    ref = None
+    first_chunk = {}
+    for name, opts, cnt in forms:
+        t1 = time.perf_counter()
+        evs, its, fb, nconv = [], [], 0, 0
+        for c0 in range(0, cnt, chunk):
+            As = [a for a, _ in mats[c0:c0 + chunk]]
+            Bs = [b for _, b in mats[c0:c0 + chunk]]
+            try:
+                res = _lib.eigs_batched(ctx, As, Bs, [k] * len(As), workers=16, keep_device=True, **opts)
+            except _lib.NoConvergence as e:  # some problem stopped at max_iter: its last Ritz values
+                res = e.results
+            ctx.sync()
+            for r in res:
+                evs.append(r[0])
+                its.append(r[2].iterations)
+                fb += int(r[2].fallback_used)
+                nconv += int(r[2].converged < k)
+            del res
+            if c0 == 0:
+                first_chunk[name] = {"seconds": round(time.perf_counter() - t1, 3),
+                                     "lobpcg_iterations_median": int(np.median(its))}
+            log(f"[bench] {name}: {c0 + len(As)}/{cnt} meshes, {time.perf_counter() - t1:.1f} s")
+        dt = time.perf_counter() - t1
+        if ref is None:
+            ref = evs
+        err = max(rel_err(e, r) for e, r in zip(evs, ref))
+        results[name] = {"meshes": cnt, "eigenpairs_per_s": round(k * cnt / dt, 3), "seconds": round(dt, 3),
+                         "lobpcg_iterations_min_median_max": [int(min(its)), int(np.median(its)), int(max(its))],
+                         "fallbacks_to_jacobi": fb, "not_converged": nconv,
+                         "eig_rel_diff_vs_chebyshev_fp64": err}
+    # heat PCG on each held-out mesh (lumped mass, t = avg edge length^2)
+    heat = {"jacobi": [0, 0.0], "learned": [0, 0.0]}
+    hfb = 0
+    for (v, t), dm in zip(held, dms):
+        Al, Bl = dm.assemble(_lib.LH_TRIA, True, np.float64)
+        tt = TriaMesh(v, t).avg_edge_length() ** 2
+        b = np.zeros(Al.n)
+        b[0] = 1.0
+        for name, pc in (("jacobi", _lib.LH_PRECOND_JACOBI), ("learned", _lib.LH_PRECOND_LEARNED)):
+            t1 = time.perf_counter()
+            _, st = _lib.pcg(ctx, Al, Bl, 1.0, tt, b, rtol=1e-10, precond=pc)
+            heat[name][0] += st.iterations
+            heat[name][1] += time.perf_counter() - t1
+            if name == "learned":
+                hfb += int(st.fallback_used)
+    net.use(False)
+    m = len(mats)
+    heat_out = {name: {"mean_iterations": round(v[0] / m, 2), "mean_ms": round(1000 * v[1] / m, 3)}
+                for name, v in heat.items()}
+    heat_out["learned"]["fallbacks_to_jacobi"] = hfb
+    plain = results["jacobi_fp64"]["eigenpairs_per_s"]
+    out.update({
+        "metric": "eigenpairs/sec (k=50), 64 held-out icosphere L7 meshes, learned preconditioner in batched LOBPCG",
+        "value": results["learned_fp64"]["eigenpairs_per_s"], "ms_per_step": round(1000 * results["learned_fp64"][
+            "seconds"], 2), "steps": 1, "warmup": 0, "dtype": "f64 (network bf16, fp32 accumulation)",
+        "config": {"workload": f"train on {args.train_count} meshes (icosphere levels 4-7, solver residuals), "
+                               f"evaluate on {m} held-out level-7 meshes ({n0} vertices each), k={k}",
+                   "train_seconds": round(train_s, 2), "train_steps": int(len(hist)),
+                   "train_loss_first_last": [float(np.mean(hist[:32])), float(np.mean(hist[-64:]))],
+                   "forms": results,
+                   "speedup_vs_plain_fp64_jacobi": {name: round(r["eigenpairs_per_s"] / plain, 3)
+                                                    for name, r in results.items()},
+                   "first_16_meshes": first_chunk,
+                   "heat_pcg_rtol_1e-10": heat_out},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_mp_fwd (one forward layer, 16-pair tiles on v_mfma_f32_16x16x32_bf16), m=64, "
+                               "in the eigensolver's locality order",
+                     "avg_us": round(us_layer, 2), "avg_us_in_mesh_order": round(us_layer_mesh_order, 2),
+                     "algorithmic_bytes": int(layer_bytes),
+                     "tflops_bf16": round(layer_flops / (us_layer * 1e-6) / 1e12, 2)},
+    })
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,7 +467,10 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the PCIe-inclusive and mixed-precision figures of the L8 line")
     ap.add_argument("--verbose", type=int, default=0)
-    ap.add_argument("--workload", default="l8", choices=["l8", "small", "tet", "pcie", "batch", "geodesic"],
+    ap.add_argument("--train-count", type=int, default=512, help="learned workload: training meshes")
+    ap.add_argument("--eval-count", type=int, default=64, help="learned workload: held-out level-7 meshes")
+    ap.add_argument("--workload", default="l8",
+                    choices=["l8", "small", "tet", "pcie", "batch", "geodesic", "learned"],
                     help="l8: the benchmark line (default); small / tet / pcie / batch: the other workloads")
     ap.add_argument("--precond", default="cheb", choices=["cheb", "jacobi", "learned", "auto"],
                     help="batch workload: Chebyshev-Jacobi (default), plain Jacobi, the learned network, or auto "

@@ -277,7 +277,8 @@ int lh_time_cheb_step(lh_ctx* ctx, lh_csr* A, lh_csr* B, int ncol, int iters, in
  * lh_mpnn_loss_grad: training loss L = (1/nc) sum_c ||S(K z_c - r_c)||^2 /
  * ||S r_c||^2 and dL/dparams (host, FP64).
  * lh_mpnn_time_layer: average duration (us) of one forward layer on an
- * n x nc block (HIP events on the context's stream, `iters` launches). */
+ * n x nc block (HIP events on the context's stream, `iters` launches), in
+ * the eigensolver's locality order when reorder = 1 (as the solver runs it). */
 int  lh_mpnn_param_count(void);
 int  lh_mpnn_create(lh_ctx* ctx, const float* params, lh_mpnn** out);
 void lh_mpnn_free(lh_mpnn* net);
@@ -289,7 +290,7 @@ int  lh_mpnn_apply(lh_mpnn* net, lh_csr* A, lh_csr* B, double alpha, double beta
 int  lh_mpnn_loss_grad(lh_mpnn* net, lh_csr* A, lh_csr* B, double alpha, double beta, const double* R, int nc,
                        double* loss, double* grad);
 int  lh_mpnn_time_layer(lh_mpnn* net, lh_csr* A, lh_csr* B, double alpha, double beta, int nc, int iters,
-                        double* avg_us);
+                        int reorder, double* avg_us);
 
 /* The eigensolver's tall-skinny block products on their own (no counterpart
  * in the reference, whose eigsh does its dense work inside ARPACK; exposed

@@ -119,7 +119,7 @@ SIGNATURES = {
     "lh_ctx_set_mpnn": (C.c_int, [_vp, _vp]),
     "lh_mpnn_apply": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp, C.c_int, _vp]),
     "lh_mpnn_loss_grad": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp, C.c_int, _dp, _vp]),
-    "lh_mpnn_time_layer": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int, _dp]),
+    "lh_mpnn_time_layer": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, _dp]),
     "lh_block_gram": (C.c_int, [_vp, C.c_int64, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int,
                                 _dp]),
     "lh_block_gram_noalloc": (C.c_int, [_vp, C.c_int64, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp]),
@@ -507,12 +507,14 @@ class Mpnn:
               "lh_mpnn_loss_grad")
         return loss.value, g
 
-    def time_layer(self, A: DeviceCSR, B, alpha: float, beta: float, ncol: int, iters: int = 20) -> float:
-        """Average microseconds of one forward layer on an n x ncol block."""
+    def time_layer(self, A: DeviceCSR, B, alpha: float, beta: float, ncol: int, iters: int = 20,
+                   reorder: bool = True) -> float:
+        """Average microseconds of one forward layer on an n x ncol block (in
+        the eigensolver's locality order when reorder)."""
         us = C.c_double()
         check(lib().lh_mpnn_time_layer(self.handle, A.handle, B.handle if B is not None else None, float(alpha),
-                                       float(beta), int(ncol), int(iters), C.byref(us)), self.ctx,
-              "lh_mpnn_time_layer")
+                                       float(beta), int(ncol), int(iters), int(bool(reorder)), C.byref(us)),
+              self.ctx, "lh_mpnn_time_layer")
         return us.value
 
     def use(self, on: bool = True):

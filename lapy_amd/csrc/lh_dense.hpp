@@ -105,6 +105,7 @@ struct LearnedPrecond {
     void setup(lh_ctx* c, lh_mpnn* net, const lh_csr* A, const lh_csr* B, double alpha, double beta);
     // W (n x m, leading dimension ldw) <- P W
     void apply(double* W, int ldw, int m);
+    void apply(float* W, int ldw, int m);
 };
 
 // breadth-first level reordering of a symmetric pattern (lh_reorder.hip)

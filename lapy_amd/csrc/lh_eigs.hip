@@ -234,7 +234,7 @@ struct Lobpcg {
             learned.setup(ctx, ctx->mpnn, A, B, -sigma, 1.0);  // K = A - sigma B
             learned_on = true;
         }
-        constexpr int kStallWindow = 8;
+        constexpr int kStallWindow = 16;
         double hist[kStallWindow] = {};  // residual measure of the last iterations
 
         // start block
@@ -274,20 +274,23 @@ struct Lobpcg {
             }
             st.max_resid = worst;
             st.converged = nconv;
-            // learned preconditioner: back to Jacobi when the geometric mean
-            // of the wanted pairs' absolute residuals ||A x - theta B x|| /
-            // ||B x|| has not halved over the last kStallWindow iterations.
-            // (A test per iteration would reject every preconditioner of
-            // Jacobi's class: LOBPCG reduces the residual by 0.5 - 0.97 per
-            // iteration with them; relative residuals can rise early on, while
-            // the Ritz values fall towards the smallest eigenvalues.)
+            // learned preconditioner: back to Jacobi when it stalls, i.e. the
+            // geometric mean of the wanted pairs' absolute residuals
+            // ||A x - theta B x|| / ||B x|| has fallen by less than 10 % over
+            // the last kStallWindow iterations.  (A halving test rejects
+            // every preconditioner of Jacobi's class on large meshes:
+            // LOBPCG reduces the residual by 0.9 - 0.99 per iteration with
+            // them at 163,842 vertices, with the network as with Jacobi, whose
+            // iterations the network cut 3.6-fold there; relative residuals
+            // can rise early on, while the Ritz values fall towards the
+            // smallest eigenvalues.)
             if (learned_on && !no_fallback) {
                 double lg = 0.0;
                 for (int j = 0; j < k; ++j) lg += std::log(std::max(res[j], 1e-300));
                 const double gm = std::exp(lg / k);
                 const double before = hist[it % kStallWindow];  // iteration it - kStallWindow
                 hist[it % kStallWindow] = gm;
-                if (it >= kStallWindow && gm > 0.5 * before) {
+                if (it >= kStallWindow && gm > 0.9 * before) {
                     learned_on = false;
                     st.fallback_used = 1;
                     if (verbose) fprintf(stderr, "[lobpcg] it %d: learned preconditioner -> Jacobi\n", it);
@@ -313,7 +316,7 @@ struct Lobpcg {
             }
             // precondition W
             if (learned_on) {
-                if constexpr (sizeof(T) == 8) learned.apply(Sp() + m, ld, m);
+                learned.apply(Sp() + m, ld, m);
             } else if (poly_degree > 1) {
                 // soft locking of the polynomial: the wanted pairs that pass
                 // the convergence test keep a plain Jacobi direction in W;
@@ -524,8 +527,6 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
     L.poly_ratio = (opts && opts->poly_ratio > 1.0) ? opts->poly_ratio : 2.0 * deg * deg;
     const int prec = opts ? opts->precision : LH_PREC_FP64;
     if (prec != LH_PREC_FP64 && prec != LH_PREC_MIXED) LH_THROW(LH_ERR_ARG, "precision mode %d unknown", prec);
-    if (prec == LH_PREC_MIXED && pc == LH_PRECOND_LEARNED)
-        LH_THROW(LH_ERR_ARG, "the mixed-precision solver takes the Jacobi / Chebyshev-Jacobi preconditioner");
     std::unique_ptr<lh_eig> e(new lh_eig());
     e->ctx = ctx;
     e->n = n;

@@ -98,11 +98,23 @@ __device__ __forceinline__ f4 mma(u16x8 a, u16x8 b, f4 c) {
 }
 __device__ __forceinline__ u16x8 zero8() { return (u16x8){0, 0, 0, 0, 0, 0, 0, 0}; }
 
-// gate of vertex features ph (4) and output feature o of layer l
-__device__ __forceinline__ float gate_z(const float4 ph, const uint16_t* __restrict__ wb, int l, int o) {
+// the gates of every layer, vertex and feature, formed once per application:
+// gam[l][i][o] = 1 + tanh(phi_i . Gam_l[:, o]) (FP32)
+__global__ void k_mp_gates(int64_t n, const float4* __restrict__ phi, const uint16_t* __restrict__ wb,
+                           float* __restrict__ gam) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= NL * n * F) return;
+    const int o = (int)(gid % F);
+    const int64_t li = gid / F;
+    const int l = (int)(li / n);
+    const int64_t i = li - (int64_t)l * n;
+    const float4 ph = phi[i];
     const uint16_t* g = wb + off_gam(l) + o;
-    return ph.x * bf2f(g[0]) + ph.y * bf2f(g[F]) + ph.z * bf2f(g[2 * F]) + ph.w * bf2f(g[3 * F]);
+    const float z = ph.x * bf2f(g[0]) + ph.y * bf2f(g[F]) + ph.z * bf2f(g[2 * F]) + ph.w * bf2f(g[3 * F]);
+    gam[gid] = 1.0f + tanhf(z);
 }
+__device__ __forceinline__ float4 ld4f(const float* __restrict__ p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ float f4at(const float4 v, int r) { return r == 0 ? v.x : r == 1 ? v.y : r == 2 ? v.z : v.w; }
 
 // ---- set-up: s, W^ and the vertex features ---------------------------------
 __global__ void k_mp_scale(int64_t n, const int32_t* __restrict__ diagpos, const double* __restrict__ kv,
@@ -177,7 +189,8 @@ __global__ void k_mp_features(int64_t n, const double* __restrict__ d, const dou
 // ---- lift and readout -------------------------------------------------------
 // H[pc][f] = bf(x_pc w_f), x = X (leading dimension ldx) times s when given;
 // one thread per (pair, 8 features)
-__global__ void k_mp_lift(int64_t n, int nc, const double* __restrict__ X, int ldx, const float* __restrict__ s,
+template <typename XT>
+__global__ void k_mp_lift(int64_t n, int nc, const XT* __restrict__ X, int ldx, const float* __restrict__ s,
                           const uint16_t* __restrict__ wb, int woff, uint16_t* __restrict__ H) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n * nc * 4) return;
@@ -206,15 +219,16 @@ __global__ void k_mp_readout(int64_t npair, const uint16_t* __restrict__ H, cons
     Y[pc] = (double)y;
 }
 // z = s (s r + b), written over Z (may alias R: same element, same thread)
-__global__ void k_mp_combine(int64_t n, int nc, const double* R, int ldr, const float* __restrict__ s,
-                             const double* __restrict__ Bv, double* Z, int ldz) {
+template <typename XT>
+__global__ void k_mp_combine(int64_t n, int nc, const XT* R, int ldr, const float* __restrict__ s,
+                             const double* __restrict__ Bv, XT* Z, int ldz) {
     const int64_t pc = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pc >= n * nc) return;
     const int64_t i = pc / nc;
     const int c = (int)(pc - i * nc);
     const float si = s[i];
     const float x = si * (float)R[i * ldr + c];
-    Z[i * ldz + c] = (double)(si * (x + (float)Bv[pc]));
+    Z[i * ldz + c] = (XT)(si * (x + (float)Bv[pc]));
 }
 // part[block][f] = sum over the block's pairs of y_pc H[pc][f] (y = X (ldx)
 // times s when given): dv (H = H3, y = dy) and du (H = dH0, y = x)
@@ -250,7 +264,7 @@ k_mp_outer(int64_t n, int nc, const double* __restrict__ X, int ldx, const float
 // when given).  Tiles of 16 pairs, one per wave, grid-stride.
 __global__ void __launch_bounds__(TBS)
 k_mp_fwd(int64_t n, int nc, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
-         const float* __restrict__ w, const float4* __restrict__ phi, const uint16_t* __restrict__ Hin,
+         const float* __restrict__ w, const float* __restrict__ gam, const uint16_t* __restrict__ Hin,
          const uint16_t* __restrict__ wb, int l, uint16_t* __restrict__ Hout, uint16_t* __restrict__ Aout) {
     const int lane = threadIdx.x & 63;
     const int pn = lane & 15, q = lane >> 4;
@@ -263,8 +277,12 @@ k_mp_fwd(int64_t n, int nc, const int32_t* __restrict__ rowptr, const int32_t* _
         aws[hf] = ld8(wb + off_ws(l) + (hf * 16 + pn) * F + 8 * q);
         awn[hf] = ld8(wb + off_wn(l) + (hf * 16 + pn) * F + 8 * q);
     }
-    const int64_t wstride = (int64_t)gridDim.x * (TBS / 64);
-    for (int64_t t = (int64_t)blockIdx.x * (TBS / 64) + (threadIdx.x >> 6); t < ntile; t += wstride) {
+    // each workgroup walks a contiguous range of tiles, the ranges of one XCD
+    // adjacent (gridDim.x is a multiple of 8): in the solver's locality
+    // order the gathered rows then sit in that XCD's L2
+    const int64_t tpb = (ntile + gridDim.x - 1) / gridDim.x;
+    const int64_t tb0 = xcd_block() * tpb, tb1 = tb0 + tpb < ntile ? tb0 + tpb : ntile;
+    for (int64_t t = tb0 + (threadIdx.x >> 6); t < tb1; t += TBS / 64) {
         const int64_t p = t * 16 + pn;
         const bool ok = p < npair;
         u16x8 h = zero8(), a = zero8();
@@ -276,12 +294,26 @@ k_mp_fwd(int64_t n, int nc, const int32_t* __restrict__ rowptr, const int32_t* _
             float g[8];
 #pragma unroll
             for (int j = 0; j < 8; ++j) g[j] = 0.f;
-            for (int32_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
-                const float wij = w[e];
-                if (wij == 0.f) continue;
-                const u16x8 hj = ld8(Hin + ((int64_t)cols[e] * nc + c) * F + 8 * q);
+            // the row's slots eight at a time (a surface row has at most eight),
+            // all their column indices, then all their rows, loaded together;
+            // slots past the row's end and the diagonal slot have weight 0
+            const int32_t e0 = rowptr[i], e1 = rowptr[i + 1];
+            for (int32_t e = e0; e < e1; e += 8) {
+                int32_t cj[8];
+                float wj[8];
 #pragma unroll
-                for (int j = 0; j < 8; ++j) g[j] += wij * bf2f(hj[j]);
+                for (int u = 0; u < 8; ++u) {
+                    const bool in = e + u < e1;
+                    cj[u] = in ? cols[e + u] : (int32_t)i;
+                    wj[u] = in ? w[e + u] : 0.f;
+                }
+                u16x8 hj[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) hj[u] = ld8(Hin + ((int64_t)cj[u] * nc + c) * F + 8 * q);
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) g[j] += wj[u] * bf2f(hj[u][j]);
             }
 #pragma unroll
             for (int j = 0; j < 8; ++j) a[j] = f2bf(g[j]);
@@ -295,15 +327,13 @@ k_mp_fwd(int64_t n, int nc, const int32_t* __restrict__ rowptr, const int32_t* _
             d[hf] = mma(awn[hf], a, d[hf]);
         }
         if (!ok) continue;
-        const float4 ph = phi[i];
+        const float* gi = gam + ((int64_t)l * n + i) * F;
 #pragma unroll
         for (int hf = 0; hf < 2; ++hf) {
+            const float4 g4 = ld4f(gi + hf * 16 + 4 * q);
             u16x4 o;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int of = hf * 16 + 4 * q + r;
-                o[r] = f2bf((1.0f + tanhf(gate_z(ph, wb, l, of))) * d[hf][r]);
-            }
+            for (int r = 0; r < 4; ++r) o[r] = f2bf(f4at(g4, r) * d[hf][r]);
             *reinterpret_cast<u16x4*>(Hout + p * F + hf * 16 + 4 * q) = o;
         }
     }
@@ -328,9 +358,9 @@ struct BwdShared<true> {
 };
 template <bool GRADS>
 __global__ void __launch_bounds__(TBS)
-k_mp_bwd_m(int64_t n, int nc, const float4* __restrict__ phi, const uint16_t* __restrict__ G,
-           const uint16_t* __restrict__ Hl, const uint16_t* __restrict__ Al, const uint16_t* __restrict__ wb, int l,
-           uint16_t* __restrict__ Mout, double* __restrict__ part) {
+k_mp_bwd_m(int64_t n, int nc, const float4* __restrict__ phi, const float* __restrict__ gam,
+           const uint16_t* __restrict__ G, const uint16_t* __restrict__ Hl, const uint16_t* __restrict__ Al,
+           const uint16_t* __restrict__ wb, int l, uint16_t* __restrict__ Mout, double* __restrict__ part) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int pn = lane & 15, q = lane >> 4;
     const int64_t npair = n * nc;
@@ -357,17 +387,22 @@ k_mp_bwd_m(int64_t n, int nc, const float4* __restrict__ phi, const uint16_t* __
     for (int k = 0; k < NFI; ++k)
 #pragma unroll
         for (int j = 0; j < 8; ++j) ggam[k][j] = 0.f;
-    const int64_t wstride = (int64_t)gridDim.x * (TBS / 64);
-    for (int64_t t = (int64_t)blockIdx.x * (TBS / 64) + wv; t < ntile; t += wstride) {
+    const int64_t tpb = (ntile + gridDim.x - 1) / gridDim.x;
+    const int64_t tb0 = xcd_block() * tpb, tb1 = tb0 + tpb < ntile ? tb0 + tpb : ntile;
+    for (int64_t t = tb0 + wv; t < tb1; t += TBS / 64) {
         const int64_t p = t * 16 + pn;
         const bool ok = p < npair;
         u16x8 tv = zero8();
         float4 ph = make_float4(0.f, 0.f, 0.f, 0.f);
+        const float* gi = gam;
         if (ok) {
-            ph = phi[p / nc];
+            const int64_t i = p / nc;
+            if (GRADS) ph = phi[i];
+            gi = gam + ((int64_t)l * n + i) * F;
             const u16x8 gv = ld8(G + p * F + 8 * q);
+            const float4 ga = ld4f(gi + 8 * q), gb = ld4f(gi + 8 * q + 4);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) tv[j] = f2bf((1.0f + tanhf(gate_z(ph, wb, l, 8 * q + j))) * bf2f(gv[j]));
+            for (int j = 0; j < 8; ++j) tv[j] = f2bf((j < 4 ? f4at(ga, j) : f4at(gb, j - 4)) * bf2f(gv[j]));
         }
         f4 d[2];
 #pragma unroll
@@ -397,9 +432,10 @@ k_mp_bwd_m(int64_t n, int nc, const float4* __restrict__ phi, const uint16_t* __
 #pragma unroll
             for (int hf = 0; hf < 2; ++hf) {
                 const u16x4 g4 = *reinterpret_cast<const u16x4*>(G + p * F + hf * 16 + 4 * q);
+                const float4 gm = ld4f(gi + hf * 16 + 4 * q);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const float th = tanhf(gate_z(ph, wb, l, hf * 16 + 4 * q + r));
+                    const float th = f4at(gm, r) - 1.0f;  // tanh(z)
                     const float dg = bf2f(g4[r]) * pre[hf][r] * (1.0f - th * th);
                     ggam[0][hf * 4 + r] += ph.x * dg;
                     ggam[1][hf * 4 + r] += ph.y * dg;
@@ -485,7 +521,7 @@ k_mp_bwd_m(int64_t n, int nc, const float4* __restrict__ phi, const uint16_t* __
 // Part 2: Gout = bf(T Ws + W^ M), T = bf(gam_l * G) recomputed
 __global__ void __launch_bounds__(TBS)
 k_mp_bwd_g(int64_t n, int nc, const int32_t* __restrict__ rowptr, const int32_t* __restrict__ cols,
-           const float* __restrict__ w, const float4* __restrict__ phi, const uint16_t* __restrict__ G,
+           const float* __restrict__ w, const float* __restrict__ gam, const uint16_t* __restrict__ G,
            const uint16_t* __restrict__ M, const uint16_t* __restrict__ wb, int l, uint16_t* __restrict__ Gout) {
     const int lane = threadIdx.x & 63;
     const int pn = lane & 15, q = lane >> 4;
@@ -494,16 +530,21 @@ k_mp_bwd_g(int64_t n, int nc, const int32_t* __restrict__ rowptr, const int32_t*
     u16x8 awsT[2];
 #pragma unroll
     for (int hf = 0; hf < 2; ++hf) awsT[hf] = ld8(wb + off_wsT(l) + (hf * 16 + pn) * F + 8 * q);
-    const int64_t wstride = (int64_t)gridDim.x * (TBS / 64);
-    for (int64_t t = (int64_t)blockIdx.x * (TBS / 64) + (threadIdx.x >> 6); t < ntile; t += wstride) {
+    // each workgroup walks a contiguous range of tiles, the ranges of one XCD
+    // adjacent (gridDim.x is a multiple of 8): in the solver's locality
+    // order the gathered rows then sit in that XCD's L2
+    const int64_t tpb = (ntile + gridDim.x - 1) / gridDim.x;
+    const int64_t tb0 = xcd_block() * tpb, tb1 = tb0 + tpb < ntile ? tb0 + tpb : ntile;
+    for (int64_t t = tb0 + (threadIdx.x >> 6); t < tb1; t += TBS / 64) {
         const int64_t p = t * 16 + pn;
         const bool ok = p < npair;
         u16x8 tv = zero8();
         if (ok) {
-            const float4 ph = phi[p / nc];
+            const float* gi = gam + ((int64_t)l * n + p / nc) * F;
             const u16x8 gv = ld8(G + p * F + 8 * q);
+            const float4 ga = ld4f(gi + 8 * q), gb = ld4f(gi + 8 * q + 4);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) tv[j] = f2bf((1.0f + tanhf(gate_z(ph, wb, l, 8 * q + j))) * bf2f(gv[j]));
+            for (int j = 0; j < 8; ++j) tv[j] = f2bf((j < 4 ? f4at(ga, j) : f4at(gb, j - 4)) * bf2f(gv[j]));
         }
         f4 d[2];
 #pragma unroll
@@ -514,17 +555,32 @@ k_mp_bwd_g(int64_t n, int nc, const int32_t* __restrict__ rowptr, const int32_t*
         float g[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) g[j] = 0.f;
-        for (int32_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
-            const float wij = w[e];
-            if (wij == 0.f) continue;
-            const uint16_t* mj = M + ((int64_t)cols[e] * nc + c) * F + 4 * q;
-            const u16x4 m0 = *reinterpret_cast<const u16x4*>(mj);
-            const u16x4 m1 = *reinterpret_cast<const u16x4*>(mj + 16);
+        // eight slots at a time, indices then rows loaded together (weight 0
+        // past the row's end and on the diagonal)
+        const int32_t e0 = rowptr[i], e1 = rowptr[i + 1];
+        for (int32_t e = e0; e < e1; e += 8) {
+            int32_t cj[8];
+            float wj[8];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                g[r] += wij * bf2f(m0[r]);
-                g[4 + r] += wij * bf2f(m1[r]);
+            for (int u = 0; u < 8; ++u) {
+                const bool in = e + u < e1;
+                cj[u] = in ? cols[e + u] : (int32_t)i;
+                wj[u] = in ? w[e + u] : 0.f;
             }
+            u16x4 m0[8], m1[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint16_t* mj = M + ((int64_t)cj[u] * nc + c) * F + 4 * q;
+                m0[u] = *reinterpret_cast<const u16x4*>(mj);
+                m1[u] = *reinterpret_cast<const u16x4*>(mj + 16);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    g[r] += wj[u] * bf2f(m0[u][r]);
+                    g[4 + r] += wj[u] * bf2f(m1[u][r]);
+                }
         }
 #pragma unroll
         for (int hf = 0; hf < 2; ++hf) {
@@ -574,7 +630,7 @@ struct MpnnRun {
     lh_ctx* ctx = nullptr;
     std::shared_ptr<Pattern> pat;
     int64_t n = 0;
-    DevBuf kv, s, w, phi;
+    DevBuf kv, s, w, phi, gam;
     DevBuf HA[NL + 1], AA[NL], HB[NL + 1], AB[NL];  // states of two forward passes (bfloat16)
     DevBuf G0, G1, Mb;                              // adjoint / back-propagation (bfloat16)
     DevBuf fa, fb;                                  // scalar fields (n x nc, FP64)
@@ -628,14 +684,24 @@ struct MpnnRun {
         LH_CHECK_LAUNCH();
         HIPX(hipStreamSynchronize(ctx->stream));  // the temporaries go out of scope
     }
+    // the gates of the current weights (before each application)
+    void gates(const uint16_t* wb) {
+        gam.ensure((size_t)NL * n * F * 4);
+        k_mp_gates<<<nblocks((int64_t)NL * n * F, 256), 256, 0, ctx->stream>>>(n, phi.as<float4>(), wb,
+                                                                              gam.as<float>());
+        LH_CHECK_LAUNCH();
+    }
     size_t hbytes(int nc) const { return (size_t)n * nc * F * 2; }
+    // workgroups of the tile kernels: a multiple of 8 (one range per XCD slot)
     unsigned tile_blocks(int nc) const {
         const int64_t ntile = (n * nc + 15) / 16;
-        return (unsigned)std::max<int64_t>(1, std::min<int64_t>((ntile + 3) / 4, (int64_t)ctx->n_cu * 8));
+        const int64_t nb = std::max<int64_t>(1, std::min<int64_t>((ntile + 3) / 4, (int64_t)ctx->n_cu * 8));
+        return (unsigned)((nb + kXCDs - 1) / kXCDs * kXCDs);
     }
     // Y (dense n x nc) = Q x, x = X (ldx), times s when scaled.  store: keep
     // every layer's input and aggregate in H, A; else ping-pong H[0], H[1]
-    void q_forward(const uint16_t* wb, const double* X, int ldx, bool scaled, int nc, DevBuf* H, DevBuf* A,
+    template <typename XT>
+    void q_forward(const uint16_t* wb, const XT* X, int ldx, bool scaled, int nc, DevBuf* H, DevBuf* A,
                    bool store, double* Y) {
         for (int l = 0; l <= (store ? NL : 1); ++l) H[l].ensure(hbytes(nc));
         if (store)
@@ -647,7 +713,7 @@ struct MpnnRun {
             uint16_t* hout = H[store ? l + 1 : ((l + 1) & 1)].as<uint16_t>();
             k_mp_fwd<<<tile_blocks(nc), TBS, 0, ctx->stream>>>(n, nc, pat->rowptr.as<int32_t>(),
                                                               pat->cols.as<int32_t>(), w.as<float>(),
-                                                              phi.as<float4>(), hin, wb, l, hout,
+                                                              gam.as<float>(), hin, wb, l, hout,
                                                               store ? A[l].as<uint16_t>() : nullptr);
         }
         k_mp_readout<<<nblocks(n * nc, 256), 256, 0, ctx->stream>>>(n * nc, H[store ? NL : (NL & 1)].as<uint16_t>(),
@@ -664,12 +730,12 @@ struct MpnnRun {
         DevBuf* g = &G0;
         DevBuf* gn = &G1;
         for (int l = NL - 1; l >= 0; --l) {
-            k_mp_bwd_m<false><<<tile_blocks(nc), TBS, 0, ctx->stream>>>(n, nc, phi.as<float4>(), g->as<uint16_t>(),
-                                                                       nullptr, nullptr, wb, l, Mb.as<uint16_t>(),
-                                                                       nullptr);
+            k_mp_bwd_m<false><<<tile_blocks(nc), TBS, 0, ctx->stream>>>(n, nc, phi.as<float4>(), gam.as<float>(),
+                                                                       g->as<uint16_t>(), nullptr, nullptr, wb, l,
+                                                                       Mb.as<uint16_t>(), nullptr);
             k_mp_bwd_g<<<tile_blocks(nc), TBS, 0, ctx->stream>>>(n, nc, pat->rowptr.as<int32_t>(),
                                                                 pat->cols.as<int32_t>(), w.as<float>(),
-                                                                phi.as<float4>(), g->as<uint16_t>(),
+                                                                gam.as<float>(), g->as<uint16_t>(),
                                                                 Mb.as<uint16_t>(), wb, l, gn->as<uint16_t>());
             std::swap(g, gn);
         }
@@ -694,15 +760,16 @@ struct MpnnRun {
         DevBuf* gn = &G1;
         const unsigned tb = tile_blocks(nc);
         for (int l = NL - 1; l >= 0; --l) {
-            k_mp_bwd_m<true><<<tb, TBS, 0, ctx->stream>>>(n, nc, phi.as<float4>(), g->as<uint16_t>(),
-                                                          H[l].as<uint16_t>(), A[l].as<uint16_t>(), wb, l,
-                                                          Mb.as<uint16_t>(), partw.as<double>());
+            k_mp_bwd_m<true><<<tb, TBS, 0, ctx->stream>>>(n, nc, phi.as<float4>(), gam.as<float>(),
+                                                          g->as<uint16_t>(), H[l].as<uint16_t>(),
+                                                          A[l].as<uint16_t>(), wb, l, Mb.as<uint16_t>(),
+                                                          partw.as<double>());
             k_mp_reduce<<<nblocks(2 * F * F, 256), 256, 0, ctx->stream>>>((int)tb, 2 * F * F, WLEN,
                                                                           partw.as<double>(), gdev + off_ws(l));
             k_mp_reduce<<<1, NFI * F, 0, ctx->stream>>>((int)tb, NFI * F, WLEN, partw.as<double>() + 2 * F * F,
                                                         gdev + off_gam(l));
             k_mp_bwd_g<<<tb, TBS, 0, ctx->stream>>>(n, nc, pat->rowptr.as<int32_t>(), pat->cols.as<int32_t>(),
-                                                    w.as<float>(), phi.as<float4>(), g->as<uint16_t>(),
+                                                    w.as<float>(), gam.as<float>(), g->as<uint16_t>(),
                                                     Mb.as<uint16_t>(), wb, l, gn->as<uint16_t>());
             std::swap(g, gn);
         }
@@ -712,10 +779,12 @@ struct MpnnRun {
         k_mp_reduce<<<1, 64, 0, ctx->stream>>>(kPairBlocks, F, F, partv.as<double>(), gdev + off_u());
         LH_CHECK_LAUNCH();
     }
-    // W (n x m, ldw) <- P W = S (I + Q^T Q) S W, in place
-    void apply(const uint16_t* wb, double* W, int ldw, int m) {
+    // W (n x m, ldw) <- P W = S (I + Q^T Q) S W, in place (FP64 or FP32 blocks)
+    template <typename XT>
+    void apply(const uint16_t* wb, XT* W, int ldw, int m) {
         fa.ensure((size_t)n * m * 8);
         fb.ensure((size_t)n * m * 8);
+        gates(wb);
         q_forward(wb, W, ldw, true, m, HA, AA, false, fa.as<double>());
         q_adjoint(wb, fa.as<double>(), m, fb.as<double>());
         k_mp_combine<<<nblocks(n * m, 256), 256, 0, ctx->stream>>>(n, m, W, ldw, s.as<float>(), fb.as<double>(), W,
@@ -727,13 +796,14 @@ struct MpnnRun {
         for (int l = 0; l < 2; ++l) HA[l].ensure(hbytes(m));
         AA[0].ensure(hbytes(m));
         HIPX(hipMemsetAsync(HA[0].p, 0, hbytes(m), ctx->stream));
+        gates(wb);
         hipEvent_t e0, e1;
         HIPX(hipEventCreate(&e0));
         HIPX(hipEventCreate(&e1));
         HIPX(hipEventRecord(e0, ctx->stream));
         for (int it = 0; it < iters; ++it)
             k_mp_fwd<<<tile_blocks(m), TBS, 0, ctx->stream>>>(n, m, pat->rowptr.as<int32_t>(), pat->cols.as<int32_t>(),
-                                                             w.as<float>(), phi.as<float4>(), HA[0].as<uint16_t>(),
+                                                             w.as<float>(), gam.as<float>(), HA[0].as<uint16_t>(),
                                                              wb, it % NL, HA[1].as<uint16_t>(), nullptr);
         HIPX(hipEventRecord(e1, ctx->stream));
         HIPX(hipEventSynchronize(e1));
@@ -753,6 +823,7 @@ void mpnn_loss_grad(MpnnRun& run, const uint16_t* wb, const double* R, int nc, d
     DevBuf a(n * nc * 8), b(n * nc * 8), Z(n * nc * 8), KZ(n * nc * 8), Tt(n * nc * 8), SR(n * nc * 8),
         gb(n * nc * 8), ga(n * nc * 8);
     // z = s (x + Q^T Q x), x = s r
+    run.gates(wb);
     run.q_forward(wb, R, nc, true, nc, run.HA, run.AA, true, a.as<double>());
     run.q_adjoint(wb, a.as<double>(), nc, b.as<double>());
     k_mp_combine<<<nblocks(n * nc, 256), 256, 0, ctx->stream>>>(n, nc, R, nc, run.s.as<float>(), b.as<double>(),
@@ -829,6 +900,7 @@ void LearnedPrecond::setup(lh_ctx* c, lh_mpnn* net_, const lh_csr* A, const lh_c
     run->setup(c, A, B, alpha, beta);
 }
 void LearnedPrecond::apply(double* W, int ldw, int m) { run->apply(net->wbf.as<uint16_t>(), W, ldw, m); }
+void LearnedPrecond::apply(float* W, int ldw, int m) { run->apply(net->wbf.as<uint16_t>(), W, ldw, m); }
 LearnedPrecond::LearnedPrecond() = default;
 LearnedPrecond::~LearnedPrecond() = default;
 
@@ -939,12 +1011,27 @@ int lh_mpnn_loss_grad(lh_mpnn* net, lh_csr* A, lh_csr* B, double alpha, double b
 }
 
 int lh_mpnn_time_layer(lh_mpnn* net, lh_csr* A, lh_csr* B, double alpha, double beta, int nc, int iters,
-                       double* avg_us) {
+                       int reorder, double* avg_us) {
     if (!net || !A || !avg_us || nc <= 0 || iters <= 0) return LH_ERR_ARG;
     lh_ctx* ctx = net->ctx;
     LH_API_BEGIN(ctx)
     HIPX(hipSetDevice(ctx->device));
     check_mpnn_args(A, B, A->n());
+    std::unique_ptr<lh_csr> Ap, Bp;
+    if (reorder && (!B || B->is_diag || B->pat == A->pat)) {
+        // the eigensolver's locality order, as the solver applies the network
+        if (!A->pat->ro) {
+            ArenaOff off;
+            auto r = std::make_shared<Reorder>();
+            r->build(ctx, *A->pat);
+            A->pat->ro = r;
+        }
+        const Reorder& ro = *A->pat->ro;
+        Ap = ro.apply(ctx, A);
+        if (B) Bp = ro.apply(ctx, B, B->is_diag ? nullptr : ro.pat);
+        A = Ap.get();
+        if (B) B = Bp.get();
+    }
     MpnnRun run;
     run.setup(ctx, A, B, alpha, beta);
     *avg_us = run.time_layers(net->wbf.as<uint16_t>(), nc, iters);

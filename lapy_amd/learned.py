@@ -94,6 +94,45 @@ class LearnedPreconditioner:
         with np.load(path, allow_pickle=False) as f:
             return cls(f["params"], ctx)
 
+    def train_collection(self, mesh_at, count: int, epochs: int = 1, steps_per_mesh: int = 2, ncol: int = 8,
+                         lr: float = 2e-3, seed: int = 0, iterations=(1, 2, 3, 5, 8, 12), log=None):
+        """Adam over a large collection on the solver's own residuals, one
+        mesh at a time: mesh ``mesh_at(i)`` ((v, t), i < count, visited in a
+        seeded random order each epoch) is assembled on the GPU, plain
+        Jacobi-LOBPCG runs on it stopped after each of ``iterations`` give
+        its residual blocks, ``steps_per_mesh`` Adam steps are taken on
+        batches of ``ncol`` of their columns, and the mesh is released.
+        Returns the loss history; ``log`` (a callable) gets progress lines."""
+        rng = np.random.default_rng(seed)
+        theta = self.params.astype(np.float64)
+        m1 = np.zeros_like(theta)
+        m2 = np.zeros_like(theta)
+        b1, b2, eps = 0.9, 0.999, 1e-8
+        hist = []
+        it = 0
+        for ep in range(epochs):
+            for pos, i in enumerate(rng.permutation(count)):
+                v, t = mesh_at(int(i))
+                dm = _lib.DeviceMesh(self.ctx, v, t)
+                A, B = dm.assemble(_lib.LH_TRIA, False, np.float64)
+                Ah, Bh = A.to_scipy_csc().tocsr(), B.to_scipy_csc().tocsr()
+                pool = solver_residuals(self.ctx, A, B, Ah, Bh, ncol, iterations, rng)
+                for _ in range(steps_per_mesh):
+                    blk = pool[int(rng.integers(len(pool)))]
+                    cols = rng.choice(blk.shape[1], size=min(ncol, blk.shape[1]), replace=False)
+                    R = np.ascontiguousarray(blk[:, cols])
+                    loss, g = self.net.loss_grad(A, B, SHIFT, 1.0, R)
+                    it += 1
+                    hist.append(loss)
+                    m1 = b1 * m1 + (1 - b1) * g
+                    m2 = b2 * m2 + (1 - b2) * g * g
+                    theta = theta - lr * (m1 / (1 - b1 ** it)) / (np.sqrt(m2 / (1 - b2 ** it)) + eps)
+                    self.params = theta.astype(np.float32)
+                del dm, A, B, Ah, Bh, pool
+                if log and (pos + 1) % 64 == 0:
+                    log(f"[mpnn] epoch {ep} mesh {pos + 1}/{count}: loss {np.mean(hist[-64 * steps_per_mesh:]):.4f}")
+        return np.asarray(hist)
+
     def train(self, meshes, steps: int = 200, ncol: int = 8, lr: float = 2e-3, seed: int = 0, log_every: int = 0,
               residuals: str = "noise"):
         """Adam on the mean loss over residual batches drawn from the given

@@ -89,17 +89,33 @@ def large_ellipsoid(seed: int = 0, level: int = 8):
     return perturbed_sphere(level, rng, axes=axes, sigma=0.01)
 
 
-def shape_batch(count: int = 256, seed: int = 0, levels=(3, 4, 5, 6)):
-    """``count`` ellipsoids of uneven size: level ~ U{levels}, axes U[0.5,2]^3,
+def shape_mesh(i: int, seed: int = 0, levels=(3, 4, 5, 6)):
+    """Mesh ``i`` of a seeded collection: level ~ U{levels}, axes U[0.5,2]^3,
     noise sigma ~ U[0, 0.05]."""
-    out = []
-    for i in range(count):
-        rng = np.random.default_rng(SEED_BASE + seed + 1000 + i)
-        lev = int(rng.choice(levels))
-        axes = rng.uniform(0.5, 2.0, size=3)
-        sig = rng.uniform(0.0, 0.05)
-        out.append(perturbed_sphere(lev, rng, axes=axes, sigma=sig))
-    return out
+    rng = np.random.default_rng(SEED_BASE + seed + 1000 + i)
+    lev = int(rng.choice(levels))
+    axes = rng.uniform(0.5, 2.0, size=3)
+    sig = rng.uniform(0.0, 0.05)
+    return perturbed_sphere(lev, rng, axes=axes, sigma=sig)
+
+
+def shape_batch(count: int = 256, seed: int = 0, levels=(3, 4, 5, 6)):
+    """``count`` ellipsoids of uneven size (``shape_mesh`` 0 .. count-1)."""
+    return [shape_mesh(i, seed, levels) for i in range(count)]
+
+
+# the learned preconditioner's collections: 512 training meshes of levels
+# 4-7 and 64 held-out level-7 meshes (seeds disjoint from each other and from
+# the 256-mesh batch)
+TRAIN_SEED, HELD_OUT_SEED = 70_000, 90_000
+
+
+def training_mesh(i: int):
+    return shape_mesh(i, TRAIN_SEED, levels=(4, 5, 6, 7))
+
+
+def held_out_mesh(i: int):
+    return shape_mesh(i, HELD_OUT_SEED, levels=(7,))
 
 
 def tet_cube(cells: int, jitter: float = 0.0, seed: int = 0):

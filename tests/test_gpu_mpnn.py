@@ -245,3 +245,46 @@ def test_auto_uses_the_network_only_where_the_polynomial_is_jacobi(trained):
     ev_a, _, st_a = _lib.eigs(ctx, A, B, 10, precond=_lib.LH_PRECOND_AUTO, poly_degree=1)
     ev_j, _, st_j = _lib.eigs(ctx, A, B, 10, poly_degree=1)
     np.testing.assert_array_equal(ev_a, ev_j)
+
+
+def test_collection_pipeline_reduced():
+    """The learned-preconditioner pipeline of the benchmark at reduced scale:
+    train_collection on 12 seeded meshes (levels 3-4, the solver's own
+    residuals, one mesh at a time), then the batched LOBPCG (k = 10) on four
+    held-out level-4 meshes with the network in FP64 and in the mixed
+    precision solve, against plain Jacobi and Chebyshev-Jacobi: the same
+    eigenvalues (every form ends at the same FP64 residual test, 1e-8 here),
+    fewer iterations than plain Jacobi; the heat PCG with the network equals
+    the Jacobi one."""
+    ctx = _lib.default_context()
+    net = learned.LearnedPreconditioner(ctx=ctx)
+    lines = []
+    hist = net.train_collection(lambda i: meshgen.shape_mesh(i, 60_000, levels=(3, 4)), 12, epochs=2,
+                                steps_per_mesh=4, ncol=8, lr=3e-3, seed=2, log=lines.append)
+    assert len(hist) == 12 * 2 * 4
+    assert np.mean(hist[-16:]) < np.mean(hist[:8])
+    held = [meshgen.shape_mesh(i, 61_000, levels=(4,)) for i in range(4)]
+    mats = [_lib.DeviceMesh(ctx, v, t).assemble(_lib.LH_TRIA, False, np.float64) for v, t in held]
+    As, Bs = [a for a, _ in mats], [b for _, b in mats]
+    net.use()
+    try:
+        forms = {"cheb": {}, "jacobi": {"poly_degree": 1}, "learned": {"precond": _lib.LH_PRECOND_LEARNED},
+                 "learned_mixed": {"precond": _lib.LH_PRECOND_LEARNED, "precision": _lib.LH_PREC_MIXED}}
+        res = {name: _lib.eigs_batched(ctx, As, Bs, [10] * 4, workers=4, **o) for name, o in forms.items()}
+        for name, r in res.items():
+            for a, b in zip(r, res["cheb"]):
+                np.testing.assert_allclose(a[0], b[0], rtol=1e-8, atol=1e-10)
+        its = {name: sum(x[2].iterations for x in r) for name, r in res.items()}
+        print("batched LOBPCG iterations (4 meshes):", its)
+        assert its["learned"] < its["jacobi"]
+        assert all(x[2].precision_mode == 1 for x in res["learned_mixed"])
+        v, t = held[0]
+        Al, Bl = _lib.DeviceMesh(ctx, v, t).assemble(_lib.LH_TRIA, True, np.float64)
+        tt = TriaMesh(v, t).avg_edge_length() ** 2
+        b = np.zeros(Al.n)
+        b[0] = 1.0
+        xj, _ = _lib.pcg(ctx, Al, Bl, 1.0, tt, b, rtol=1e-10)
+        xl, _ = _lib.pcg(ctx, Al, Bl, 1.0, tt, b, rtol=1e-10, precond=_lib.LH_PRECOND_LEARNED)
+        np.testing.assert_allclose(xl, xj, rtol=0, atol=1e-8 * np.abs(xj).max())
+    finally:
+        net.use(False)
