@@ -150,6 +150,24 @@ int lh_assemble_into(lh_ctx* ctx, lh_mesh* m, int kind, int lump,
 enum { LH_DG_GRADIENT = 0, LH_DG_DIVERGENCE = 1, LH_DG_GEODESIC_RHS = 2 };
 int lh_diffgeo(lh_ctx* ctx, lh_mesh* m, int mode, const double* in, int nf, double* out);
 
+/* ---- principal curvatures (triangle meshes) ---------------------------------
+ * lh_curvature replaces TriaMesh.curvature (lapy/tria_mesh.py:1078-1219):
+ * per vertex (nv rows, host outputs, any may be NULL) the unit principal
+ * directions u_min, u_max (nv x 3 row-major), the curvatures c_min <= c_max,
+ * c_mean = (c_min + c_max) / 2 and c_gauss = c_min c_max (taken before the
+ * two are ordered, as in the reference), and the normals (nv x 3), after
+ * `smoothit` neighbour-averaging steps of the vertex tensors.
+ * lh_curvature_tria replaces TriaMesh.curvature_tria (lapy/tria_mesh.py:
+ * 1221-1279): per triangle u_min, u_max (ne x 3) and c_min, c_max (ne), the
+ * input of LH_TRIA_ANISO.  Eigenvector signs: the largest-magnitude
+ * component is positive; a triangle's corners' u_min are turned to agree
+ * with the first corner's before they are averaged (the reference keeps
+ * LAPACK's signs).  FP64 arithmetic; the mesh must be oriented. */
+int lh_curvature(lh_ctx* ctx, lh_mesh* m, int smoothit, double* u_min, double* u_max, double* c_min,
+                 double* c_max, double* c_mean, double* c_gauss, double* normals);
+int lh_curvature_tria(lh_ctx* ctx, lh_mesh* m, int smoothit, double* u_min, double* u_max, double* c_min,
+                      double* c_max);
+
 /* ---- CSR matrices ----------------------------------------------------------- */
 /* A CSR of a symmetric matrix equals its CSC, which is how the façade hands
  * back scipy.sparse.csc_matrix (lapy/solver.py:100-101). */

@@ -111,6 +111,8 @@ SIGNATURES = {
     "lh_time_cheb_step": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     "lh_eigs_batched": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_double, _vp, C.c_int, _vp, _vp, _vp]),
     "lh_eigs_autotune": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_double, _vp, _vp, _vp]),
+    "lh_curvature": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lh_curvature_tria": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
     "lh_mpnn_param_count": (C.c_int, []),
     "lh_mpnn_create": (C.c_int, [_vp, _vp, _vp]),
     "lh_mpnn_free": (None, [_vp]),
@@ -645,3 +647,23 @@ def small_symeig(ctx: Context, H, nev: int, profile: bool = False):
     ph = np.zeros(7) if profile else None
     check(lib().lh_small_symeig(ctx.handle, N, nev, _ptr(H), _ptr(ev), _ptr(Z), _ptr(ph)), ctx, "lh_small_symeig")
     return (ev, Z, ph) if profile else (ev, Z)
+
+
+def curvature(ctx: Context, dm: "DeviceMesh", smoothit: int):
+    """lh_curvature: (u_min, u_max, c_min, c_max, c_mean, c_gauss, normals)."""
+    nv = dm.nv
+    u_min, u_max, normals = (np.empty((nv, 3)) for _ in range(3))
+    c_min, c_max, c_mean, c_gauss = (np.empty(nv) for _ in range(4))
+    check(lib().lh_curvature(ctx.handle, dm.handle, int(smoothit), _ptr(u_min), _ptr(u_max), _ptr(c_min), _ptr(c_max),
+                             _ptr(c_mean), _ptr(c_gauss), _ptr(normals)), ctx, "lh_curvature")
+    return u_min, u_max, c_min, c_max, c_mean, c_gauss, normals
+
+
+def curvature_tria(ctx: Context, dm: "DeviceMesh", smoothit: int):
+    """lh_curvature_tria: (u_min, u_max, c_min, c_max) per triangle."""
+    ne = dm.ne
+    u_min, u_max = np.empty((ne, 3)), np.empty((ne, 3))
+    c_min, c_max = np.empty(ne), np.empty(ne)
+    check(lib().lh_curvature_tria(ctx.handle, dm.handle, int(smoothit), _ptr(u_min), _ptr(u_max), _ptr(c_min),
+                                  _ptr(c_max)), ctx, "lh_curvature_tria")
+    return u_min, u_max, c_min, c_max

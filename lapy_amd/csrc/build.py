@@ -1,9 +1,11 @@
 """Build liblapyhip.so for gfx950 with hipcc (no cmake).
 
 Every .hip source is compiled to an object with its own flags, then linked
-into lapy_amd/_lib/liblapyhip.so.  The assembly source is compiled with
--ffp-contract=off: its element arithmetic must not fuse a*b+c so that it
-reproduces numpy's rounding exactly.
+into lapy_amd/_lib/liblapyhip.so.  The assembly and curvature sources are
+compiled with -ffp-contract=off: their element arithmetic must not fuse
+a*b+c so that it reproduces numpy's rounding (the curvature's dihedral angles
+are arccos of dot products near 1, where a one-ulp difference in the dot
+product is amplified).
 """
 
 from __future__ import annotations
@@ -42,6 +44,7 @@ SOURCES = {
     "lh_pcg.hip": [],
     "lh_mpnn.hip": [],
     "lh_rrlib.hip": [],
+    "lh_curv.hip": ["-ffp-contract=off"],
 }
 # rocSOLVER (and its rocBLAS) only for the Rayleigh-Ritz of wide trial bases
 LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]

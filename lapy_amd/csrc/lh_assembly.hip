@@ -1492,3 +1492,6 @@ int lh_diffgeo(lh_ctx* ctx, lh_mesh* m, int mode, const double* in, int nf, doub
 }
 
 }  // extern "C"
+
+// the symbolic phase for other translation units (curvature)
+void lh_build_topology(lh_ctx* ctx, lh_mesh* m) { build_topology(ctx, m); }

@@ -143,6 +143,10 @@ struct lh_ctx {
 // frees ctx->blas (lh_rrlib.hip)
 void release_blas(lh_ctx* ctx);
 
+// incidence lists, CSR pattern and AoS coordinates of a mesh, built once
+// (lh_assembly.hip)
+void lh_build_topology(lh_ctx* ctx, lh_mesh* m);
+
 // device address of lh_block_gram_noalloc's fixed scratch on `device`, looked
 // up once per device with that device current (lh_mfma.hip; called by
 // lh_ctx_create)

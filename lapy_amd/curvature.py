@@ -1,23 +1,28 @@
-"""Principal curvatures of a triangle mesh: the input of the anisotropic
-Laplace-Beltrami assembly (``Solver(aniso=...)``).
+"""Principal curvatures of a triangle mesh on the GPU: the input of the
+anisotropic Laplace-Beltrami assembly (``Solver(aniso=...)``).
 
-Host-side mesh preprocessing, as in the reference's mesh class
-(``lapy/tria_mesh.py:1078-1279``): an O(V + E) pass whose result, two unit
-directions and two curvatures per triangle, is uploaded with the geometry for
-the GPU assembly. The estimator is the edge-based curvature tensor of Alliez,
-Cohen-Steiner, Devillers, Levy and Desbrun ("Anisotropic Polygonal
-Remeshing", ACM TOG 2003): each inner edge contributes its signed dihedral
-angle times its (mean-normalised) length times ``e e^T`` for its unit
-direction ``e``; a vertex averages the contributions of its edges, the six
-tensor entries are smoothed three times by neighbour averaging, and the
-per-vertex 3x3 eigenproblem gives the normal (the eigenvector most aligned
-with the vertex normal) and the two principal directions.
+``curvature`` and ``curvature_tria`` replace lapy/tria_mesh.py:1078-1219 and
+:1221-1279 with the lh_curvature / lh_curvature_tria kernels (edge terms,
+vertex tensors, smoothing, per-vertex 3x3 eigenproblems and the per-triangle
+projection all on the device).  Two differences to the reference, both where
+it is not determined: an eigenvector's sign is chosen so that its
+largest-magnitude component is positive (the reference keeps LAPACK's), and
+``curvature_tria`` turns each corner's ``u_min`` to agree with the first
+corner's before averaging (the reference averages the corners as LAPACK left
+them).  The anisotropic assembly does not depend on the sign of a triangle's
+direction.
+
+``tria_normals`` and ``vertex_normals`` are the host mesh utilities of
+lapy/tria_mesh.py:750-817.
 """
+
+from __future__ import annotations
 
 import sys
 
 import numpy as np
-from scipy import sparse
+
+from . import _lib
 
 
 def _check_oriented(t, n):
@@ -39,8 +44,8 @@ def tria_normals(v, t):
 
 def vertex_normals(v, t):
     """Unit vertex normals: the sum of the incident triangles' corner cross
-    products, i.e. area-weighted triangle normals (``tria_mesh.py:775-817``).
-    Raises ``ValueError`` for a mesh that is not consistently oriented."""
+    products (``tria_mesh.py:775-817``). Raises ``ValueError`` for a mesh that
+    is not consistently oriented."""
     n = v.shape[0]
     _check_oriented(t, n)
     v0, v1, v2 = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
@@ -53,124 +58,38 @@ def vertex_normals(v, t):
     return acc / ln[:, None]
 
 
-def inner_edges(v, t):
-    """Inner edges of an oriented mesh (``tria_mesh.py:1019-1076``):
-    ``vids`` (E, 2) with ``vids[:,0] < vids[:,1]``, sorted lexicographically,
-    and ``tids`` (E, 2): the triangle holding the half edge ``vids[:,0] ->
-    vids[:,1]`` and the triangle holding the opposite half edge. Boundary
-    edges (one triangle) are left out."""
-    n = v.shape[0]
-    _check_oriented(t, n)
-    nt = t.shape[0]
-    a = t.reshape(-1)
-    b = t[:, [1, 2, 0]].reshape(-1)
-    tri = np.repeat(np.arange(nt), 3)
-    fwd = a < b
-    key_f = a[fwd].astype(np.int64) * n + b[fwd]
-    key_b = b[~fwd].astype(np.int64) * n + a[~fwd]
-    keys, i_f, i_b = np.intersect1d(key_f, key_b, assume_unique=True, return_indices=True)
-    vids = np.stack([keys // n, keys % n], axis=1)
-    tids = np.stack([tri[fwd][i_f], tri[~fwd][i_b]], axis=1)
-    return vids, tids
+def _device_mesh(v, t, dm):
+    v = np.asarray(v)
+    t = np.asarray(t)
+    if t.ndim != 2 or t.shape[1] != 3:
+        raise ValueError("curvature needs a triangle mesh")
+    _check_oriented(t, v.shape[0])
+    if dm is not None:
+        return dm
+    if v.dtype not in (np.float32, np.float64):
+        v = v.astype(np.float64)
+    return _lib.DeviceMesh(_lib.default_context(), v, t)
 
 
-def _smoothing_matrix(v, t):
-    """Row-stochastic neighbour average (``tria_mesh.py:1696-1715``): row i
-    holds ``1 / deg_i`` at each of its ``deg_i`` distinct edge neighbours. The
-    edge adjacency is binarised first (``tria_mesh.py:1707``), so an inner
-    edge (two triangles) and a boundary edge (one) weigh the same. The
-    reference weights both by the vertex area, which cancels; a vertex of zero
-    area keeps a zero row, as there."""
-    n = v.shape[0]
-    i = t[:, [0, 1, 1, 2, 2, 0]].reshape(-1)
-    j = t[:, [1, 0, 2, 1, 0, 2]].reshape(-1)
-    adj = sparse.csr_matrix((np.ones(i.size), (i, j)), shape=(n, n))
-    adj.sum_duplicates()
-    adj.data[:] = 1.0
-    v0, v1, v2 = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
-    cr = np.cross(v1 - v0, v2 - v0)
-    area = 0.5 * np.sqrt(np.sum(cr * cr, axis=1))
-    varea = np.bincount(t.reshape(-1), np.repeat(area, 3), minlength=n) / 3.0
-    deg = np.asarray(adj.sum(axis=1)).ravel()
-    scale = np.where((varea * deg) != 0, 1.0 / np.where(deg == 0, 1.0, deg), 0.0)
-    return sparse.diags(scale) @ adj
-
-
-def curvature(v, t, smoothit=3):
-    """Per-vertex principal curvatures (``tria_mesh.py:1078-1219``).
+def curvature(v, t, smoothit: int = 3, dm=None):
+    """Per-vertex principal curvatures (``tria_mesh.py:1078-1219``) on the GPU.
 
     Returns ``u_min, u_max`` (V, 3) unit principal directions, ``c_min, c_max``
-    (V,) with ``c_min <= c_max``, ``c_mean = (c_min + c_max) / 2`` and
+    (V,) with ``c_min <= c_max``, ``c_mean = (c_min + c_max) / 2``,
     ``c_gauss = c_min * c_max`` (taken before the two are ordered, as in the
-    reference), and ``normals`` (V, 3), the tensor's normal eigenvector turned
-    towards the vertex normal. ``u_max`` is oriented so that
-    ``u_min x u_max`` points along ``normals``.
+    reference) and ``normals`` (V, 3), turned towards the vertex normals;
+    ``u_max`` is oriented so that ``u_min x u_max`` points along ``normals``.
+    ``dm``: an existing device mesh of (v, t).
     """
-    v = np.asarray(v)
-    t = np.asarray(t)
-    n = v.shape[0]
-    vids, tids = inner_edges(v, t)
-    tn = tria_normals(v, t)
-    n0, n1 = tn[tids[:, 0]], tn[tids[:, 1]]
-    angle = np.arccos(np.clip(np.sum(n0 * n1, axis=1), -1.0, 1.0))
-    evec = v[vids[:, 1]] - v[vids[:, 0]]
-    elen = np.sqrt(np.sum(evec * evec, axis=1))
-    # signed dihedral angle: negative where the surface is convex (outward normals)
-    angle = angle * -np.sign(np.sum(np.cross(n0, n1) * evec, axis=1))
-    elen[elen < sys.float_info.epsilon] = 1
-    e = evec / elen[:, None]
-    elen = elen / np.mean(elen)
-    # upper triangle of e e^T: xx, xy, xz, yy, yz, zz
-    ee = np.stack([e[:, 0] * e[:, 0], e[:, 0] * e[:, 1], e[:, 0] * e[:, 2],
-                   e[:, 1] * e[:, 1], e[:, 1] * e[:, 2], e[:, 2] * e[:, 2]], axis=1)
-    ee = ee * (angle * elen)[:, None]
-    ends = np.concatenate([vids[:, 0], vids[:, 1]])
-    vv = np.stack([np.bincount(ends, np.concatenate([ee[:, c], ee[:, c]]), minlength=n)
-                   for c in range(6)], axis=1)
-    deg = np.bincount(ends, minlength=n).astype(np.float64)
-    deg[deg == 0] = 1
-    vv = vv / deg[:, None]
-    mat = _smoothing_matrix(v, t)
-    for _ in range(smoothit):
-        vv = mat @ vv
-    tens = vv[:, [0, 1, 2, 1, 3, 4, 2, 4, 5]].reshape(n, 3, 3)
-    evals, evecs = np.linalg.eigh(tens)
-    vn = vertex_normals(v, t)
-    # the eigenvector most aligned with the vertex normal first (not the
-    # smallest |eigenvalue|, which is ambiguous on flat regions)
-    order = np.argsort(-np.abs(np.einsum("vrk,vr->vk", evecs, vn)), axis=1)
-    evals = np.take_along_axis(evals, order, axis=1)
-    evecs = np.take_along_axis(evecs, order[:, None, :], axis=2)
-    normals = evecs[:, :, 0].copy()
-    u_max = evecs[:, :, 1].copy()
-    u_min = evecs[:, :, 2].copy()
-    c_max = evals[:, 2].copy()
-    c_min = evals[:, 1].copy()
-    c_mean = (c_min + c_max) / 2.0
-    c_gauss = c_min * c_max
-    swap = c_min > c_max
-    c_min[swap], c_max[swap] = c_max[swap], c_min[swap]
-    u_min[swap], u_max[swap] = u_max[swap], u_min[swap]
-    normals = normals * np.sign(np.sum(normals * vn, axis=1))[:, None]
-    flip = np.sum(np.cross(u_min, u_max) * normals, axis=1) < 0
-    u_max[flip] = -u_max[flip]
-    return u_min, u_max, c_min, c_max, c_mean, c_gauss, normals
+    dm = _device_mesh(v, t, dm)
+    return _lib.curvature(dm.ctx, dm, int(smoothit))
 
 
-def curvature_tria(v, t, smoothit=3):
+def curvature_tria(v, t, smoothit: int = 3, dm=None):
     """Per-triangle principal directions and curvatures
-    (``tria_mesh.py:1221-1279``): the vertex values averaged over each
-    triangle's corners; ``u_min`` projected into the triangle plane and
+    (``tria_mesh.py:1221-1279``) on the GPU: the vertex values averaged over
+    each triangle's corners, ``u_min`` projected into the triangle plane and
     normalised, ``u_max = normal x u_min``. Returns ``u_min, u_max`` (F, 3)
     and ``c_min, c_max`` (F,)."""
-    v = np.asarray(v)
-    t = np.asarray(t)
-    u_min, _u_max, c_min, c_max, _m, _g, _n = curvature(v, t, smoothit)
-    tumin = np.sum((u_min / 3.0)[t], axis=1)
-    tcmin = np.sum((c_min / 3.0)[t], axis=1)
-    tcmax = np.sum((c_max / 3.0)[t], axis=1)
-    tn = tria_normals(v, t)
-    tumin = tumin - tn * np.sum(tn * tumin, axis=1)[:, None]
-    tumin = tumin / np.maximum(np.sqrt(np.sum(tumin * tumin, axis=1)), 1e-8)[:, None]
-    tumax = np.cross(tn, tumin)
-    return tumin, tumax, tcmin, tcmax
+    dm = _device_mesh(v, t, dm)
+    return _lib.curvature_tria(dm.ctx, dm, int(smoothit))

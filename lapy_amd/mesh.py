@@ -76,13 +76,13 @@ class TriaMesh:
     def curvature(self, smoothit: int = 3):
         """Per-vertex principal curvatures and directions
         (lapy/tria_mesh.py:1078-1219): ``u_min, u_max, c_min, c_max, c_mean,
-        c_gauss, normals``."""
-        return curvature.curvature(self.v, self.t, smoothit)
+        c_gauss, normals``, on the GPU."""
+        return curvature.curvature(self.v, self.t, smoothit, dm=device_mesh(self))
 
     def curvature_tria(self, smoothit: int = 3):
         """Per-triangle principal directions and curvatures, the input of
-        ``Solver(aniso=...)`` (lapy/tria_mesh.py:1221-1279)."""
-        return curvature.curvature_tria(self.v, self.t, smoothit)
+        ``Solver(aniso=...)`` (lapy/tria_mesh.py:1221-1279), on the GPU."""
+        return curvature.curvature_tria(self.v, self.t, smoothit, dm=device_mesh(self))
 
 
 class TetMesh:

@@ -1,5 +1,6 @@
-"""Principal curvatures (lapy_amd.curvature, the input of Solver(aniso=...)),
-pinned by what the geometry fixes: a plane has zero curvature, a sphere is
+"""Principal curvatures (the input of Solver(aniso=...)): the oracle's
+restatement (oracle/curvature.py, which the GPU curvature is checked
+against) pinned by what the geometry fixes: a plane has zero curvature, a sphere is
 umbilic (equal principal curvatures, normals radial), a cylinder's flat
 principal direction is its axis, and the estimator is scale-invariant (edge
 lengths are normalised by their mean) and rotation-equivariant. A wrong
@@ -8,7 +9,8 @@ tensor entry (e.g. xz and yy exchanged) breaks the equivariance check."""
 import numpy as np
 import pytest
 
-from lapy_amd import TriaMesh, curvature, meshgen
+from lapy_amd import TriaMesh, meshgen
+from oracle import curvature
 
 
 def _rotation(seed):
@@ -119,16 +121,15 @@ def test_scale_invariant_and_rotation_equivariant():
 
 def test_curvature_tria_frame():
     v, t = meshgen.perturbed_sphere(3, np.random.default_rng(5), axes=(1.2, 0.9, 1.0), sigma=0.02)
-    mesh = TriaMesh(v, t)
-    u1, u2, c1, c2 = mesh.curvature_tria(smoothit=3)
-    tn = mesh.tria_normals()
+    u1, u2, c1, c2 = curvature.curvature_tria(v, t, smoothit=3)
+    tn = curvature.tria_normals(v, t)
     assert u1.shape == u2.shape == (len(t), 3) and c1.shape == c2.shape == (len(t),)
     np.testing.assert_allclose(np.linalg.norm(u1, axis=1), 1.0, atol=1e-12)
     np.testing.assert_allclose(np.linalg.norm(u2, axis=1), 1.0, atol=1e-12)
     np.testing.assert_allclose(np.sum(u1 * tn, axis=1), 0.0, atol=1e-12)
     np.testing.assert_allclose(np.sum(u1 * u2, axis=1), 0.0, atol=1e-12)
     np.testing.assert_allclose(np.sum(np.cross(u1, u2) * tn, axis=1), 1.0, atol=1e-12)
-    _, _, cmin, cmax, _, _, _ = mesh.curvature(3)
+    _, _, cmin, cmax, _, _, _ = curvature.curvature(v, t, 3)
     np.testing.assert_allclose(c1, cmin[t].mean(axis=1), rtol=1e-12, atol=1e-15)
     np.testing.assert_allclose(c2, cmax[t].mean(axis=1), rtol=1e-12, atol=1e-15)
 
