@@ -134,6 +134,20 @@ int lh_assemble(lh_ctx* ctx, lh_mesh* m, int kind, int lump, int out_dtype,
 int lh_assemble_into(lh_ctx* ctx, lh_mesh* m, int kind, int lump,
                      const void* u1, const void* u2, const void* aniso_mat,
                      lh_csr* A, lh_csr* B);
+/* Row order of the triangle assembly (LH_TRIA, LH_TRIA_MASS; no counterpart
+ * in the reference, results are identical in every order): the rows are
+ * processed in the mesh's vertex order, or in the pattern's breadth-first
+ * locality order with the coordinates gathered from a copy in that order
+ * (each row still written at its own CSR offset and summed in ascending
+ * element order).  LH_ASM_ORDER_AUTO takes the locality order for meshes of
+ * at least 65,536 used vertices (fewer than 2^24, rows of at most 64
+ * entries) from their second numeric assembly on, or once a solve has built
+ * the pattern's order (building the order costs more than one assembly);
+ * LH_ASM_ORDER_LOCALITY takes it whenever the mesh is eligible. */
+#define LH_ASM_ORDER_AUTO     0
+#define LH_ASM_ORDER_MESH     1
+#define LH_ASM_ORDER_LOCALITY 2
+int lh_ctx_set_assembly_order(lh_ctx* ctx, int order);
 
 /* ---- gradient / divergence on triangles and tetrahedra ----------------------
  * mode LH_DG_GRADIENT: in = nf vertex functions (nv x nf, float64, row-major),

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "liblapyhip.so")
 LH_OK, LH_ERR_ARG, LH_ERR_OOM, LH_ERR_HIP, LH_ERR_NOCONV, LH_ERR_NUMERIC = range(6)
 LH_F64, LH_F32 = 0, 1
 LH_TRIA, LH_TRIA_ANISO, LH_TRIA_MASS, LH_TET = 0, 1, 2, 3
+LH_ASM_ORDER_AUTO, LH_ASM_ORDER_MESH, LH_ASM_ORDER_LOCALITY = 0, 1, 2
 LH_DG_GRADIENT, LH_DG_DIVERGENCE, LH_DG_GEODESIC_RHS = 0, 1, 2
 LH_PREC_FP64, LH_PREC_MIXED = 0, 1
 LH_PRECOND_JACOBI, LH_PRECOND_LEARNED, LH_PRECOND_NONE, LH_PRECOND_AUTO = 0, 1, 2, 3
@@ -119,6 +120,7 @@ SIGNATURES = {
     "lh_mpnn_set_params": (C.c_int, [_vp, _vp]),
     "lh_mpnn_get_params": (C.c_int, [_vp, _vp]),
     "lh_ctx_set_mpnn": (C.c_int, [_vp, _vp]),
+    "lh_ctx_set_assembly_order": (C.c_int, [_vp, C.c_int]),
     "lh_mpnn_apply": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp, C.c_int, _vp]),
     "lh_mpnn_loss_grad": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp, C.c_int, _dp, _vp]),
     "lh_mpnn_time_layer": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, _dp]),
@@ -200,6 +202,11 @@ class Context:
 
     def sync(self):
         check(lib().lh_sync(self.handle), self, "lh_sync")
+
+    def set_assembly_order(self, order: int):
+        """Row order of the triangle assembly (LH_ASM_ORDER_*); the values
+        do not depend on it."""
+        check(lib().lh_ctx_set_assembly_order(self.handle, int(order)), self, "lh_ctx_set_assembly_order")
 
 
 _default_ctx = None

@@ -22,7 +22,7 @@
 // degenerate elements; only then is mean(vol) computed with numpy's own
 // summation order (pairwise sums over 8192-element buffer chunks) and the
 // numeric kernel run again with the clamp value.
-#include "lh_internal.hpp"
+#include "lh_dense.hpp"
 
 #include <cfloat>
 #include <cmath>
@@ -392,6 +392,78 @@ __device__ __forceinline__ void lds_window_end(const OT* sA, const OT* sB, OT* _
     }
 }
 
+// one triangle's entries in the row of its corner l (v1, v2, v3 in the
+// element's vertex order): the values for the row's "next" column (oa: t2
+// for corner 0, t1 for corner 1, t2 for corner 2), its "previous" column
+// (ob: t3, t3, t1), the diagonal (od) and the mass terms, with numpy's
+// operation order.  e, u1, u2, am: the element and the anisotropic inputs
+// (LH_TRIA_ANISO only).
+template <typename OT>
+struct TriEntries {
+    OT oa, ob, od, bij, bii, bl;
+};
+
+template <typename VT, typename OT, int KIND, bool LUMP>
+__device__ __forceinline__ TriEntries<OT> tria_entries(int l, const V3<VT>& v1, const V3<VT>& v2, const V3<VT>& v3,
+                                                       int clamp, VT vm, int32_t* __restrict__ degen, int64_t e,
+                                                       const double* __restrict__ u1, const double* __restrict__ u2,
+                                                       const double* __restrict__ am) {
+    TriEntries<OT> c{};
+    V3<VT> v2mv1 = vsub(v2, v1), v3mv2 = vsub(v3, v2), v1mv3 = vsub(v1, v3);
+    V3<VT> cr = vcross(v3mv2, v1mv3);
+    VT vol;
+    if constexpr (KIND == LH_TRIA_MASS) vol = VT(0.5) * dsqrt(vdot(cr, cr));
+    else vol = VT(2) * dsqrt(vdot(cr, cr));
+    if (vol < VT(kEps)) {
+        if (clamp) vol = vm;
+        else atomicOr(degen, 1);
+    }
+    if constexpr (KIND == LH_TRIA) {
+        // a12 (t1,t2) = (v3-v2).(v1-v3) / vol, a23 (t2,t3) = (v1-v3).(v2-v1) / vol,
+        // a31 (t3,t1) = (v2-v1).(v3-v2) / vol.  Row t1 takes a12 -> t2 and
+        // a31 -> t3, row t2 a12 -> t1 and a23 -> t3, row t3 a23 -> t2 and
+        // a31 -> t1: only the corner's two terms are formed (products commute
+        // and -a31 - a23 == -a23 - a31 exactly, so the values are those of
+        // the three-term form)
+        const V3<VT>& xa = l == 2 ? v1mv3 : v3mv2;
+        const V3<VT>& xb = l == 2 ? v2mv1 : v1mv3;
+        const V3<VT>& ya = l == 1 ? v1mv3 : v2mv1;
+        const V3<VT>& yb = l == 1 ? v2mv1 : v3mv2;
+        const VT aa = vdot(xa, xb) / vol;
+        const VT ab = vdot(ya, yb) / vol;
+        c.oa = (OT)aa;
+        c.ob = (OT)ab;
+        c.od = (OT)(-aa - ab);
+    } else if constexpr (KIND == LH_TRIA_ANISO) {
+        // lapy/solver.py:259-280: edges projected on (u1, u2), weighted
+        const double* pu1 = u1 + 3 * e;
+        const double* pu2 = u2 + 3 * e;
+        V3<double> U1{pu1[0], pu1[1], pu1[2]}, U2{pu2[0], pu2[1], pu2[2]};
+        V3<double> d21{(double)v2mv1.x, (double)v2mv1.y, (double)v2mv1.z};
+        V3<double> d32{(double)v3mv2.x, (double)v3mv2.y, (double)v3mv2.z};
+        V3<double> d13{(double)v1mv3.x, (double)v1mv3.y, (double)v1mv3.z};
+        double w0 = am[2 * e], w1 = am[2 * e + 1];
+        double p21_0 = vdot(U1, d21), p21_1 = vdot(U2, d21);
+        double p32_0 = vdot(U1, d32), p32_1 = vdot(U2, d32);
+        double p13_0 = vdot(U1, d13), p13_1 = vdot(U2, d13);
+        double dv = (double)vol;
+        double a12 = ((p32_0 * w0) * p13_0 + (p32_1 * w1) * p13_1) / dv;
+        double a23 = ((p13_0 * w0) * p21_0 + (p13_1 * w1) * p21_1) / dv;
+        double a31 = ((p21_0 * w0) * p32_0 + (p21_1 * w1) * p32_1) / dv;
+        if (l == 0) { c.oa = (OT)a12; c.ob = (OT)a31; c.od = (OT)(-a12 - a31); }
+        else if (l == 1) { c.oa = (OT)a12; c.ob = (OT)a23; c.od = (OT)(-a12 - a23); }
+        else { c.oa = (OT)a23; c.ob = (OT)a31; c.od = (OT)(-a31 - a23); }
+    }
+    if constexpr (LUMP) {
+        if constexpr (KIND == LH_TRIA_MASS) c.bl = (OT)(vol / VT(3));
+        else c.bl = (OT)(vol / VT(12));
+    } else {
+        if constexpr (KIND == LH_TRIA_MASS) { c.bij = (OT)(vol / VT(12)); c.bii = (OT)(vol / VT(6)); }
+        else { c.bij = (OT)(vol / VT(48)); c.bii = (OT)(vol / VT(24)); }
+    }
+    return c;
+}
+
 // triangles: KIND in {LH_TRIA, LH_TRIA_ANISO, LH_TRIA_MASS}; one matrix row
 template <typename VT, typename OT, int KIND, bool LUMP>
 __device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ rowptr,
@@ -447,54 +519,15 @@ __device__ __forceinline__ void tria_row(int64_t i, const int32_t* __restrict__ 
             v2 = load_xyz(xyz, i2);
             v3 = load_xyz(xyz, i3);
         }
-        V3<VT> v2mv1 = vsub(v2, v1), v3mv2 = vsub(v3, v2), v1mv3 = vsub(v1, v3);
-        V3<VT> cr = vcross(v3mv2, v1mv3);
-        VT vol;
-        if constexpr (KIND == LH_TRIA_MASS) vol = VT(0.5) * dsqrt(vdot(cr, cr));
-        else vol = VT(2) * dsqrt(vdot(cr, cr));
-        if (vol < VT(kEps)) {
-            if (clamp) vol = vm;
-            else atomicOr(degen, 1);
-        }
-        if constexpr (HAS_A) {
-            // a12 (t1,t2), a23 (t2,t3), a31 (t3,t1); row t1: a12 -> t2, a31 -> t3
-            // row t2: a12 -> t1, a23 -> t3; row t3: a23 -> t2, a31 -> t1
-            if constexpr (KIND == LH_TRIA) {
-                VT a12 = vdot(v3mv2, v1mv3) / vol;
-                VT a23 = vdot(v1mv3, v2mv1) / vol;
-                VT a31 = vdot(v2mv1, v3mv2) / vol;
-                if (l == 0) { c.oa = (OT)a12; c.ob = (OT)a31; c.od = (OT)(-a12 - a31); }
-                else if (l == 1) { c.oa = (OT)a12; c.ob = (OT)a23; c.od = (OT)(-a12 - a23); }
-                else { c.oa = (OT)a23; c.ob = (OT)a31; c.od = (OT)(-a31 - a23); }
-            } else {
-                // lapy/solver.py:259-280: edges projected on (u1, u2), weighted
-                const int64_t e = inc[p] >> 2;
-                const double* pu1 = u1 + 3 * e;
-                const double* pu2 = u2 + 3 * e;
-                V3<double> U1{pu1[0], pu1[1], pu1[2]}, U2{pu2[0], pu2[1], pu2[2]};
-                V3<double> d21{(double)v2mv1.x, (double)v2mv1.y, (double)v2mv1.z};
-                V3<double> d32{(double)v3mv2.x, (double)v3mv2.y, (double)v3mv2.z};
-                V3<double> d13{(double)v1mv3.x, (double)v1mv3.y, (double)v1mv3.z};
-                double w0 = am[2 * e], w1 = am[2 * e + 1];
-                double p21_0 = vdot(U1, d21), p21_1 = vdot(U2, d21);
-                double p32_0 = vdot(U1, d32), p32_1 = vdot(U2, d32);
-                double p13_0 = vdot(U1, d13), p13_1 = vdot(U2, d13);
-                double dv = (double)vol;
-                double a12 = ((p32_0 * w0) * p13_0 + (p32_1 * w1) * p13_1) / dv;
-                double a23 = ((p13_0 * w0) * p21_0 + (p13_1 * w1) * p21_1) / dv;
-                double a31 = ((p21_0 * w0) * p32_0 + (p21_1 * w1) * p32_1) / dv;
-                if (l == 0) { c.oa = (OT)a12; c.ob = (OT)a31; c.od = (OT)(-a12 - a31); }
-                else if (l == 1) { c.oa = (OT)a12; c.ob = (OT)a23; c.od = (OT)(-a12 - a23); }
-                else { c.oa = (OT)a23; c.ob = (OT)a31; c.od = (OT)(-a31 - a23); }
-            }
-        }
-        if constexpr (LUMP) {
-            if constexpr (KIND == LH_TRIA_MASS) c.bl = (OT)(vol / VT(3));
-            else c.bl = (OT)(vol / VT(12));
-        } else {
-            if constexpr (KIND == LH_TRIA_MASS) { c.bij = (OT)(vol / VT(12)); c.bii = (OT)(vol / VT(6)); }
-            else { c.bij = (OT)(vol / VT(48)); c.bii = (OT)(vol / VT(24)); }
-        }
+        const TriEntries<OT> te = tria_entries<VT, OT, KIND, LUMP>(l, v1, v2, v3, clamp, vm, degen,
+                                                                   KIND == LH_TRIA_ANISO ? (int64_t)(inc[p] >> 2) : 0,
+                                                                   u1, u2, am);
+        c.oa = te.oa;
+        c.ob = te.ob;
+        c.od = te.od;
+        c.bij = te.bij;
+        c.bii = te.bii;
+        c.bl = te.bl;
         return c;
     };
     // sums in ascending element order, as the reference's duplicate sums
@@ -575,6 +608,233 @@ k_tria_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __rest
         tria_row<VT, OT, KIND, LUMP>(i, rowptr, cols, diagpos, lrowptr, inc_ptr, inc, incv, xyz, u1, u2, am,
                                      clamp, vol_mean, RowAcc<OT>{Aval, Bval, 0},
                                      RowVerts<VT>{nullptr, nullptr, nullptr, nullptr}, Bval, degen);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// triangle rows in locality order
+//
+// The mesh's own vertex numbering (icosphere subdivision order, scanner
+// order) puts a row's neighbours far apart, so the three vertices each
+// incident element gathers come from HBM again and again.  The scheduled form
+// processes the rows in the pattern's breadth-first order (the order the
+// solvers use), gathers coordinates from a copy of them in that order and
+// reads one 8-byte record per incidence (the two other vertices in that
+// order, the corner and the two slot offsets) instead of the 16-byte
+// vertex-index record.  Each row is still summed in ascending element order
+// and written at its own CSR offset: the values are bit-identical to the
+// mesh-order kernel.  Workgroups of one XCD take a contiguous range of the
+// order, so their neighbourhoods share that XCD's L2.
+// ---------------------------------------------------------------------------
+constexpr int kSchedBS = 256;
+constexpr int kSchedMaxLen = 64;           // slot offsets in 6 bits
+constexpr int64_t kSchedMaxN = 1 << 24;    // vertex ids in 24 bits
+constexpr int64_t kSchedAutoMinN = 65536;  // LH_ASM_ORDER_AUTO threshold
+
+}  // namespace
+
+struct AsmSched {
+    int64_t n = 0;
+    std::shared_ptr<Reorder> ro;
+    DevBuf rows;  // int4 per row in locality order, n+1: {row, CSR offset, first record, len | diag << 8}
+    DevBuf rec;   // uint2 per incidence: {a | off_a << 24 | corner << 30, b | off_b << 24}
+    DevBuf xyz;   // (x, y, z, 0) per vertex in locality order, vertex dtype
+};
+
+namespace {
+
+// per row in locality order: its incidence count (for the record offsets),
+// and whether it fits the record format
+__global__ void k_sched_count(int64_t n, const int32_t* __restrict__ perm, const int32_t* __restrict__ rowptr,
+                              const int32_t* __restrict__ inc_ptr, int32_t* __restrict__ cnt,
+                              int32_t* __restrict__ bad) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int32_t i = perm[q];
+    cnt[q] = inc_ptr[i + 1] - inc_ptr[i];
+    if (rowptr[i + 1] - rowptr[i] > kSchedMaxLen) atomicOr(bad, 1);
+}
+
+template <typename VT>
+__global__ void k_sched_fill(int64_t n, const int32_t* __restrict__ perm, const int32_t* __restrict__ iperm,
+                             const int32_t* __restrict__ rowptr, const int32_t* __restrict__ diagpos,
+                             const int32_t* __restrict__ inc_ptr, const int4* __restrict__ incv,
+                             const int32_t* __restrict__ rec0, const VT* __restrict__ xyz, int4* __restrict__ rows,
+                             uint2* __restrict__ rec, VT* __restrict__ pxyz) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > n) return;
+    if (q == n) {
+        rows[n] = make_int4(0, 0, rec0[n], 0);
+        return;
+    }
+    const int32_t i = perm[q];
+    const int32_t r0 = rowptr[i], len = rowptr[i + 1] - r0;
+    const int32_t dpos = len > 0 ? diagpos[i] - r0 : 0;
+    const int32_t base = rec0[q];
+    rows[q] = make_int4(i, r0, base, len | (dpos << 8));
+    for (int k = 0; k < 4; ++k) pxyz[4 * q + k] = xyz[4 * (int64_t)i + k];
+    const int32_t pa = inc_ptr[i], pb = inc_ptr[i + 1];
+    for (int32_t p = pa; p < pb; ++p) {
+        const int4 ev = incv[p];
+        const uint32_t w = (uint32_t)ev.w;
+        const uint32_t l = w & 3u, oa = (w >> 2) & kSlotSearch, ob = w >> 17;
+        int32_t ja, jb;
+        if (l == 0) { ja = ev.y; jb = ev.z; }
+        else if (l == 1) { ja = ev.x; jb = ev.z; }
+        else { ja = ev.y; jb = ev.x; }
+        // rows of at most kSchedMaxLen entries: oa, ob < 64 (checked by k_sched_count)
+        rec[base + (p - pa)] = make_uint2((uint32_t)iperm[ja] | (oa << 24) | (l << 30),
+                                          (uint32_t)iperm[jb] | (ob << 24));
+    }
+}
+
+template <typename VT, typename OT, int KIND, bool LUMP>
+__device__ __forceinline__ void tria_row_sched(int64_t q, int4 ri, int32_t rec1, const uint2* __restrict__ rec,
+                                               const VT* __restrict__ pxyz, const int32_t* __restrict__ lrowptr,
+                                               int clamp, VT vm, OT* __restrict__ ar, OT* __restrict__ br,
+                                               OT* __restrict__ Bval, int32_t* __restrict__ degen) {
+    constexpr bool HAS_A = (KIND != LH_TRIA_MASS);
+    const int32_t rec0 = ri.z;
+    if (rec1 == rec0) return;  // unused vertex: empty row
+    const int dpos = (ri.w >> 8) & 0xff;
+    const V3<VT> vi = load_xyz(pxyz, (int32_t)q);
+    OT bl = OT(0);
+    // the diagonal's sums stay in registers (same additions in the same order)
+    OT ad = OT(0), bd = OT(0);
+    struct Contrib {
+        int sa, sb;
+        TriEntries<OT> v;
+    };
+    auto eval = [&](const uint2 r) -> Contrib {
+        const int l = (int)(r.x >> 30);
+        const int sa = (int)((r.x >> 24) & 63u), sb = (int)((r.y >> 24) & 63u);
+        const V3<VT> va = load_xyz(pxyz, (int32_t)(r.x & 0xffffffu)), vb = load_xyz(pxyz, (int32_t)(r.y & 0xffffffu));
+        V3<VT> v1, v2, v3;
+        if (l == 0) { v1 = vi; v2 = va; v3 = vb; }
+        else if (l == 1) { v1 = va; v2 = vi; v3 = vb; }
+        else { v1 = vb; v2 = va; v3 = vi; }
+        return {sa, sb, tria_entries<VT, OT, KIND, LUMP>(l, v1, v2, v3, clamp, vm, degen, 0, nullptr, nullptr, nullptr)};
+    };
+    // sums in ascending element order, as the reference's duplicate sums
+    // (a triangle with a repeated vertex sends an off-diagonal term to the
+    // diagonal slot: it joins the register sum at its place in the order)
+    auto apply = [&](const Contrib& c) {
+        if constexpr (HAS_A) {
+            if (c.sa == dpos) ad += c.v.oa;
+            else ar[c.sa] += c.v.oa;
+            if (c.sb == dpos) ad += c.v.ob;
+            else ar[c.sb] += c.v.ob;
+            ad += c.v.od;
+        }
+        if constexpr (LUMP) {
+            bl += c.v.bl;
+        } else {
+            if (c.sa == dpos) bd += c.v.bij;
+            else br[c.sa] += c.v.bij;
+            if (c.sb == dpos) bd += c.v.bij;
+            else br[c.sb] += c.v.bij;
+            bd += c.v.bii;
+        }
+    };
+    int32_t p = rec0;
+    for (; p + 2 < rec1; p += 3) {
+        const uint2 e0 = rec[p], e1 = rec[p + 1], e2 = rec[p + 2];
+        const Contrib c0 = eval(e0);
+        const Contrib c1 = eval(e1);
+        const Contrib c2 = eval(e2);
+        apply(c0);
+        apply(c1);
+        apply(c2);
+    }
+    if (p + 1 < rec1) {
+        const uint2 e0 = rec[p], e1 = rec[p + 1];
+        const Contrib c0 = eval(e0);
+        const Contrib c1 = eval(e1);
+        apply(c0);
+        apply(c1);
+    } else if (p < rec1) {
+        apply(eval(rec[p]));
+    }
+    if constexpr (HAS_A) ar[dpos] = ad;
+    if constexpr (LUMP) Bval[lrowptr[ri.x]] = bl;
+    else br[dpos] = bd;
+}
+
+// grid: a multiple of kXCDs workgroups of kSchedBS rows (the blocks past the
+// last row return at once)
+template <typename VT, typename OT, int KIND, bool LUMP>
+__global__ void __launch_bounds__(kSchedBS)
+k_tria_rows_sched(int64_t n, const int4* __restrict__ rows, const uint2* __restrict__ rec,
+                  const VT* __restrict__ pxyz, const int32_t* __restrict__ lrowptr, int clamp,
+                  const VT* __restrict__ vol_mean, OT* __restrict__ Aval, OT* __restrict__ Bval,
+                  int32_t* __restrict__ degen, int cap) {
+    constexpr bool HAS_A = (KIND != LH_TRIA_MASS);
+    constexpr bool HAS_B = !LUMP;
+    __shared__ int32_t s_loc[kSchedBS + 1];  // window offset of each row (exclusive scan of the lengths)
+    __shared__ int32_t s_r0[kSchedBS];       // CSR offset of each row
+    __shared__ int32_t s_wsum[kSchedBS / 64];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    OT* sA = reinterpret_cast<OT*>(lds_raw);
+    OT* sB = HAS_A ? sA + cap : sA;
+    const int64_t q0 = xcd_block() * kSchedBS;
+    if (q0 >= n) return;  // whole workgroup
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int64_t q = q0 + tid;
+    int4 ri = make_int4(0, 0, 0, 0);
+    int32_t rec1 = 0;
+    if (q < n) {
+        ri = rows[q];
+        rec1 = rows[q + 1].z;
+    }
+    const int len = ri.w & 0xff;
+    // exclusive scan of the row lengths over the workgroup
+    int v = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    if (lane == 63) s_wsum[wid] = v;
+    __syncthreads();
+    int pre = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kSchedBS / 64; ++w) {
+        pre += w < wid ? s_wsum[w] : 0;
+        total += s_wsum[w];
+    }
+    const int loc = pre + v - len;
+    s_loc[tid] = loc;
+    if (tid == kSchedBS - 1) s_loc[kSchedBS] = loc + len;
+    s_r0[tid] = ri.y;
+    const VT vm = clamp ? vol_mean[0] : VT(0);
+    if (total <= cap) {
+        for (int s = tid; s < total; s += kSchedBS) {
+            if (HAS_A) sA[s] = OT(0);
+            if (HAS_B) sB[s] = OT(0);
+        }
+        __syncthreads();
+        if (q < n)
+            tria_row_sched<VT, OT, KIND, LUMP>(q, ri, rec1, rec, pxyz, lrowptr, clamp, vm, sA + loc, sB + loc, Bval,
+                                               degen);
+        __syncthreads();
+        // window -> CSR: slot s belongs to the last row whose window starts at or before it
+        for (int s = tid; s < total; s += kSchedBS) {
+            int lo = 0, hi = kSchedBS - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_loc[mid] <= s) lo = mid;
+                else hi = mid - 1;
+            }
+            const int64_t dst = (int64_t)s_r0[lo] + (s - s_loc[lo]);
+            if (HAS_A) Aval[dst] = sA[s];
+            if (HAS_B) Bval[dst] = sB[s];
+        }
+    } else if (q < n) {
+        OT* ar = HAS_A ? Aval + ri.y : nullptr;
+        OT* br = HAS_B ? Bval + ri.y : nullptr;
+        if (HAS_A) zero_row(ar, 0, len);
+        if (HAS_B) zero_row(br, 0, len);
+        tria_row_sched<VT, OT, KIND, LUMP>(q, ri, rec1, rec, pxyz, lrowptr, clamp, vm, ar, br, Bval, degen);
     }
 }
 
@@ -922,11 +1182,98 @@ void clamp_value(lh_ctx* ctx, lh_mesh* m, int kind, DevBuf& vm) {
 // workgroups per CU (105 -> 81 us at 655k vertices)
 constexpr bool kStageTriaXyz = false;
 
+// the locality-ordered row schedule of a triangle mesh, built on first use
+// and kept with the mesh (nullptr: the mesh does not fit the record format)
+template <typename VT>
+AsmSched* asm_schedule(lh_ctx* ctx, lh_mesh* m) {
+    if (m->sched_state == 1) return m->sched.get();
+    if (m->sched_state == -1) return nullptr;
+    const int64_t n = m->n;
+    if (m->nodes != 3 || n < 1 || n >= kSchedMaxN) {
+        m->sched_state = -1;
+        return nullptr;
+    }
+    ArenaOff off;  // kept with the mesh, beyond this call
+    Pattern& p = *m->full;
+    pattern_reorder(ctx, p);
+    const int32_t* perm = p.ro->perm.as<int32_t>();
+    const int32_t* iperm = p.ro->iperm.as<int32_t>();
+    DevBuf cnt(n * 4), bad(4);
+    memset_d(ctx, bad.p, 0, 4);
+    k_sched_count<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, perm, p.rowptr.as<int32_t>(),
+                                                            m->inc_ptr.as<int32_t>(), cnt.as<int32_t>(),
+                                                            bad.as<int32_t>());
+    LH_CHECK_LAUNCH();
+    int32_t hbad = 0;
+    d2h(ctx, &hbad, bad.p, 4);
+    if (hbad) {
+        m->sched_state = -1;
+        return nullptr;
+    }
+    auto sc = std::make_shared<AsmSched>();
+    sc->n = n;
+    sc->ro = p.ro;
+    DevBuf rec0((n + 1) * 4);
+    exclusive_scan_i32(ctx, cnt.as<int32_t>(), rec0.as<int32_t>(), n);
+    int32_t total = 0;
+    d2h(ctx, &total, rec0.as<int32_t>() + n, 4);
+    sc->rows.alloc((n + 1) * sizeof(int4));
+    sc->rec.alloc((size_t)std::max<int32_t>(total, 1) * sizeof(uint2));
+    sc->xyz.alloc(n * 4 * sizeof(VT));
+    k_sched_fill<VT><<<nblocks(n + 1, 256), 256, 0, ctx->stream>>>(
+        n, perm, iperm, p.rowptr.as<int32_t>(), p.diagpos.as<int32_t>(), m->inc_ptr.as<int32_t>(),
+        m->incv.as<int4>(), rec0.as<int32_t>(), m->xyz.as<VT>(), sc->rows.as<int4>(), sc->rec.as<uint2>(),
+        sc->xyz.as<VT>());
+    LH_CHECK_LAUNCH();
+    HIPX(hipStreamSynchronize(ctx->stream));
+    m->sched = sc;
+    m->sched_state = 1;
+    return sc.get();
+}
+
+template <typename VT, typename OT>
+void launch_sched(lh_ctx* ctx, lh_mesh* m, const AsmSched& sc, int kind, int lump, int clamp, const VT* vm, OT* Aval,
+                  OT* Bval, int32_t* degen) {
+    const Pattern& p = *m->full;
+    const int64_t n = m->n;
+    const unsigned nb = (nblocks(n, kSchedBS) + kXCDs - 1) / kXCDs * kXCDs;
+    const bool has_a = kind != LH_TRIA_MASS, has_b = !lump;
+    const int64_t per_slot = (int64_t)(has_a + has_b) * sizeof(OT);
+    const int64_t avg_row = (p.nnz + n - 1) / n;
+    const int cap = per_slot == 0 ? 0
+                                  : (int)std::min<int64_t>((65536 / per_slot) / 64 * 64,
+                                                           (avg_row * kSchedBS * 5 / 4 + 63) / 64 * 64);
+    const size_t shm = (size_t)cap * per_slot;
+    const int32_t* lrp = m->lumped->rowptr.as<int32_t>();
+    const int4* rows = sc.rows.as<int4>();
+    const uint2* rec = sc.rec.as<uint2>();
+    const VT* pxyz = sc.xyz.as<VT>();
+#define SCHED_LAUNCH(K, L)                                                                                    \
+    k_tria_rows_sched<VT, OT, K, L><<<nb, kSchedBS, shm, ctx->stream>>>(n, rows, rec, pxyz, lrp, clamp, vm, Aval, \
+                                                                        Bval, degen, cap)
+    if (kind == LH_TRIA) { if (lump) SCHED_LAUNCH(LH_TRIA, true); else SCHED_LAUNCH(LH_TRIA, false); }
+    else { if (lump) SCHED_LAUNCH(LH_TRIA_MASS, true); else SCHED_LAUNCH(LH_TRIA_MASS, false); }
+#undef SCHED_LAUNCH
+    LH_CHECK_LAUNCH();
+}
+
 template <typename VT, typename OT>
 void launch_numeric(lh_ctx* ctx, lh_mesh* m, int kind, int lump, const double* u1, const double* u2,
                     const double* am, int clamp, const VT* vm, OT* Aval, OT* Bval, int32_t* degen) {
     const Pattern& p = *m->full;
     const int64_t n = m->n;
+    // automatic order: the locality order pays for its schedule (and the
+    // pattern's order, unless a solve has built it) from a mesh's second
+    // numeric assembly on (time stepping, repeated solves)
+    const bool auto_sched = n >= kSchedAutoMinN && (m->numeric_runs > 0 || m->full->ro);
+    ++m->numeric_runs;
+    if ((kind == LH_TRIA || kind == LH_TRIA_MASS) && ctx->asm_order != LH_ASM_ORDER_MESH &&
+        (ctx->asm_order == LH_ASM_ORDER_LOCALITY || auto_sched)) {
+        if (const AsmSched* sc = asm_schedule<VT>(ctx, m)) {
+            launch_sched<VT, OT>(ctx, m, *sc, kind, lump, clamp, vm, Aval, Bval, degen);
+            return;
+        }
+    }
     const int32_t* lrp = m->lumped->rowptr.as<int32_t>();
     const int bs = kind == LH_TET ? 64 : 128;
     const unsigned nb = nblocks(n, bs);
@@ -1495,3 +1842,9 @@ int lh_diffgeo(lh_ctx* ctx, lh_mesh* m, int mode, const double* in, int nf, doub
 
 // the symbolic phase for other translation units (curvature)
 void lh_build_topology(lh_ctx* ctx, lh_mesh* m) { build_topology(ctx, m); }
+
+extern "C" int lh_ctx_set_assembly_order(lh_ctx* ctx, int order) {
+    if (!ctx || order < LH_ASM_ORDER_AUTO || order > LH_ASM_ORDER_LOCALITY) return LH_ERR_ARG;
+    ctx->asm_order = order;
+    return LH_OK;
+}

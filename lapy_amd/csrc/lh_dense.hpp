@@ -123,4 +123,9 @@ struct Reorder {
     void unpermute(lh_ctx* ctx, const double* in, int ldi, double* out, int ldo, int64_t n, int m) const;
 };
 
+// the locality order of a pattern: built on the first request (outside any
+// call arena) and kept with the pattern; concurrent first requests build it
+// once
+std::shared_ptr<Reorder> pattern_reorder(lh_ctx* ctx, Pattern& p);
+
 }  // namespace lh

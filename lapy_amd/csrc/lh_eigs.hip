@@ -73,7 +73,7 @@ struct Lobpcg {
     ChebPrecond* cheb = &own_cheb;  // a second phase shares the first one's
     bool cheb_ready = false;
     // locality reordering (solver-internal)
-    Reorder ro;
+    std::shared_ptr<Reorder> ro;      // the pattern's (kept with it)
     const Reorder* ext_ro = nullptr;  // a second phase's: the first phase's order
     std::unique_ptr<lh_csr> Ap, Bp;
     bool reordered = false;
@@ -188,9 +188,9 @@ struct Lobpcg {
     void run(double* out_evals, DevBuf* out_vecs) {
         double t_start = now_us();
         if (!ext_ro && n >= 4096 && (B->is_diag || B->pat == A->pat)) {
-            ro.build(ctx, *A->pat);
-            Ap = ro.apply(ctx, A);
-            Bp = ro.apply(ctx, B, B->is_diag ? nullptr : ro.pat);
+            ro = pattern_reorder(ctx, *A->pat);
+            Ap = ro->apply(ctx, A);
+            Bp = ro->apply(ctx, B, B->is_diag ? nullptr : ro->pat);
             A = Ap.get();
             B = Bp.get();
             reordered = true;
@@ -397,7 +397,7 @@ struct Lobpcg {
             ArenaOff keep;  // the result outlives the solver's buffers
             out_vecs->alloc(n * k * 8);
         }
-        const Reorder* order = ext_ro ? ext_ro : (reordered ? &ro : nullptr);
+        const Reorder* order = ext_ro ? ext_ro : (reordered ? ro.get() : nullptr);
         if constexpr (sizeof(T) == 8) {
             if (order) order->unpermute(ctx, Sp(), ld, out_vecs->as<double>(), k, n, k);
             else copy_block<double, double>(ctx, n, k, Sp(), ld, out_vecs->as<double>(), k);
@@ -565,7 +565,7 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
         copy_block<float, double>(ctx, n, m, L1.Sp(), L1.ld, X0.as<double>(), m);
         L.A = L1.A;
         L.B = L1.B;
-        L.ext_ro = L1.reordered ? &L1.ro : nullptr;
+        L.ext_ro = L1.reordered ? L1.ro.get() : nullptr;
         L.cheb = L1.cheb;
         L.cheb_ready = L.poly_degree > 1;
         L.x0 = X0.as<double>();

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -138,6 +139,7 @@ struct lh_ctx {
     void* host_small = nullptr;   // pinned host staging (hipHostMalloc)
     size_t host_small_bytes = 0;
     void* blas = nullptr;  // rocBLAS handle of the wide Rayleigh-Ritz solver, made on first use
+    int asm_order = 0;     // LH_ASM_ORDER_*: row order of the triangle assembly
 };
 
 // frees ctx->blas (lh_rrlib.hip)
@@ -155,15 +157,19 @@ void noalloc_scratch_init(int device);
 namespace lh {
 struct Reorder;
 }
+// locality-ordered row schedule of the triangle assembly (lh_assembly.hip)
+struct AsmSched;
 // CSR pattern (rows sorted, int32), shared by A and consistent B.
 struct Pattern {
     int64_t n = 0, nnz = 0;
     DevBuf rowptr;   // n+1 int32
     DevBuf cols;     // nnz int32
     DevBuf diagpos;  // n int32 (-1: row without diagonal)
-    // locality reordering of this pattern, built by the first solve that
-    // wants it and kept for later solves on the same pattern
+    // locality reordering of this pattern, built by the first solve (or
+    // assembly) that wants it and kept for later ones on the same pattern
+    // (lh::pattern_reorder)
     std::shared_ptr<lh::Reorder> ro;
+    std::mutex ro_mu;
 };
 
 struct lh_csr {
@@ -192,6 +198,9 @@ struct lh_mesh {
     DevBuf xyz;                          // AoS copy (x, y, z, 0) in the vertex dtype
     std::shared_ptr<Pattern> full;       // consistent pattern
     std::shared_ptr<Pattern> lumped;     // diagonal-of-used-vertices pattern
+    std::shared_ptr<AsmSched> sched;     // built on first use by a large triangle mesh
+    int sched_state = 0;                 // 0: not tried, 1: built, -1: mesh not eligible
+    int64_t numeric_runs = 0;            // numeric assembly launches so far
 };
 
 struct lh_eig {

@@ -1020,13 +1020,8 @@ int lh_mpnn_time_layer(lh_mpnn* net, lh_csr* A, lh_csr* B, double alpha, double 
     std::unique_ptr<lh_csr> Ap, Bp;
     if (reorder && (!B || B->is_diag || B->pat == A->pat)) {
         // the eigensolver's locality order, as the solver applies the network
-        if (!A->pat->ro) {
-            ArenaOff off;
-            auto r = std::make_shared<Reorder>();
-            r->build(ctx, *A->pat);
-            A->pat->ro = r;
-        }
-        const Reorder& ro = *A->pat->ro;
+        const auto rop = pattern_reorder(ctx, *A->pat);
+        const Reorder& ro = *rop;
         Ap = ro.apply(ctx, A);
         if (B) Bp = ro.apply(ctx, B, B->is_diag ? nullptr : ro.pat);
         A = Ap.get();

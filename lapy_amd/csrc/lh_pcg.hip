@@ -18,20 +18,28 @@
 // such an A is regular (its smallest eigenvalue ~1e-9) and is solved as it
 // stands, as SuperLU does, with no projection.
 //
-// One iteration is two launches (Jacobi or no preconditioner):
-//   k_cg_dir:    q = K z + beta q, p = z + beta p (so q = K p without a
-//                second product), block partials of p.q; the last block to
-//                finish sums the partials in a fixed order and forms alpha;
-//   k_cg_update: x += alpha p, r -= alpha q, z = D^-1 r, block partials of
-//                r.z and r.r; the last block forms beta, retires converged
-//                columns and sets a done flag when none is left.
-// K z is a CSR product with LPR lanes per row, the row's entries split over
-// the lanes and summed by lane shuffles.  Systems of 4,096 rows or more are
-// solved in the breadth-first locality order of the pattern (built once per
-// pattern and kept with it), and each workgroup walks one contiguous range of
-// rows, with the ranges of one XCD adjacent: the rows gathered by a range
-// then sit in that XCD's L2.  Sums over blocks are always taken in block
-// order, so results do not depend on scheduling.  The host checks the done
+// One iteration is two launches (Jacobi or no preconditioner), and no kernel
+// waits for another's blocks: each kernel reads the block partials the
+// previous one wrote, every block summing them itself in the same fixed order
+// (so all blocks agree on alpha and beta, and results do not depend on
+// scheduling):
+//   k_cg_dir(_ell): beta from the partials of r.z and r.r, then
+//                q = K z + beta q, p = z + beta p (so q = K p without a
+//                second product), and block partials of p.q (and of sum(r)
+//                for a singular system);
+//   k_cg_update: alpha from the partials of p.q, then x += alpha p,
+//                r -= alpha q (its mean removed for a singular system),
+//                z = D^-1 r, and block partials of r.z and r.r; block 0
+//                counts the iteration and sets a done flag when every column
+//                has converged.
+// The partials of r.z and r.r alternate between two buffers by iteration
+// parity, so the direction kernel never reads what the update of the same
+// iteration writes.  K z is a product on the operator's ELL copy, one thread
+// per row (CSR with LPR lanes per row when rows are long).  Systems of 4,096
+// rows or more are solved in the breadth-first locality order of the pattern
+// (built once per pattern and kept with it), and each workgroup walks one
+// contiguous range of rows, with the ranges of one XCD adjacent: the rows
+// gathered by a range then sit in that XCD's L2.  The host checks the done
 // flag once per chunk of iterations; iterations enqueued after convergence
 // exit at once.
 #include "lh_dense.hpp"
@@ -745,13 +753,7 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
     // built once per pattern
     std::shared_ptr<Reorder> ro;
     if (n >= 4096 && precond != LH_PRECOND_LEARNED) {
-        if (!A->pat->ro) {
-            ArenaOff off;  // kept with the pattern, beyond this call
-            auto r = std::make_shared<Reorder>();
-            r->build(ctx, *A->pat);
-            A->pat->ro = r;
-        }
-        ro = A->pat->ro;
+        ro = pattern_reorder(ctx, *A->pat);
     }
     std::unique_ptr<lh_csr> Kp;
     const Pattern* kpat = A->pat.get();

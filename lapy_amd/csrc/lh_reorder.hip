@@ -131,6 +131,17 @@ __global__ void k_gather_rows(const T* __restrict__ v, const int32_t* __restrict
 
 }  // namespace
 
+std::shared_ptr<Reorder> pattern_reorder(lh_ctx* ctx, Pattern& p) {
+    std::lock_guard<std::mutex> lk(p.ro_mu);
+    if (!p.ro) {
+        ArenaOff off;  // kept with the pattern, beyond this call
+        auto r = std::make_shared<Reorder>();
+        r->build(ctx, p);
+        p.ro = r;
+    }
+    return p.ro;
+}
+
 void Reorder::build(lh_ctx* ctx, const Pattern& p) {
     const int64_t n = p.n;
     const int32_t* rp = p.rowptr.as<int32_t>();

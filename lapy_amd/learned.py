@@ -22,9 +22,9 @@ stopped runs.
 Use: ``net.use()`` makes it the context's LH_PRECOND_LEARNED for
 ``Solver.eigs(precond="learned")`` and ``heat.diffusion``'s PCG; both fall
 back to plain Jacobi when convergence stalls: LOBPCG when the geometric mean of
-the wanted pairs' residuals has not halved over the last 8 iterations, PCG when
-the residual falls by less than half per iteration over three 4-iteration checks
-running.
+the wanted pairs' residuals has fallen by less than 10 % over the last 16
+iterations, PCG when the residual falls by less than half per iteration over
+three 4-iteration checks running.
 """
 
 from __future__ import annotations

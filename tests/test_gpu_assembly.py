@@ -221,3 +221,86 @@ def test_avg_edge_length_gpu(expected):
     assert tet32.avg_edge_length() == fem.avg_edge_length(tet32.v, tet32.t)
     big = meshgen.tet_cube(30, jitter=0.1)
     assert TetMesh(*big).avg_edge_length() == fem.avg_edge_length(*big)
+
+
+def _assemble_order(v, t, kind, lump, order, dtype=np.float64):
+    ctx = _lib.default_context()
+    ctx.set_assembly_order(order)
+    try:
+        return gpu_assemble(v, t, kind, lump, dtype)
+    finally:
+        ctx.set_assembly_order(_lib.LH_ASM_ORDER_AUTO)
+
+
+def _clique_and_islands(nc=50, islands=20000):
+    """A 50-vertex clique (triangles (i, j, j+1)) beside isolated triangles:
+    the clique's rows hold 50 entries while the average row holds 4, so the
+    locality-ordered workgroup holding them exceeds its LDS window and
+    accumulates in global memory."""
+    rng = np.random.default_rng(3)
+    tri = [[i, j, (j + 1) % nc] for i in range(nc) for j in range(nc) if i != j and i != (j + 1) % nc]
+    base = nc + 3 * np.arange(islands)[:, None]
+    t = np.vstack([np.array(tri), base + np.arange(3)[None, :]])
+    v = rng.standard_normal((nc + 3 * islands, 3))
+    return v, t
+
+
+def _order_cases():
+    rng = np.random.default_rng(12)
+    yield "ellipsoid_l5", meshgen.perturbed_sphere(5, rng, axes=(1.3, 0.8, 1.0), sigma=0.02)
+    v, t = _read_off(data_path("square-mesh.off"))
+    yield "square_open_f32", (v, t)
+    v, t = meshgen.icosphere(3)
+    v = np.concatenate([v, [[5.0, 5, 5], [6.0, 6, 6]]])
+    t = np.where(t >= 17, t + 1, t)
+    yield "unused_vertices", (np.insert(v, 17, [[9.0, 9, 9]], axis=0), t)
+    v, t = meshgen.icosphere(3)
+    t = t.copy()
+    t[3] = [t[3, 0], t[3, 0], t[3, 1]]
+    t[40] = [t[40, 0], t[40, 1], t[40, 1]]
+    yield "degenerate_clamp", (v, t)
+    yield "clique_global_path", _clique_and_islands()
+
+
+@pytest.mark.parametrize("case", ["ellipsoid_l5", "square_open_f32", "unused_vertices", "degenerate_clamp",
+                                  "clique_global_path"])
+def test_locality_order_is_bit_identical(case):
+    """The locality-ordered triangle assembly (rows in the pattern's BFS
+    order, coordinates from a copy in that order, 8-byte incidence records)
+    gives the mesh-order kernel's values bit for bit: every row is summed in
+    ascending element order and written at its own CSR offset."""
+    v, t = dict(_order_cases())[case]
+    print(f"case: assembly order {case} nv={len(v)} ne={len(t)}", flush=True)
+    for kind, lump, dtype in (("tria", False, np.float64), ("tria", True, np.float64), ("mass", False, np.float64),
+                              ("mass", True, np.float64), ("tria", False, np.float32)):
+        A0, B0 = _assemble_order(v, t, kind, lump, _lib.LH_ASM_ORDER_MESH, dtype)
+        A1, B1 = _assemble_order(v, t, kind, lump, _lib.LH_ASM_ORDER_LOCALITY, dtype)
+        for m0, m1 in ((A0, A1), (B0, B1)):
+            if m0 is None:
+                continue
+            assert np.array_equal(m0.indptr, m1.indptr) and np.array_equal(m0.indices, m1.indices)
+            assert m0.data.dtype == m1.data.dtype
+            assert np.array_equal(m0.data, m1.data), (kind, lump, dtype)
+
+
+def test_locality_order_against_oracle_and_auto_at_l7():
+    """The locality order against the oracle's element-order sums, and the
+    automatic choice (locality order from 65,536 vertices) at level 7."""
+    print("case: locality-order assembly vs oracle (L4) and auto vs mesh order (L7)", flush=True)
+    v, t = meshgen.perturbed_sphere(4, np.random.default_rng(2), axes=(1.2, 0.9, 1.0), sigma=0.02)
+    ctx = _lib.default_context()
+    ctx.set_assembly_order(_lib.LH_ASM_ORDER_LOCALITY)
+    try:
+        for lump in (False, True):
+            check_against_oracle(v, t, "tria", lump)
+        check_against_oracle(v, t, "mass", False)
+    finally:
+        ctx.set_assembly_order(_lib.LH_ASM_ORDER_AUTO)
+    v, t = meshgen.perturbed_sphere(7, np.random.default_rng(3), axes=(1.2, 0.9, 1.0), sigma=0.01)
+    A0, B0 = _assemble_order(v, t, "tria", False, _lib.LH_ASM_ORDER_MESH)
+    # automatic order: mesh order on the first assembly, locality order on the second
+    dm = _lib.DeviceMesh(ctx, v, t)
+    for _ in range(2):
+        a, b = dm.assemble(_lib.LH_TRIA, False, np.float64)
+        A1, B1 = a.to_scipy_csc(), b.to_scipy_csc()
+        assert np.array_equal(A0.data, A1.data) and np.array_equal(B0.data, B1.data)

@@ -88,3 +88,29 @@ def test_diffgeo_dispatch_errors():
                     (diffgeo.compute_geodesic_f, np.zeros(3))):
         with pytest.raises(ValueError, match='Geometry type "Polygon" unknown'):
             fn(Polygon(), arg)
+
+
+def test_corner_two_term_cotangent_form_is_exact():
+    """The triangle kernels form only the corner's two cotangent terms: for
+    corner l the operand pairs (x, y) are selected as in lh_assembly.hip's
+    tria_entries, and the values equal the three-term form's
+    (a12, a31 | a12, a23 | a23, a31, diagonal -a - b) bit for bit, with numpy's
+    ((p0 + p1) + p2) dot and the element's 2|cross| volume."""
+    rng = np.random.default_rng(0)
+
+    def dot(a, b):
+        p = a * b
+        return (p[0] + p[1]) + p[2]
+
+    for _ in range(20000):
+        v1, v2, v3 = rng.standard_normal((3, 3)) * rng.choice([1e-3, 1.0, 1e3])
+        d21, d32, d13 = v2 - v1, v3 - v2, v1 - v3
+        cr = np.cross(d32, d13)
+        vol = 2.0 * np.sqrt(dot(cr, cr))
+        a12, a23, a31 = dot(d32, d13) / vol, dot(d13, d21) / vol, dot(d21, d32) / vol
+        three = {0: (a12, a31, -a12 - a31), 1: (a12, a23, -a12 - a23), 2: (a23, a31, -a31 - a23)}
+        for l in range(3):
+            xa, xb = (d13, d21) if l == 2 else (d32, d13)
+            ya, yb = (d13, d21) if l == 1 else (d21, d32)
+            aa, ab = dot(xa, xb) / vol, dot(ya, yb) / vol
+            assert (aa, ab, -aa - ab) == three[l]
