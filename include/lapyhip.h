@@ -237,12 +237,37 @@ int lh_eigs(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma,
  * out[i]). */
 int lh_eigs_batched(lh_ctx* ctx, int nmesh, lh_csr* const* A, lh_csr* const* B, const int* k, double sigma,
                     const lh_eig_opts* opts, int workers, lh_eig** out, int* status, lh_stats* stats);
-/* Autotuner of the batched solver: solves the given problems with 1, 2, 4, 8
- * and 16 workers, keeps the fastest count in the context (used by
- * lh_eigs_batched with workers = 0) and reports it and the five wall times
- * (seconds, nullable). */
+/* Autotuner of the batched solver (no counterpart in the reference): solves
+ * the given problems with 1, 2, 4, 8 and 16 workers, keeps the fastest count
+ * in the context (used by lh_eigs_batched with workers = 0) and reports it
+ * and the five wall times (seconds, nullable); then, for each problem shape
+ * not yet in the context's table, times the ELL SpMM pair on the solver's
+ * block width with 64, 128 and 256 threads per workgroup and keeps the
+ * fastest (used by every solve on the context and its batch workers). */
 int lh_eigs_autotune(lh_ctx* ctx, int nmesh, lh_csr* const* A, lh_csr* const* B, const int* k, double sigma,
                      const lh_eig_opts* opts, int* best_workers, double* seconds);
+/* The context's tuned launch shapes, for a cache kept across processes
+ * (lapy_amd.tuning stores them keyed by lh_device_arch): one entry per
+ * matrix shape (n, nnz), with the k and block width it was tuned on and the
+ * SpMM workgroup size (64, 128 or 256).  lh_tune_set adds an entry or
+ * replaces the one of the same shape; none of these may run while a solve on
+ * the context does.  lh_ctx_get_workers / lh_ctx_set_workers: the tuned
+ * worker count of lh_eigs_batched (0: not tuned).  lh_device_arch: the
+ * device's architecture name (e.g. "gfx950:sramecc+:xnack-"). */
+typedef struct {
+    int64_t n;
+    int64_t nnz;
+    int32_t k;
+    int32_t ncol;
+    int32_t block;
+    int32_t reserved;
+} lh_tune_entry;
+int lh_tune_count(lh_ctx* ctx, int* count);
+int lh_tune_get(lh_ctx* ctx, int i, lh_tune_entry* out);
+int lh_tune_set(lh_ctx* ctx, const lh_tune_entry* e);
+int lh_ctx_get_workers(lh_ctx* ctx, int* workers);
+int lh_ctx_set_workers(lh_ctx* ctx, int workers);
+int lh_device_arch(lh_ctx* ctx, char* buf, int len);
 /* Copy results to host: evals (k), evecs (n x k, column-major = numpy
  * Fortran order, i.e. evecs[:, j] is contiguous). Either may be NULL. */
 int  lh_eig_download(lh_eig* e, double* evals, double* evecs);

@@ -112,6 +112,12 @@ SIGNATURES = {
     "lh_time_cheb_step": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     "lh_eigs_batched": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_double, _vp, C.c_int, _vp, _vp, _vp]),
     "lh_eigs_autotune": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_double, _vp, _vp, _vp]),
+    "lh_tune_count": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "lh_tune_get": (C.c_int, [_vp, C.c_int, _vp]),
+    "lh_tune_set": (C.c_int, [_vp, _vp]),
+    "lh_ctx_get_workers": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "lh_ctx_set_workers": (C.c_int, [_vp, C.c_int]),
+    "lh_device_arch": (C.c_int, [_vp, C.c_char_p, C.c_int]),
     "lh_curvature": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lh_curvature_tria": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
     "lh_mpnn_param_count": (C.c_int, []),
@@ -179,6 +185,12 @@ def check(code, ctx=None, what=""):
     raise LapyHipError(code, msg)
 
 
+class TuneEntry(C.Structure):
+    """lh_tune_entry: a tuned ELL SpMM workgroup size for one matrix shape."""
+    _fields_ = [("n", C.c_int64), ("nnz", C.c_int64), ("k", C.c_int32), ("ncol", C.c_int32), ("block", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 class Context:
     """One HIP device + stream (lh_ctx)."""
 
@@ -202,6 +214,32 @@ class Context:
 
     def sync(self):
         check(lib().lh_sync(self.handle), self, "lh_sync")
+
+    def arch(self) -> str:
+        """The device's architecture name (lh_device_arch)."""
+        buf = C.create_string_buffer(256)
+        check(lib().lh_device_arch(self.handle, buf, 256), self, "lh_device_arch")
+        return buf.value.decode()
+
+    def tuning(self):
+        """(tuned worker count or 0, [dict(n, nnz, k, ncol, block), ...])."""
+        w = C.c_int(0)
+        check(lib().lh_ctx_get_workers(self.handle, C.byref(w)), self, "lh_ctx_get_workers")
+        cnt = C.c_int(0)
+        check(lib().lh_tune_count(self.handle, C.byref(cnt)), self, "lh_tune_count")
+        out = []
+        for i in range(cnt.value):
+            e = TuneEntry()
+            check(lib().lh_tune_get(self.handle, i, C.byref(e)), self, "lh_tune_get")
+            out.append({"n": e.n, "nnz": e.nnz, "k": e.k, "ncol": e.ncol, "block": e.block})
+        return w.value, out
+
+    def set_tuning(self, workers: int, entries):
+        """Install a tuned worker count and SpMM shapes (as returned by tuning())."""
+        check(lib().lh_ctx_set_workers(self.handle, int(workers)), self, "lh_ctx_set_workers")
+        for d in entries:
+            e = TuneEntry(int(d["n"]), int(d["nnz"]), int(d["k"]), int(d["ncol"]), int(d["block"]), 0)
+            check(lib().lh_tune_set(self.handle, C.byref(e)), self, "lh_tune_set")
 
     def set_assembly_order(self, order: int):
         """Row order of the triangle assembly (LH_ASM_ORDER_*); the values

@@ -508,7 +508,7 @@ lh_eig* eigs_impl(lh_ctx* ctx, lh_csr* A, lh_csr* B, int k, double sigma, const 
         // the learned network where the automatic polynomial would be plain
         // Jacobi (small meshes), Chebyshev-Jacobi elsewhere
         const bool small = (opts && opts->poly_degree > 0) ? opts->poly_degree == 1 : auto_deg < 4;
-        const bool fp64 = !opts || opts->precision == LH_PREC_FP64;  // the mixed solve takes no network
+        const bool fp64 = !opts || opts->precision == LH_PREC_FP64;  // auto picks the network for FP64 solves only
         pc = (ctx->mpnn && small && fp64) ? LH_PRECOND_LEARNED : LH_PRECOND_JACOBI;
     }
     if (pc == LH_PRECOND_LEARNED && !ctx->mpnn)
@@ -598,6 +598,7 @@ void ensure_workers(lh_ctx* ctx, int nw) {
         lh_ctx* w = nullptr;
         const int c = lh_ctx_create(ctx->device, &w);
         if (c != LH_OK) LH_THROW(c, "could not create a worker context");
+        w->tune = ctx->tune;  // the workers use the context's tuned launch shapes
         ctx->workers.push_back(w);
     }
 }
@@ -725,6 +726,51 @@ int lh_eigs_batched(lh_ctx* ctx, int nmesh, lh_csr* const* A, lh_csr* const* B, 
     LH_API_END(ctx)
 }
 
+namespace {
+
+// the ELL SpMM pair's workgroup size for one problem shape: 64, 128 and 256
+// threads timed (HIP events, 20 launches each) on the solver's block width,
+// the fastest kept in the context's table (a shape already there is kept)
+void tune_spmm_shape(lh_ctx* ctx, const lh_csr* A, const lh_csr* B, int k, const lh_eig_opts* opts) {
+    const int64_t n = A->n();
+    const int m = block_width(k, opts);
+    if (3 * m > n || n <= 384 || A->dtype != LH_F64) return;  // the dense path: no SpMM
+    if (tuned_spmm_block(ctx, n, A->nnz())) return;
+    Ell e;
+    e.build(ctx, A, B);
+    if (!e.W) return;
+    DevBuf X((size_t)n * m * 8), YA((size_t)n * m * 8), YB((size_t)n * m * 8);
+    memset_d(ctx, X.p, 0, (size_t)n * m * 8);
+    double* yb = e.two ? YB.as<double>() : nullptr;
+    static const int cand[3] = {64, 128, 256};
+    hipEvent_t e0, e1;
+    HIPX(hipEventCreate(&e0));
+    HIPX(hipEventCreate(&e1));
+    double best = 1e300;
+    int bb = 0;
+    for (int c = 0; c < 3; ++c) {
+        if (!spmm_ell<double, double>(ctx, e, X.as<double>(), YA.as<double>(), yb, m, m, m, 0, cand[c])) break;
+        HIPX(hipEventRecord(e0, ctx->stream));
+        for (int it = 0; it < 20; ++it)
+            spmm_ell<double, double>(ctx, e, X.as<double>(), YA.as<double>(), yb, m, m, m, 0, cand[c]);
+        HIPX(hipEventRecord(e1, ctx->stream));
+        HIPX(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPX(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            bb = cand[c];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!bb) return;
+    std::lock_guard<std::mutex> lk(ctx->tune->mu);
+    ctx->tune->spmm.push_back(SpmmTune{n, A->nnz(), k, m, bb});
+}
+
+}  // namespace
+
 int lh_eigs_autotune(lh_ctx* ctx, int nmesh, lh_csr* const* A, lh_csr* const* B, const int* k, double sigma,
                      const lh_eig_opts* opts, int* best_workers, double* seconds) {
     if (!ctx || nmesh <= 0 || !A || !B || !k) return LH_ERR_ARG;
@@ -749,7 +795,58 @@ int lh_eigs_autotune(lh_ctx* ctx, int nmesh, lh_csr* const* A, lh_csr* const* B,
     }
     ctx->tuned_workers = bw;
     if (best_workers) *best_workers = bw;
+    // then the SpMM's workgroup size per problem shape
+    for (int i = 0; i < nmesh; ++i) tune_spmm_shape(ctx, A[i], B[i], k[i], opts);
     LH_API_END(ctx)
+}
+
+int lh_tune_count(lh_ctx* ctx, int* count) {
+    if (!ctx || !count) return LH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->tune->mu);
+    *count = (int)ctx->tune->spmm.size();
+    return LH_OK;
+}
+
+int lh_tune_get(lh_ctx* ctx, int i, lh_tune_entry* out) {
+    if (!ctx || !out) return LH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->tune->mu);
+    if (i < 0 || i >= (int)ctx->tune->spmm.size()) return LH_ERR_ARG;
+    const SpmmTune& t = ctx->tune->spmm[i];
+    *out = lh_tune_entry{t.n, t.nnz, t.k, t.ncol, t.block, 0};
+    return LH_OK;
+}
+
+int lh_tune_set(lh_ctx* ctx, const lh_tune_entry* e) {
+    if (!ctx || !e || e->n <= 0 || e->nnz < 0 || (e->block != 64 && e->block != 128 && e->block != 256))
+        return LH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(ctx->tune->mu);
+    for (SpmmTune& t : ctx->tune->spmm)
+        if (t.n == e->n && t.nnz == e->nnz) {
+            t = SpmmTune{e->n, e->nnz, e->k, e->ncol, e->block};
+            return LH_OK;
+        }
+    ctx->tune->spmm.push_back(SpmmTune{e->n, e->nnz, e->k, e->ncol, e->block});
+    return LH_OK;
+}
+
+int lh_ctx_get_workers(lh_ctx* ctx, int* workers) {
+    if (!ctx || !workers) return LH_ERR_ARG;
+    *workers = ctx->tuned_workers;
+    return LH_OK;
+}
+
+int lh_ctx_set_workers(lh_ctx* ctx, int workers) {
+    if (!ctx || workers < 0 || workers > 64) return LH_ERR_ARG;
+    ctx->tuned_workers = workers;
+    return LH_OK;
+}
+
+int lh_device_arch(lh_ctx* ctx, char* buf, int len) {
+    if (!ctx || !buf || len <= 0) return LH_ERR_ARG;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) return LH_ERR_HIP;
+    std::snprintf(buf, (size_t)len, "%s", prop.gcnArchName);
+    return LH_OK;
 }
 
 int lh_eig_download(lh_eig* e, double* evals, double* evecs) {

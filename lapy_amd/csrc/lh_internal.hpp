@@ -122,6 +122,19 @@ struct ArenaOff {
 // --------------------------------------------------------------------------
 // handles
 // --------------------------------------------------------------------------
+// tuned workgroup size of the ELL SpMM for one problem shape (lh_eigs_autotune,
+// lh_tune_set), looked up by the matrix's (n, nnz); 256 threads when absent.
+// k and ncol (the solver's block width) record what it was tuned on.
+// Shared by a context and its batch workers; not changed while solves run.
+struct SpmmTune {
+    int64_t n, nnz;
+    int k, ncol, block;
+};
+struct TuneTable {
+    std::mutex mu;
+    std::vector<SpmmTune> spmm;
+};
+
 struct lh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -140,6 +153,7 @@ struct lh_ctx {
     size_t host_small_bytes = 0;
     void* blas = nullptr;  // rocBLAS handle of the wide Rayleigh-Ritz solver, made on first use
     int asm_order = 0;     // LH_ASM_ORDER_*: row order of the triangle assembly
+    std::shared_ptr<TuneTable> tune = std::make_shared<TuneTable>();
 };
 
 // frees ctx->blas (lh_rrlib.hip)
@@ -260,7 +274,7 @@ std::unique_ptr<lh_csr> csr_to_f32(lh_ctx* ctx, const lh_csr* s);
 // the slots' columns and values and eight gathers go out together.
 struct Ell {
     static constexpr int kMaxW = 32;
-    int64_t n = 0;
+    int64_t n = 0, nnz = 0;
     int W = 0;          // 0: not built (rows too long, or a diagonal matrix)
     bool two = false;   // vb holds the second matrix
     int dtype = LH_F64;
@@ -270,8 +284,13 @@ struct Ell {
 // YA = a X (and YB = b X when the ELL holds two matrices and YB is given);
 // which = 1: YB = b X.  Returns false (nothing launched) when the copy is not
 // built or the block is not 16-byte vectorisable; the caller then uses CSR.
+// block: threads per workgroup (64, 128 or 256); 0 takes the context's tuned
+// size for this shape (256 when untuned).
 template <typename VT, typename XT>
-bool spmm_ell(lh_ctx* ctx, const Ell& e, const XT* X, XT* YA, XT* YB, int ncol, int ldx, int ldy, int which = 0);
+bool spmm_ell(lh_ctx* ctx, const Ell& e, const XT* X, XT* YA, XT* YB, int ncol, int ldx, int ldy, int which = 0,
+              int block = 0);
+// the tuned ELL SpMM workgroup size for a matrix shape (0: none)
+int tuned_spmm_block(const lh_ctx* ctx, int64_t n, int64_t nnz);
 
 // 16-byte vector of a block row: 4 floats or 2 doubles per lane
 template <typename T> struct Vec16;

@@ -288,6 +288,7 @@ void Ell::build(lh_ctx* ctx, const lh_csr* a, const lh_csr* b) {
     if (a->is_diag) return;
     if (b && (b->is_diag || b->pat != a->pat || b->dtype != a->dtype)) b = nullptr;
     n = a->n();
+    nnz = a->nnz();
     dtype = a->dtype;
     const Pattern& p = *a->pat;
     DevBuf wmax(8);
@@ -315,8 +316,17 @@ void Ell::build(lh_ctx* ctx, const lh_csr* a, const lh_csr* b) {
     two = b != nullptr;
 }
 
+int tuned_spmm_block(const lh_ctx* ctx, int64_t n, int64_t nnz) {
+    if (!ctx->tune) return 0;
+    std::lock_guard<std::mutex> lk(ctx->tune->mu);
+    for (const SpmmTune& t : ctx->tune->spmm)
+        if (t.n == n && t.nnz == nnz) return t.block;
+    return 0;
+}
+
 template <typename VT, typename XT>
-bool spmm_ell(lh_ctx* ctx, const Ell& e, const XT* X, XT* YA, XT* YB, int ncol, int ldx, int ldy, int which) {
+bool spmm_ell(lh_ctx* ctx, const Ell& e, const XT* X, XT* YA, XT* YB, int ncol, int ldx, int ldy, int which,
+              int block) {
     if (e.W == 0 || (which == 1 && !e.two)) return false;
     const bool pair = which == 0 && e.two && YB != nullptr;
     XT* y = (which == 1) ? YB : YA;
@@ -330,11 +340,13 @@ bool spmm_ell(lh_ctx* ctx, const Ell& e, const XT* X, XT* YA, XT* YB, int ncol, 
     const VT* a = (which == 1) ? e.vb.as<VT>() : e.va.as<VT>();
     const VT* b = e.vb.as<VT>();
     const int lanes = vec_lanes(ncol, v);
-    const unsigned nb = nblocks_xcd(n * lanes, 256);
+    if (block == 0) block = tuned_spmm_block(ctx, n, e.nnz);
+    if (block != 64 && block != 128) block = 256;
+    const unsigned nb = nblocks_xcd(n * lanes, block);
 #define LH_ELL_LAUNCH(L)                                                                                     \
     do {                                                                                                     \
-        if (pair) k_spmm_e<VT, XT, L, true><<<nb, 256, 0, ctx->stream>>>(n, e.W, ec, a, b, X, YA, YB, ncol, ldx, ldy); \
-        else k_spmm_e<VT, XT, L, false><<<nb, 256, 0, ctx->stream>>>(n, e.W, ec, a, nullptr, X, y, nullptr, ncol, ldx, ldy); \
+        if (pair) k_spmm_e<VT, XT, L, true><<<nb, block, 0, ctx->stream>>>(n, e.W, ec, a, b, X, YA, YB, ncol, ldx, ldy); \
+        else k_spmm_e<VT, XT, L, false><<<nb, block, 0, ctx->stream>>>(n, e.W, ec, a, nullptr, X, y, nullptr, ncol, ldx, ldy); \
     } while (0)
     switch (lanes) {
         case 1: LH_ELL_LAUNCH(1); break;
@@ -349,8 +361,8 @@ bool spmm_ell(lh_ctx* ctx, const Ell& e, const XT* X, XT* YA, XT* YB, int ncol, 
     LH_CHECK_LAUNCH();
     return true;
 }
-template bool spmm_ell<double, double>(lh_ctx*, const Ell&, const double*, double*, double*, int, int, int, int);
-template bool spmm_ell<float, float>(lh_ctx*, const Ell&, const float*, float*, float*, int, int, int, int);
+template bool spmm_ell<double, double>(lh_ctx*, const Ell&, const double*, double*, double*, int, int, int, int, int);
+template bool spmm_ell<float, float>(lh_ctx*, const Ell&, const float*, float*, float*, int, int, int, int, int);
 
 template void spmm<double, double>(lh_ctx*, const lh_csr*, const double*, double*, int, int);
 template void spmm<float, double>(lh_ctx*, const lh_csr*, const double*, double*, int, int);

@@ -35,13 +35,16 @@ def _shapedna_dict(geom, k, evals, evecs) -> dict:
 
 
 def compute_shapedna_batch(geoms, k: int = 50, lump: bool = False, workers: int = 0, autotune: bool = False,
-                           **eig_opts) -> list:
+                           tune_cache=None, **eig_opts) -> list:
     """ShapeDNA of a collection of meshes in one call: what a loop of
     ``compute_shapedna(geom, k, lump)`` over ``geoms`` returns (lapy/
     shapedna.py:96-166), with the meshes assembled on the GPU and their
     eigenproblems solved concurrently (lh_eigs_batched; ``workers = 0`` uses
-    the autotuned or default count, ``autotune=True`` tunes it on this batch
-    first).  Anisotropic assembly is not batched: use compute_shapedna."""
+    the autotuned or default count, ``autotune=True`` tunes the worker count
+    and the SpMM workgroup sizes on this batch first).  ``tune_cache``: a
+    JSON file of tuned shapes (lapy_amd.tuning), loaded first and tuned and
+    updated only for shapes it lacks.  Anisotropic assembly is not batched:
+    use compute_shapedna."""
     if not geoms:
         return []
     ctx = _lib.default_context()
@@ -57,7 +60,11 @@ def compute_shapedna_batch(geoms, k: int = 50, lump: bool = False, workers: int 
         Bs.append(b)
         keep.append(dm)
     ks = [k] * len(geoms)
-    if autotune:
+    if tune_cache is not None:
+        from . import tuning
+
+        tuning.autotune_cached(As, Bs, ks, tune_cache, ctx, **eig_opts)
+    elif autotune:
         _lib.eigs_autotune(ctx, As, Bs, ks, **eig_opts)
     res = _lib.eigs_batched(ctx, As, Bs, ks, workers=workers, **eig_opts)
     return [_shapedna_dict(g, k, r[0], r[1]) for g, r in zip(geoms, res)]

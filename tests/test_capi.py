@@ -83,7 +83,8 @@ def test_struct_layout_matches_header(tmp_path):
     cc = shutil.which("gcc") or shutil.which("cc")
     if cc is None:
         pytest.skip("no host C compiler")
-    structs = {"lh_stats": _lib.Stats, "lh_eig_opts": _lib.EigOpts, "lh_pcg_opts": _lib.PcgOpts}
+    structs = {"lh_stats": _lib.Stats, "lh_eig_opts": _lib.EigOpts, "lh_pcg_opts": _lib.PcgOpts,
+               "lh_tune_entry": _lib.TuneEntry}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "lapyhip.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

@@ -119,3 +119,42 @@ def test_batched_with_the_learned_preconditioner():
     # cleared on the context: the workers no longer see it
     with pytest.raises(ValueError):
         _lib.eigs_batched(ctx, As, Bs, ks, workers=2, precond=_lib.LH_PRECOND_LEARNED)
+
+
+def test_autotune_spmm_shapes_and_persistent_cache(tmp_path):
+    """The tuner's second stage picks an SpMM workgroup size per matrix shape;
+    the context's table survives a save and a load into a fresh context; the
+    workgroup size does not change the results (bit for bit); a cached shape
+    is not tuned again."""
+    from lapy_amd import tuning
+
+    print("case: autotune SpMM shapes (L4, L5, L4) + tuning cache", flush=True)
+    rng = np.random.default_rng(31)
+    meshes = [meshgen.perturbed_sphere(lv, rng, axes=(1.2, 0.9, 1.0), sigma=0.01) for lv in (4, 5, 4)]
+    ctx = _lib.Context(0)
+    mats = [_lib.DeviceMesh(ctx, v, t).assemble(_lib.LH_TRIA, False, np.float64) for v, t in meshes]
+    As, Bs = [a for a, _ in mats], [b for _, b in mats]
+    ks = [20, 20, 20]
+    best, secs = _lib.eigs_autotune(ctx, As, Bs, ks)
+    workers, entries = ctx.tuning()
+    assert workers == best
+    shapes = {(a.n, a.nnz) for a in As}
+    assert {(e["n"], e["nnz"]) for e in entries} == shapes
+    assert all(e["block"] in (64, 128, 256) and e["k"] == 20 for e in entries)
+    path = str(tmp_path / "tune.json")
+    tuning.save(path, ctx)
+    ctx2 = _lib.Context(0)
+    assert tuning.load(path, ctx2) == len(shapes)
+    assert ctx2.tuning() == ctx.tuning()
+    # results do not depend on the workgroup size
+    res = {}
+    for blk in (64, 256):
+        c = _lib.Context(0)
+        c.set_tuning(4, [dict(e, block=blk) for e in entries])
+        m = [_lib.DeviceMesh(c, v, t).assemble(_lib.LH_TRIA, False, np.float64) for v, t in meshes]
+        out = _lib.eigs_batched(c, [a for a, _ in m], [b for _, b in m], ks)
+        res[blk] = [r[0] for r in out]
+    for x, y in zip(res[64], res[256]):
+        assert np.array_equal(x, y)
+    # the cache covers these shapes: no second tuning
+    assert tuning.autotune_cached(As, Bs, ks, path, ctx2) is False
