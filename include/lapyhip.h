@@ -291,6 +291,15 @@ typedef struct lh_pcg_opts {
                               back to Jacobi (restarting CG) when the residual
                               falls by less than half per iteration over three
                               4-iteration convergence checks running         */
+    int32_t poly_degree;   /* LH_PRECOND_JACOBI: degree of the Chebyshev-Jacobi
+                              polynomial z = p(D^-1 K) D^-1 r (symmetric
+                              positive definite, on [bmax / (2 deg^2), bmax],
+                              bmax the Gershgorin bound of D^-1 K); 0 or 1:
+                              plain Jacobi (the default: on one right-hand
+                              side each Chebyshev step costs ~0.4 of a
+                              Jacobi-CG iteration, so fewer iterations save
+                              little time); at most 256                      */
+    int32_t reserved;
 } lh_pcg_opts;
 int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta,
            const double* b, int nrhs, const uint8_t* dmask, double* x,

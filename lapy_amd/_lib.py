@@ -77,6 +77,8 @@ class PcgOpts(C.Structure):
         ("rtol", C.c_double),
         ("verbose", C.c_int32),
         ("no_fallback", C.c_int32),
+        ("poly_degree", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -666,15 +668,16 @@ class EigHandle:
 
 
 def pcg(ctx: Context, A: DeviceCSR, B, alpha: float, beta: float, b, x0=None, dmask=None, rtol=1e-12,
-        max_iter=0, precond=LH_PRECOND_JACOBI, verbose=0, no_fallback=False):
-    """Solve (alpha*B + beta*A) x = b (b: (n,) or (n, nrhs)); returns (x, stats)."""
+        max_iter=0, precond=LH_PRECOND_JACOBI, verbose=0, no_fallback=False, poly_degree=0):
+    """Solve (alpha*B + beta*A) x = b (b: (n,) or (n, nrhs)); returns (x, stats).
+    poly_degree (Jacobi): Chebyshev-Jacobi polynomial degree (0 or 1: plain Jacobi)."""
     b = np.asarray(b, dtype=np.float64)
     squeeze = b.ndim == 1
     b2 = b[:, None] if squeeze else b
     bf = np.asfortranarray(b2)
     x = np.asfortranarray(np.zeros_like(b2) if x0 is None else np.asarray(x0, dtype=np.float64).reshape(b2.shape))
     dm = None if dmask is None else np.ascontiguousarray(dmask, dtype=np.uint8)
-    opts = PcgOpts(precond, max_iter, rtol, verbose, int(bool(no_fallback)))
+    opts = PcgOpts(precond, max_iter, rtol, verbose, int(bool(no_fallback)), int(poly_degree), 0)
     st = Stats()
     code = lib().lh_pcg(ctx.handle, A.handle, B.handle if B is not None else None, float(alpha), float(beta),
                         _ptr(bf), bf.shape[1], _ptr(dm), _ptr(x), C.byref(opts), C.byref(st))

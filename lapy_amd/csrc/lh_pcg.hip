@@ -34,8 +34,11 @@
 //                has converged.
 // The partials of r.z and r.r alternate between two buffers by iteration
 // parity, so the direction kernel never reads what the update of the same
-// iteration writes.  K z is a product on the operator's ELL copy, one thread
-// per row (CSR with LPR lanes per row when rows are long).  Systems of 4,096
+// iteration writes.  With the Chebyshev-Jacobi preconditioner (a polynomial
+// degree in the options) the update leaves z alone, deg - 1 fused Chebyshev
+// steps on the ELL copy form z = p(D^-1 K) D^-1 r, and one more launch
+// writes the partials of r.z.  K z is a product on the operator's ELL copy,
+// one thread per row (CSR with LPR lanes per row when rows are long).  Systems of 4,096
 // rows or more are solved in the breadth-first locality order of the pattern
 // (built once per pattern and kept with it), and each workgroup walks one
 // contiguous range of rows, with the ranges of one XCD adjacent: the rows
@@ -463,6 +466,80 @@ k_cg_rz(int64_t n, int m, int c0, int mc, const double* __restrict__ R, const do
     }
 }
 
+// ---- Chebyshev-Jacobi preconditioner: z = p(D^-1 K) D^-1 r -------------------
+// The Chebyshev semi-iteration (two-step form) for K z = r on [amin, bmax],
+// started from z_0 = 0: z_1 = a D^-1 r, then
+//   z_{k+1} = w_k (z_k - z_{k-1} + a D^-1 (r - K z_k)) + z_{k-1},
+// a = 2 / (bmax + amin), written over z_{k-1} (row-local), z_k gathered on
+// the operator's ELL copy (padding slots point at the row with weight 0).
+// The map r -> z is a fixed polynomial in D^-1 K times D^-1: symmetric, and
+// positive definite when p > 0 on the spectrum, which the residual
+// polynomial's bound below 1 on [amin, bmax] gives (and p(0) > 0 below it).
+// Converged solves (status[0]) skip the work.
+__global__ void __launch_bounds__(kBS)
+k_pc_cheb_init(int64_t n, int m, const double* __restrict__ dinv, const double* __restrict__ R, double a,
+               double* __restrict__ Z1, double* __restrict__ Z0, const int* __restrict__ status) {
+    if (status && status[0]) return;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n * m) return;
+    const int64_t i = g / m;
+    Z1[g] = a * dinv[i] * R[g];
+    Z0[g] = 0.0;
+}
+
+template <int W>
+__global__ void __launch_bounds__(kEBS)
+k_pc_cheb_step(int64_t n, int m, const int32_t* __restrict__ ecols, const double* __restrict__ ev,
+               const double* __restrict__ dinv, const double* __restrict__ R, const double* __restrict__ Zk,
+               double* __restrict__ Zp, double a, double w, const int* __restrict__ status) {
+    if (status && status[0]) return;
+    const int64_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t js[W];
+    double ws[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        js[q] = ecols[i * W + q];
+        ws[q] = ev[i * W + q];
+    }
+    const double ad = a * dinv[i];
+    for (int c = 0; c < m; ++c) {
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) acc += ws[q] * Zk[(int64_t)js[q] * m + c];
+        const int64_t o = i * m + c;
+        const double zp = Zp[o];
+        Zp[o] = w * (Zk[o] - zp + ad * (R[o] - acc)) + zp;
+    }
+}
+
+// nb_dir block maxima of the Gershgorin bound sum_j |K_ij| dinv_i of D^-1 K
+template <int LPR>
+__global__ void __launch_bounds__(kBS)
+k_gersh(int64_t n, const int32_t* __restrict__ rowptr, const double* __restrict__ kv,
+        const double* __restrict__ dinv, double* __restrict__ part) {
+    __shared__ double sh[kBS / 64];
+    constexpr int GPB = kBS / LPR;
+    const int lane = threadIdx.x % LPR;
+    double mx = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * GPB + threadIdx.x / LPR; i < n; i += (int64_t)gridDim.x * GPB) {
+        double acc = 0.0;
+        for (int32_t q = rowptr[i] + lane; q < rowptr[i + 1]; q += LPR) acc += fabs(kv[q]);
+#pragma unroll
+        for (int o = LPR / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, LPR);
+        if (lane == 0) mx = fmax(mx, acc * dinv[i]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = 0.0;
+        for (int k = 0; k < kBS / 64; ++k) b = fmax(b, sh[k]);
+        part[blockIdx.x] = b;
+    }
+}
+
 // ---- set-up reductions -------------------------------------------------------
 // partials of sum_i A[i, c] * (B ? B[i, c] : 1) (one per block), and the
 // total from them by a one-block kernel
@@ -671,6 +748,43 @@ struct Launcher {
         }
         LH_CHECK_LAUNCH();
     }
+    // nb_dir block maxima of sum_j |K_ij| dinv_i
+    void gersh(const double* dinv, double* part) {
+        switch (lpr) {
+            case 4: k_gersh<4><<<nb_dir, kBS, 0, ctx->stream>>>(n, rp, kv, dinv, part); break;
+            case 8: k_gersh<8><<<nb_dir, kBS, 0, ctx->stream>>>(n, rp, kv, dinv, part); break;
+            case 16: k_gersh<16><<<nb_dir, kBS, 0, ctx->stream>>>(n, rp, kv, dinv, part); break;
+            default: k_gersh<32><<<nb_dir, kBS, 0, ctx->stream>>>(n, rp, kv, dinv, part); break;
+        }
+        LH_CHECK_LAUNCH();
+    }
+    // Z = p(D^-1 K) D^-1 R (degree deg, on [amin, bmax]) with T as the second
+    // work block; needs the ELL copy
+    void cheb(int deg, double amin, double bmax, const double* dinv, const double* R, double* Z, double* T,
+              const int* status) {
+        const double a = 2.0 / (bmax + amin), mu = (bmax - amin) / (bmax + amin);
+        // the last iterate must land in Z: x_1 goes to Z when deg - 1 is even
+        double* zk = ((deg - 1) % 2 == 0) ? Z : T;
+        double* zp = (zk == Z) ? T : Z;
+        k_pc_cheb_init<<<nblocks(n * m, kBS), kBS, 0, ctx->stream>>>(n, m, dinv, R, a, zk, zp, status);
+        LH_CHECK_LAUNCH();
+        double w = 1.0;
+        for (int k = 1; k < deg; ++k) {
+            w = (k == 1) ? 1.0 / (1.0 - 0.5 * mu * mu) : 1.0 / (1.0 - 0.25 * mu * mu * w);
+        const unsigned nbr = nblocks_xcd(n, kEBS);  // one thread per row
+#define LH_PCS(W_) \
+    k_pc_cheb_step<W_><<<nbr, kEBS, 0, ctx->stream>>>(n, m, ecols, ev, dinv, R, zk, zp, a, w, status)
+            switch (ew) {
+                case 8: LH_PCS(8); break;
+                case 16: LH_PCS(16); break;
+                case 24: LH_PCS(24); break;
+                default: LH_PCS(32); break;
+            }
+#undef LH_PCS
+            LH_CHECK_LAUNCH();
+            std::swap(zk, zp);
+        }
+    }
     // nb_dir block maxima of |(K 1)_i| dinv_i
     void rowsum_rel(const double* dinv, double* part) {
         switch (lpr) {
@@ -703,6 +817,9 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
     const int precond = opts ? opts->precond : LH_PRECOND_JACOBI;
     if (precond != LH_PRECOND_JACOBI && precond != LH_PRECOND_LEARNED && precond != LH_PRECOND_NONE)
         LH_THROW(LH_ERR_ARG, "preconditioner %d unknown", precond);
+    // Chebyshev-Jacobi degree (Jacobi only): 0 or 1 plain Jacobi
+    const int pdeg_opt = opts ? opts->poly_degree : 0;
+    if (pdeg_opt < 0 || pdeg_opt > 256) LH_THROW(LH_ERR_ARG, "poly_degree must be in [0, 256], got %d", pdeg_opt);
     if (precond == LH_PRECOND_LEARNED && !ctx->mpnn)
         LH_THROW(LH_ERR_ARG, "LH_PRECOND_LEARNED needs a network on the context (lh_ctx_set_mpnn)");
     if ((size_t)m * 16 + 64 > ctx->host_small_bytes) LH_THROW(LH_ERR_ARG, "too many right-hand sides (%d)", m);
@@ -844,6 +961,21 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
     }
     if (precond == LH_PRECOND_NONE)
         k_dinv_unit<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, dinv.as<double>());
+    // Chebyshev-Jacobi: degree, and the interval [bmax / (2 deg^2), bmax] with
+    // bmax the Gershgorin bound of D^-1 K (above its largest eigenvalue)
+    const int pdeg = (precond == LH_PRECOND_JACOBI && L.ew && pdeg_opt > 1) ? pdeg_opt : 1;
+    double cheb_amin = 0.0, cheb_bmax = 2.0;
+    if (pdeg > 1) {
+        L.gersh(dinv.as<double>(), s.pq);
+        std::vector<double> hg(L.nb_dir);
+        d2h(ctx, hg.data(), s.pq, (size_t)L.nb_dir * 8);
+        double g = 0.0;
+        for (double v : hg) g = std::max(g, v);
+        cheb_bmax = g > 0.0 ? g : 2.0;
+        cheb_amin = cheb_bmax / (2.0 * pdeg * pdeg);
+    }
+    DevBuf dT;
+    if (pdeg > 1) dT.alloc((size_t)n * m * 8);
     bool learned_on = false;
     LearnedPrecond learned;
     if (precond == LH_PRECOND_LEARNED) {
@@ -906,7 +1038,10 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
     // a first direction at iteration k: z = M^-1 r and the partials of r.z
     // into parity k % 2, P = Q = 0 (so p = z, q = K z whatever beta is)
     auto start_direction = [&](bool jac, int64_t k) {
-        if (jac) {
+        if (jac && pdeg > 1) {
+            L.cheb(pdeg, cheb_amin, cheb_bmax, dinv.as<double>(), dR.as<double>(), dZ.as<double>(), dT.as<double>(),
+                   nullptr);
+        } else if (jac) {
             HIPX(hipMemcpyAsync(dZ.p, dR.p, blk, hipMemcpyDeviceToDevice, ctx->stream));
             rowscale<double>(ctx, n, m, dZ.as<double>(), m, dinv.as<double>());
         } else {
@@ -928,10 +1063,14 @@ int lh_pcg(lh_ctx* ctx, lh_csr* A, lh_csr* B, double alpha, double beta, const d
         for (int kk = 0; kk < chunk; ++kk) {
             const int par = (int)((enq + kk) & 1);
             L.dir(par, singular, dZ.as<double>(), dP.as<double>(), dQ.as<double>(), dR.as<double>(), s);
-            L.update(par, singular, !learned_on, dinv.as<double>(), dP.as<double>(), dQ.as<double>(),
+            L.update(par, singular, !learned_on && pdeg == 1, dinv.as<double>(), dP.as<double>(), dQ.as<double>(),
                      dX.as<double>(), dR.as<double>(), dZ.as<double>(), s);
             if (learned_on) {
                 apply_learned();
+                L.rz(dR.as<double>(), dZ.as<double>(), s.rz[par ^ 1], s.status);
+            } else if (pdeg > 1) {
+                L.cheb(pdeg, cheb_amin, cheb_bmax, dinv.as<double>(), dR.as<double>(), dZ.as<double>(),
+                       dT.as<double>(), s.status);
                 L.rz(dR.as<double>(), dZ.as<double>(), s.rz[par ^ 1], s.status);
             }
             st.spmm_calls++;

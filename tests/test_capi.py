@@ -71,7 +71,7 @@ def test_stats_struct_layout():
     # lh_stats: 4 int32, 5 double, 5 int32 (+ 4 bytes of padding to 8)
     assert ctypes.sizeof(_lib.Stats) == 4 * 4 + 5 * 8 + 5 * 4 + 4
     assert ctypes.sizeof(_lib.EigOpts) == 4 * 4 + 8 + 8 + 4 * 2 + 8 + 4 * 2
-    assert ctypes.sizeof(_lib.PcgOpts) == 4 * 2 + 8 + 4 * 2
+    assert ctypes.sizeof(_lib.PcgOpts) == 4 * 2 + 8 + 4 * 4
 
 
 def test_struct_layout_matches_header(tmp_path):

@@ -157,3 +157,24 @@ def test_tet_geodesics_known_answer_cube():
     gf = diffgeo.compute_geodesic_f(mesh, u)
     assert np.isclose(gf.max(), 0.69931495), gf.max()
     rel_close(gf, odg.geodesic_f(v, t, u), 1e-8)
+
+
+def test_geodesics_l7_against_oracle():
+    """At 163,842 vertices (the locality-ordered PCG of the singular Poisson
+    solve), from the same heat solution (m = 50, so that the heat reaches the
+    whole mesh without underflow), the distances equal the oracle's
+    splu-based geodesic within 1e-8 of the largest distance."""
+    from scipy.sparse.linalg import splu
+
+    print("case: geodesics L7 ellipsoid against the oracle", flush=True)
+    v, t = meshgen.perturbed_sphere(7, np.random.default_rng(17), axes=(1.3, 0.8, 1.1), sigma=0.01)
+    a, b = fem.fem_tria(v, t, lump=True)
+    tt = 50.0 * float(TriaMesh(v, t).avg_edge_length()) ** 2
+    b0 = np.zeros(len(v))
+    b0[0] = 1.0
+    u = splu((b + tt * a).tocsc()).solve(b0)
+    gf = diffgeo.compute_geodesic_f(TriaMesh(v, t), u)
+    ref = odg.geodesic_f(v, t, u)
+    err = rel_close(gf, ref, 1e-8)
+    print(f"  max distance {gf.max():.6f}, relative error {err:.2e}", flush=True)
+    assert gf[0] < 0.01 * gf.max()

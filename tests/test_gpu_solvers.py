@@ -347,3 +347,51 @@ def test_pcg_many_rhs_warm_start_and_singular(square):
     for c in range(3):
         assert abs(xs[:, c].mean()) < 1e-12 * np.abs(xs[:, c]).max()
         assert np.linalg.norm(as_ @ xs[:, c] - pb[:, c]) <= 1e-10 * np.linalg.norm(pb[:, c])
+
+
+@pytest.mark.parametrize("deg", [4, 8, 16])
+def test_pcg_chebyshev_jacobi_preconditioner(deg):
+    """lh_pcg with the Chebyshev-Jacobi polynomial z = p(D^-1 K) D^-1 r as the
+    CG preconditioner: Dirichlet rows with 3 right-hand sides against the
+    oracle's splu, the heat operator (M + t A) and the singular Neumann
+    system (mean-free solution), each to the same tolerance as plain Jacobi,
+    in fewer iterations."""
+    from scipy import sparse
+    from scipy.sparse.linalg import splu
+
+    from lapy_amd import _lib
+
+    print(f"case: Chebyshev-Jacobi PCG degree {deg} (L5 ellipsoid)", flush=True)
+    ctx = _lib.default_context()
+    v, t = meshgen.perturbed_sphere(5, np.random.default_rng(3), axes=(1.3, 0.8, 1.0), sigma=0.02)
+    fs = Solver(TriaMesh(v, t), lump=True)
+    a, b = fs.stiffness, fs.mass
+    A = _lib.DeviceCSR.from_scipy(ctx, a)
+    B = _lib.DeviceCSR.from_scipy(ctx, b)
+    n = a.shape[0]
+    rng = np.random.default_rng(9)
+    rhs = rng.standard_normal((n, 3))
+    dmask = np.zeros(n, np.uint8)
+    dmask[rng.choice(n, 20, replace=False)] = 1
+    free = np.flatnonzero(dmask == 0)
+    x, st = _lib.pcg(ctx, A, None, 0.0, 1.0, rhs, dmask=dmask, rtol=1e-12, poly_degree=deg)
+    _, st_j = _lib.pcg(ctx, A, None, 0.0, 1.0, rhs, dmask=dmask, rtol=1e-12, poly_degree=1)
+    x_o = splu(sparse.csc_matrix(a[free][:, free])).solve(rhs[free])
+    np.testing.assert_allclose(x[free], x_o, rtol=1e-8, atol=1e-10 * np.abs(x_o).max())
+    assert np.all(x[dmask == 1] == 0.0)
+    print("  dirichlet iterations: chebyshev", st.iterations, "jacobi", st_j.iterations, flush=True)
+    assert st.iterations < st_j.iterations
+    # heat operator M + t A (lumped M), one column
+    tt = 0.01
+    h, st_h = _lib.pcg(ctx, A, B, 1.0, tt, rhs[:, 0], rtol=1e-12, poly_degree=deg)
+    k = sparse.csc_matrix(b + tt * a)
+    np.testing.assert_allclose(h, splu(k).solve(rhs[:, 0]), rtol=1e-8, atol=1e-10 * np.abs(h).max())
+    # singular: mean-free solution of A x = b - mean(b)
+    xs, st_s = _lib.pcg(ctx, A, None, 0.0, 1.0, rhs[:, :2], rtol=1e-12, poly_degree=deg)
+    _, st_sj = _lib.pcg(ctx, A, None, 0.0, 1.0, rhs[:, :2], rtol=1e-12, poly_degree=1)
+    pb = rhs[:, :2] - rhs[:, :2].mean(axis=0)
+    for c in range(2):
+        assert abs(xs[:, c].mean()) < 1e-12 * np.abs(xs[:, c]).max()
+        assert np.linalg.norm(a @ xs[:, c] - pb[:, c]) <= 1e-10 * np.linalg.norm(pb[:, c])
+    print("  singular iterations: chebyshev", st_s.iterations, "jacobi", st_sj.iterations, flush=True)
+    assert st_s.iterations < st_sj.iterations
