@@ -136,8 +136,8 @@ int lh_assemble_into(lh_ctx* ctx, lh_mesh* m, int kind, int lump,
                      lh_csr* A, lh_csr* B);
 /* Row order of the triangle assembly (LH_TRIA, LH_TRIA_MASS; no counterpart
  * in the reference, results are identical in every order): the rows are
- * processed in the mesh's vertex order, or in the pattern's breadth-first
- * locality order with the coordinates gathered from a copy in that order
+ * processed in the mesh's vertex order, or in the pattern's locality order
+ * (the vertices' Morton codes sorted, as the solvers use) with the coordinates gathered from a copy in that order
  * (each row still written at its own CSR offset and summed in ascending
  * element order).  LH_ASM_ORDER_AUTO takes the locality order for meshes of
  * at least 65,536 used vertices (fewer than 2^24, rows of at most 64

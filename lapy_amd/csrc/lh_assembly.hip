@@ -192,6 +192,54 @@ __global__ void k_aos(int64_t nv, const VT* __restrict__ x, const VT* __restrict
     xyz[4 * i + 3] = VT(0);
 }
 
+// block minima and maxima of the vertex coordinates (rows < n): part[6 b ..]
+template <typename VT>
+__global__ void __launch_bounds__(256) k_bbox(int64_t n, const VT* __restrict__ xyz, double* __restrict__ part) {
+    __shared__ double sh[6][256];
+    double mn[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, mx[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; ++c) {
+            const double v = (double)xyz[4 * i + c];
+            mn[c] = fmin(mn[c], v);
+            mx[c] = fmax(mx[c], v);
+        }
+    for (int c = 0; c < 3; ++c) {
+        sh[c][threadIdx.x] = mn[c];
+        sh[3 + c][threadIdx.x] = mx[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int c = threadIdx.x;
+        double r = sh[c][0];
+        for (int k = 1; k < 256; ++k) r = c < 3 ? fmin(r, sh[c][k]) : fmax(r, sh[c][k]);
+        part[6 * blockIdx.x + c] = r;
+    }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x30000ffu;
+    x = (x | (x << 8)) & 0x300f00fu;
+    x = (x | (x << 4)) & 0x30c30c3u;
+    x = (x | (x << 2)) & 0x9249249u;
+    return x;
+}
+
+// 30-bit Morton code of each vertex on a 1024^3 grid over the bounding box
+template <typename VT>
+__global__ void k_morton(int64_t n, const VT* __restrict__ xyz, double lx, double ly, double lz, double sx,
+                         double sy, double sz, int32_t* __restrict__ key) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    auto q = [](double v, double lo, double s) {
+        const double t = (v - lo) * s;
+        return (uint32_t)(t < 0.0 ? 0.0 : (t > 1023.0 ? 1023.0 : t));
+    };
+    const uint32_t qx = q((double)xyz[4 * i], lx, sx), qy = q((double)xyz[4 * i + 1], ly, sy),
+                   qz = q((double)xyz[4 * i + 2], lz, sz);
+    key[i] = (int32_t)(spread10(qx) | (spread10(qy) << 1) | (spread10(qz) << 2));
+}
+
 // Row i's columns: the sorted unique vertices of all elements incident to i
 // (emitted by repeated "smallest value above the last one").  FILL=false
 // counts, FILL=true writes cols and the diagonal position.
@@ -617,8 +665,8 @@ k_tria_rows(int64_t n, const int32_t* __restrict__ rowptr, const int32_t* __rest
 // The mesh's own vertex numbering (icosphere subdivision order, scanner
 // order) puts a row's neighbours far apart, so the three vertices each
 // incident element gathers come from HBM again and again.  The scheduled form
-// processes the rows in the pattern's breadth-first order (the order the
-// solvers use), gathers coordinates from a copy of them in that order and
+// processes the rows in the pattern's locality order (the order the solvers
+// use: the vertices' Morton codes sorted), gathers coordinates from a copy of them in that order and
 // reads one 8-byte record per incidence (the two other vertices in that
 // order, the corner and the two slot offsets) instead of the 16-byte
 // vertex-index record.  Each row is still summed in ascending element order
@@ -1143,6 +1191,34 @@ void build_topology(lh_ctx* ctx, lh_mesh* m) {
                                                           p->diagpos.as<int32_t>());
     LH_CHECK_LAUNCH();
     m->full = p;
+    // the pattern's locality order will sort the vertices' Morton codes
+    {
+        const unsigned nbb = std::max(1u, std::min<unsigned>(nblocks(n, 256), 256u));
+        DevBuf bb((size_t)nbb * 6 * 8);
+        if (m->vdtype == LH_F64) k_bbox<double><<<nbb, 256, 0, ctx->stream>>>(n, m->xyz.as<double>(), bb.as<double>());
+        else k_bbox<float><<<nbb, 256, 0, ctx->stream>>>(n, m->xyz.as<float>(), bb.as<double>());
+        LH_CHECK_LAUNCH();
+        std::vector<double> hb((size_t)nbb * 6);
+        d2h(ctx, hb.data(), bb.p, hb.size() * 8);
+        double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+        for (unsigned b = 0; b < nbb; ++b)
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = std::min(lo[c], hb[6 * b + c]);
+                hi[c] = std::max(hi[c], hb[6 * b + 3 + c]);
+            }
+        const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+        if (n > 0 && std::isfinite(ext) && ext > 0.0) {
+            const double sc = 1023.0 / ext;  // one cell size on all axes
+            p->order_key.alloc(n * 4);
+            if (m->vdtype == LH_F64)
+                k_morton<double><<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, m->xyz.as<double>(), lo[0], lo[1], lo[2],
+                                                                           sc, sc, sc, p->order_key.as<int32_t>());
+            else
+                k_morton<float><<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, m->xyz.as<float>(), lo[0], lo[1], lo[2],
+                                                                          sc, sc, sc, p->order_key.as<int32_t>());
+            LH_CHECK_LAUNCH();
+        }
+    }
     if (nodes == 3) {
         k_inc_slots<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, p->rowptr.as<int32_t>(), p->cols.as<int32_t>(),
                                                               m->inc_ptr.as<int32_t>(), m->incv.as<int4>());

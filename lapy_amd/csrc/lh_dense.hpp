@@ -108,7 +108,8 @@ struct LearnedPrecond {
     void apply(float* W, int ldw, int m);
 };
 
-// breadth-first level reordering of a symmetric pattern (lh_reorder.hip)
+// locality reordering of a symmetric pattern: sorted Morton codes of a mesh's
+// vertices, or breadth-first levels (lh_reorder.hip)
 struct Reorder {
     DevBuf perm;   // new row -> old row
     DevBuf iperm;  // old row -> new row

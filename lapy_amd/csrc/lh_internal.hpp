@@ -184,6 +184,10 @@ struct Pattern {
     // (lh::pattern_reorder)
     std::shared_ptr<lh::Reorder> ro;
     std::mutex ro_mu;
+    // optional per-row sort key of the locality order (the Morton code of a
+    // mesh vertex, set with the mesh's pattern); without it the order is a
+    // breadth-first search of the pattern
+    DevBuf order_key;
 };
 
 struct lh_csr {

@@ -39,7 +39,7 @@
 // steps on the ELL copy form z = p(D^-1 K) D^-1 r, and one more launch
 // writes the partials of r.z.  K z is a product on the operator's ELL copy,
 // one thread per row (CSR with LPR lanes per row when rows are long).  Systems of 4,096
-// rows or more are solved in the breadth-first locality order of the pattern
+// rows or more are solved in the locality order of the pattern
 // (built once per pattern and kept with it), and each workgroup walks one
 // contiguous range of rows, with the ranges of one XCD adjacent: the rows
 // gathered by a range then sit in that XCD's L2.  The host checks the done

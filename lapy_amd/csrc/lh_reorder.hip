@@ -1,16 +1,21 @@
-// Bandwidth-reducing symmetric reordering of a CSR matrix (breadth-first
-// level order), computed on the GPU and applied inside the iterative solvers.
+// Locality-improving symmetric reordering of a CSR matrix, computed on the GPU
+// and applied inside the iterative solvers.
 //
 // Mesh vertex numberings are often far from local (1-to-4 subdivision
 // appends every edge midpoint at the end, so most edges join vertices far
 // apart in index).  The SpMM-type kernels gather one row of a block per
-// nonzero; with a level order, the rows a workgroup gathers lie within a few
-// BFS levels of its own, so the gathers hit the XCD's L2 instead of HBM.
+// nonzero; in a local order the rows a workgroup gathers lie close to its
+// own, so the gathers hit the XCD's L2 instead of HBM.  A mesh's pattern
+// carries the Morton codes of its vertices (30 bits on a 1024^3 grid over the
+// bounding box, computed with the pattern): the order sorts them, one radix
+// sort.  A pattern without coordinates (a matrix handed in by the caller)
+// gets breadth-first levels instead, one launch per level.
 //
-// Deterministic: BFS levels are graph distances, and the permutation is a
-// stable sort of (level, original index) with an LSD radix sort whose ranks
-// are computed per workgroup without atomics.  Results are mapped back to the
-// caller's numbering, so the permutation is invisible outside the solver.
+// Deterministic: the permutation is a stable sort of (key, original index),
+// key = Morton code or BFS level (a graph distance), with an LSD radix sort
+// whose ranks are computed per workgroup without atomics.  Results are mapped
+// back to the caller's numbering, so the permutation is invisible outside the
+// solver.
 #include "lh_dense.hpp"
 
 namespace lh {
@@ -131,25 +136,16 @@ __global__ void k_gather_rows(const T* __restrict__ v, const int32_t* __restrict
 
 }  // namespace
 
-std::shared_ptr<Reorder> pattern_reorder(lh_ctx* ctx, Pattern& p) {
-    std::lock_guard<std::mutex> lk(p.ro_mu);
-    if (!p.ro) {
-        ArenaOff off;  // kept with the pattern, beyond this call
-        auto r = std::make_shared<Reorder>();
-        r->build(ctx, p);
-        p.ro = r;
-    }
-    return p.ro;
-}
-
-void Reorder::build(lh_ctx* ctx, const Pattern& p) {
+// breadth-first levels of the pattern's graph into level (n entries): from
+// vertex 0, then again from the first vertex of the deepest level (a
+// pseudo-peripheral start gives narrower levels); unreached vertices get the
+// level after the deepest.  Returns the number of levels.
+static int32_t bfs_levels(lh_ctx* ctx, const Pattern& p, DevBuf& level) {
     const int64_t n = p.n;
     const int32_t* rp = p.rowptr.as<int32_t>();
     const int32_t* cl = p.cols.as<int32_t>();
-    DevBuf level(n * 4), changed(64);
+    DevBuf changed(64);
     int32_t last = 0;
-    // BFS from vertex 0, then again from a vertex of the last level (a
-    // pseudo-peripheral start gives narrower levels)
     int32_t start = 0;
     for (int round = 0; round < 2; ++round) {
         k_fill_i32<<<nblocks(n, 256), 256, 0, ctx->stream>>>(level.as<int32_t>(), n, -1);
@@ -183,13 +179,40 @@ void Reorder::build(lh_ctx* ctx, const Pattern& p) {
     }
     k_unreached<<<nblocks(n, 256), 256, 0, ctx->stream>>>(n, level.as<int32_t>(), last + 1);
     LH_CHECK_LAUNCH();
-    nlevels = last + 2;
-    // stable LSD radix sort of (level, index) by level, three 8-bit digits
+    return last + 2;
+}
+
+std::shared_ptr<Reorder> pattern_reorder(lh_ctx* ctx, Pattern& p) {
+    std::lock_guard<std::mutex> lk(p.ro_mu);
+    if (!p.ro) {
+        ArenaOff off;  // kept with the pattern, beyond this call
+        auto r = std::make_shared<Reorder>();
+        r->build(ctx, p);
+        p.ro = r;
+    }
+    return p.ro;
+}
+
+void Reorder::build(lh_ctx* ctx, const Pattern& p) {
+    const int64_t n = p.n;
+    const int32_t* rp = p.rowptr.as<int32_t>();
+    const int32_t* cl = p.cols.as<int32_t>();
+    DevBuf level(n * 4);
+    int key_bits = 24;
+    if (p.order_key.p) {
+        // a mesh's Morton codes: sort by them (30 bits) instead of a search
+        HIPX(hipMemcpyAsync(level.p, p.order_key.p, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+        key_bits = 32;
+        nlevels = 0;
+    } else {
+        nlevels = bfs_levels(ctx, p, level);
+    }
+    // stable LSD radix sort of (key, index) by key, 8-bit digits
     const int nb = (int)((n + RBS - 1) / RBS);
     DevBuf k2(n * 4), v1(n * 4), v2(n * 4), hist((size_t)256 * nb * 4), off(((size_t)256 * nb + 1) * 4);
     k_iota<<<nblocks(n, 256), 256, 0, ctx->stream>>>(v1.as<int32_t>(), n);
     int32_t *kin = level.as<int32_t>(), *kout = k2.as<int32_t>(), *vin = v1.as<int32_t>(), *vout = v2.as<int32_t>();
-    for (int shift = 0; shift < 24; shift += 8) {
+    for (int shift = 0; shift < key_bits; shift += 8) {
         k_digit_hist<<<nb, RBS, 0, ctx->stream>>>(kin, n, shift, hist.as<int32_t>(), nb);
         LH_CHECK_LAUNCH();
         exclusive_scan_i32(ctx, hist.as<int32_t>(), off.as<int32_t>(), (int64_t)256 * nb);

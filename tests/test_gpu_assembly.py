@@ -265,7 +265,7 @@ def _order_cases():
 @pytest.mark.parametrize("case", ["ellipsoid_l5", "square_open_f32", "unused_vertices", "degenerate_clamp",
                                   "clique_global_path"])
 def test_locality_order_is_bit_identical(case):
-    """The locality-ordered triangle assembly (rows in the pattern's BFS
+    """The locality-ordered triangle assembly (rows in the pattern's locality
     order, coordinates from a copy in that order, 8-byte incidence records)
     gives the mesh-order kernel's values bit for bit: every row is summed in
     ascending element order and written at its own CSR offset."""
