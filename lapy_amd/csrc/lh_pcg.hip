@@ -126,6 +126,61 @@ __device__ __forceinline__ double allsum_partials(const double* __restrict__ par
     return block_allsum(t, sh);
 }
 
+// two block sums with one pair of barriers (each formed as block_allsum forms
+// it); sh holds 2 * blockDim / 64 doubles
+__device__ __forceinline__ void block_allsum2(double a, double b, double* sh, double& ra, double& rb) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_down(a, o, 64);
+        b += __shfl_down(b, o, 64);
+    }
+    const int nw = (int)(blockDim.x >> 6);
+    __syncthreads();  // sh free (an earlier call's readers are done)
+    if ((threadIdx.x & 63) == 0) {
+        sh[threadIdx.x >> 6] = a;
+        sh[nw + (threadIdx.x >> 6)] = b;
+    }
+    __syncthreads();
+    double ta = 0.0, tb = 0.0;
+    for (int k = 0; k < nw; ++k) {
+        ta += sh[k];
+        tb += sh[nw + k];
+    }
+    ra = ta;
+    rb = tb;
+}
+
+// K such sums at once: every partial load is issued before the first
+// reduction waits, and one pair of barriers serves all K (each sum is formed
+// in the order allsum_partials uses, so the results are the same).  sh holds
+// K * blockDim / 64 doubles.
+template <int K>
+__device__ __forceinline__ void allsum_partials_k(const double* const (&part)[K], const int (&np)[K], int m, int c,
+                                                  double* sh, double (&out)[K]) {
+    double t[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        t[q] = 0.0;
+        for (int b = threadIdx.x; b < np[q]; b += blockDim.x) t[q] += part[q][(int64_t)b * m + c];
+    }
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) t[q] += __shfl_down(t[q], o, 64);
+    const int nw = (int)(blockDim.x >> 6);
+    __syncthreads();  // sh free (an earlier call's readers are done)
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < K; ++q) sh[q * nw + (threadIdx.x >> 6)] = t[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        double r = 0.0;
+        for (int k = 0; k < nw; ++k) r += sh[q * nw + k];
+        out[q] = r;
+    }
+}
+
 // CG state on the device (m columns).  Iteration k reads r_k.z_k and r_k.r_k
 // from the partial buffers of parity k % 2 and the update writes those of
 // r_{k+1} into parity (k + 1) % 2, so a kernel never reads partials that
@@ -150,7 +205,7 @@ k_cg_dir(int64_t n, int m, int c0, int mc, int par, const int32_t* __restrict__ 
          const int32_t* __restrict__ cols, const double* __restrict__ kv, const double* __restrict__ Z,
          double* __restrict__ P, double* __restrict__ Q, const double* __restrict__ R, CgState s) {
     if (s.status[0]) return;  // converged: the enqueued iterations end here
-    __shared__ double sh[kIBS / 64];
+    __shared__ double sh[4 * kIBS / 64];
     constexpr int GPB = kIBS / LPR;
     const int lane = threadIdx.x % LPR, grp = threadIdx.x / LPR;
     const int64_t rpb = (n + gridDim.x - 1) / gridDim.x;
@@ -163,9 +218,11 @@ k_cg_dir(int64_t n, int m, int c0, int mc, int par, const int32_t* __restrict__ 
         if (c < mc) {
             // beta_k = rz_k / rz_{k-1}; 0 for a converged column (and at k = 0,
             // where P and Q hold zeros anyway)
-            const double rzk = allsum_partials(s.rz[par], s.nb_upd, m, c0 + c, sh);
-            const double rzo = allsum_partials(s.rz[par ^ 1], s.nb_upd, m, c0 + c, sh);
-            const double rrk = allsum_partials(s.rr[par], s.nb_upd, m, c0 + c, sh);
+            const double* const pp[3] = {s.rz[par], s.rz[par ^ 1], s.rr[par]};
+            const int np[3] = {s.nb_upd, s.nb_upd, s.nb_upd};
+            double v[3];
+            allsum_partials_k<3>(pp, np, m, c0 + c, sh, v);
+            const double rzk = v[0], rzo = v[1], rrk = v[2];
             bet[c] = (rrk > s.tol2[c0 + c] && rzo != 0.0) ? rzk / rzo : 0.0;
         }
     }
@@ -203,11 +260,11 @@ k_cg_dir(int64_t n, int m, int c0, int mc, int par, const int32_t* __restrict__ 
 #pragma unroll
     for (int c = 0; c < MC; ++c) {
         if (c >= mc) continue;
-        const double bp = block_allsum(pq[c], sh);
-        if (threadIdx.x == 0) s.pq[(int64_t)blockIdx.x * m + c0 + c] = bp;
-        if (SING) {
-            const double bs = block_allsum(sr[c], sh);
-            if (threadIdx.x == 0) s.sr[(int64_t)blockIdx.x * m + c0 + c] = bs;
+        double bp, bs;
+        block_allsum2(pq[c], SING ? sr[c] : 0.0, sh, bp, bs);
+        if (threadIdx.x == 0) {
+            s.pq[(int64_t)blockIdx.x * m + c0 + c] = bp;
+            if (SING) s.sr[(int64_t)blockIdx.x * m + c0 + c] = bs;
         }
     }
 }
@@ -226,7 +283,7 @@ k_cg_dir_ell(int64_t n, int m, int c0, int mc, int par, const int32_t* __restric
              const double* __restrict__ ev, const double* __restrict__ Z, double* __restrict__ P,
              double* __restrict__ Q, const double* __restrict__ R, CgState s) {
     if (s.status[0]) return;
-    __shared__ double sh[kEBS / 64];
+    __shared__ double sh[4 * kEBS / 64];
     const int64_t rpb = (n + gridDim.x - 1) / gridDim.x;
     const int64_t rb = xcd_block() * rpb, re = rb + rpb < n ? rb + rpb : n;
     double bet[MC], pq[MC], sr[MC];
@@ -235,9 +292,11 @@ k_cg_dir_ell(int64_t n, int m, int c0, int mc, int par, const int32_t* __restric
         bet[c] = 0.0;
         pq[c] = sr[c] = 0.0;
         if (c < mc) {
-            const double rzk = allsum_partials(s.rz[par], s.nb_upd, m, c0 + c, sh);
-            const double rzo = allsum_partials(s.rz[par ^ 1], s.nb_upd, m, c0 + c, sh);
-            const double rrk = allsum_partials(s.rr[par], s.nb_upd, m, c0 + c, sh);
+            const double* const pp[3] = {s.rz[par], s.rz[par ^ 1], s.rr[par]};
+            const int np[3] = {s.nb_upd, s.nb_upd, s.nb_upd};
+            double v[3];
+            allsum_partials_k<3>(pp, np, m, c0 + c, sh, v);
+            const double rzk = v[0], rzo = v[1], rrk = v[2];
             bet[c] = (rrk > s.tol2[c0 + c] && rzo != 0.0) ? rzk / rzo : 0.0;
         }
     }
@@ -306,11 +365,11 @@ k_cg_dir_ell(int64_t n, int m, int c0, int mc, int par, const int32_t* __restric
 #pragma unroll
     for (int c = 0; c < MC; ++c) {
         if (c >= mc) continue;
-        const double bp = block_allsum(pq[c], sh);
-        if (threadIdx.x == 0) s.pq[(int64_t)blockIdx.x * m + c0 + c] = bp;
-        if (SING) {
-            const double bs = block_allsum(sr[c], sh);
-            if (threadIdx.x == 0) s.sr[(int64_t)blockIdx.x * m + c0 + c] = bs;
+        double bp, bs;
+        block_allsum2(pq[c], SING ? sr[c] : 0.0, sh, bp, bs);
+        if (threadIdx.x == 0) {
+            s.pq[(int64_t)blockIdx.x * m + c0 + c] = bp;
+            if (SING) s.sr[(int64_t)blockIdx.x * m + c0 + c] = bs;
         }
     }
 }
@@ -323,7 +382,7 @@ k_cg_update(int64_t n, int m, int c0, int mc, int par, const double* __restrict_
             const double* __restrict__ Q, double* __restrict__ X, double* __restrict__ R, double* __restrict__ Z,
             CgState s, int last_chunk) {
     if (s.status[0]) return;
-    __shared__ double sh[kIBS / 64];
+    __shared__ double sh[4 * kIBS / 64];
     // contiguous rows per block, an even count; a thread takes two adjacent
     // rows per step (one 16-byte load per operand for one column)
     int64_t rpb = (n + gridDim.x - 1) / gridDim.x;
@@ -336,13 +395,15 @@ k_cg_update(int64_t n, int m, int c0, int mc, int par, const double* __restrict_
         al[c] = mu[c] = rz[c] = rr[c] = 0.0;
         if (c < mc) {
             const int cc = c0 + c;
-            const double rzk = allsum_partials(s.rz[par], s.nb_upd, m, cc, sh);
-            const double rrk = allsum_partials(s.rr[par], s.nb_upd, m, cc, sh);
-            const double pqk = allsum_partials(s.pq, s.nb_dir, m, cc, sh);
+            const double* const pp[4] = {s.rz[par], s.rr[par], s.pq, s.sr};
+            const int np[4] = {s.nb_upd, s.nb_upd, s.nb_dir, SING ? s.nb_dir : 0};
+            double v[4];
+            allsum_partials_k<4>(pp, np, m, cc, sh, v);
+            const double rzk = v[0], rrk = v[1], pqk = v[2];
             const bool act = rrk > s.tol2[cc];
             nact += act ? 1 : 0;
             al[c] = (act && pqk != 0.0) ? rzk / pqk : 0.0;
-            if (SING && act) mu[c] = allsum_partials(s.sr, s.nb_dir, m, cc, sh) / (double)n;
+            if (SING && act) mu[c] = v[3] / (double)n;
             if (blockIdx.x == 0 && threadIdx.x == 0) s.rr_out[cc] = rrk;
         }
     }
@@ -421,11 +482,11 @@ k_cg_update(int64_t n, int m, int c0, int mc, int par, const double* __restrict_
 #pragma unroll
     for (int c = 0; c < MC; ++c) {
         if (c >= mc) continue;
-        const double b1 = block_allsum(rr[c], sh);
-        if (threadIdx.x == 0) s.rr[par ^ 1][(int64_t)blockIdx.x * m + c0 + c] = b1;
-        if (JAC) {
-            const double b2 = block_allsum(rz[c], sh);
-            if (threadIdx.x == 0) s.rz[par ^ 1][(int64_t)blockIdx.x * m + c0 + c] = b2;
+        double b1, b2;
+        block_allsum2(rr[c], JAC ? rz[c] : 0.0, sh, b1, b2);
+        if (threadIdx.x == 0) {
+            s.rr[par ^ 1][(int64_t)blockIdx.x * m + c0 + c] = b1;
+            if (JAC) s.rz[par ^ 1][(int64_t)blockIdx.x * m + c0 + c] = b2;
         }
     }
     // status for the host (and the early exit of later launches): every block
