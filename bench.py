@@ -62,6 +62,93 @@ def _oracle_eigs_seconds(v, t, k, threads):
     return t1 - t0, t2 - t1
 
 
+def _oracle_mesh_seconds(v, t, k):
+    """Oracle pipeline on one triangle mesh with one BLAS thread (a pool
+    worker of the batch baseline; module level so that spawn can pickle it)."""
+    a, e = _oracle_eigs_seconds(v, t, k, 1)
+    return a + e
+
+
+def _host_info():
+    return f"CPU: {_cpu_model()}, os.cpu_count()={os.cpu_count()}, sched_getaffinity={len(os.sched_getaffinity(0))}"
+
+
+def cpu_baseline_small(v, t, k, runs=5):
+    """The oracle on the small sphere itself: median of `runs` runs, one thread."""
+    secs = sorted(sum(_oracle_eigs_seconds(v, t, k, 1)) for _ in range(runs))
+    med = secs[len(secs) // 2]
+    return {"value": round(k / med, 4), "unit": "eigenpairs/s", "cores": 1, "kind": "port",
+            "sample": f"oracle pipeline (numpy assembly + splu + eigsh shift-invert) on the same L5 sphere "
+                      f"({len(v)} vertices), k={k}, median of {runs} runs ({med:.3f} s), BLAS threads=1. "
+                      + _host_info()}
+
+
+def cpu_baseline_tet(k, cells=24):
+    """The oracle on a bounded tetrahedral cube of `cells`^3 cells (the same
+    construction as the 64^3 workload): assembly, splu + eigsh, and the heat
+    solve timed separately, one thread.  SuperLU's fill on a 3-D Laplacian
+    grows much faster than the vertex count, so the full 64^3 cube is not a
+    bounded sample."""
+    import threadpoolctl
+
+    from lapy_amd import meshgen
+    from oracle import fem
+
+    v, t = meshgen.volume_cube(cells=cells)
+    with threadpoolctl.threadpool_limits(1):
+        t0 = time.perf_counter()
+        a, b = fem.fem_tetra(v, t)
+        t1 = time.perf_counter()
+        fem.eigs(a, b, k=k)
+        t2 = time.perf_counter()
+        fem.diffusion(v, t, 0, m=1e-3 / fem.avg_edge_length(v, t) ** 2)
+        t3 = time.perf_counter()
+    return {"value": round(k / (t2 - t0), 4), "unit": "eigenpairs/s", "cores": 1, "kind": "port",
+            "sample": f"oracle pipeline on the {cells}^3-cell tetrahedral cube ({len(v)} vertices, {len(t)} tets), "
+                      f"k={k}, one run, BLAS threads=1: assembly {t1 - t0:.2f} s, splu + eigsh {t2 - t1:.2f} s, "
+                      f"heat solve (splu of M + tA) {t3 - t2:.2f} s. " + _host_info(),
+            "heat_seconds": round(t3 - t2, 4)}
+
+
+def _stratified(meshes, stride):
+    """Every `stride`-th mesh of each level (vertex count) of the batch, so
+    that the sample's level mix is the batch's."""
+    by_level = {}
+    for v, t in meshes:
+        by_level.setdefault(len(v), []).append((v, t))
+    return [m for nv in sorted(by_level) for m in by_level[nv][::stride]]
+
+
+def cpu_baseline_batch(meshes, k, seq_stride=16, pool_stride=4, max_procs=16):
+    """The oracle on level-stratified samples of the batch: the reference's
+    sequential loop (one thread) over every `seq_stride`-th mesh of each
+    level, and a spawn pool of one-thread workers over the host's cores (at
+    most `max_procs`) on every `pool_stride`-th mesh of each level, enough
+    meshes per worker to balance the sizes."""
+    import multiprocessing as mp
+
+    seq_s = _stratified(meshes, seq_stride)
+    t0 = time.perf_counter()
+    for v, t in seq_s:
+        _oracle_mesh_seconds(v, t, k)
+    seq = time.perf_counter() - t0
+    pool_s = _stratified(meshes, pool_stride)
+    procs = max(1, min(max_procs, len(os.sched_getaffinity(0))))
+    with mp.get_context("spawn").Pool(procs) as pool:
+        pool.starmap(_oracle_mesh_seconds, [(seq_s[0][0], seq_s[0][1], 2)] * procs)  # workers import scipy
+        t0 = time.perf_counter()
+        pool.starmap(_oracle_mesh_seconds, [(v, t, k) for v, t in pool_s[::-1]], chunksize=1)  # largest first
+        par = time.perf_counter() - t0
+    avg = sum(len(v) for v, _ in meshes) / len(meshes)
+    return {"value": round(k * len(seq_s) / seq, 4), "unit": "eigenpairs/s", "cores": 1, "kind": "port",
+            "sample": f"oracle pipeline (as lapy shapedna.compute_shapedna per mesh), k={k}, on level-stratified "
+                      f"samples of the batch ({avg:.0f} vertices per mesh on average): sequential loop over "
+                      f"{len(seq_s)} meshes ({sum(len(v) for v, _ in seq_s) / len(seq_s):.0f} vertices on average) "
+                      f"{seq:.2f} s; pool of {procs} one-thread workers over {len(pool_s)} meshes "
+                      f"({sum(len(v) for v, _ in pool_s) / len(pool_s):.0f} on average) {par:.2f} s. " + _host_info(),
+            "pool": {"value": round(k * len(pool_s) / par, 4), "processes": procs}}
+
+
 def cpu_baseline(level: int, k: int, thread_level: int):
     """The oracle timed on this host's cores, on the metric's own mesh
     (icosphere level `level`, one run with BLAS threads = 1: SuperLU, ARPACK
@@ -194,6 +281,8 @@ def run_workload(args, ctx, _lib, meshgen):
                 ref = np.asarray(gold["heat_probe_values"])
                 herr = float(np.max(np.abs(u[idx] - ref)) / np.max(np.abs(ref)))
             out["config"].update({"heat_seconds": round(th, 4), "heat_rtol": 1e-10, "heat_probe_rel_err": herr})
+        if not args.no_cpu_baseline:
+            out["cpu_baseline"] = cpu_baseline_small(v, t, k) if args.workload == "small" else cpu_baseline_tet(k)
     elif args.workload == "geodesic":
         from lapy_amd import TriaMesh, diffgeo
 
@@ -286,6 +375,8 @@ def run_workload(args, ctx, _lib, meshgen):
                                "fallbacks": int(sum(r[2].fallback_used for r in res)),
                                "autotune_seconds": secs, "lobpcg_iterations_min_max": [min(its), max(its)],
                                "eig_rel_err_vs_oracle": err}})
+        if not args.no_cpu_baseline:
+            out["cpu_baseline"] = cpu_baseline_batch(meshes, k)
     elif args.workload == "learned":
         run_learned(args, ctx, _lib, meshgen, out)
         return
