@@ -76,15 +76,46 @@ void exclusive_scan_i32(lh_ctx* ctx, const int32_t* in, int32_t* out, int64_t n)
     LH_CHECK_LAUNCH();
 }
 
+// Small transfers go through the context's pinned staging buffer: a copy
+// between device and pinned memory is one DMA (or kernel) on the stream,
+// where pageable memory takes the runtime's shared staging path, which
+// serialises the concurrent solver streams of the batched eigensolver.
+void d2h_n(lh_ctx* ctx, const Xfer* x, int cnt) {
+    size_t total = 0;
+    for (int i = 0; i < cnt; ++i) total += (x[i].bytes + 63) & ~size_t(63);
+    if (!ctx->host_stage || total > ctx->host_stage_bytes) {
+        for (int i = 0; i < cnt; ++i)
+            if (x[i].bytes) HIPX(hipMemcpyAsync(x[i].host, x[i].dev, x[i].bytes, hipMemcpyDeviceToHost, ctx->stream));
+        HIPX(hipStreamSynchronize(ctx->stream));
+        return;
+    }
+    char* stage = static_cast<char*>(ctx->host_stage);
+    size_t off = 0;
+    for (int i = 0; i < cnt; ++i) {
+        if (x[i].bytes) HIPX(hipMemcpyAsync(stage + off, x[i].dev, x[i].bytes, hipMemcpyDeviceToHost, ctx->stream));
+        off += (x[i].bytes + 63) & ~size_t(63);
+    }
+    HIPX(hipStreamSynchronize(ctx->stream));
+    off = 0;
+    for (int i = 0; i < cnt; ++i) {
+        if (x[i].bytes) std::memcpy(x[i].host, stage + off, x[i].bytes);
+        off += (x[i].bytes + 63) & ~size_t(63);
+    }
+}
 void d2h(lh_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (!bytes) return;
-    HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    HIPX(hipStreamSynchronize(ctx->stream));
+    const Xfer x{dst, src, bytes};
+    d2h_n(ctx, &x, 1);
 }
 void h2d(lh_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (!bytes) return;
-    HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-    HIPX(hipStreamSynchronize(ctx->stream));
+    if (ctx->host_stage && bytes <= ctx->host_stage_bytes) {
+        std::memcpy(ctx->host_stage, src, bytes);
+        HIPX(hipMemcpyAsync(dst, ctx->host_stage, bytes, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIPX(hipStreamSynchronize(ctx->stream));  // the source may be reused on return
 }
 void memset_d(lh_ctx* ctx, void* dst, int v, size_t bytes) {
     if (!bytes) return;
@@ -192,8 +223,12 @@ int lh_ctx_create(int device, lh_ctx** out) {
         noalloc_scratch_init(device);
         ctx->host_small_bytes = 1 << 20;
         HIPX(hipHostMalloc(&ctx->host_small, ctx->host_small_bytes, hipHostMallocDefault));
+        ctx->host_stage_bytes = 256 << 10;
+        HIPX(hipHostMalloc(&ctx->host_stage, ctx->host_stage_bytes, hipHostMallocDefault));
     } catch (const lh_error& e) {
         fprintf(stderr, "lh_ctx_create: %s\n", e.msg.c_str());
+        if (ctx->host_small) (void)hipHostFree(ctx->host_small);
+        if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return e.code;
@@ -212,6 +247,7 @@ void lh_ctx_destroy(lh_ctx* ctx) {
     ctx->scratch.release();
     ctx->small.release();
     if (ctx->host_small) (void)hipHostFree(ctx->host_small);
+    if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
     release_blas(ctx);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

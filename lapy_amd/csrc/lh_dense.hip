@@ -425,19 +425,32 @@ void rr_solve(lh_ctx* ctx, int N, int nev, double* GA, double* GM, double* C, do
 }
 
 // T with T^T G T = I for a small SPD Gram G (m x m, m <= 128): G is
-// equilibrated to unit diagonal (D G D), optionally shifted, factorised
-// G_s = L L^T in LDS, and T = D L^-T.  status 1 on a non-positive pivot.
-__global__ void __launch_bounds__(RB) k_cholinv(int m, const double* __restrict__ G, double* __restrict__ T,
-                                                double shift, int* __restrict__ status) {
-    // row stride m + 1 (odd for even m): a column's entries fall on
-    // different LDS banks
-    __shared__ double L[kCholMax * (kCholMax + 1)];
-    __shared__ double dsc[kCholMax];
+// equilibrated to unit diagonal (D G D), optionally shifted, and T = D U
+// with U upper triangular and U^T G_s U = I, the inverse of G_s's Cholesky
+// factor transposed.  U comes from symmetric elimination on [G_s | I]:
+// step j takes the pivot p = S_jj of the Schur complement S (the square of
+// the Cholesky factor's diagonal), subtracts r r^T / p from S (r = row j of
+// S, zero at indices <= j) and U[:, j] r^T / p from U, and scales column j
+// of U by 1 / sqrt(p) at the end (Gram-Schmidt of the unit vectors in the
+// G_s inner product).  S and U live in registers: 512 threads form 16
+// half-waves, thread (hw, ln) holding entries [hw + 16a][ln + 32b]; row j of
+// S and column j of U are published to LDS, double-buffered by the parity of
+// j, so one barrier per step suffices.  status 1 on a non-positive diagonal
+// or a pivot <= 1e-12 (reported in ((double*)status)[1]); otherwise the
+// smallest pivot is reported there (below 1e-2 the caller repeats the pass).
+constexpr int CHOL_NT = 512;
+
+template <int CB>
+__global__ void __launch_bounds__(CHOL_NT) k_cholinv(int m, const double* __restrict__ G, double* __restrict__ T,
+                                                     double shift, int* __restrict__ status) {
+    constexpr int RA = 2 * CB;
+    __shared__ double dsc[kCholMax], piv[kCholMax];
+    __shared__ double rbuf[2][32 * CB], ubuf[2][32 * CB];
     __shared__ int flag[1];
-    const int tid = threadIdx.x;
-    const int lw = (m % 2 == 0) ? m + 1 : m;
+    const int tid = threadIdx.x, hw = tid >> 5, ln = tid & 31;
     if (tid == 0) flag[0] = 0;
-    for (int i = tid; i < m; i += RB) {
+    __syncthreads();
+    for (int i = tid; i < m; i += CHOL_NT) {
         const double d = G[i * m + i];
         dsc[i] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
         if (!(d > 0.0)) flag[0] = 1;
@@ -447,68 +460,116 @@ __global__ void __launch_bounds__(RB) k_cholinv(int m, const double* __restrict_
         if (tid == 0) status[0] = 1;
         return;
     }
-    for (int idx = tid; idx < m * m; idx += RB) {
-        const int i = idx / m, j = idx - (idx / m) * m;
-        double v = 0.5 * (G[i * m + j] + G[j * m + i]) * dsc[i] * dsc[j];
-        if (i == j) v = 1.0 + shift;
-        L[i * lw + j] = v;
+    // S and U padded to 32 CB with the identity: the padding has zero
+    // coupling, is never a pivot and stays unchanged, so no bounds tests
+    // are needed inside the elimination
+    double S[RA][CB], U[RA][CB];
+#pragma unroll
+    for (int a = 0; a < RA; ++a) {
+        const int i = hw + 16 * a;
+#pragma unroll
+        for (int b = 0; b < CB; ++b) {
+            const int c = ln + 32 * b;
+            double v = (i == c) ? 1.0 : 0.0;
+            if (i < m && c < m && i != c) v = 0.5 * (G[i * m + c] + G[c * m + i]) * dsc[i] * dsc[c];
+            if (i == c && i < m) v = 1.0 + shift;
+            S[a][b] = v;
+            U[a][b] = (i == c) ? 1.0 : 0.0;
+        }
     }
-    __syncthreads();
-    // right-looking Cholesky; lanes own rows of the trailing matrix.  The
-    // smallest pivot (relative to the unit diagonal) is reported in
-    // ((double*)status)[1]; below 1e-12 the block counts as rank deficient.
-    // Column j is read unscaled during its step and divided by its pivot
-    // root after the loop (step j writes only columns > j): one barrier per
-    // step.
-    __shared__ double ldg[kCholMax];  // the pivot roots, L's diagonal
-    double minpiv = 1.0;
     for (int j = 0; j < m; ++j) {
-        const double djj = L[j * lw + j];
-        minpiv = fmin(minpiv, djj);
-        if (!(djj > 1e-12)) {
-            if (tid == 0) { status[0] = 1; reinterpret_cast<double*>(status)[1] = djj; }
+        const int par = j & 1;
+        // publish row j of S (the pivot at j, zero before it) and column j of U
+        if (hw == (j & 15)) {
+            const int aj = j >> 4;
+#pragma unroll
+            for (int a = 0; a < RA; ++a)
+                if (a == aj) {
+#pragma unroll
+                    for (int b = 0; b < CB; ++b) {
+                        const int c = ln + 32 * b;
+                        rbuf[par][c] = (c >= j) ? S[a][b] : 0.0;
+                    }
+                }
+        }
+        if (ln == (j & 31)) {
+            const int bj = j >> 5;
+#pragma unroll
+            for (int b = 0; b < CB; ++b)
+                if (b == bj) {
+#pragma unroll
+                    for (int a = 0; a < RA; ++a) ubuf[par][hw + 16 * a] = U[a][b];  // zero below row j
+                }
+        }
+        __syncthreads();
+        const double p = rbuf[par][j];
+        if (!(p > 1e-12)) {  // uniform
+            if (tid == 0) {
+                status[0] = 1;
+                reinterpret_cast<double*>(status)[1] = p;
+            }
             return;
         }
-        const double ljj = sqrt(djj);
-        if (tid == 0) ldg[j] = ljj;
-        for (int i = j + 1 + (tid >> 4); i < m; i += RB / 16) {
-            const double lij = L[i * lw + j] / ljj;
-            for (int kk = j + 1 + (tid & 15); kk <= i; kk += 16) L[i * lw + kk] -= lij * (L[kk * lw + j] / ljj);
+        if (tid == 0) piv[j] = p;
+        const double ip = 1.0 / p;
+        double fc[CB];
+#pragma unroll
+        for (int b = 0; b < CB; ++b) {
+            const int c = ln + 32 * b;
+            fc[b] = (c > j) ? rbuf[par][c] * ip : 0.0;
         }
-        __syncthreads();
-    }
-    for (int idx = tid; idx < m * m; idx += RB) {
-        const int i = idx / m, j = idx - (idx / m) * m;
-        if (i > j) L[i * lw + j] /= ldg[j];
-        else if (i == j) L[i * lw + i] = ldg[i];
+        // S -= r fc^T (rows <= j: r is zero there, or row j, which is
+        // finished); U -= U[:, j] fc^T; columns <= j have fc = 0
+        const int b_lo = (j >= 31) ? (j - 31) / 32 + 1 : 0;
+#pragma unroll
+        for (int a = 0; a < RA; ++a) {
+            const double ri = rbuf[par][hw + 16 * a];
+            const double ui = ubuf[par][hw + 16 * a];
+#pragma unroll
+            for (int b = 0; b < CB; ++b) {
+                if (b < b_lo) continue;
+                S[a][b] -= ri * fc[b];
+                U[a][b] -= ui * fc[b];
+            }
+        }
     }
     __syncthreads();
-    // Tu = L^-T (upper triangular) by back substitution over rows i, all
-    // columns c >= i in parallel, the dot split over 8 lanes per column.  Tu
-    // is kept in the (unused) upper triangle of L, L's diagonal in ldg.
-    for (int i = m - 1; i >= 0; --i) {
-        const int c = i + (tid >> 3), l8 = tid & 7;
-        double s = 0.0;
-        if (c < m)
-            for (int r = i + 1 + l8; r <= c; r += 8) s += L[r * lw + i] * L[r * lw + c];
-        s += __shfl_xor(s, 1, 64);
-        s += __shfl_xor(s, 2, 64);
-        s += __shfl_xor(s, 4, 64);
-        __syncthreads();  // all reads of row i's inputs done before row i is written
-        if (c < m && l8 == 0) L[i * lw + c] = ((c == i ? 1.0 : 0.0) - s) / ldg[i];
-        __syncthreads();
+    double isq[CB];
+#pragma unroll
+    for (int b = 0; b < CB; ++b) {
+        const int c = ln + 32 * b;
+        isq[b] = (c < m) ? 1.0 / sqrt(piv[c]) : 0.0;
     }
-    // T = D Tu
-    for (int idx = tid; idx < m * m; idx += RB) {
-        const int i = idx / m, c = idx - (idx / m) * m;
-        T[idx] = (c >= i) ? dsc[i] * L[i * lw + c] : 0.0;
+#pragma unroll
+    for (int a = 0; a < RA; ++a) {
+        const int i = hw + 16 * a;
+        if (i >= m) continue;
+        const double di = dsc[i];
+#pragma unroll
+        for (int b = 0; b < CB; ++b) {
+            const int c = ln + 32 * b;
+            if (c < m) T[i * m + c] = (c >= i) ? di * U[a][b] * isq[b] : 0.0;
+        }
     }
-    if (tid == 0) { status[0] = 0; reinterpret_cast<double*>(status)[1] = minpiv; }
+    if (tid < 64) {  // the smallest pivot (a running minimum in the loop costs registers)
+        double mp = 1.0;
+        for (int j = tid; j < m; j += 64) mp = fmin(mp, piv[j]);
+        for (int off = 32; off > 0; off >>= 1) mp = fmin(mp, __shfl_xor(mp, off, 64));
+        if (tid == 0) {
+            status[0] = 0;
+            reinterpret_cast<double*>(status)[1] = mp;
+        }
+    }
 }
 
 void cholinv(lh_ctx* ctx, int m, double* G, double* T, double shift, int* status) {
-    if (m > kCholMax) LH_THROW(LH_ERR_ARG, "cholinv: block width %d exceeds %d", m, kCholMax);
-    k_cholinv<<<1, RB, 0, ctx->stream>>>(m, G, T, shift, status);
+    if (m > kCholMax || m < 1) LH_THROW(LH_ERR_ARG, "cholinv: block width %d outside 1..%d", m, kCholMax);
+    switch ((m + 31) / 32) {
+        case 1: k_cholinv<1><<<1, CHOL_NT, 0, ctx->stream>>>(m, G, T, shift, status); break;
+        case 2: k_cholinv<2><<<1, CHOL_NT, 0, ctx->stream>>>(m, G, T, shift, status); break;
+        case 3: k_cholinv<3><<<1, CHOL_NT, 0, ctx->stream>>>(m, G, T, shift, status); break;
+        default: k_cholinv<4><<<1, CHOL_NT, 0, ctx->stream>>>(m, G, T, shift, status); break;
+    }
     LH_CHECK_LAUNCH();
 }
 

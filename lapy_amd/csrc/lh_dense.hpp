@@ -4,6 +4,39 @@
 
 namespace lh {
 
+// ---- cross-lane sums in VALU (DPP and gfx950's permlane swaps; no LDS) ----
+// Every lane of the group receives the sum, with the same bits in every lane
+// (each stage adds a lane's value and its partner's, and the partner adds the
+// same two numbers).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+// over the 16 lanes of a DPP row
+__device__ __forceinline__ double row_sum16(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+    v += dpp_f64<0x141>(v);  // row_half_mirror: 7 - lane within 8
+    v += dpp_f64<0x140>(v);  // row_mirror: 15 - lane within 16
+    return v;
+}
+// over the 32 lanes of a half-wave
+__device__ __forceinline__ double half_sum32(double v) {
+    v = row_sum16(v);
+    const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
+    return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+// over the 64 lanes of a wave
+__device__ __forceinline__ double wave_sum64(double v) {
+    v = half_sum32(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(v), __double2loint(v), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(v), __double2hiint(v), false, false);
+    return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+
 // C (p x q, ldc, FP64) = alpha * U^T V over n rows; sym (p == q): the
 // product is known to be symmetric, only its upper 64 x 64 tiles are formed
 // and the lower ones mirror them

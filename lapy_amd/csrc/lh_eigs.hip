@@ -81,7 +81,7 @@ struct Lobpcg {
     DevBuf S, S2, AS, MS, dinv;
     DevBuf GA, GM, Cm, Gs, Ts, theta_d, rrwork, status, cs1, cs2, wsg, wsc;
     DevBuf lockbuf;  // soft locking of the polynomial: column norms (m doubles), then the column order (m int32)
-    std::vector<double> theta, res;
+    std::vector<double> theta, theta_new, res;
     lh_stats st{};
 
     T* Sp() { return S.as<T>(); }
@@ -175,11 +175,14 @@ struct Lobpcg {
             rr_solve_wide(ctx, N, m, GA.as<double>(), GM.as<double>(), Cm.as<double>(), theta_d.as<double>(),
                           rrwork.as<double>(), status.as<int>());
         }
-        int s = read_status();
+        int s = 0;
+        theta_new.resize(m);
+        const Xfer x[2] = {{&s, status.p, sizeof(int)}, {theta_new.data(), theta_d.p, (size_t)m * 8}};
+        d2h_n(ctx, x, 2);
         st.rr_calls++;
         st.t_rr_us += now_us() - t0;
-        if (s) return false;
-        d2h(ctx, theta.data(), theta_d.p, m * 8);
+        if (s) return false;  // theta keeps the last good Ritz values
+        theta.swap(theta_new);
         return true;
     }
 
@@ -260,8 +263,8 @@ struct Lobpcg {
             // residual into W and the convergence test
             residual_norms<T>(ctx, n, m, ASp(), MSp(), ld, theta_d.as<double>(), Sp() + m, ld, cs1.as<double>(),
                               cs2.as<double>(), wsc);
-            d2h(ctx, r2.data(), cs1.p, m * 8);
-            d2h(ctx, mx2.data(), cs2.p, m * 8);
+            const Xfer rx[2] = {{r2.data(), cs1.p, (size_t)m * 8}, {mx2.data(), cs2.p, (size_t)m * 8}};
+            d2h_n(ctx, rx, 2);
             const double floor = 1e-2 * std::fabs(theta[k - 1]);
             int nconv = 0;
             double worst = 0.0;

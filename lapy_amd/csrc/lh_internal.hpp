@@ -149,8 +149,10 @@ struct lh_ctx {
     Arena arena;
     int tuned_workers = 0;  // 0: not tuned
     lh_mpnn* mpnn = nullptr;  // network used by LH_PRECOND_LEARNED (not owned)
-    void* host_small = nullptr;   // pinned host staging (hipHostMalloc)
+    void* host_small = nullptr;   // pinned host memory of the PCG's status polling (hipHostMalloc)
     size_t host_small_bytes = 0;
+    void* host_stage = nullptr;   // pinned staging of d2h / h2d / d2h_n
+    size_t host_stage_bytes = 0;
     void* blas = nullptr;  // rocBLAS handle of the wide Rayleigh-Ritz solver, made on first use
     int asm_order = 0;     // LH_ASM_ORDER_*: row order of the triangle assembly
     std::shared_ptr<TuneTable> tune = std::make_shared<TuneTable>();
@@ -252,6 +254,13 @@ __device__ __forceinline__ int64_t xcd_block() {
 void exclusive_scan_i32(lh_ctx* ctx, const int32_t* in, int32_t* out, int64_t n);
 // host-synchronous small copy helpers
 void d2h(lh_ctx* ctx, void* dst, const void* src, size_t bytes);
+// several device -> host copies behind one stream synchronisation
+struct Xfer {
+    void* host;
+    const void* dev;
+    size_t bytes;
+};
+void d2h_n(lh_ctx* ctx, const Xfer* x, int cnt);
 void h2d(lh_ctx* ctx, void* dst, const void* src, size_t bytes);
 void memset_d(lh_ctx* ctx, void* dst, int v, size_t bytes);
 

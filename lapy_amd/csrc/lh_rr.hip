@@ -36,10 +36,7 @@ constexpr int PK = NMAX * (NMAX + 1) / 2;
 
 __device__ __forceinline__ int rowbase(int i) { return i * (i + 1) / 2; }
 
-__device__ __forceinline__ double wsum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
+__device__ __forceinline__ double wsum(double v) { return wave_sum64(v); }
 __device__ double bsum(double v, double* red) {
     v = wsum(v);
     __syncthreads();
@@ -143,124 +140,170 @@ __device__ void tri_lu_solve(const double* ws, int N, double* y) {
 }
 
 // ---- 1. tridiagonalisation ------------------------------------------------
-// The packed lower triangle of H lives in LDS.  Step k needs three barriers:
-// every wave reads column k (below the diagonal) straight from the packed
-// matrix and forms the reflector's norm, alpha, v0 and tau itself (the same
-// summation order in every wave, so all agree) while wave 0 publishes v
-// contiguously and parks it in global memory (barrier 1); the matvec
-// p = tau H22 v is split over row groups (4 lanes per trailing row while the
-// trailing block is long, up to 32 when it is short) and published
-// (barrier 2); every wave forms K = tau (p . v) / 2 itself, and the rank-2
-// update H22 -= v w^T + w v^T with w = p - K v runs on the packed rows
-// (barrier 3).
-__device__ __forceinline__ int lanes_per_row(int L) { return L > 128 ? 4 : (L > 64 ? 8 : (L > 32 ? 16 : 32)); }
+// H lives in registers: 512 threads form 32 row groups of 16 lanes (one DPP
+// row each), and the thread (group g, lane l) holds H[g + 32a][l + 16b] for
+// a < CB, b < 2 CB (CB = ceil(N / 32)), a block-cyclic layout in which every
+// row is spread over the 16 lanes of one group, so that row sums are DPP
+// sums within a row.  Step k (Householder reflector of row k = column k
+// below the diagonal) costs two barriers:
+//  A. the group owning row k forms the norm of H[k][k+1:], alpha, v0 and
+//     tau, and publishes v (zero at indices <= k) to LDS and to global
+//     memory (barrier 1);
+//  B. every thread multiplies its block by v; row sums give p = tau H v for
+//     the group's rows, which is published with the group's share of p . v
+//     (barrier 2);
+//  C. every thread forms K = tau (p . v) / 2 and w = p - K v and applies
+//     H -= v w^T + w v^T to its block in registers.
+// v, p and the shares of p . v are double-buffered by the parity of k, so
+// step k + 1 may overwrite nothing that a thread still reads from step k.
+// Rows and columns <= k are finished: p is zero there, so the update leaves
+// them unchanged, and register rows that lie wholly in them are skipped.
+// The step's critical path is the owning group's reflector (a norm, a square
+// root and a division in one wave, ~4,000 cycles), which the other waves
+// wait for at barrier 1.
+constexpr int TRI_NT = 512;
 
-__global__ void __launch_bounds__(NB)
+template <int CB>
+__global__ void __launch_bounds__(TRI_NT)
 k_rr_tri(int N, int nev, const double* __restrict__ G, int ldg, double* __restrict__ work,
          unsigned long long* __restrict__ prof) {
-    __shared__ double H[PK];  // packed lower triangle of H
-    __shared__ double sv[NMAX], sp[NMAX], sd[NMAX], se[NMAX], stau[NMAX];
+    constexpr int RA = CB, CC = 2 * CB;
+    __shared__ double vbuf[2][NMAX], pbuf[2][NMAX], kpart[2][32];
+    __shared__ double sd[NMAX], se[NMAX], stau[NMAX];
     RRWork W(work, N, nev);
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x, g = tid >> 4, l = tid & 15;
     stamp(prof, 0);
-    {
-        const int rl = tid >> 2, gl = tid & 3;
-        for (int i = rl; i < N; i += NB / 4) {
-            const int b = rowbase(i);
-            for (int j = gl; j <= i; j += 4) H[b + j] = 0.5 * (G[(int64_t)i * ldg + j] + G[(int64_t)j * ldg + i]);
+    double h[RA][CC];
+#pragma unroll
+    for (int a = 0; a < RA; ++a) {
+        const int i = g + 32 * a;
+#pragma unroll
+        for (int b = 0; b < CC; ++b) {
+            const int j = l + 16 * b;
+            h[a][b] = (i < N && j < N) ? 0.5 * (G[(int64_t)i * ldg + j] + G[(int64_t)j * ldg + i]) : 0.0;
         }
     }
-    __syncthreads();
     stamp(prof, 1);
     for (int k = 0; k < N - 2; ++k) {
-        const int L = N - k - 1;
-        // column k below the diagonal: x_c = H[k+1+c][k]
-        auto colx = [&](int c) -> double { return H[rowbase(k + 1 + c) + k]; };
-        double part = 0.0;
-        for (int c = lane; c < L; c += 64) {
-            const double x = colx(c);
-            part += x * x;
-        }
-        const double xn2 = wsum(part);
-        const double x0 = colx(0);
-        if (tid == 0) sd[k] = H[rowbase(k) + k];
-        if (xn2 == 0.0) {
-            if (tid == 0) { se[k] = 0.0; stau[k] = 0.0; }
-            continue;  // H22 untouched: no barrier needed
-        }
-        const double alpha = (x0 >= 0.0) ? -sqrt(xn2) : sqrt(xn2);
-        const double v0 = x0 - alpha;
-        const double t = 2.0 / (xn2 - x0 * x0 + v0 * v0);
-        if (tid == 0) { se[k] = alpha; stau[k] = t; }
-        // wave 0 publishes v contiguously (and parks it in global memory)
-        if (tid < 64)
-            for (int c = lane; c < L; c += 64) {
-                const double v = (c == 0) ? v0 : colx(c);
-                sv[c] = v;
-                W.R[(int64_t)k * N + c] = v;
-            }
-        __syncthreads();
-        auto vof = [&](int c) -> double { return sv[c]; };
-        const int lpr = lanes_per_row(L);
-        const int r = tid / lpr, gl = tid % lpr;  // NB / lpr >= L row groups
-        // p = t * H22 v ; row r of H22 = packed row i = k+1+r (columns <= i
-        // contiguous) plus column i below it (entries j > i, stride j)
-        {
-            double s0 = 0.0, s1 = 0.0;
-            if (r < L) {
-                const int i = k + 1 + r;
-                const double* hr = H + rowbase(i) + (k + 1);
-                int c = gl;
-                for (; c + lpr <= r; c += 2 * lpr) {
-                    s0 += hr[c] * vof(c);
-                    s1 += hr[c + lpr] * vof(c + lpr);
+        const int par = k & 1;
+        // A. reflector of row k (owned by group k % 32, register row k / 32)
+        if (g == (k & 31)) {
+            const int ak = k >> 5;
+            double x[CC];
+#pragma unroll
+            for (int b = 0; b < CC; ++b) x[b] = 0.0;
+#pragma unroll
+            for (int a = 0; a < RA; ++a)
+                if (a == ak) {
+#pragma unroll
+                    for (int b = 0; b < CC; ++b) x[b] = h[a][b];
                 }
-                if (c <= r) s0 += hr[c] * vof(c);
-                c = r + 1 + gl;
-                for (; c + lpr < L; c += 2 * lpr) {
-                    s0 += H[rowbase(k + 1 + c) + i] * vof(c);
-                    s1 += H[rowbase(k + 1 + c + lpr) + i] * vof(c + lpr);
-                }
-                if (c < L) s0 += H[rowbase(k + 1 + c) + i] * vof(c);
+            if (l == (k & 15)) {
+#pragma unroll
+                for (int b = 0; b < CC; ++b)
+                    if (b == (k >> 4)) sd[k] = x[b];
             }
-            double s = s0 + s1;
-            for (int off = lpr >> 1; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-            if (r < L && gl == 0) sp[r] = t * s;
+            double part = 0.0, x0l = 0.0;
+#pragma unroll
+            for (int b = 0; b < CC; ++b) {
+                const int j = l + 16 * b;
+                if (j <= k || j >= N) x[b] = 0.0;
+                part += x[b] * x[b];
+                if (b == ((k + 1) >> 4)) x0l = x[b];
+            }
+            const double xn2 = row_sum16(part);
+            const int src = ((g & 3) << 4) + ((k + 1) & 15);  // lane of H[k][k+1] in this wave
+            const double x0 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x0l), src),
+                                               __builtin_amdgcn_readlane(__double2loint(x0l), src));
+            double alpha = 0.0, v0 = 0.0, t = 0.0;
+            if (xn2 != 0.0) {
+                alpha = (x0 >= 0.0) ? -sqrt(xn2) : sqrt(xn2);
+                v0 = x0 - alpha;
+                t = 2.0 / (xn2 - x0 * x0 + v0 * v0);
+            }
+#pragma unroll
+            for (int b = 0; b < CC; ++b) {
+                const int j = l + 16 * b;
+                const double v = (t == 0.0) ? 0.0 : ((j == k + 1) ? v0 : x[b]);
+                if (j < N) vbuf[par][j] = v;
+                if (j > k && j < N) W.R[(int64_t)k * N + (j - k - 1)] = v;
+            }
+            if (l == 0) {
+                se[k] = alpha;
+                stau[k] = t;
+            }
         }
         __syncthreads();
-        double pv = 0.0;
-        for (int c = lane; c < L; c += 64) pv += sp[c] * vof(c);
-        const double K = 0.5 * t * wsum(pv);
-        if (r < L) {
-            double* hr = H + rowbase(k + 1 + r) + (k + 1);
-            const double vr = vof(r), wr = sp[r] - K * vr;
-            int c = gl;
-            for (; c + lpr <= r; c += 2 * lpr) {
-                const double va = vof(c), vb = vof(c + lpr);
-                const double wa = sp[c] - K * va, wb = sp[c + lpr] - K * vb;
-                const double h0 = hr[c], h1 = hr[c + lpr];
-                hr[c] = h0 - (vr * wa + wr * va);
-                hr[c + lpr] = h1 - (vr * wb + wr * vb);
-            }
-            if (c <= r) {
-                const double va = vof(c), wa = sp[c] - K * va;
-                hr[c] -= vr * wa + wr * va;
-            }
+        const double t = stau[k];
+        if (t == 0.0) continue;  // H untouched (uniform)
+        // B. p = tau H v for this group's rows
+        // register rows a < a_lo hold rows <= k only and are skipped; within
+        // a row, finished columns get exact zero updates (v and w are zero
+        // there), which costs less than a branch per block
+        const int a_lo = (k >= 31) ? (k - 31) / 32 + 1 : 0;
+        double vc[CC];
+#pragma unroll
+        for (int b = 0; b < CC; ++b) {
+            const int j = l + 16 * b;
+            vc[b] = (j < N) ? vbuf[par][j] : 0.0;
         }
+        double kp = 0.0;
+#pragma unroll
+        for (int a = 0; a < RA; ++a) {
+            const int i = g + 32 * a;
+            double p = 0.0;
+            if (a >= a_lo) {
+                double s = 0.0;
+#pragma unroll
+                for (int b = 0; b < CC; ++b) s += h[a][b] * vc[b];
+                p = t * row_sum16(s);
+            }
+            if (i <= k || i >= N) p = 0.0;
+            if (i < N) kp += p * vbuf[par][i];
+            if (l == a && i < N) pbuf[par][i] = p;
+        }
+        if (l == 0) kpart[par][g] = kp;
         __syncthreads();
+        // C. rank-2 update H -= v w^T + w v^T, w = p - K v (v and p at this
+        // thread's rows read back from LDS); every half-wave holds the 32
+        // shares of p . v
+        const double K = 0.5 * t * half_sum32(kpart[par][tid & 31]);
+        double wc[CC];
+#pragma unroll
+        for (int b = 0; b < CC; ++b) {
+            const int j = l + 16 * b;
+            wc[b] = ((j < N) ? pbuf[par][j] : 0.0) - K * vc[b];
+        }
+#pragma unroll
+        for (int a = 0; a < RA; ++a) {
+            const int i = g + 32 * a;
+            if (a < a_lo || i >= N) continue;
+            const double vr = vbuf[par][i];
+            const double wr = pbuf[par][i] - K * vr;
+#pragma unroll
+            for (int b = 0; b < CC; ++b)
+                h[a][b] -= vr * wc[b] + wr * vc[b];
+        }
+    }
+    // the last 2 x 2 block (or the 1 x 1 matrix)
+#pragma unroll
+    for (int a = 0; a < RA; ++a) {
+        const int i = g + 32 * a;
+#pragma unroll
+        for (int b = 0; b < CC; ++b) {
+            const int j = l + 16 * b;
+            if (N >= 2 && i == N - 2 && j == N - 2) sd[N - 2] = h[a][b];
+            if (N >= 2 && i == N - 1 && j == N - 2) se[N - 2] = h[a][b];
+            if (i == N - 1 && j == N - 1) sd[N - 1] = h[a][b];
+        }
     }
     if (tid == 0) {
-        if (N >= 2) {
-            sd[N - 2] = H[rowbase(N - 2) + N - 2];
-            se[N - 2] = H[rowbase(N - 1) + N - 2];
-            stau[N - 2] = 0.0;
-        }
-        sd[N - 1] = H[rowbase(N - 1) + N - 1];
+        if (N >= 2) stau[N - 2] = 0.0;
         stau[N - 1] = 0.0;
         se[N - 1] = 0.0;
     }
     __syncthreads();
-    for (int i = tid; i < N; i += NB) {
+    for (int i = tid; i < N; i += TRI_NT) {
         W.d[i] = sd[i];
         W.e[i] = se[i];
         W.tau[i] = stau[i];
@@ -362,34 +405,52 @@ k_rr_invit(int N, int nev, double* __restrict__ work) {
 }
 
 // ---- 4. back-transformation Z = Q Y ----------------------------------------
+// One wave per eigenvector (rows q, q + 64, q + 128 in lane q), one
+// workgroup each, so the vectors spread over CUs.  The N - 2 reflector steps
+// are dependent; each one's entries are loaded BACK_PD steps ahead into a
+// register ring, which hides the L2 latency, and the wave sums are DPP sums.
+constexpr int BACK_PD = 4;
+
 __global__ void __launch_bounds__(64)
 k_rr_back(int N, int nev, double* __restrict__ work, unsigned long long* __restrict__ prof) {
+    __shared__ double tau[NMAX];
     stamp(prof, 4);
     RRWork W(work, N, nev);
     const int g = blockIdx.x, q = threadIdx.x;
-    // rows q, q + 64, q + 128 of the vector
-    double z0 = 0.0, z1 = 0.0, z2 = 0.0;
+    for (int i = q; i < N; i += 64) tau[i] = W.tau[i];
     const double* yg = W.Y + (int64_t)g * N;
-    if (q < N) z0 = yg[q];
-    if (q + 64 < N) z1 = yg[q + 64];
-    if (q + 128 < N) z2 = yg[q + 128];
-    // reflector k acts on rows k+1 .. N-1 with v_k[r - k - 1]
+    double z0 = (q < N) ? yg[q] : 0.0;
+    double z1 = (q + 64 < N) ? yg[q + 64] : 0.0;
+    double z2 = (q + 128 < N) ? yg[q + 128] : 0.0;
+    // reflector k acts on rows k+1 .. N-1 with v_k[row - k - 1] = R[k][row - k - 1]
     auto vload = [&](int k, int row) -> double {
         return (k >= 0 && row >= k + 1 && row < N) ? W.R[(int64_t)k * N + (row - k - 1)] : 0.0;
     };
-    int k = N - 3;
-    double n0 = vload(k, q), n1 = vload(k, q + 64), n2 = vload(k, q + 128);
-    for (; k >= 0; --k) {
-        const double v0 = n0, v1 = n1, v2 = n2;
-        n0 = vload(k - 1, q);
-        n1 = vload(k - 1, q + 64);
-        n2 = vload(k - 1, q + 128);
-        const double t = W.tau[k];
-        if (t == 0.0) continue;
-        const double s = t * wsum(v0 * z0 + v1 * z1 + v2 * z2);
-        z0 -= s * v0;
-        z1 -= s * v1;
-        z2 -= s * v2;
+    double ring[BACK_PD][3];
+#pragma unroll
+    for (int d = 0; d < BACK_PD; ++d) {
+        const int k = N - 3 - d;
+        ring[d][0] = vload(k, q);
+        ring[d][1] = vload(k, q + 64);
+        ring[d][2] = vload(k, q + 128);
+    }
+    __syncthreads();  // tau staged
+    for (int k0 = N - 3; k0 >= 0; k0 -= BACK_PD) {
+#pragma unroll
+        for (int d = 0; d < BACK_PD; ++d) {
+            const int k = k0 - d;
+            if (k < 0) break;
+            const double v0 = ring[d][0], v1 = ring[d][1], v2 = ring[d][2];
+            ring[d][0] = vload(k - BACK_PD, q);
+            ring[d][1] = vload(k - BACK_PD, q + 64);
+            ring[d][2] = vload(k - BACK_PD, q + 128);
+            const double t = tau[k];
+            if (t == 0.0) continue;
+            const double sdot = t * wsum(v0 * z0 + v1 * z1 + v2 * z2);
+            z0 -= sdot * v0;
+            z1 -= sdot * v1;
+            z2 -= sdot * v2;
+        }
     }
     if (q < N) W.Z[(int64_t)q * nev + g] = z0;
     if (q + 64 < N) W.Z[(int64_t)(q + 64) * nev + g] = z1;
@@ -477,7 +538,14 @@ void rr_std(lh_ctx* ctx, int N, int nev, const double* G, int ldg, double* C, do
             int* status, unsigned long long* prof) {
     if (N > kRRStdMaxN || nev > N || N < 1 || nev < 1)
         LH_THROW(LH_ERR_ARG, "rr_std: N=%d nev=%d out of range", N, nev);
-    k_rr_tri<<<1, NB, 0, ctx->stream>>>(N, nev, G, ldg, work, prof);
+    switch ((N + 31) / 32) {
+        case 1: k_rr_tri<1><<<1, TRI_NT, 0, ctx->stream>>>(N, nev, G, ldg, work, prof); break;
+        case 2: k_rr_tri<2><<<1, TRI_NT, 0, ctx->stream>>>(N, nev, G, ldg, work, prof); break;
+        case 3: k_rr_tri<3><<<1, TRI_NT, 0, ctx->stream>>>(N, nev, G, ldg, work, prof); break;
+        case 4: k_rr_tri<4><<<1, TRI_NT, 0, ctx->stream>>>(N, nev, G, ldg, work, prof); break;
+        case 5: k_rr_tri<5><<<1, TRI_NT, 0, ctx->stream>>>(N, nev, G, ldg, work, prof); break;
+        default: k_rr_tri<6><<<1, TRI_NT, 0, ctx->stream>>>(N, nev, G, ldg, work, prof); break;
+    }
     LH_CHECK_LAUNCH();
     k_rr_bisect<<<nev, 64, 0, ctx->stream>>>(N, nev, work, theta, prof);
     LH_CHECK_LAUNCH();
