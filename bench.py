@@ -371,6 +371,7 @@ def run_workload(args, ctx, _lib, meshgen):
                     "value": round(256 * k * args.steps / total, 4), "ms_per_step": round(1000 * total / args.steps, 2),
                     "config": {"workload": "256 ellipsoids, icosphere levels 3-6, FP64 assembly + k=50 eigenpairs each",
                                "vertices_total": int(sum(len(v) for v, _ in meshes)), "k": k, "workers": best,
+                               "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                                "precond": args.precond,
                                "fallbacks": int(sum(r[2].fallback_used for r in res)),
                                "autotune_seconds": secs, "lobpcg_iterations_min_max": [min(its), max(its)],
@@ -575,9 +576,12 @@ def main():
         print(json.dumps({"error": "lapy_amd runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
-    # the batched eigensolver runs one stream per worker; let HIP map up to 16
-    # of them to hardware queues (an explicit setting in the environment wins)
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+    # the batched eigensolver runs one stream per worker: for the workloads
+    # that use it, let HIP map up to 16 streams to hardware queues (its
+    # default is 4, which serialises the streams sharing a queue); read by the
+    # HIP runtime at its initialisation, so it is set before the library loads
+    if args.workload in ("batch", "learned"):
+        os.environ["GPU_MAX_HW_QUEUES"] = "16"
     from lapy_amd import _lib, meshgen
 
     ctx = _lib.default_context()

@@ -511,7 +511,7 @@ __global__ void __launch_bounds__(CHOL_NT) k_cholinv(int m, const double* __rest
             return;
         }
         if (tid == 0) piv[j] = p;
-        const double ip = 1.0 / p;
+        const double ip = rcp_nr(p);  // 1e-12 < p <= ~1 (unit diagonal): the short-latency form
         double fc[CB];
 #pragma unroll
         for (int b = 0; b < CB; ++b) {

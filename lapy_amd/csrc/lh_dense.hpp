@@ -4,6 +4,30 @@
 
 namespace lh {
 
+// ---- short-latency reciprocal and square root --------------------------------
+// The hardware reciprocal (square root) estimate refined by Newton
+// (Goldschmidt) steps to within an ulp or two: about half the dependent
+// instructions of the correctly rounded division and square root.  For
+// normal, finite positive arguments away from the exponent range's ends.
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double sqrt_nr(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y, h = 0.5 * y;  // s ~ sqrt(x), h ~ 1 / (2 sqrt(x))
+    double r = fma(-s, h, 0.5);
+    s = fma(s, r, s);
+    h = fma(h, r, h);
+    r = fma(-s, h, 0.5);
+    s = fma(s, r, s);
+    h = fma(h, r, h);
+    return fma(fma(-s, s, x), h, s);
+}
+
 // ---- cross-lane sums in VALU (DPP and gfx950's permlane swaps; no LDS) ----
 // Every lane of the group receives the sum, with the same bits in every lane
 // (each stage adds a lane's value and its partner's, and the partner adds the

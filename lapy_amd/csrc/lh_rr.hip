@@ -69,13 +69,16 @@ struct RRWork {
     }
 };
 
+// number of eigenvalues of T below x (negative pivots of T - x I).  The
+// caller scales T so that ||T|| <= 1: the pivots then lie between pivmin and
+// ~1 / pivmin, where the short-latency reciprocal is accurate
 __device__ int sturm(const double* d, const double* e2, int N, double x, double pivmin) {
     int cnt = 0;
     double q = d[0] - x;
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < 0.0;
     for (int i = 1; i < N; ++i) {
-        q = (d[i] - x) - e2[i - 1] / q;
+        q = (d[i] - x) - e2[i - 1] * rcp_nr(q);
         if (fabs(q) < pivmin) q = -pivmin;
         cnt += q < 0.0;
     }
@@ -217,9 +220,19 @@ k_rr_tri(int N, int nev, const double* __restrict__ G, int ldg, double* __restri
                                                __builtin_amdgcn_readlane(__double2loint(x0l), src));
             double alpha = 0.0, v0 = 0.0, t = 0.0;
             if (xn2 != 0.0) {
-                alpha = (x0 >= 0.0) ? -sqrt(xn2) : sqrt(xn2);
-                v0 = x0 - alpha;
-                t = 2.0 / (xn2 - x0 * x0 + v0 * v0);
+                // the short-latency forms on the step's critical path (the
+                // exact ones for extreme magnitudes; xn2 is the same in every
+                // lane, so one branch runs)
+                if (xn2 > 1e-280 && xn2 < 1e280) {
+                    const double sq = sqrt_nr(xn2);
+                    alpha = (x0 >= 0.0) ? -sq : sq;
+                    v0 = x0 - alpha;
+                    t = 2.0 * rcp_nr(xn2 - x0 * x0 + v0 * v0);
+                } else {
+                    alpha = (x0 >= 0.0) ? -sqrt(xn2) : sqrt(xn2);
+                    v0 = x0 - alpha;
+                    t = 2.0 / (xn2 - x0 * x0 + v0 * v0);
+                }
             }
 #pragma unroll
             for (int b = 0; b < CC; ++b) {
@@ -331,12 +344,14 @@ k_rr_bisect(int N, int nev, double* __restrict__ work, double* __restrict__ thet
     stamp(prof, 3);
     RRWork W(work, N, nev);
     const int j = blockIdx.x, q = threadIdx.x;
+    // T scaled by a power of two (exact) to ||T|| <= 1
+    const double sc = ldexp(1.0, -ilogb(W.bnd[2]) - 1);
     for (int i = q; i < N; i += 64) {
-        sd[i] = W.d[i];
-        se2[i] = (i < N - 1) ? W.e[i] * W.e[i] : 0.0;
+        sd[i] = W.d[i] * sc;
+        se2[i] = (i < N - 1) ? (W.e[i] * sc) * (W.e[i] * sc) : 0.0;
     }
     __syncthreads();
-    const double glo = W.bnd[0], ghi = W.bnd[1], tnorm = W.bnd[2];
+    const double glo = W.bnd[0] * sc, ghi = W.bnd[1] * sc, tnorm = W.bnd[2] * sc;
     const double pivmin = 1e-290 + 1e-300 * tnorm;
     double a = glo - 1e-14 * tnorm, b = ghi + 1e-14 * tnorm;
     for (int it = 0; it < 24; ++it) {
@@ -352,8 +367,9 @@ k_rr_bisect(int N, int nev, double* __restrict__ work, double* __restrict__ thet
         if (b - a <= 2.2e-16 * (fabs(a) + fabs(b)) + 1e-17 * tnorm) break;
     }
     if (q == 0) {
-        W.lam[j] = 0.5 * (a + b);
-        theta[j] = 0.5 * (a + b);
+        const double lam = 0.5 * (a + b) / sc;
+        W.lam[j] = lam;
+        theta[j] = lam;
     }
 }
 
