@@ -313,9 +313,10 @@ def run_workload(args, ctx, _lib, meshgen):
             out["cpu_baseline"] = {"value": round(1.0 / ct, 4), "unit": "fields/s", "cores": 1, "kind": "port",
                                    "sample": f"oracle geodesic_f (splu) on the L6 ellipsoid, {len(vs)} vertices"}
     elif args.workload == "batch":
-        meshes = meshgen.shape_batch(count=256)
+        nb = args.batch_count
+        meshes = meshgen.shape_batch(count=nb)
         k = 50
-        gold = golden("batch_256_k50.json")
+        gold = golden("batch_256_k50.json") if nb == 256 else None
         base = [_lib.DeviceMesh(ctx, v, t) for v, t in meshes]
         # preconditioner: Chebyshev-Jacobi (default), plain Jacobi, or the
         # learned network, trained first (untimed) on six other meshes of
@@ -340,18 +341,22 @@ def run_workload(args, ctx, _lib, meshgen):
             ab = [m.assemble(_lib.LH_TRIA, False, np.float64) for m in ms]
             return ms, [a for a, _ in ab], [b for _, b in ab]
 
-        # autotune the worker count on a sample of 32 meshes of every size
-        ms, As, Bs = assemble_all()
-        sample = list(range(0, 256, 8))
-        t0 = time.perf_counter()
-        best, secs = _lib.eigs_autotune(ctx, [As[i] for i in sample], [Bs[i] for i in sample], [k] * len(sample),
-                                        **popts)
-        log(f"[bench] autotune: best workers {best}, sample seconds {secs} ({time.perf_counter() - t0:.1f} s)")
-        del ms, As, Bs
+        # autotune the worker count on a sample of every eighth mesh (unless
+        # a count is given)
+        if args.batch_workers > 0:
+            best, secs = args.batch_workers, None
+        else:
+            ms, As, Bs = assemble_all()
+            sample = list(range(0, nb, 8))
+            t0 = time.perf_counter()
+            best, secs = _lib.eigs_autotune(ctx, [As[i] for i in sample], [Bs[i] for i in sample],
+                                            [k] * len(sample), **popts)
+            log(f"[bench] autotune: best workers {best}, sample seconds {secs} ({time.perf_counter() - t0:.1f} s)")
+            del ms, As, Bs
         for w in range(args.warmup):
             t1 = time.perf_counter()
             ms, As, Bs = assemble_all()
-            res = _lib.eigs_batched(ctx, As, Bs, [k] * 256, keep_device=True, **popts)
+            res = _lib.eigs_batched(ctx, As, Bs, [k] * nb, workers=best, keep_device=True, **popts)
             ctx.sync()
             del res, ms, As, Bs
             log(f"[bench] batch warmup {w}: {time.perf_counter() - t1:.2f} s")
@@ -359,7 +364,7 @@ def run_workload(args, ctx, _lib, meshgen):
         for st_i in range(args.steps):
             t1 = time.perf_counter()
             ms, As, Bs = assemble_all()
-            res = _lib.eigs_batched(ctx, As, Bs, [k] * 256, keep_device=True, **popts)
+            res = _lib.eigs_batched(ctx, As, Bs, [k] * nb, workers=best, keep_device=True, **popts)
             ctx.sync()
             log(f"[bench] batch step {st_i}: {time.perf_counter() - t1:.2f} s")
         total = time.perf_counter() - t0
@@ -367,9 +372,9 @@ def run_workload(args, ctx, _lib, meshgen):
         if gold:
             err = max(rel_err(r[0], g) for r, g in zip(res, gold["evals"]))
         its = [r[2].iterations for r in res]
-        out.update({"metric": "eigenpairs/sec (k=50), 256-mesh ShapeDNA batch",
-                    "value": round(256 * k * args.steps / total, 4), "ms_per_step": round(1000 * total / args.steps, 2),
-                    "config": {"workload": "256 ellipsoids, icosphere levels 3-6, FP64 assembly + k=50 eigenpairs each",
+        out.update({"metric": f"eigenpairs/sec (k=50), {nb}-mesh ShapeDNA batch",
+                    "value": round(nb * k * args.steps / total, 4), "ms_per_step": round(1000 * total / args.steps, 2),
+                    "config": {"workload": f"{nb} ellipsoids, icosphere levels 3-6, FP64 assembly + k=50 eigenpairs each",
                                "vertices_total": int(sum(len(v) for v, _ in meshes)), "k": k, "workers": best,
                                "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
                                "precond": args.precond,
@@ -564,6 +569,9 @@ def main():
     ap.add_argument("--workload", default="l8",
                     choices=["l8", "small", "tet", "pcie", "batch", "geodesic", "learned"],
                     help="l8: the benchmark line (default); small / tet / pcie / batch: the other workloads")
+    ap.add_argument("--batch-count", type=int, default=256, help="batch workload: number of meshes")
+    ap.add_argument("--batch-workers", type=int, default=0,
+                    help="batch workload: solver workers (0: autotuned on every eighth mesh)")
     ap.add_argument("--precond", default="cheb", choices=["cheb", "jacobi", "learned", "auto"],
                     help="batch workload: Chebyshev-Jacobi (default), plain Jacobi, the learned network, or auto "
                          "(the network on the meshes where the polynomial would be plain Jacobi)")
