@@ -433,9 +433,10 @@ void rr_solve(lh_ctx* ctx, int N, int nev, double* GA, double* GM, double* C, do
 // S, zero at indices <= j) and U[:, j] r^T / p from U, and scales column j
 // of U by 1 / sqrt(p) at the end (Gram-Schmidt of the unit vectors in the
 // G_s inner product).  S and U live in registers: 512 threads form 16
-// half-waves, thread (hw, ln) holding entries [hw + 16a][ln + 32b]; row j of
-// S and column j of U are published to LDS, double-buffered by the parity of
-// j, so one barrier per step suffices.  status 1 on a non-positive diagonal
+// half-waves, thread (hw, ln) holding entries [hw + 16a][ln + 32b].  Columns
+// are eliminated in pairs: rows j, j + 1 of S and columns j, j + 1 of U are
+// published to LDS, double-buffered by the pair's parity, so one barrier per
+// two columns suffices.  status 1 on a non-positive diagonal
 // or a pivot <= 1e-12 (reported in ((double*)status)[1]); otherwise the
 // smallest pivot is reported there (below 1e-2 the caller repeats the pass).
 constexpr int CHOL_NT = 512;
@@ -444,8 +445,8 @@ template <int CB>
 __global__ void __launch_bounds__(CHOL_NT) k_cholinv(int m, const double* __restrict__ G, double* __restrict__ T,
                                                      double shift, int* __restrict__ status) {
     constexpr int RA = 2 * CB;
-    __shared__ double dsc[kCholMax], piv[kCholMax];
-    __shared__ double rbuf[2][32 * CB], ubuf[2][32 * CB];
+    __shared__ double dsc[kCholMax], piv[kCholMax + 1];
+    __shared__ double rbuf[2][2][32 * CB], ubuf[2][2][32 * CB];
     __shared__ int flag[1];
     const int tid = threadIdx.x, hw = tid >> 5, ln = tid & 31;
     if (tid == 0) flag[0] = 0;
@@ -477,59 +478,86 @@ __global__ void __launch_bounds__(CHOL_NT) k_cholinv(int m, const double* __rest
             U[a][b] = (i == c) ? 1.0 : 0.0;
         }
     }
-    for (int j = 0; j < m; ++j) {
-        const int par = j & 1;
-        // publish row j of S (the pivot at j, zero before it) and column j of U
-        if (hw == (j & 15)) {
-            const int aj = j >> 4;
+    // two columns per step (one barrier): rows j and j + 1 of S and columns j
+    // and j + 1 of U are published; every thread corrects row j + 1 for the
+    // elimination of column j itself (r2' = r2 - f r1, f = S[j][j+1] / p1)
+    // and applies both rank-1 updates.  An odd m pairs its last column with
+    // the identity padding (pivot 1, no coupling).
+    for (int j = 0; j < m; j += 2) {
+        const int par = (j >> 1) & 1;
 #pragma unroll
-            for (int a = 0; a < RA; ++a)
-                if (a == aj) {
+        for (int h = 0; h < 2; ++h) {
+            const int jr = j + h;
+            if (hw == (jr & 15)) {
+                const int aj = jr >> 4;
 #pragma unroll
-                    for (int b = 0; b < CB; ++b) {
-                        const int c = ln + 32 * b;
-                        rbuf[par][c] = (c >= j) ? S[a][b] : 0.0;
+                for (int a = 0; a < RA; ++a)
+                    if (a == aj) {
+#pragma unroll
+                        for (int b = 0; b < CB; ++b) {
+                            const int c = ln + 32 * b;
+                            rbuf[par][h][c] = (c >= jr) ? S[a][b] : 0.0;
+                        }
                     }
-                }
-        }
-        if (ln == (j & 31)) {
-            const int bj = j >> 5;
+            }
+            if (ln == (jr & 31)) {
+                const int bj = jr >> 5;
 #pragma unroll
-            for (int b = 0; b < CB; ++b)
-                if (b == bj) {
+                for (int b = 0; b < CB; ++b)
+                    if (b == bj) {
 #pragma unroll
-                    for (int a = 0; a < RA; ++a) ubuf[par][hw + 16 * a] = U[a][b];  // zero below row j
-                }
+                        for (int a = 0; a < RA; ++a) ubuf[par][h][hw + 16 * a] = U[a][b];  // zero below row jr
+                    }
+            }
         }
         __syncthreads();
-        const double p = rbuf[par][j];
-        if (!(p > 1e-12)) {  // uniform
+        const double p1 = rbuf[par][0][j];
+        if (!(p1 > 1e-12)) {  // uniform
             if (tid == 0) {
                 status[0] = 1;
-                reinterpret_cast<double*>(status)[1] = p;
+                reinterpret_cast<double*>(status)[1] = p1;
             }
             return;
         }
-        if (tid == 0) piv[j] = p;
-        const double ip = rcp_nr(p);  // 1e-12 < p <= ~1 (unit diagonal): the short-latency form
-        double fc[CB];
+        const double ip1 = rcp_nr(p1);  // 1e-12 < p <= ~1 (unit diagonal): the short-latency form
+        const double f = rbuf[par][0][j + 1] * ip1;
+        const double p2 = rbuf[par][1][j + 1] - f * rbuf[par][0][j + 1];
+        if (!(p2 > 1e-12)) {  // uniform
+            if (tid == 0) {
+                status[0] = 1;
+                reinterpret_cast<double*>(status)[1] = p2;
+            }
+            return;
+        }
+        const double ip2 = rcp_nr(p2);
+        if (tid == 0) {
+            piv[j] = p1;
+            piv[j + 1] = p2;
+        }
+        double fc1[CB], fc2[CB];
 #pragma unroll
         for (int b = 0; b < CB; ++b) {
             const int c = ln + 32 * b;
-            fc[b] = (c > j) ? rbuf[par][c] * ip : 0.0;
+            const double r1 = rbuf[par][0][c];
+            fc1[b] = (c > j) ? r1 * ip1 : 0.0;
+            fc2[b] = (c > j + 1) ? (rbuf[par][1][c] - f * r1) * ip2 : 0.0;
         }
-        // S -= r fc^T (rows <= j: r is zero there, or row j, which is
-        // finished); U -= U[:, j] fc^T; columns <= j have fc = 0
+        // S -= r1 fc1^T + r2' fc2^T, U -= u1 fc1^T + u2' fc2^T (r1 and r2' are
+        // zero at rows before j; rows j and j + 1 are finished); columns <= j
+        // have zero coefficients
         const int b_lo = (j >= 31) ? (j - 31) / 32 + 1 : 0;
 #pragma unroll
         for (int a = 0; a < RA; ++a) {
-            const double ri = rbuf[par][hw + 16 * a];
-            const double ui = ubuf[par][hw + 16 * a];
+            const int i = hw + 16 * a;
+            const double r1i = rbuf[par][0][i];
+            const double r2i = (i > j) ? rbuf[par][1][i] - f * r1i : 0.0;
+            const double u1i = ubuf[par][0][i];
+            const double u2i = ubuf[par][1][i] - f * u1i;
 #pragma unroll
             for (int b = 0; b < CB; ++b) {
                 if (b < b_lo) continue;
-                S[a][b] -= ri * fc[b];
-                U[a][b] -= ui * fc[b];
+                S[a][b] -= r1i * fc1[b] + r2i * fc2[b];
+                U[a][b] -= u1i * fc1[b] + u2i * fc2[b];
             }
         }
     }
