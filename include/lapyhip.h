@@ -403,6 +403,15 @@ int lh_block_gram_noalloc(lh_ctx* ctx, int64_t n, const double* U, int ldu, int 
  * N x N host matrix H (N <= 192).  evals (nev), Z (N x nev row-major); when
  * phase_us is not NULL it receives 7 phase durations in microseconds. */
 int lh_small_symeig(lh_ctx* ctx, int N, int nev, const double* H, double* evals, double* Z, double* phase_us);
+/* The eigensolver's Cholesky-QR factor on its own (no counterpart in the
+ * reference; exposed for its tests): the upper triangular m x m T with
+ * T^T G T = I for the symmetric positive definite m x m host matrix G
+ * (m <= 128, row-major, shift added to the equilibrated diagonal), computed
+ * on one workgroup as in the block solver.  min_pivot (nullable) receives the
+ * smallest pivot of the equilibrated matrix.  LH_ERR_NUMERIC when G is not
+ * numerically positive definite (a pivot <= 1e-12 relative to its unit
+ * diagonal). */
+int lh_small_cholinv(lh_ctx* ctx, int m, const double* G, double shift, double* T, double* min_pivot);
 
 #ifdef __cplusplus
 }

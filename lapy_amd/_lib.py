@@ -111,6 +111,7 @@ SIGNATURES = {
     "lh_time_assembly": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, _dp]),
     "lh_time_spmm": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _dp]),
     "lh_small_symeig": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "lh_small_cholinv": (C.c_int, [_vp, C.c_int, _vp, C.c_double, _vp, _dp]),
     "lh_time_cheb_step": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp]),
     "lh_eigs_batched": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_double, _vp, C.c_int, _vp, _vp, _vp]),
     "lh_eigs_autotune": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_double, _vp, _vp, _vp]),
@@ -695,6 +696,19 @@ def small_symeig(ctx: Context, H, nev: int, profile: bool = False):
     ph = np.zeros(7) if profile else None
     check(lib().lh_small_symeig(ctx.handle, N, nev, _ptr(H), _ptr(ev), _ptr(Z), _ptr(ph)), ctx, "lh_small_symeig")
     return (ev, Z, ph) if profile else (ev, Z)
+
+
+def small_cholinv(ctx: Context, G, shift: float = 0.0):
+    """The block solver's Cholesky-QR factor of a small SPD matrix (m <= 128):
+    (T, min_pivot) with T upper triangular and T^T (G + shift D^-2) T = I,
+    D = diag(G)^-1/2 (lh_small_cholinv)."""
+    G = np.ascontiguousarray(G, dtype=np.float64)
+    m = G.shape[0]
+    T = np.empty((m, m))
+    mp = C.c_double(0.0)
+    check(lib().lh_small_cholinv(ctx.handle, m, _ptr(G), float(shift), _ptr(T), C.byref(mp)), ctx,
+          "lh_small_cholinv")
+    return T, mp.value
 
 
 def curvature(ctx: Context, dm: "DeviceMesh", smoothit: int):

@@ -901,3 +901,22 @@ void jacobi_diag(lh_ctx* ctx, const lh_csr* A, const lh_csr* B, double alpha, do
 }
 
 }  // namespace lh
+
+using namespace lh;
+
+extern "C" int lh_small_cholinv(lh_ctx* ctx, int m, const double* G, double shift, double* T, double* min_pivot) {
+    if (!ctx || !G || !T || m < 1 || m > kCholMax || !(shift >= 0.0)) return LH_ERR_ARG;
+    LH_API_BEGIN(ctx)
+    HIPX(hipSetDevice(ctx->device));
+    DevBuf dG((size_t)m * m * 8), dT((size_t)m * m * 8), st(64);
+    h2d(ctx, dG.p, G, (size_t)m * m * 8);
+    HIPX(hipMemsetAsync(st.p, 0, 64, ctx->stream));
+    cholinv(ctx, m, dG.as<double>(), dT.as<double>(), shift, st.as<int>());
+    double sv[2];
+    d2h(ctx, sv, st.p, sizeof(sv));
+    if (min_pivot) *min_pivot = sv[1];
+    if (reinterpret_cast<int*>(sv)[0])
+        LH_THROW(LH_ERR_NUMERIC, "lh_small_cholinv: not numerically positive definite (pivot %g)", sv[1]);
+    d2h(ctx, T, dT.p, (size_t)m * m * 8);
+    LH_API_END(ctx)
+}
