@@ -54,7 +54,10 @@ typedef struct lh_stats {
     double  max_resid;       /* largest final relative residual               */
     double  t_setup_us;      /* host-measured phase times (microseconds)      */
     double  t_iter_us;
-    double  t_rr_us;
+    double  t_rr_us;         /* eigensolver: host time in the Rayleigh-Ritz
+                                calls; those inside the iteration are issued
+                                without waiting (their results are read with
+                                the next residual norms)                      */
     double  t_total_us;
     int32_t spmm_calls;
     int32_t rr_calls;

@@ -164,8 +164,9 @@ struct Lobpcg {
     // the small problem is a standard one; the generalized path (Cholesky of
     // S^T B S, through rocSOLVER) is kept for trial bases wider than the LDS
     // solver takes.
-    bool rayleigh_ritz(int N) {
-        double t0 = now_us();
+    // launches only; the status and the Ritz values (theta_d) are read by
+    // rayleigh_ritz or, inside the iteration, with the next residual norms
+    void rr_issue(int N) {
         gram<T>(ctx, n, Sp(), ld, N, ASp(), ld, N, GA.as<double>(), N, wsg, 1.0, true);
         if (N <= kRRStdMaxN) {
             rr_std(ctx, N, m, GA.as<double>(), N, Cm.as<double>(), theta_d.as<double>(), rrwork.as<double>(),
@@ -175,15 +176,36 @@ struct Lobpcg {
             rr_solve_wide(ctx, N, m, GA.as<double>(), GM.as<double>(), Cm.as<double>(), theta_d.as<double>(),
                           rrwork.as<double>(), status.as<int>());
         }
+        st.rr_calls++;
+    }
+    bool rr_pending = false;  // an in-loop Rayleigh-Ritz whose status and theta are not read yet
+    bool rayleigh_ritz(int N) {
+        double t0 = now_us();
+        rr_issue(N);
         int s = 0;
         theta_new.resize(m);
         const Xfer x[2] = {{&s, status.p, sizeof(int)}, {theta_new.data(), theta_d.p, (size_t)m * 8}};
         d2h_n(ctx, x, 2);
-        st.rr_calls++;
         st.t_rr_us += now_us() - t0;
         if (s) return false;  // theta keeps the last good Ritz values
         theta.swap(theta_new);
         return true;
+    }
+    // reads a pending in-loop Rayleigh-Ritz's status and theta together with
+    // the residual norms (one synchronisation)
+    void read_residuals(std::vector<double>& r2, std::vector<double>& mx2, int it) {
+        const Xfer rx[2] = {{r2.data(), cs1.p, (size_t)m * 8}, {mx2.data(), cs2.p, (size_t)m * 8}};
+        if (!rr_pending) {
+            d2h_n(ctx, rx, 2);
+            return;
+        }
+        int s = 0;
+        theta_new.resize(m);
+        const Xfer x[4] = {rx[0], rx[1], {&s, status.p, sizeof(int)}, {theta_new.data(), theta_d.p, (size_t)m * 8}};
+        d2h_n(ctx, x, 4);
+        rr_pending = false;
+        if (s) LH_THROW(LH_ERR_NUMERIC, "Rayleigh-Ritz failed at iteration %d", it - 1);
+        theta.swap(theta_new);
     }
 
     // out_evals / out_vecs: the result (a second phase or an FP64 solve);
@@ -256,15 +278,13 @@ struct Lobpcg {
         std::vector<double> r2(m), mx2(m);
         std::vector<char> conv(m, 0);
         std::vector<int32_t> perm;
-        std::vector<char> lock_host((size_t)m * 12);
         std::vector<double> gm_hist;  // first phase: residual history
         constexpr int kMinFp32Iters = 6;
         for (it = 0; it < max_iter; ++it) {
             // residual into W and the convergence test
             residual_norms<T>(ctx, n, m, ASp(), MSp(), ld, theta_d.as<double>(), Sp() + m, ld, cs1.as<double>(),
                               cs2.as<double>(), wsc);
-            const Xfer rx[2] = {{r2.data(), cs1.p, (size_t)m * 8}, {mx2.data(), cs2.p, (size_t)m * 8}};
-            d2h_n(ctx, rx, 2);
+            read_residuals(r2, mx2, it);
             const double floor = 1e-2 * std::fabs(theta[k - 1]);
             int nconv = 0;
             double worst = 0.0;
@@ -334,10 +354,14 @@ struct Lobpcg {
                 if (m - ma >= 8) {
                     for (int j = 0; j < k; ++j)
                         if (conv[j]) perm.push_back(j);
-                    double* nrm = reinterpret_cast<double*>(lock_host.data());
+                    // staged in the context's pinned buffer and copied without a
+                    // wait: the buffer is next written after the next residual
+                    // read, which waits for this copy (stream order)
+                    char* pin = static_cast<char*>(ctx->host_small);
+                    double* nrm = reinterpret_cast<double*>(pin);
                     for (int c = 0; c < m; ++c) nrm[c] = r2[perm[c]];  // r2: ||w_c||^2
-                    std::memcpy(lock_host.data() + (size_t)m * 8, perm.data(), (size_t)m * 4);
-                    h2d(ctx, lockbuf.p, lock_host.data(), (size_t)m * 12);
+                    std::memcpy(pin + (size_t)m * 8, perm.data(), (size_t)m * 4);
+                    HIPX(hipMemcpyAsync(lockbuf.p, pin, (size_t)m * 12, hipMemcpyHostToDevice, ctx->stream));
                     T* W = Sp() + m;
                     T* tmp = S2.as<T>() + m;  // S2 is free until the update below
                     gather_cols<T>(ctx, n, m, W, ld, tmp, ld,
@@ -371,7 +395,12 @@ struct Lobpcg {
                 }
             }
             const int N = m + nb;
-            if (!rayleigh_ritz(N)) LH_THROW(LH_ERR_NUMERIC, "Rayleigh-Ritz failed at iteration %d", it);
+            {
+                const double t0 = now_us();
+                rr_issue(N);  // status and theta are read with the next residual norms
+                rr_pending = true;
+                st.t_rr_us += now_us() - t0;
+            }
             // X_new = S C ; P_new = S[:, m:N] C[m:N, :]
             T* S2p = S2.as<T>();
             gemm_nn<T>(ctx, n, N, m, Sp(), ld, Cm.as<double>(), m, S2p, ld, 1.0, 0.0);
@@ -380,6 +409,15 @@ struct Lobpcg {
             std::swap(S, S2);
             spmm_AB(Sp(), ASp(), MSp(), m);
             have_p = true;
+        }
+        if (rr_pending) {  // the loop ended at max_iter: read the last Rayleigh-Ritz
+            int s = 0;
+            theta_new.resize(m);
+            const Xfer x[2] = {{&s, status.p, sizeof(int)}, {theta_new.data(), theta_d.p, (size_t)m * 8}};
+            d2h_n(ctx, x, 2);
+            rr_pending = false;
+            if (s) LH_THROW(LH_ERR_NUMERIC, "Rayleigh-Ritz failed at iteration %d", it - 1);
+            theta.swap(theta_new);
         }
         HIPX(hipStreamSynchronize(ctx->stream));
         st.iterations = it;

@@ -149,7 +149,8 @@ struct lh_ctx {
     Arena arena;
     int tuned_workers = 0;  // 0: not tuned
     lh_mpnn* mpnn = nullptr;  // network used by LH_PRECOND_LEARNED (not owned)
-    void* host_small = nullptr;   // pinned host memory of the PCG's status polling (hipHostMalloc)
+    void* host_small = nullptr;   // pinned host memory (hipHostMalloc): the PCG's status polling, the
+                                  // eigensolver's soft-locking uploads (never both at once on one context)
     size_t host_small_bytes = 0;
     void* host_stage = nullptr;   // pinned staging of d2h / h2d / d2h_n
     size_t host_stage_bytes = 0;
