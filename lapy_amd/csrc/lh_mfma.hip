@@ -211,11 +211,16 @@ k_gram_mfma_reduce(int nwaves, int ntile, int p, int q, int tv, int sym, const d
 // GROUPED (q <= 256): a workgroup owns all column blocks of its rows, so the
 // rows of U are fetched from HBM once for all of them.  INPLACE (Y = U,
 // p = q <= 256, grouped): the workgroup writes only after every wave of it has
-// read its rows of U.
-template <bool VEC, bool INPLACE, bool GROUPED, typename T>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+// read its rows of U.  A wave's tile is 64 rows x 16 NB columns: NB = 4 for
+// large n; NB = 1 for small n, where 64-column tiles leave most of the GPU idle
+// and each wave's chain of dependent matrix-core steps is four times shorter
+// (U, small enough to stay in the caches, is then read once per 16 columns).
+// nq counts the tile-wide column blocks.
+template <bool VEC, bool INPLACE, bool GROUPED, int NB, typename T>
+__global__ void __launch_bounds__(NB == 4 ? 256 : 1024) __attribute__((amdgpu_waves_per_eu(2)))
 k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restrict__ Cm, int ldc, T* Y, int ldy,
             double alpha, double beta, int upper, int nq) {
+    constexpr int TW = 16 * NB;
     typedef typename Mf<T>::v4 v4;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int il = lane & 15, kg = lane >> 4;
@@ -224,19 +229,19 @@ k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restri
     if (INPLACE || GROUPED) {
         const int rsub = (int)(blockDim.x >> 6) / nq;
         rw = xcd_block() * 64 * rsub + (int64_t)(wv / nq) * 64;
-        n0 = (wv % nq) * 64;
+        n0 = (wv % nq) * TW;
         if (!INPLACE && rw >= n) return;  // whole wave (no barrier follows)
     } else {
         rw = xcd_block() * 256 + (int64_t)wv * 64;
-        n0 = blockIdx.y * 64;
+        n0 = blockIdx.y * TW;
         if (rw >= n) return;  // whole wave
     }
-    const int kend = upper ? min(p, n0 + 64) : p;
-    v4 acc[4][4];
+    const int kend = upper ? min(p, n0 + TW) : p;
+    v4 acc[4][NB];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = zero4<T>();
+        for (int b = 0; b < NB; ++b) acc[a][b] = zero4<T>();
     for (int k0 = 0; k0 < kend; k0 += 16) {
         const int kc = k0 + 4 * kg;  // this lane's 4 summed indices
         v4 a[4];
@@ -245,11 +250,11 @@ k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restri
             const int64_t r = rw + 16 * bi + il;
             a[bi] = (r < n) ? load4<VEC, T>(U + r * ldu, kc, kend) : zero4<T>();
         }
-        T b[4][4];
+        T b[4][NB];
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
-            for (int bj = 0; bj < 4; ++bj) {
+            for (int bj = 0; bj < NB; ++bj) {
                 const int col = n0 + 16 * bj + il;
                 b[jj][bj] = (kc + jj < kend && col < q) ? (T)Cm[(int64_t)(kc + jj) * ldc + col] : T(0);
             }
@@ -258,7 +263,7 @@ k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restri
 #pragma unroll
             for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = Mf<T>::mma(a[bi][jj], b[jj][bj], acc[bi][bj]);
+                for (int bj = 0; bj < NB; ++bj) acc[bi][bj] = Mf<T>::mma(a[bi][jj], b[jj][bj], acc[bi][bj]);
     }
     if (INPLACE) __syncthreads();
     // per row block: all of its Y values are read before any is written (a
@@ -266,13 +271,13 @@ k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restri
     // interleaving them would cost one memory round trip per value)
 #pragma unroll
     for (int bi = 0; bi < 4; ++bi) {
-        T yv[4][4];
+        T yv[4][NB];
         if (beta != 0.0) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t row = rw + 16 * bi + Mf<T>::drow(kg, r);
 #pragma unroll
-                for (int bj = 0; bj < 4; ++bj) {
+                for (int bj = 0; bj < NB; ++bj) {
                     const int col = n0 + 16 * bj + il;
                     yv[r][bj] = (row < n && col < q) ? Y[row * ldy + col] : T(0);
                 }
@@ -283,7 +288,7 @@ k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restri
             const int64_t row = rw + 16 * bi + Mf<T>::drow(kg, r);
             if (row >= n) continue;
 #pragma unroll
-            for (int bj = 0; bj < 4; ++bj) {
+            for (int bj = 0; bj < NB; ++bj) {
                 const int col = n0 + 16 * bj + il;
                 if (col >= q) continue;
                 double v = alpha * (double)acc[bi][bj][r];
@@ -359,34 +364,62 @@ void gram_mfma(lh_ctx* ctx, int64_t n, const T* U, int ldu, int p, const T* V, i
     gram_launch<T>(ctx, g, n, U, ldu, p, V, ldv, q, sym, ws.as<double>(), C, ldc, alpha);
 }
 
+// 16-column tiles when 64-column ones would give fewer than 512 waves
+inline bool gemm_narrow(int64_t n, int q) { return (n + 63) / 64 * ((q + 63) / 64) < 512; }
+
 template <typename T>
 void gemm_mfma(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const double* C, int ldc, T* Y, int ldy,
                double alpha, double beta, bool upper) {
     if (n == 0 || q == 0) return;
     const bool vec = vec_ok(U, ldu);
-    const int nq = (q + 63) / 64;
     const int up = upper ? 1 : 0;
+    if (gemm_narrow(n, q)) {
+        const int nq = (q + 15) / 16;
+        if (nq > 1 && nq <= 16) {
+            // grouped: nq 16-column blocks x rsub row blocks of 64 per workgroup
+            const int rsub = std::max(1, 16 / nq);
+            const int bs = 64 * nq * rsub;
+            const unsigned nb = nblocks_xcd(n, 64 * rsub);
+            if (vec)
+                k_gemm_mfma<true, false, true, 1, T><<<nb, bs, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy,
+                                                                                alpha, beta, up, nq);
+            else
+                k_gemm_mfma<false, false, true, 1, T><<<nb, bs, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy,
+                                                                                 alpha, beta, up, nq);
+        } else {
+            dim3 grid(nblocks_xcd(n, 256), (unsigned)nq);
+            if (vec)
+                k_gemm_mfma<true, false, false, 1, T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy,
+                                                                                    alpha, beta, up, nq);
+            else
+                k_gemm_mfma<false, false, false, 1, T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y,
+                                                                                     ldy, alpha, beta, up, nq);
+        }
+        LH_CHECK_LAUNCH();
+        return;
+    }
+    const int nq = (q + 63) / 64;
     if (nq > 1 && nq <= 4) {
         // grouped: nq column blocks x rsub row blocks of 64 per workgroup
         const int rsub = nq == 2 ? 2 : 1;
         const int bs = 64 * nq * rsub;
         const unsigned nb = nblocks_xcd(n, 64 * rsub);
         if (vec)
-            k_gemm_mfma<true, false, true, T><<<nb, bs, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta,
-                                                                         up, nq);
+            k_gemm_mfma<true, false, true, 4, T><<<nb, bs, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha,
+                                                                            beta, up, nq);
         else
-            k_gemm_mfma<false, false, true, T><<<nb, bs, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha,
-                                                                          beta, up, nq);
+            k_gemm_mfma<false, false, true, 4, T><<<nb, bs, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha,
+                                                                             beta, up, nq);
         LH_CHECK_LAUNCH();
         return;
     }
     dim3 grid(nblocks_xcd(n, 256), (unsigned)nq);
     if (vec)
-        k_gemm_mfma<true, false, false, T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha, beta,
-                                                                          up, nq);
+        k_gemm_mfma<true, false, false, 4, T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha,
+                                                                             beta, up, nq);
     else
-        k_gemm_mfma<false, false, false, T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha,
-                                                                           beta, up, nq);
+        k_gemm_mfma<false, false, false, 4, T><<<grid, 256, 0, ctx->stream>>>(n, p, q, U, ldu, C, ldc, Y, ldy, alpha,
+                                                                              beta, up, nq);
     LH_CHECK_LAUNCH();
 }
 
@@ -396,17 +429,33 @@ void gemm_mfma_inplace(lh_ctx* ctx, int64_t n, int p, T* X, int ldx, const doubl
     if (n == 0 || p == 0) return;
     if (p > 256) LH_THROW(LH_ERR_ARG, "in-place block update needs width <= 256, got %d", p);
     const bool vec = vec_ok(X, ldx);
+    const int up = upper ? 1 : 0;
+    if (gemm_narrow(n, p)) {
+        // a workgroup covers all p columns of its rows: nq <= 16 waves of 16
+        // columns, rsub row blocks of 64
+        const int nq = (p + 15) / 16;
+        const int rsub = std::max(1, 16 / nq);
+        const int bs = 64 * nq * rsub;
+        const unsigned nb = nblocks_xcd(n, 64 * rsub);
+        if (vec)
+            k_gemm_mfma<true, true, true, 1, T><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha,
+                                                                           0.0, up, nq);
+        else
+            k_gemm_mfma<false, true, true, 1, T><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha,
+                                                                            0.0, up, nq);
+        LH_CHECK_LAUNCH();
+        return;
+    }
     const int nq = (p + 63) / 64;
     const int rsub = nq == 1 ? 4 : (nq == 2 ? 2 : 1);
     const int bs = 64 * nq * rsub;
     const unsigned nb = nblocks_xcd(n, 64 * rsub);
-    const int up = upper ? 1 : 0;
     if (vec)
-        k_gemm_mfma<true, true, true, T><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0, up,
-                                                                     nq);
+        k_gemm_mfma<true, true, true, 4, T><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0,
+                                                                        up, nq);
     else
-        k_gemm_mfma<false, true, true, T><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0, up,
-                                                                      nq);
+        k_gemm_mfma<false, true, true, 4, T><<<nb, bs, 0, ctx->stream>>>(n, p, p, X, ldx, C, ldc, X, ldx, alpha, 0.0,
+                                                                         up, nq);
     LH_CHECK_LAUNCH();
 }
 

@@ -67,6 +67,9 @@ def test_gram_deterministic():
     (3001, 64, 256, 64, 256, -1.0, 0.5),
     (2049, 48, 300, 48, 300, 1.0, 0.0),
     (17, 3, 5, 3, 5, 2.0, 1.0),
+    (3001, 56, 48, 56, 48, 1.0, 0.5),          # three 16-column tiles: 960-thread workgroups
+    (40000, 192, 64, 192, 192, 1.0, 0.0),      # n large enough for 64-column tiles
+    (33000, 128, 128, 192, 128, -1.0, 1.0),    # 64-column tiles, two per workgroup
 ])
 def test_gemm_matches_numpy(n, p, q, ldu, ldy, alpha, beta):
     rng = np.random.default_rng(n * 7 + p)
@@ -81,8 +84,10 @@ def test_gemm_matches_numpy(n, p, q, ldu, ldy, alpha, beta):
     assert np.array_equal(Y[:, q:], Y0[:, q:])
 
 
+# small n run on 16-column tiles, n >= 32,768 (at 64 columns) on 64-column ones
 @pytest.mark.parametrize("n,p,ld,upper", [(10000, 64, 192, True), (10007, 128, 192, True), (4099, 64, 192, False),
-                                          (3001, 192, 192, False), (1001, 72, 73, True), (17, 5, 5, False)])
+                                          (3001, 192, 192, False), (1001, 72, 73, True), (17, 5, 5, False),
+                                          (40000, 128, 192, True), (40000, 64, 64, False)])
 def test_gemm_in_place(n, p, ld, upper):
     """X <- X C in place (the Cholesky-QR and Ritz rotations): every
     workgroup reads all columns of its rows before any of them is written."""
