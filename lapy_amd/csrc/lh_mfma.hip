@@ -217,7 +217,7 @@ k_gram_mfma_reduce(int nwaves, int ntile, int p, int q, int tv, int sym, const d
 // (U, small enough to stay in the caches, is then read once per 16 columns).
 // nq counts the tile-wide column blocks.
 template <bool VEC, bool INPLACE, bool GROUPED, int NB, typename T>
-__global__ void __launch_bounds__(NB == 4 ? 256 : 1024) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ void __launch_bounds__(NB == 4 ? 256 : 512) __attribute__((amdgpu_waves_per_eu(2)))
 k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restrict__ Cm, int ldc, T* Y, int ldy,
             double alpha, double beta, int upper, int nq) {
     constexpr int TW = 16 * NB;
@@ -242,15 +242,14 @@ k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restri
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc[a][b] = zero4<T>();
-    for (int k0 = 0; k0 < kend; k0 += 16) {
-        const int kc = k0 + 4 * kg;  // this lane's 4 summed indices
-        v4 a[4];
+    // one K step's operands: 16 summed indices, this lane's 4 of them
+    auto load_step = [&](int k0, v4 (&a)[4], T (&b)[4][NB]) {
+        const int kc = k0 + 4 * kg;
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi) {
             const int64_t r = rw + 16 * bi + il;
-            a[bi] = (r < n) ? load4<VEC, T>(U + r * ldu, kc, kend) : zero4<T>();
+            a[bi] = (r < n && k0 < kend) ? load4<VEC, T>(U + r * ldu, kc, kend) : zero4<T>();
         }
-        T b[4][NB];
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
@@ -258,12 +257,39 @@ k_gemm_mfma(int64_t n, int p, int q, const T* U, int ldu, const double* __restri
                 const int col = n0 + 16 * bj + il;
                 b[jj][bj] = (kc + jj < kend && col < q) ? (T)Cm[(int64_t)(kc + jj) * ldc + col] : T(0);
             }
+    };
+    auto mma_step = [&](const v4 (&a)[4], const T (&b)[4][NB]) {
 #pragma unroll
         for (int jj = 0; jj < 4; ++jj)
 #pragma unroll
             for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
                 for (int bj = 0; bj < NB; ++bj) acc[bi][bj] = Mf<T>::mma(a[bi][jj], b[jj][bj], acc[bi][bj]);
+    };
+    if constexpr (NB == 1) {
+        // 16-column tiles serve small n, where a step's loads are not hidden
+        // by other waves: the next step's operands are loaded before this
+        // step's matrix-core products
+        v4 a[4], an[4];
+        T b[4][NB], bn[4][NB];
+        load_step(0, a, b);
+        for (int k0 = 0; k0 < kend; k0 += 16) {
+            load_step(k0 + 16, an, bn);  // zeros past kend
+            mma_step(a, b);
+#pragma unroll
+            for (int bi = 0; bi < 4; ++bi) a[bi] = an[bi];
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+                for (int bj = 0; bj < NB; ++bj) b[jj][bj] = bn[jj][bj];
+        }
+    } else {
+        for (int k0 = 0; k0 < kend; k0 += 16) {
+            v4 a[4];
+            T b[4][NB];
+            load_step(k0, a, b);
+            mma_step(a, b);
+        }
     }
     if (INPLACE) __syncthreads();
     // per row block: all of its Y values are read before any is written (a
@@ -375,9 +401,10 @@ void gemm_mfma(lh_ctx* ctx, int64_t n, int p, int q, const T* U, int ldu, const 
     const int up = upper ? 1 : 0;
     if (gemm_narrow(n, q)) {
         const int nq = (q + 15) / 16;
-        if (nq > 1 && nq <= 16) {
-            // grouped: nq 16-column blocks x rsub row blocks of 64 per workgroup
-            const int rsub = std::max(1, 16 / nq);
+        if (nq > 1 && nq <= 8) {
+            // grouped: nq 16-column blocks x rsub row blocks of 64 per
+            // workgroup (at most 512 threads)
+            const int rsub = std::max(1, 8 / nq);
             const int bs = 64 * nq * rsub;
             const unsigned nb = nblocks_xcd(n, 64 * rsub);
             if (vec)
@@ -430,11 +457,11 @@ void gemm_mfma_inplace(lh_ctx* ctx, int64_t n, int p, T* X, int ldx, const doubl
     if (p > 256) LH_THROW(LH_ERR_ARG, "in-place block update needs width <= 256, got %d", p);
     const bool vec = vec_ok(X, ldx);
     const int up = upper ? 1 : 0;
-    if (gemm_narrow(n, p)) {
-        // a workgroup covers all p columns of its rows: nq <= 16 waves of 16
-        // columns, rsub row blocks of 64
+    if (p <= 128 && gemm_narrow(n, p)) {
+        // a workgroup covers all p columns of its rows: nq <= 8 waves of 16
+        // columns, rsub row blocks of 64 (at most 512 threads)
         const int nq = (p + 15) / 16;
-        const int rsub = std::max(1, 16 / nq);
+        const int rsub = std::max(1, 8 / nq);
         const int bs = 64 * nq * rsub;
         const unsigned nb = nblocks_xcd(n, 64 * rsub);
         if (vec)
