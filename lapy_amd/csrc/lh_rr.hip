@@ -474,6 +474,10 @@ k_rr_back(int N, int nev, double* __restrict__ work, unsigned long long* __restr
 }
 
 // ---- 5. orthonormalisation in the small space and output --------------------
+// 4 x 4 tiles of Z in the update: with N nev + nev^2 <= PK and N <= NMAX
+// there are at most 864 (ceil(N/4) ceil(nev/4), maximised over the
+// admissible shapes), one per thread of the 1024
+constexpr int POL_UT = 1;
 __global__ void __launch_bounds__(NB)
 k_rr_polish(int N, int nev, double* __restrict__ work, double* __restrict__ C, int* __restrict__ status,
             unsigned long long* __restrict__ prof) {
@@ -483,60 +487,160 @@ k_rr_polish(int N, int nev, double* __restrict__ work, double* __restrict__ C, i
     RRWork W(work, N, nev);
     const int tid = threadIdx.x;
     const int nz = N * nev;
-    if (nz + nev * nev <= PK) {
+    if (nz + nev * nev <= PK && ((N + 3) / 4) * ((nev + 3) / 4) <= NB * POL_UT) {
         double* E = Zs + nz;
         for (int idx = tid; idx < nz; idx += NB) Zs[idx] = W.Z[idx];
         __syncthreads();
-        for (int pass = 0; pass < 3; ++pass) {
-            // E = Z^T Z - I (symmetric: lower entries computed, mirrored)
-            double emax = 0.0;
-            for (int idx = tid; idx < nev * nev; idx += NB) {
-                const int a = idx / nev, b = idx - a * nev;
-                if (b > a) continue;
-                double s0 = 0.0, s1 = 0.0;
-                int i = 0;
-                for (; i + 1 < N; i += 2) {
-                    s0 += Zs[i * nev + a] * Zs[i * nev + b];
-                    s1 += Zs[(i + 1) * nev + a] * Zs[(i + 1) * nev + b];
+        if (nev >= 32) {
+            // both products in 4 x 4 register tiles: every LDS value loaded serves
+            // four multiply-adds
+            const int nt = (nev + 3) / 4, nr = (N + 3) / 4;
+            const int ntri = nt * (nt + 1) / 2, nupd = nr * nt;
+            for (int pass = 0; pass < 3; ++pass) {
+                // E = Z^T Z - I: lower-triangle tiles, mirrored
+                double emax = 0.0;
+                for (int t = tid; t < ntri; t += NB) {
+                    int ta = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+                    while ((ta + 1) * (ta + 2) / 2 <= t) ++ta;
+                    while (ta * (ta + 1) / 2 > t) --ta;
+                    const int a0 = 4 * ta, b0 = 4 * (t - ta * (ta + 1) / 2);
+                    double sacc[4][4];
+    #pragma unroll
+                    for (int u = 0; u < 4; ++u)
+    #pragma unroll
+                        for (int w = 0; w < 4; ++w) sacc[u][w] = 0.0;
+                    for (int i = 0; i < N; ++i) {
+                        const double* zr = Zs + i * nev;
+                        double za[4], zb[4];
+    #pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            za[u] = (a0 + u < nev) ? zr[a0 + u] : 0.0;
+                            zb[u] = (b0 + u < nev) ? zr[b0 + u] : 0.0;
+                        }
+    #pragma unroll
+                        for (int u = 0; u < 4; ++u)
+    #pragma unroll
+                            for (int w = 0; w < 4; ++w) sacc[u][w] += za[u] * zb[w];
+                    }
+    #pragma unroll
+                    for (int u = 0; u < 4; ++u)
+    #pragma unroll
+                        for (int w = 0; w < 4; ++w) {
+                            const int a = a0 + u, b = b0 + w;
+                            if (a >= nev || b > a) continue;
+                            const double ev = sacc[u][w] - (a == b ? 1.0 : 0.0);
+                            E[a * nev + b] = ev;
+                            E[b * nev + a] = ev;
+                            emax = fmax(emax, fabs(ev));
+                        }
                 }
-                if (i < N) s0 += Zs[i * nev + a] * Zs[i * nev + b];
-                const double ev = (s0 + s1) - (a == b ? 1.0 : 0.0);
-                E[a * nev + b] = ev;
-                E[b * nev + a] = ev;
-                emax = fmax(emax, fabs(ev));
-            }
-            // block max of |E|
-            for (int off = 32; off > 0; off >>= 1) emax = fmax(emax, __shfl_xor(emax, off, 64));
-            __syncthreads();
-            if ((tid & 63) == 0) red[tid >> 6] = emax;
-            __syncthreads();
-            double em = 0.0;
-#pragma unroll
-            for (int w = 0; w < NB / 64; ++w) em = fmax(em, red[w]);
-            // Newton-Schulz needs ||E|| < 1; a basis that far off is left as it is
-            if (em >= 0.5 || (pass > 0 && em <= 1e-8)) break;
-            // Z <- Z - Z E / 2 ; each thread owns entries idx = tid + NB * u
-            constexpr int U = (PK + NB - 1) / NB;
-            double nz_[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int idx = tid + NB * u;
-                nz_[u] = 0.0;
-                if (idx < nz) {
-                    const int i = idx / nev, c = idx - i * nev;
-                    double s = 0.0;
-                    for (int h = 0; h < nev; ++h) s += Zs[i * nev + h] * E[h * nev + c];
-                    nz_[u] = Zs[idx] - 0.5 * s;
+                // block max of |E|
+                for (int off = 32; off > 0; off >>= 1) emax = fmax(emax, __shfl_xor(emax, off, 64));
+                __syncthreads();
+                if ((tid & 63) == 0) red[tid >> 6] = emax;
+                __syncthreads();
+                double em = 0.0;
+    #pragma unroll
+                for (int w = 0; w < NB / 64; ++w) em = fmax(em, red[w]);
+                // Newton-Schulz needs ||E|| < 1; a basis that far off is left as it is
+                if (em >= 0.5 || (pass > 0 && em <= 1e-8)) break;
+                // Z <- Z - Z E / 2: one 4 x 4 tile of Z per thread, written after a
+                // barrier once every tile has been read
+                const bool has = tid < nupd;
+                const int i0 = 4 * (tid / nt), c0 = 4 * (tid % nt);
+                double sacc[4][4];
+    #pragma unroll
+                for (int u = 0; u < 4; ++u)
+    #pragma unroll
+                    for (int w = 0; w < 4; ++w) sacc[u][w] = 0.0;
+                if (has) {
+                    for (int h = 0; h < nev; ++h) {
+                        const double* er = E + h * nev;
+                        double zi[4], eh[4];
+    #pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            zi[u] = (i0 + u < N) ? Zs[(i0 + u) * nev + h] : 0.0;
+                            eh[u] = (c0 + u < nev) ? er[c0 + u] : 0.0;
+                        }
+    #pragma unroll
+                        for (int u = 0; u < 4; ++u)
+    #pragma unroll
+                            for (int w = 0; w < 4; ++w) sacc[u][w] += zi[u] * eh[w];
+                    }
+    #pragma unroll
+                    for (int u = 0; u < 4; ++u)
+    #pragma unroll
+                        for (int w = 0; w < 4; ++w) {
+                            const int i = i0 + u, c = c0 + w;
+                            sacc[u][w] = (i < N && c < nev) ? Zs[i * nev + c] - 0.5 * sacc[u][w] : 0.0;
+                        }
                 }
+                __syncthreads();
+                if (has) {
+    #pragma unroll
+                    for (int u = 0; u < 4; ++u)
+    #pragma unroll
+                        for (int w = 0; w < 4; ++w) {
+                            const int i = i0 + u, c = c0 + w;
+                            if (i < N && c < nev) Zs[i * nev + c] = sacc[u][w];
+                        }
+                }
+                __syncthreads();
+                if (em <= 1e-8) break;  // one step leaves ~ em^2: rounding level
             }
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int idx = tid + NB * u;
-                if (idx < nz) Zs[idx] = nz_[u];
+        } else {
+            // few columns: one output per thread keeps more threads busy
+            for (int pass = 0; pass < 3; ++pass) {
+                // E = Z^T Z - I (symmetric: lower entries computed, mirrored)
+                double emax = 0.0;
+                for (int idx = tid; idx < nev * nev; idx += NB) {
+                    const int a = idx / nev, b = idx - a * nev;
+                    if (b > a) continue;
+                    double s0 = 0.0, s1 = 0.0;
+                    int i = 0;
+                    for (; i + 1 < N; i += 2) {
+                        s0 += Zs[i * nev + a] * Zs[i * nev + b];
+                        s1 += Zs[(i + 1) * nev + a] * Zs[(i + 1) * nev + b];
+                    }
+                    if (i < N) s0 += Zs[i * nev + a] * Zs[i * nev + b];
+                    const double ev = (s0 + s1) - (a == b ? 1.0 : 0.0);
+                    E[a * nev + b] = ev;
+                    E[b * nev + a] = ev;
+                    emax = fmax(emax, fabs(ev));
+                }
+                // block max of |E|
+                for (int off = 32; off > 0; off >>= 1) emax = fmax(emax, __shfl_xor(emax, off, 64));
+                __syncthreads();
+                if ((tid & 63) == 0) red[tid >> 6] = emax;
+                __syncthreads();
+                double em = 0.0;
+    #pragma unroll
+                for (int w = 0; w < NB / 64; ++w) em = fmax(em, red[w]);
+                // Newton-Schulz needs ||E|| < 1; a basis that far off is left as it is
+                if (em >= 0.5 || (pass > 0 && em <= 1e-8)) break;
+                // Z <- Z - Z E / 2 ; each thread owns entries idx = tid + NB * u
+                constexpr int U = (PK + NB - 1) / NB;
+                double nz_[U];
+    #pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int idx = tid + NB * u;
+                    nz_[u] = 0.0;
+                    if (idx < nz) {
+                        const int i = idx / nev, c = idx - i * nev;
+                        double s = 0.0;
+                        for (int h = 0; h < nev; ++h) s += Zs[i * nev + h] * E[h * nev + c];
+                        nz_[u] = Zs[idx] - 0.5 * s;
+                    }
+                }
+                __syncthreads();
+    #pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int idx = tid + NB * u;
+                    if (idx < nz) Zs[idx] = nz_[u];
+                }
+                __syncthreads();
+                if (em <= 1e-8) break;  // one step leaves ~ em^2: rounding level
             }
-            __syncthreads();
-            if (em <= 1e-8) break;  // one step leaves ~ em^2: rounding level
         }
         for (int idx = tid; idx < nz; idx += NB) C[idx] = Zs[idx];
     } else {
