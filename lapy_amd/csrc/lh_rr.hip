@@ -393,15 +393,19 @@ k_rr_invit(int N, int nev, double* __restrict__ work) {
     if (!leader) return;  // whole wave
     int gend = j + 1;
     while (gend < nev && lam[gend] - lam[gend - 1] <= 1e-7 * fmax(fabs(lam[gend]), 1e-6 * tnorm)) ++gend;
+    // T and the shifts scaled by a power of two (exact) to ||T|| <= 1: the
+    // iterates grow like 1 / (eps ||T||) and their squared norms must stay
+    // finite and normal whatever the magnitude of H
+    const double sc = ldexp(1.0, -ilogb(tnorm) - 1);
     for (int i = q; i < N; i += 64) {
-        sd[i] = W.d[i];
-        se[i] = W.e[i];
+        sd[i] = W.d[i] * sc;
+        se[i] = W.e[i] * sc;
     }
-    const double tiny = 2.2e-16 * tnorm;
+    const double tiny = 2.2e-16 * tnorm * sc;
     for (int g = j; g < gend; ++g) {
         for (int i = q; i < N; i += 64) y[i] = 1.0 + 0.1 * (double)((7 * i + 3 * g) % 11);
         __syncthreads();
-        if (q == 0) tri_factor(sd, se, N, lam[g] + (double)(g - j) * 8.0 * tiny, tiny, ws);
+        if (q == 0) tri_factor(sd, se, N, lam[g] * sc + (double)(g - j) * 8.0 * tiny, tiny, ws);
         for (int itr = 0; itr < 2; ++itr) {
             __syncthreads();
             if (q == 0) tri_lu_solve(ws, N, y);

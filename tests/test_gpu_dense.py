@@ -50,6 +50,27 @@ def test_clusters_and_multiplicities():
     check(sym_with_spectrum(np.sort(lam), 1), 64)
 
 
+def test_diagonal_and_block_diagonal():
+    """A diagonal matrix has a zero column below every diagonal entry (every
+    reflector is the identity, tau = 0); two decoupled blocks give zeros in
+    part of every column."""
+    rng = np.random.default_rng(3)
+    d = rng.uniform(-2.0, 5.0, 96)
+    check(np.diag(d), 20)
+    H = np.zeros((100, 100))
+    H[:40, :40] = sym_with_spectrum(np.linspace(1.0, 2.0, 40), 5)
+    H[40:, 40:] = sym_with_spectrum(np.linspace(0.5, 3.0, 60), 6)
+    check(H, 30)
+
+
+@pytest.mark.parametrize("scale", [1e-150, 1e150])
+def test_extreme_magnitudes(scale):
+    """Norms beyond the range of the short-latency square root and reciprocal
+    take the exact ones; the bisection works on T scaled by a power of two."""
+    lam = np.sort(np.random.default_rng(7).uniform(-3.0, 10.0, 168))
+    check(sym_with_spectrum(lam, 9) * scale, 56)
+
+
 def test_lobpcg_like_wide_spectrum():
     rng = np.random.default_rng(5)
     lam = np.sort(np.concatenate([[1e-12], rng.uniform(0.4, 60.0, 63), 10 ** rng.uniform(2, 5.7, 128)]))
