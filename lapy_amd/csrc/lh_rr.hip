@@ -1,24 +1,27 @@
 // Rayleigh-Ritz eigensolver for an orthonormal trial basis: the smallest nev
 // eigenpairs of a small dense symmetric H (N <= 192), FP64.
 //
-// Five kernels on the library stream:
-//  1. k_rr_tri (one workgroup, 1024 threads): the lower triangle of H is held
-//     packed in LDS (N(N+1)/2 doubles, 148 KB at N = 192) and reduced to
-//     tridiagonal form by Householder reflections, three barriers per step.
+// Five kernels on the library stream, each bound by the latency of its
+// dependent steps on one workgroup (or one wave per eigenpair):
+//  1. k_rr_tri (one workgroup, 512 threads): H is held in registers, each
+//     row spread over the 16 lanes of one DPP row, and reduced to
+//     tridiagonal form by Householder reflections, two barriers per step.
 //     The reflectors, the tridiagonal T and its Gershgorin bounds are parked
 //     in global memory.
-//  2. k_rr_bisect (one wave per wanted eigenvalue): multisection on T, the
-//     64 lanes evaluating Sturm counts at 64 interior points (65-fold
-//     bracket reduction per round).
+//  2. k_rr_bisect (one wave per wanted eigenvalue): multisection on T scaled
+//     by a power of two to ||T|| <= 1, the 64 lanes evaluating Sturm counts
+//     at 64 interior points (65-fold bracket reduction per round).
 //  3. k_rr_invit (one wave per eigenvalue; only the first of a group of
 //     numerically equal eigenvalues works): inverse iteration with the LU
-//     factors of T - lambda I in LDS, Gram-Schmidt inside the group.
+//     factors of T - lambda I (same scaling) in LDS, Gram-Schmidt inside the
+//     group.
 //  4. k_rr_back (one wave per eigenvector): Z := Q Y, the reflectors applied
-//     in reverse order, the next one loaded while the current is applied.
+//     in reverse order, loaded four steps ahead into registers.
 //  5. k_rr_polish (one workgroup): Newton-Schulz orthonormalisation of Z in
 //     the small space, Z := Z (3 I - Z^T Z) / 2 (inverse iteration leaves
 //     orthogonality errors ~ eps ||T|| / gap; each step squares them; up to
-//     three steps, none for a basis with ||Z^T Z - I|| >= 1/2).
+//     three steps, none for a basis with ||Z^T Z - I|| >= 1/2), its products
+//     in 4 x 4 register tiles from 32 columns up.
 //
 // Replaces the dense Rayleigh-Ritz step ARPACK performs inside eigsh
 // (lapy/solver.py:713-715) for the block solver.
