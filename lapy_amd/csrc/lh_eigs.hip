@@ -13,8 +13,13 @@
 //   W = D^-1 (AX - BX Theta),  D = diag(A - sigma*B)      (Jacobi)
 //   W <- W - X (BX)^T W (twice),  Cholesky-QR in the B inner product
 //   P <- P - X (BX)^T P - W (BW)^T P,  Cholesky-QR (dropped on breakdown)
-//   Rayleigh-Ritz on span(S) with the one-workgroup FP64 solver (lh_dense)
+//   Rayleigh-Ritz on span(S) with the one-workgroup FP64 solver (lh_rr.hip)
 //   X <- S C,  P <- S[:, m:] C[m:]
+//
+// The host waits for the device at the residual norms (which also bring the
+// previous Rayleigh-Ritz's status and Ritz values) and at each Cholesky-QR
+// pass's pivot check; with the Jacobi and Chebyshev preconditioners
+// everything else is queued without waiting.
 //
 // Convergence: ||A x_j - theta_j B x_j||_2 / ||B x_j||_2 <= tol * scale_j with
 // scale_j = max(|theta_j|, 1e-2 * |theta_{k-1}|) for the k wanted columns.
@@ -22,13 +27,11 @@
 
 #include <algorithm>
 #include <atomic>
-#include <mutex>
-#include <thread>
-
-#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <thread>
 
 using namespace lh;
 
