@@ -9,7 +9,8 @@
 //    H = L^-1 GA L^-T, Householder tridiagonalisation, bisection for the
 //    wanted (smallest) eigenvalues, inverse iteration for their vectors,
 //    back-transformation, C = L^-T Z.
-//  * cholinv: T = L^-T for G = L L^T (Cholesky-QR of a block).
+//  * cholinv: T = L^-T for G = L L^T (Cholesky-QR of a block), by symmetric
+//    elimination with the Schur complement and T in registers.
 //
 // These replace the dense steps ARPACK performs inside eigsh
 // (lapy/solver.py:713-715); the algorithm is block LOBPCG, not Lanczos.

@@ -17,10 +17,10 @@
 // rows and columns is undone when the wave writes its partial tile.  The
 // partials are summed in fixed order by a second kernel (deterministic).
 //
-// Update: a wave owns 64 rows x 64 columns of Y; per group of 16 columns of U
-// (the summed index) each lane loads 4 consecutive values of its row (one
-// 32-byte load), which serve as the A operand of 4 successive MFMAs, and the
-// matching 4 rows of C (L2-resident) as B.
+// Update: a wave owns 64 rows x 64 columns of Y (64 x 16 for small n); per
+// group of 16 columns of U (the summed index) each lane loads 4 consecutive
+// values of its row (one 32-byte load), which serve as the A operand of 4
+// successive MFMAs, and the matching 4 rows of C (L2-resident) as B.
 #include "lh_dense.hpp"
 
 #include <mutex>
