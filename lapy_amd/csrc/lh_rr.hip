@@ -88,9 +88,12 @@ __device__ int sturm(const double* d, const double* e2, int N, double x, double 
     return cnt;
 }
 
-// LU with partial pivoting of T - lam I into ws (u0, u1, u2, l, piv: 5N)
+// LU with partial pivoting of T - lam I into ws (1/u0, u1, u2, l, piv: 5N).
+// The caller scales T to ||T|| <= 1, so the pivots (a zero or sub-normal one
+// replaced by tiny) have ordinary magnitudes and their reciprocals come from
+// the short-latency form, stored for the solves
 __device__ void tri_factor(const double* d, const double* e, int N, double lam, double tiny, double* ws) {
-    double* u0 = ws;
+    double* iu0 = ws;
     double* u1 = ws + N;
     double* u2 = ws + 2 * N;
     double* lm = ws + 3 * N;
@@ -100,26 +103,28 @@ __device__ void tri_factor(const double* d, const double* e, int N, double lam, 
         const double sub = e[i];
         const double a1 = d[i + 1] - lam;
         const double b1 = (i + 1 < N - 1) ? e[i + 1] : 0.0;
-        if (fabs(c0) >= fabs(sub)) {
-            const double piv = (c0 != 0.0) ? c0 : tiny;
-            const double l = sub / piv;
-            u0[i] = piv; u1[i] = c1; u2[i] = c2; lm[i] = l; pv[i] = 0.0;
+        if (fabs(c0) >= fabs(sub) || !(fabs(sub) > 1e-300)) {
+            const double piv = (fabs(c0) > 1e-300) ? c0 : tiny;
+            const double ip = rcp_nr(piv);
+            const double l = sub * ip;
+            iu0[i] = ip; u1[i] = c1; u2[i] = c2; lm[i] = l; pv[i] = 0.0;
             c0 = a1 - l * c1;
             c1 = b1 - l * c2;
         } else {
-            const double l = c0 / sub;
-            u0[i] = sub; u1[i] = a1; u2[i] = b1; lm[i] = l; pv[i] = 1.0;
+            const double ip = rcp_nr(sub);
+            const double l = c0 * ip;
+            iu0[i] = ip; u1[i] = a1; u2[i] = b1; lm[i] = l; pv[i] = 1.0;
             c0 = c1 - l * a1;
             c1 = c2 - l * b1;
         }
         c2 = 0.0;
     }
-    u0[N - 1] = (fabs(c0) > tiny) ? c0 : tiny;
+    iu0[N - 1] = rcp_nr((fabs(c0) > tiny) ? c0 : tiny);
 }
 
 // solve with the factorisation; y overwritten
 __device__ void tri_lu_solve(const double* ws, int N, double* y) {
-    const double* u0 = ws;
+    const double* iu0 = ws;
     const double* u1 = ws + N;
     const double* u2 = ws + 2 * N;
     const double* lm = ws + 3 * N;
@@ -131,13 +136,13 @@ __device__ void tri_lu_solve(const double* ws, int N, double* y) {
         y[i] = a;
         a = b - lm[i] * a;
     }
-    double x2 = a / u0[N - 1];
+    double x2 = a * iu0[N - 1];
     y[N - 1] = x2;
     if (N >= 2) {
-        double x1 = (y[N - 2] - u1[N - 2] * x2) / u0[N - 2];
+        double x1 = (y[N - 2] - u1[N - 2] * x2) * iu0[N - 2];
         y[N - 2] = x1;
         for (int i = N - 3; i >= 0; --i) {
-            double x0 = (y[i] - u1[i] * x1 - u2[i] * x2) / u0[i];
+            double x0 = (y[i] - u1[i] * x1 - u2[i] * x2) * iu0[i];
             y[i] = x0;
             x2 = x1;
             x1 = x0;
