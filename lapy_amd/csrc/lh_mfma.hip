@@ -16,6 +16,8 @@
 // the columns 4 i + bi of U and 4 j + bj of V.  That permutation of the tile's
 // rows and columns is undone when the wave writes its partial tile.  The
 // partials are summed in fixed order by a second kernel (deterministic).
+// Narrow blocks (at most 9 useful blocks of 16 x 16) use a variant whose
+// blocks hold contiguous columns, so the blocks beyond p or q are skipped.
 //
 // Update: a wave owns 64 rows x 64 columns of Y (64 x 16 for small n); per
 // group of 16 columns of U (the summed index) each lane loads 4 consecutive
@@ -156,6 +158,78 @@ k_gram_mfma(int64_t n, int64_t rows_per_wave, int nwaves, const T* __restrict__ 
                 const int i = 4 * Mf<T>::drow(kg, r) + bi;  // tile-local column of U
                 const int j = 4 * il + bj;                  // tile-local column of V
                 dst[i * 64 + j] = (double)acc[bi][bj][r];
+            }
+}
+
+// Gram of narrow blocks (p, q <= 64, at most 9 useful 16 x 16 blocks): block
+// (bi, bj) holds the contiguous columns 16 bi .. 16 bi + 15 of U and 16 bj ..
+// 16 bj + 15 of V, so the blocks wholly beyond p or q are skipped (a 16-column
+// Gram runs one matrix-core block instead of 16).  Lane l loads U[r][16 bi +
+// l % 16] of row r = 4 s + l / 16 (128 contiguous bytes per 16 lanes).  The
+// partial tile has the layout of k_gram_mfma's; only its blocks bi < nbu,
+// bj < nbv are written, which hold every entry i < p, j < q.
+template <typename T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_gram_mfma_narrow(int64_t n, int64_t rows_per_wave, int nwaves, const T* __restrict__ U, int ldu, int p,
+                   const T* __restrict__ V, int ldv, int q, double* __restrict__ part) {
+    typedef typename Mf<T>::v4 v4;
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= nwaves) return;  // whole wave exits together
+    const int il = lane & 15, kg = lane >> 4;
+    const int nbu = (p + 15) >> 4, nbv = (q + 15) >> 4;
+    const int64_t rb = (int64_t)w * rows_per_wave;
+    const int64_t re = (rb + rows_per_wave < n) ? rb + rows_per_wave : n;
+    v4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = zero4<T>();
+    auto load_rows = [&](int64_t r0, T (&a)[2][4], T (&b)[2][4]) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int64_t r = r0 + 4 * s + kg;
+            const bool in = r < re;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int cc = 16 * c + il;
+                a[s][c] = (c < nbu && in && cc < p) ? U[r * ldu + cc] : T(0);
+                b[s][c] = (c < nbv && in && cc < q) ? V[r * ldv + cc] : T(0);
+            }
+        }
+    };
+    T a[2][4], b[2][4];
+    if (rb < re) load_rows(rb, a, b);
+    for (int64_t r0 = rb; r0 < re; r0 += 8) {
+        T an[2][4], bn[2][4];
+        const bool more = r0 + 8 < re;
+        if (more) load_rows(r0 + 8, an, bn);
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 4; ++bj)
+                    if (bi < nbu && bj < nbv) acc[bi][bj] = Mf<T>::mma(a[s][bi], b[s][bj], acc[bi][bj]);
+        if (more) {
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    a[s][c] = an[s][c];
+                    b[s][c] = bn[s][c];
+                }
+        }
+    }
+    double* dst = part + (int64_t)w * 4096;  // one tile
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
+            if (bi < nbu && bj < nbv) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    dst[(16 * bi + Mf<T>::drow(kg, r)) * 64 + 16 * bj + il] = (double)acc[bi][bj][r];
             }
 }
 
@@ -362,12 +436,17 @@ GramPlan gram_plan(const lh_ctx* ctx, int64_t n, int p, int q, bool sym, int64_t
     return g;
 }
 
+// one output tile of which at most 9 of the 16 matrix-core blocks are needed
+inline bool gram_narrow(int p, int q) { return p <= 64 && q <= 64 && ((p + 15) / 16) * ((q + 15) / 16) <= 9; }
+
 template <typename T>
 void gram_launch(lh_ctx* ctx, const GramPlan& g, int64_t n, const T* U, int ldu, int p, const T* V, int ldv, int q,
                  bool sym, double* part, double* C, int ldc, double alpha) {
     const bool vec = vec_ok(U, ldu) && vec_ok(V, ldv);
     dim3 grid((unsigned)((g.nw + 3) / 4), (unsigned)g.ntile);
-    if (vec)
+    if (gram_narrow(p, q))
+        k_gram_mfma_narrow<T><<<grid, 256, 0, ctx->stream>>>(n, g.rpw, (int)g.nw, U, ldu, p, V, ldv, q, part);
+    else if (vec)
         k_gram_mfma<true, T><<<grid, 256, 0, ctx->stream>>>(n, g.rpw, (int)g.nw, U, ldu, p, V, ldv, q, g.tv,
                                                             sym ? 1 : 0, part);
     else

@@ -20,6 +20,13 @@ pytestmark = pytest.mark.gpu
     (3001, 56, 112, 168, 168),
     (2000, 100, 37, 101, 37),
     (17, 5, 3, 7, 3),
+    # narrow blocks (at most 9 of the 16 matrix-core blocks of the tile used)
+    (10242, 16, 16, 16, 16),
+    (10242, 48, 48, 48, 48),
+    (5001, 33, 47, 48, 47),
+    (8000, 16, 48, 64, 48),
+    (4000, 64, 32, 64, 32),
+    (1001, 40, 20, 41, 21),
 ])
 def test_gram_matches_numpy(n, p, q, ldu, ldv):
     rng = np.random.default_rng(n + p + q)
@@ -31,7 +38,8 @@ def test_gram_matches_numpy(n, p, q, ldu, ldv):
     np.testing.assert_allclose(C, ref, rtol=0, atol=1e-13 * scale * 8)
 
 
-@pytest.mark.parametrize("n,p,ld", [(10000, 64, 192), (5003, 128, 192), (4099, 192, 192), (1001, 72, 73)])
+@pytest.mark.parametrize("n,p,ld", [(10000, 64, 192), (5003, 128, 192), (4099, 192, 192), (1001, 72, 73),
+                                    (10242, 48, 48), (999, 33, 35)])
 def test_gram_symmetric_tiles(n, p, ld):
     """S^T (A S) with A SPD: the symmetric form fills the lower tiles from the
     upper ones, so tiles off the diagonal are exactly mirrored."""
@@ -124,6 +132,8 @@ def _f32(a):
     (3001, 56, 112, 168, False),
     (2000, 100, 37, 101, False),
     (17, 5, 3, 7, False),
+    (10242, 48, 48, 48, True),
+    (3001, 16, 40, 41, False),
 ])
 def test_gram_fp32_matrix_cores(n, p, q, ld, sym):
     """FP32 blocks (the mixed-precision eigensolver's): products of the
