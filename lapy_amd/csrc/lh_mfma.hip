@@ -266,7 +266,24 @@ k_gram_mfma_reduce(int nwaves, int ntile, int p, int q, int tv, int sym, const d
         if (i < p && j < q) {
             const double* src = part + o;
             const int64_t stride = (int64_t)ntile * 4096;
-            for (int c = g; c < nwaves; c += 8) s += src[c * stride];
+            // eight loads in flight, summed in the order of c
+            int c = g;
+            for (; c + 56 < nwaves; c += 64) {
+                double x[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) x[u] = src[(int64_t)(c + 8 * u) * stride];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += x[u];
+            }
+            if (c + 24 < nwaves) {
+                double x[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) x[u] = src[(int64_t)(c + 8 * u) * stride];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) s += x[u];
+                c += 32;
+            }
+            for (; c < nwaves; c += 8) s += src[c * stride];
         }
     }
     sh[g][threadIdx.x & 63] = s;
