@@ -1,7 +1,8 @@
 """The eigensolver's tall-skinny FP64 block products (matrix-core kernels)
 against numpy: Gram C = U^T V (full and symmetric-tile forms) and the update
 Y = alpha U C + beta Y, over widths that are and are not multiples of the
-64-wide tiles, leading dimensions larger than the width, odd leading
+64-wide tiles, narrow blocks (at most 9 useful 16 x 16 matrix-core blocks, the
+Gram's narrow kernel), leading dimensions larger than the width, odd leading
 dimensions (unaligned rows: the element-wise load path) and row counts that
 are not multiples of the 16-row groups."""
 
